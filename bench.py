@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Fit steps/s of the plain image fitter at 1280x720 with 60,000 splats.
+
+One step = ImageFitter::step of the reference (src/pipelines/image_fitter.cpp:103-112):
+zero grads, activation+projection, tile assign + sort + ranges, compositing
+with fused MSE, compositing backward, projection/activation backward, Adam.
+Inputs are synthetic and resident in HBM before the timed region starts.
+
+Prints ONE JSON line. `roofline` is for the compositing backward (the
+dominant kernel), timed with HIP events on the fitter's stream at the state
+reached after the timed steps; its algorithmic work (splat evaluations) is
+counted by the oracle on the same state. `cpu_baseline` is the oracle's fp32
+OpenMP transcription of the same step on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fit steps/s @720p 60k splats; time-to-30dB PSNR learned-init vs plain"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+# Algorithmic FP32 work per splat evaluation of the compositing backward
+# (FMA = 2 FLOP): falloff, alpha, the colour / alpha / conic gradients.
+BWD_FLOP_PER_EVAL = 60.0
+
+CONFIGS = {
+    # name: (W, H, N, n_ellipses, seed)
+    "c1": (1280, 720, 60000, 128, 1000),
+    "c0": (256, 256, 4096, 32, 0),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--graph", type=int, default=1, help="replay the step as a HIP graph")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--ppt", type=int, default=int(os.environ.get("DS_PPT", "4")),
+                    help="compositing pixels per thread (1, 2 or 4)")
+    return ap.parse_args()
+
+
+def roofline_bwd(ds, oracle, fit, W, H, N, reps=20):
+    L = ds.lib()
+    g = ds._lib.Gauss3D()
+    s2 = ds._lib.Splat2D()
+    gs2 = ds._lib.Splat2D()
+    grad_rgb = C.c_void_p()
+    ds.check(L.ds_fitter_views(fit.h, C.byref(g), C.byref(s2), C.byref(gs2), C.byref(grad_rgb)),
+             "ds_fitter_views")
+    ras = C.c_void_p()
+    ds.check(L.ds_fitter_get_raster(fit.h, C.byref(ras)), "ds_fitter_get_raster")
+    stream = fit.stream()
+    fit.forward()
+    fit.synchronize()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    ds.check(L.ds_event_create(C.byref(e0)), "event")
+    ds.check(L.ds_event_create(C.byref(e1)), "event")
+    ds.check(L.ds_raster_bwd(ras, C.byref(s2), grad_rgb, C.byref(gs2), stream), "warm")
+    ds.check(L.ds_event_record(e0, stream), "record")
+    for _ in range(reps):
+        ds.check(L.ds_raster_bwd(ras, C.byref(s2), grad_rgb, C.byref(gs2), stream),
+                 "ds_raster_bwd")
+    ds.check(L.ds_event_record(e1, stream), "record")
+    ds.check(L.ds_event_synchronize(e1), "sync")
+    ms = C.c_float()
+    ds.check(L.ds_event_elapsed_ms(e0, e1, C.byref(ms)), "elapsed")
+    avg_s = ms.value / reps * 1e-3
+    L.ds_event_destroy(e0)
+    L.ds_event_destroy(e1)
+    # algorithmic work on the same state, counted by the oracle
+    st = fit.raster_state()
+    s2_host = ds.download_ptr(s2.pos_x, np.float32, 10 * N)
+    g_host = ds.download_ptr(grad_rgb.value, np.float32, 3 * W * H)
+    _, e_bwd = oracle.comp_bwd(s2_host, N, st["flatten_ids"], st["tile_ranges"], W, H, st["T"],
+                               st["last_ids"], g_host, dtype=np.float32)
+    flops = e_bwd * BWD_FLOP_PER_EVAL
+    achieved = flops / avg_s / 1e12
+    return {"kernel": "k_composite_bwd", "bound": "mfma", "unit": "TFLOP/s",
+            "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "avg_launch_ms": round(avg_s * 1e3, 4), "splat_evals_per_launch": int(e_bwd),
+            "flop_per_eval": BWD_FLOP_PER_EVAL, "n_isects": int(st["n_isects"]),
+            "note": "fp32 vector (VALU) work priced against the 157.3 TFLOP/s fp32 peak"}
+
+
+def cpu_baseline(oracle, params, target, W, H, N, pv, cam, lr6, seconds):
+    threads = oracle.set_threads(0)
+    fit = oracle.Fit(params, target, N, W, H, 0, pv, cam, lr6)
+    for _ in range(2):
+        fit.step()
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        fit.step()
+        k += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or k >= 200:
+            break
+    fit.close()
+    return {"value": round(k / el, 4), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{k} fp32 steps of the same {W}x{H}/{N}-splat fit after 2 warm-up steps "
+                      f"(oracle/ds_oracle.cpp, OpenMP, {threads} threads), {el:.1f} s"}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on exactly one GPU: every box here has one GPU and this build "
+              "measures one GPU's throughput (multi-GPU sharding is design-only).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import diffsplat_amd as ds
+
+    W, H, N, n_ell, seed = CONFIGS[args.config]
+    target = ds.target_image(seed, W, H, n_ell)
+    params = ds.frustum_splats(seed, N, aspect=W / H)
+    view, proj, cam = ds.fitter_camera(W, H)
+    lr = {"lr_pos": 1e-2, "lr_sh_dc": 1e-2, "lr_scale": 5e-3, "lr_rot": 6e-4,
+          "lr_opacity": 6e-3}
+
+    fit = ds.ImageFitter(W, H, N, view=view, proj=proj, cam_pos=cam, lr=lr)
+    fit.set_target(target)
+    fit.upload_params(params)
+    ras = C.c_void_p()
+    ds.check(ds.lib().ds_fitter_get_raster(fit.h, C.byref(ras)), "ds_fitter_get_raster")
+    ds.check(ds.lib().ds_raster_set_tuning(ras, args.ppt), "ds_raster_set_tuning")
+    fit.set_graph(bool(args.graph))
+    fit.step(args.warmup)
+    fit.synchronize()
+    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    fit.step(args.steps)
+    fit.synchronize()
+    el = time.perf_counter() - t0
+    print(f"[bench] {args.steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
+    fit.forward()
+    loss = fit.loss()
+    value = args.steps / el
+
+    roof = None
+    cpu = None
+    if not args.no_roofline or not args.no_cpu:
+        import oracle
+    if not args.no_roofline:
+        roof = roofline_bwd(ds, oracle, fit, W, H, N)
+        print(f"[bench] roofline {roof}", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        pv = ds.mat_mul(proj, view)
+        lr6 = np.array([lr["lr_pos"], lr["lr_scale"], lr["lr_rot"], lr["lr_sh_dc"], 6e-3,
+                        lr["lr_opacity"]], np.float32)
+        cpu = cpu_baseline(oracle, params, target, W, H, N, pv, cam, lr6, args.cpu_seconds)
+        print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain image fitter {W}x{H}, {N} random-init splats in a 60 deg "
+                               f"pinhole frustum, {n_ell}-ellipse synthetic target, "
+                               "fwd+bwd+Adam per step",
+                   "width": W, "height": H, "splats": N, "graph": bool(args.graph),
+                   "pix_per_thread": args.ppt,
+                   "loss_after": round(loss, 6),
+                   "psnr_db_after": round(ds.psnr_from_loss(loss), 3),
+                   "steps_trained": args.steps + args.warmup},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["vs_cpu_baseline"] = round(value / cpu["value"], 2)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,328 @@
+"""diffsplat_amd -- differentiable 3D Gaussian splatting for MI355X (gfx950).
+
+The compute path is libdiffsplat.so (hand-written HIP kernels behind the C-ABI
+in include/ds_api.h, with the reference's layer classes in C++ on top). This
+package is a thin Python handle on that library: synthetic workloads, camera
+matrices and the fitter / renderer / rasterizer objects.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from ._lib import (AdamGroup, AdamHParams, DeviceArray, DiffsplatError, FitterConfig,
+                   RasterConfig, RasterView, check, download_ptr, farr, fptr, gauss_view,
+                   has_gpu, lib, splat2d_view, splat3d_view)
+
+__all__ = [
+    "lib", "has_gpu", "check", "DiffsplatError", "DeviceArray", "gauss_view", "splat3d_view",
+    "splat2d_view", "sh_bands", "slab_floats", "target_image", "frustum_splats",
+    "cluster_scene", "fitter_init", "perspective", "ortho", "look_at", "mat_mul",
+    "fitter_camera", "orbit_cameras", "ImageFitter", "SplatRenderer", "Rasterizer",
+    "psnr_from_loss",
+]
+
+SH_C0 = 0.28209479177387814
+
+
+def sh_bands(deg: int) -> int:
+    return (0, 3, 8, 15)[max(0, min(3, deg))]
+
+
+def slab_floats(count: int, deg: int) -> int:
+    return (14 + 3 * sh_bands(deg)) * count
+
+
+# ---------------------------------------------------------------- workloads
+def target_image(seed: int, width: int, height: int, n_ellipses: int) -> np.ndarray:
+    out = np.empty(width * height * 3, np.float32)
+    lib().ds_gen_target_image(seed, width, height, n_ellipses, fptr(out))
+    return out
+
+
+def frustum_splats(seed: int, count: int, fovy_deg: float = 60.0, aspect: float = 1.0,
+                   zmin: float = 2.0, zmax: float = 6.0, ls_mean: float = -3.0,
+                   ls_std: float = 0.3, logit_opacity: float = 0.1) -> np.ndarray:
+    out = np.empty(slab_floats(count, 0), np.float32)
+    lib().ds_gen_frustum_splats(seed, count, fovy_deg, aspect, zmin, zmax, ls_mean, ls_std,
+                                logit_opacity, fptr(out))
+    return out
+
+
+def cluster_scene(seed: int, count: int, sh_degree: int = 3, n_clusters: int = 64,
+                  ls_mean: float = -4.5, ls_std: float = 0.5) -> np.ndarray:
+    out = np.empty(slab_floats(count, sh_degree), np.float32)
+    lib().ds_gen_cluster_scene(seed, count, sh_degree, n_clusters, ls_mean, ls_std, fptr(out))
+    return out
+
+
+def fitter_init(seed: int, count: int, width: int, height: int) -> np.ndarray:
+    out = np.empty(slab_floats(count, 0), np.float32)
+    lib().ds_gen_fitter_init(seed, count, width, height, fptr(out))
+    return out
+
+
+# ---------------------------------------------------------------- cameras
+def perspective(fovy_rad: float, aspect: float, znear: float, zfar: float) -> np.ndarray:
+    m = np.empty(16, np.float32)
+    lib().ds_mat_perspective(fovy_rad, aspect, znear, zfar, fptr(m))
+    return m
+
+
+def ortho(l, r, b, t, n, f) -> np.ndarray:
+    m = np.empty(16, np.float32)
+    lib().ds_mat_ortho(l, r, b, t, n, f, fptr(m))
+    return m
+
+
+def look_at(eye, center, up) -> np.ndarray:
+    m = np.empty(16, np.float32)
+    e, c, u = (np.asarray(v, np.float32) for v in (eye, center, up))
+    lib().ds_mat_look_at(fptr(e), fptr(c), fptr(u), fptr(m))
+    return m
+
+
+def mat_mul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    m = np.empty(16, np.float32)
+    lib().ds_mat_mul(fptr(np.ascontiguousarray(a, np.float32)),
+                     fptr(np.ascontiguousarray(b, np.float32)), fptr(m))
+    return m
+
+
+def fitter_camera(width: int, height: int, fovy_deg: float = 60.0):
+    """Pinhole camera at the origin looking down -z (the frustum workloads)."""
+    view = np.eye(4, dtype=np.float32).reshape(-1)
+    proj = perspective(math.radians(fovy_deg), width / height, 0.5, 20.0)
+    return view, proj, np.zeros(3, np.float32)
+
+
+def orbit_cameras(n: int, width: int, height: int, radius: float = 3.0,
+                  fovy_deg: float = 50.0, elevation: float = 0.35):
+    """n look-at cameras on a circle of `radius` around the origin."""
+    proj = perspective(math.radians(fovy_deg), width / height, 0.1, 20.0)
+    cams = []
+    for k in range(n):
+        a = 2.0 * math.pi * k / n
+        eye = np.array([radius * math.cos(a), radius * elevation, radius * math.sin(a)],
+                       np.float32)
+        cams.append((look_at(eye, (0, 0, 0), (0, 1, 0)), proj, eye))
+    return cams
+
+
+def psnr_from_loss(loss: float) -> float:
+    """PSNR of the reference's loss (sum over channels / pixels): 10 log10(3 / loss)."""
+    return 10.0 * math.log10(3.0 / max(loss, 1e-30))
+
+
+# ---------------------------------------------------------------- objects
+class Rasterizer:
+    """ds_raster: tile assign + sort + compositing, forward and backward."""
+
+    def __init__(self, width: int, height: int, count_cap: int, isect_cap: int = 0,
+                 keep_keys: bool = False, grad_mode: int = 0):
+        cfg = RasterConfig(width, height, count_cap, isect_cap, int(keep_keys), int(grad_mode))
+        h = C.c_void_p()
+        check(lib().ds_raster_create(C.byref(cfg), C.byref(h)), "ds_raster_create")
+        self.h = h.value
+        self.width, self.height = width, height
+
+    def forward(self, splat2d, out_rgb: DeviceArray, target=None, loss=None, grad=None,
+                stream=None) -> None:
+        check(lib().ds_raster_fwd(self.h, C.byref(splat2d), out_rgb.ptr,
+                                  target.ptr if target else None, loss.ptr if loss else None,
+                                  grad.ptr if grad else None, stream), "ds_raster_fwd")
+
+    def backward(self, splat2d, grad_rgb: DeviceArray, grad_splat2d, stream=None) -> None:
+        check(lib().ds_raster_bwd(self.h, C.byref(splat2d), grad_rgb.ptr,
+                                  C.byref(grad_splat2d), stream), "ds_raster_bwd")
+
+    def counts(self, stream=None):
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().ds_raster_counts(self.h, stream, C.byref(a), C.byref(b), C.byref(c)),
+              "ds_raster_counts")
+        return a.value, b.value, c.value
+
+    def state(self):
+        """Downloads the forward state: dict of numpy arrays (synchronises)."""
+        check(lib().ds_synchronize(), "ds_synchronize")
+        v = RasterView()
+        check(lib().ds_raster_get_view(self.h, C.byref(v)), "ds_raster_get_view")
+        n_isects, visible, wanted = self.counts()
+        ntiles = v.num_tiles_x * v.num_tiles_y
+        npix = v.width * v.height
+        st = {
+            "n_isects": n_isects, "visible": visible, "wanted": wanted,
+            "flatten_ids": download_ptr(v.flatten_ids, np.uint32, n_isects),
+            "tile_ranges": download_ptr(v.tile_ranges, np.int32, 2 * ntiles),
+            "T": download_ptr(v.transmittance, np.float32, npix),
+            "last_ids": download_ptr(v.last_ids, np.int32, npix),
+            "num_tiles_x": v.num_tiles_x, "num_tiles_y": v.num_tiles_y,
+        }
+        if v.isect_keys_sorted:
+            st["keys"] = download_ptr(v.isect_keys_sorted, np.uint64, n_isects)
+        return st
+
+    def close(self):
+        if self.h:
+            lib().ds_raster_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ImageFitter:
+    """ds_fitter: the reference's ImageFitter (activ -> proj -> raster -> MSE -> Adam)."""
+
+    def __init__(self, width: int, height: int, count: int, *, view=None, proj=None,
+                 cam_pos=None, sh_degree: int = 0, lr=None, fused: bool = True,
+                 isect_cap: int = 0, grad_mode: int = 0):
+        cfg = FitterConfig()
+        lib().ds_fitter_default_config(width, height, count, C.byref(cfg))
+        cfg.sh_degree = sh_degree
+        if view is not None:
+            cfg.view[:] = [float(x) for x in np.asarray(view, np.float32).reshape(-1)]
+        if proj is not None:
+            cfg.proj[:] = [float(x) for x in np.asarray(proj, np.float32).reshape(-1)]
+        if cam_pos is not None:
+            cfg.cam_pos[:] = [float(x) for x in cam_pos]
+        for k, v in (lr or {}).items():
+            setattr(cfg, k, float(v))
+        cfg.isect_cap = isect_cap
+        cfg.fused = int(fused)
+        cfg.grad_mode = int(grad_mode)
+        h = C.c_void_p()
+        check(lib().ds_fitter_create(C.byref(cfg), C.byref(h)), "ds_fitter_create")
+        self.h = h.value
+        self.cfg = cfg
+        self.width, self.height, self.count, self.sh_degree = width, height, count, sh_degree
+
+    def set_target(self, rgb: np.ndarray) -> None:
+        rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1)
+        assert rgb.size == self.width * self.height * 3
+        check(lib().ds_fitter_set_target(self.h, fptr(rgb)), "ds_fitter_set_target")
+
+    def upload_params(self, slab: np.ndarray) -> None:
+        slab = np.ascontiguousarray(slab, np.float32).reshape(-1)
+        assert slab.size == slab_floats(self.count, self.sh_degree)
+        check(lib().ds_fitter_upload_params(self.h, fptr(slab)), "ds_fitter_upload_params")
+
+    def params(self) -> np.ndarray:
+        out = np.empty(slab_floats(self.count, self.sh_degree), np.float32)
+        check(lib().ds_fitter_download_params(self.h, fptr(out)), "ds_fitter_download_params")
+        return out
+
+    def grads(self) -> np.ndarray:
+        out = np.empty(slab_floats(self.count, self.sh_degree), np.float32)
+        check(lib().ds_fitter_download_grads(self.h, fptr(out)), "ds_fitter_download_grads")
+        return out
+
+    def step(self, n: int = 1) -> None:
+        check(lib().ds_fitter_step(self.h, n), "ds_fitter_step")
+
+    def forward(self) -> None:
+        check(lib().ds_fitter_forward(self.h), "ds_fitter_forward")
+
+    def loss(self) -> float:
+        v = C.c_float()
+        check(lib().ds_fitter_loss(self.h, C.byref(v)), "ds_fitter_loss")
+        return v.value
+
+    def synchronize(self) -> None:
+        check(lib().ds_fitter_synchronize(self.h), "ds_fitter_synchronize")
+
+    def image(self) -> np.ndarray:
+        out = np.empty(self.width * self.height * 3, np.float32)
+        check(lib().ds_fitter_download_image(self.h, fptr(out)), "ds_fitter_download_image")
+        return out
+
+    def set_graph(self, on: bool) -> None:
+        check(lib().ds_fitter_set_graph(self.h, int(on)), "ds_fitter_set_graph")
+
+    def stream(self):
+        s = C.c_void_p()
+        check(lib().ds_fitter_get_stream(self.h, C.byref(s)), "ds_fitter_get_stream")
+        return s.value
+
+    def raster_state(self):
+        r = C.c_void_p()
+        check(lib().ds_fitter_get_raster(self.h, C.byref(r)), "ds_fitter_get_raster")
+        ras = Rasterizer.__new__(Rasterizer)
+        ras.h, ras.width, ras.height = r.value, self.width, self.height
+        try:
+            return ras.state()
+        finally:
+            ras.h = None
+
+    def close(self):
+        if self.h:
+            lib().ds_fitter_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SplatRenderer:
+    """ds_renderer: the reference's SplatRenderer (forward only, SH degree 0-3)."""
+
+    def __init__(self, width: int, height: int, count: int, sh_degree: int,
+                 isect_cap: int = 0):
+        h = C.c_void_p()
+        check(lib().ds_renderer_create(width, height, count, sh_degree, isect_cap, C.byref(h)),
+              "ds_renderer_create")
+        self.h = h.value
+        self.width, self.height, self.count, self.sh_degree = width, height, count, sh_degree
+
+    def upload_params(self, slab: np.ndarray) -> None:
+        slab = np.ascontiguousarray(slab, np.float32).reshape(-1)
+        assert slab.size == slab_floats(self.count, self.sh_degree)
+        check(lib().ds_renderer_upload_params(self.h, fptr(slab)), "ds_renderer_upload_params")
+
+    def render(self, view, proj, cam_pos) -> None:
+        v = np.ascontiguousarray(view, np.float32)
+        p = np.ascontiguousarray(proj, np.float32)
+        c = np.ascontiguousarray(cam_pos, np.float32)
+        check(lib().ds_renderer_render(self.h, fptr(v), fptr(p), fptr(c)), "ds_renderer_render")
+
+    def synchronize(self) -> None:
+        check(lib().ds_renderer_synchronize(self.h), "ds_renderer_synchronize")
+
+    def image(self) -> np.ndarray:
+        out = np.empty(self.width * self.height * 3, np.float32)
+        check(lib().ds_renderer_download_image(self.h, fptr(out)), "ds_renderer_download_image")
+        return out
+
+    def stream(self):
+        s = C.c_void_p()
+        check(lib().ds_renderer_get_stream(self.h, C.byref(s)), "ds_renderer_get_stream")
+        return s.value
+
+    def raster_state(self):
+        r = C.c_void_p()
+        check(lib().ds_renderer_get_raster(self.h, C.byref(r)), "ds_renderer_get_raster")
+        ras = Rasterizer.__new__(Rasterizer)
+        ras.h, ras.width, ras.height = r.value, self.width, self.height
+        try:
+            return ras.state()
+        finally:
+            ras.h = None
+
+    def close(self):
+        if self.h:
+            lib().ds_renderer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,580 @@
+// C-ABI entry points: runtime helpers, layer kernels, MSE, Adam, and the
+// fitter / renderer objects (wrapping the host C++ classes).
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+
+#include "ds_internal.hpp"
+#include "host/pipelines.hpp"
+
+#ifndef DS_VERSION
+#define DS_VERSION "0.1.0"
+#endif
+
+namespace ds {
+
+static thread_local std::string g_last_error;
+
+ds_status set_error(ds_status st, const std::string &msg) {
+    g_last_error = msg;
+    return st;
+}
+void clear_error() { g_last_error.clear(); }
+
+}  // namespace ds
+
+using namespace ds;
+
+#define DS_GUARD_BEGIN try {
+#define DS_GUARD_END                                                       \
+    }                                                                      \
+    catch (const std::exception &e) {                                      \
+        return set_error(DS_ERR_HIP, e.what());                            \
+    }                                                                      \
+    catch (...) {                                                          \
+        return set_error(DS_ERR_HIP, "unknown C++ exception");             \
+    }
+
+extern "C" {
+
+const char *ds_status_string(ds_status s) {
+    switch (s) {
+        case DS_OK: return "DS_OK";
+        case DS_ERR_INVALID_ARG: return "DS_ERR_INVALID_ARG";
+        case DS_ERR_HIP: return "DS_ERR_HIP";
+        case DS_ERR_OUT_OF_MEMORY: return "DS_ERR_OUT_OF_MEMORY";
+        case DS_ERR_CAPACITY: return "DS_ERR_CAPACITY";
+        case DS_ERR_STATE: return "DS_ERR_STATE";
+    }
+    return "DS_ERR_UNKNOWN";
+}
+
+const char *ds_last_error(void) { return g_last_error.c_str(); }
+
+const char *ds_version(void) { return "diffsplat_amd " DS_VERSION " (gfx950)"; }
+
+int ds_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+ds_status ds_device_name(char *buf, size_t len) {
+    clear_error();
+    DS_REQUIRE(buf && len > 0, "NULL buffer");
+    buf[0] = 0;
+    if (ds_device_count() == 0) return DS_OK;
+    hipDeviceProp_t p;
+    DS_TRY_HIP(hipGetDeviceProperties(&p, 0));
+    std::strncpy(buf, p.gcnArchName[0] ? p.gcnArchName : p.name, len - 1);
+    buf[len - 1] = 0;
+    return DS_OK;
+}
+
+ds_status ds_malloc(void **dptr, size_t bytes) {
+    clear_error();
+    DS_REQUIRE(dptr, "NULL dptr");
+    *dptr = nullptr;
+    if (bytes == 0) return DS_OK;
+    DS_TRY_HIP(hipMalloc(dptr, bytes));
+    return DS_OK;
+}
+
+ds_status ds_free(void *dptr) {
+    clear_error();
+    if (dptr) DS_TRY_HIP(hipFree(dptr));
+    return DS_OK;
+}
+
+ds_status ds_memcpy_htod(void *dst, const void *src, size_t bytes) {
+    clear_error();
+    if (bytes == 0) return DS_OK;
+    DS_REQUIRE(dst && src, "NULL pointer");
+    DS_TRY_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return DS_OK;
+}
+
+ds_status ds_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+    clear_error();
+    if (bytes == 0) return DS_OK;
+    DS_REQUIRE(dst && src, "NULL pointer");
+    DS_TRY_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return DS_OK;
+}
+
+ds_status ds_memcpy_dtod(void *dst, const void *src, size_t bytes, ds_stream_t s) {
+    clear_error();
+    if (bytes == 0) return DS_OK;
+    DS_REQUIRE(dst && src, "NULL pointer");
+    DS_TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(s)));
+    return DS_OK;
+}
+
+ds_status ds_fill_u32(void *dptr, uint32_t value, size_t count, ds_stream_t s) {
+    clear_error();
+    if (count == 0) return DS_OK;
+    DS_REQUIRE(dptr, "NULL pointer");
+    return fill_u32(dptr, value, count, as_stream(s));
+}
+
+ds_status ds_synchronize(void) {
+    clear_error();
+    DS_TRY_HIP(hipDeviceSynchronize());
+    return DS_OK;
+}
+
+ds_status ds_stream_create(ds_stream_t *s) {
+    clear_error();
+    DS_REQUIRE(s, "NULL stream pointer");
+    hipStream_t st = nullptr;
+    DS_TRY_HIP(hipStreamCreate(&st));
+    *s = st;
+    return DS_OK;
+}
+
+ds_status ds_stream_destroy(ds_stream_t s) {
+    clear_error();
+    if (s) DS_TRY_HIP(hipStreamDestroy(as_stream(s)));
+    return DS_OK;
+}
+
+ds_status ds_stream_synchronize(ds_stream_t s) {
+    clear_error();
+    DS_TRY_HIP(hipStreamSynchronize(as_stream(s)));
+    return DS_OK;
+}
+
+ds_status ds_event_create(ds_event_t *e) {
+    clear_error();
+    DS_REQUIRE(e, "NULL event pointer");
+    hipEvent_t ev = nullptr;
+    DS_TRY_HIP(hipEventCreate(&ev));
+    *e = ev;
+    return DS_OK;
+}
+
+ds_status ds_event_destroy(ds_event_t e) {
+    clear_error();
+    if (e) DS_TRY_HIP(hipEventDestroy(reinterpret_cast<hipEvent_t>(e)));
+    return DS_OK;
+}
+
+ds_status ds_event_record(ds_event_t e, ds_stream_t s) {
+    clear_error();
+    DS_REQUIRE(e, "NULL event");
+    DS_TRY_HIP(hipEventRecord(reinterpret_cast<hipEvent_t>(e), as_stream(s)));
+    return DS_OK;
+}
+
+ds_status ds_event_synchronize(ds_event_t e) {
+    clear_error();
+    DS_REQUIRE(e, "NULL event");
+    DS_TRY_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(e)));
+    return DS_OK;
+}
+
+ds_status ds_event_elapsed_ms(ds_event_t a, ds_event_t b, float *ms) {
+    clear_error();
+    DS_REQUIRE(a && b && ms, "NULL argument");
+    DS_TRY_HIP(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(a),
+                                   reinterpret_cast<hipEvent_t>(b)));
+    return DS_OK;
+}
+
+// ------------------------------------------------------------------ layers
+
+static ds_status check_gauss(const ds_gauss3d *g, int deg, const char *what) {
+    DS_REQUIRE(g, std::string(what) + ": NULL view");
+    DS_REQUIRE(deg >= 0 && deg <= 3, std::string(what) + ": sh_degree must be in [0,3]");
+    DS_REQUIRE(g->count >= 0, std::string(what) + ": negative count");
+    if (g->count == 0) return DS_OK;
+    DS_REQUIRE(g->pos_x && g->pos_y && g->pos_z && g->scale_x && g->scale_y && g->scale_z &&
+                   g->rot_w && g->rot_x && g->rot_y && g->rot_z && g->sh_dc_r && g->sh_dc_g &&
+                   g->sh_dc_b && g->logit_opacity,
+               std::string(what) + ": NULL field");
+    if (deg > 0) {
+        DS_REQUIRE(g->sh_num_bands >= ds_sh_bands(deg),
+                   std::string(what) + ": sh_num_bands smaller than the SH degree needs");
+        DS_REQUIRE(g->sh_rest_r && g->sh_rest_g && g->sh_rest_b,
+                   std::string(what) + ": NULL sh_rest field");
+    }
+    return DS_OK;
+}
+
+static Cam3 cam_of(const float *c) { return c ? Cam3{c[0], c[1], c[2]} : Cam3{0.f, 0.f, 0.f}; }
+
+static PV pv_of(const float *m) {
+    PV p;
+    std::memcpy(p.m, m, sizeof(p.m));
+    return p;
+}
+
+ds_status ds_activ_fwd(const ds_gauss3d *in, const ds_splat3d *out, int deg,
+                       const float cam_pos[3], ds_stream_t s) {
+    clear_error();
+    DS_TRY(check_gauss(in, deg, "ds_activ_fwd"));
+    DS_REQUIRE(out && out->count >= in->count, "ds_activ_fwd: output view too small");
+    return launch_activ_fwd(*in, *out, deg, cam_of(cam_pos), as_stream(s));
+}
+
+ds_status ds_activ_bwd(const ds_gauss3d *in, const ds_splat3d *out, const ds_splat3d *gout,
+                       const ds_gauss3d *gin, int deg, const float cam_pos[3], ds_stream_t s) {
+    clear_error();
+    DS_TRY(check_gauss(in, deg, "ds_activ_bwd"));
+    DS_TRY(check_gauss(gin, deg, "ds_activ_bwd grad_in"));
+    DS_REQUIRE(out && gout, "ds_activ_bwd: NULL view");
+    return launch_activ_bwd(*in, *out, *gout, *gin, deg, cam_of(cam_pos), as_stream(s));
+}
+
+ds_status ds_project_fwd(const ds_splat3d *in, const ds_splat2d *out, const float pv[16],
+                         ds_stream_t s) {
+    clear_error();
+    DS_REQUIRE(in && out && pv, "ds_project_fwd: NULL argument");
+    return launch_project_fwd(*in, *out, pv_of(pv), as_stream(s));
+}
+
+ds_status ds_project_bwd(const ds_splat3d *in, const ds_splat2d *gout, const ds_splat3d *gin,
+                         const float pv[16], ds_stream_t s) {
+    clear_error();
+    DS_REQUIRE(in && gout && gin && pv, "ds_project_bwd: NULL argument");
+    return launch_project_bwd(*in, *gout, *gin, pv_of(pv), as_stream(s));
+}
+
+ds_status ds_activ_project_fwd(const ds_gauss3d *in, const ds_splat2d *out, int deg,
+                               const float cam_pos[3], const float pv[16], ds_stream_t s) {
+    clear_error();
+    DS_TRY(check_gauss(in, deg, "ds_activ_project_fwd"));
+    DS_REQUIRE(out && pv, "ds_activ_project_fwd: NULL argument");
+    return launch_activ_project_fwd(*in, *out, deg, cam_of(cam_pos), pv_of(pv), as_stream(s));
+}
+
+ds_status ds_activ_project_bwd(const ds_gauss3d *in, const ds_splat2d *gout,
+                               const ds_gauss3d *gin, int deg, const float cam_pos[3],
+                               const float pv[16], ds_stream_t s) {
+    clear_error();
+    DS_TRY(check_gauss(in, deg, "ds_activ_project_bwd"));
+    DS_TRY(check_gauss(gin, deg, "ds_activ_project_bwd grad_in"));
+    DS_REQUIRE(gout && pv, "ds_activ_project_bwd: NULL argument");
+    return launch_activ_project_bwd(*in, *gout, *gin, deg, cam_of(cam_pos), pv_of(pv),
+                                    as_stream(s));
+}
+
+ds_status ds_mse_fwd(const float *pixels, const float *target, int64_t num_pixels,
+                     float *loss_dev, ds_stream_t s) {
+    clear_error();
+    DS_REQUIRE(loss_dev && num_pixels >= 0, "ds_mse_fwd: bad argument");
+    DS_REQUIRE(num_pixels == 0 || (pixels && target), "ds_mse_fwd: NULL image");
+    return launch_mse_fwd(pixels, target, num_pixels, loss_dev, as_stream(s));
+}
+
+ds_status ds_mse_bwd(const float *pixels, const float *target, int64_t num_pixels, float *grad,
+                     ds_stream_t s) {
+    clear_error();
+    DS_REQUIRE(num_pixels >= 0, "ds_mse_bwd: bad argument");
+    DS_REQUIRE(num_pixels == 0 || (pixels && target && grad), "ds_mse_bwd: NULL pointer");
+    return launch_mse_bwd(pixels, target, num_pixels, grad, as_stream(s));
+}
+
+static ds_status check_adam(const ds_adam_group *g, int n, const ds_adam_hparams *hp) {
+    DS_REQUIRE(hp, "ds_adam_step: NULL hparams");
+    DS_REQUIRE(n >= 0 && n <= DS_ADAM_MAX_GROUPS, "ds_adam_step: n_groups out of range");
+    DS_REQUIRE(n == 0 || g, "ds_adam_step: NULL groups");
+    for (int k = 0; k < n; ++k) {
+        DS_REQUIRE(g[k].n >= 0, "ds_adam_step: negative group size");
+        DS_REQUIRE(g[k].n == 0 || (g[k].param && g[k].grad && g[k].m && g[k].v),
+                   "ds_adam_step: NULL group pointer");
+    }
+    return DS_OK;
+}
+
+ds_status ds_adam_step(const ds_adam_group *groups, int n_groups, const ds_adam_hparams *hp,
+                       int32_t step, ds_stream_t s) {
+    clear_error();
+    DS_TRY(check_adam(groups, n_groups, hp));
+    DS_REQUIRE(step >= 1, "ds_adam_step: step must be >= 1");
+    return launch_adam(groups, n_groups, *hp, step, nullptr, as_stream(s));
+}
+
+ds_status ds_adam_step_dev(const ds_adam_group *groups, int n_groups, const ds_adam_hparams *hp,
+                           int32_t *step_dev, ds_stream_t s) {
+    clear_error();
+    DS_TRY(check_adam(groups, n_groups, hp));
+    DS_REQUIRE(step_dev, "ds_adam_step_dev: NULL step counter");
+    return launch_adam(groups, n_groups, *hp, 0, step_dev, as_stream(s));
+}
+
+// ------------------------------------------------------------------ fitter
+
+struct ds_fitter {
+    dsh::ImageFitter impl;
+};
+
+void ds_fitter_default_config(int width, int height, int count, ds_fitter_config *cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->width = width;
+    cfg->height = height;
+    cfg->count = count;
+    cfg->sh_degree = 0;
+    dsh::ImageFitter::referenceCamera(width, height, cfg->view, cfg->proj);
+    cfg->cam_pos[0] = 0.f;
+    cfg->cam_pos[1] = 0.f;
+    cfg->cam_pos[2] = 5.f;
+    dsh::Gaussian3DAdamConfig d;
+    cfg->lr_pos = d.lr_pos;
+    cfg->lr_scale = d.lr_scale;
+    cfg->lr_rot = d.lr_rot;
+    cfg->lr_sh_dc = d.lr_sh_dc;
+    cfg->lr_sh_rest = d.lr_sh_rest;
+    cfg->lr_opacity = d.lr_opacity;
+    cfg->beta1 = d.adam.beta1;
+    cfg->beta2 = d.adam.beta2;
+    cfg->epsilon = d.adam.epsilon;
+    cfg->isect_cap = 0;
+    cfg->fused = 1;
+    cfg->grad_mode = DS_GRAD_EXACT;
+}
+
+ds_status ds_fitter_create(const ds_fitter_config *cfg, ds_fitter **out) {
+    clear_error();
+    DS_REQUIRE(cfg && out, "NULL argument");
+    DS_REQUIRE(cfg->width > 0 && cfg->height > 0 && cfg->count >= 0, "bad fitter size");
+    DS_REQUIRE(cfg->sh_degree >= 0 && cfg->sh_degree <= 3, "sh_degree must be in [0,3]");
+    DS_GUARD_BEGIN
+    auto f = std::make_unique<ds_fitter>();
+    dsh::FitterOptions o;
+    o.width = cfg->width;
+    o.height = cfg->height;
+    o.count = cfg->count;
+    o.sh_degree = cfg->sh_degree;
+    std::memcpy(o.view, cfg->view, sizeof(o.view));
+    std::memcpy(o.proj, cfg->proj, sizeof(o.proj));
+    std::memcpy(o.cam_pos, cfg->cam_pos, sizeof(o.cam_pos));
+    o.lr.lr_pos = cfg->lr_pos;
+    o.lr.lr_scale = cfg->lr_scale;
+    o.lr.lr_rot = cfg->lr_rot;
+    o.lr.lr_sh_dc = cfg->lr_sh_dc;
+    o.lr.lr_sh_rest = cfg->lr_sh_rest;
+    o.lr.lr_opacity = cfg->lr_opacity;
+    o.lr.adam.beta1 = cfg->beta1;
+    o.lr.adam.beta2 = cfg->beta2;
+    o.lr.adam.epsilon = cfg->epsilon;
+    o.isect_cap = cfg->isect_cap;
+    o.fused = cfg->fused != 0;
+    DS_REQUIRE(cfg->grad_mode == DS_GRAD_EXACT || cfg->grad_mode == DS_GRAD_REFERENCE,
+               "grad_mode must be DS_GRAD_EXACT or DS_GRAD_REFERENCE");
+    o.grad_mode = cfg->grad_mode;
+    f->impl.init(o);
+    f->impl.initLayers();
+    *out = f.release();
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_destroy(ds_fitter *f) {
+    clear_error();
+    DS_GUARD_BEGIN
+    delete f;
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_set_target(ds_fitter *f, const float *host_rgb) {
+    clear_error();
+    DS_REQUIRE(f && host_rgb, "NULL argument");
+    DS_GUARD_BEGIN
+    f->impl.setTarget(host_rgb);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_upload_params(ds_fitter *f, const float *host_slab) {
+    clear_error();
+    DS_REQUIRE(f && host_slab, "NULL argument");
+    DS_GUARD_BEGIN
+    f->impl.uploadParams(host_slab);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_download_params(ds_fitter *f, float *host_slab) {
+    clear_error();
+    DS_REQUIRE(f && host_slab, "NULL argument");
+    DS_GUARD_BEGIN
+    f->impl.downloadParams(host_slab);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_download_grads(ds_fitter *f, float *host_slab) {
+    clear_error();
+    DS_REQUIRE(f && host_slab, "NULL argument");
+    DS_GUARD_BEGIN
+    f->impl.downloadGrads(host_slab);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_step(ds_fitter *f, int n_steps) {
+    clear_error();
+    DS_REQUIRE(f && n_steps >= 0, "bad argument");
+    DS_GUARD_BEGIN
+    for (int k = 0; k < n_steps; ++k) f->impl.step();
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_forward(ds_fitter *f) {
+    clear_error();
+    DS_REQUIRE(f, "NULL fitter");
+    DS_GUARD_BEGIN
+    f->impl.forwardOnly();
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_loss(ds_fitter *f, float *loss) {
+    clear_error();
+    DS_REQUIRE(f && loss, "NULL argument");
+    DS_GUARD_BEGIN
+    *loss = f->impl.getLoss();
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_synchronize(ds_fitter *f) {
+    clear_error();
+    DS_REQUIRE(f, "NULL fitter");
+    DS_TRY_HIP(hipStreamSynchronize(f->impl.getStream()));
+    return DS_OK;
+}
+
+ds_status ds_fitter_download_image(ds_fitter *f, float *host_rgb) {
+    clear_error();
+    DS_REQUIRE(f && host_rgb, "NULL argument");
+    DS_TRY_HIP(hipStreamSynchronize(f->impl.getStream()));
+    const size_t bytes = (size_t)f->impl.width() * f->impl.height() * 3 * sizeof(float);
+    DS_TRY_HIP(hipMemcpy(host_rgb, f->impl.getOutput(), bytes, hipMemcpyDeviceToHost));
+    return DS_OK;
+}
+
+ds_status ds_fitter_get_raster(ds_fitter *f, ds_raster **r) {
+    clear_error();
+    DS_REQUIRE(f && r, "NULL argument");
+    *r = f->impl.raster();
+    return DS_OK;
+}
+
+ds_status ds_fitter_get_stream(ds_fitter *f, ds_stream_t *s) {
+    clear_error();
+    DS_REQUIRE(f && s, "NULL argument");
+    *s = f->impl.getStream();
+    return DS_OK;
+}
+
+ds_status ds_fitter_set_graph(ds_fitter *f, int enable) {
+    clear_error();
+    DS_REQUIRE(f, "NULL fitter");
+    f->impl.setUseGraph(enable != 0);
+    return DS_OK;
+}
+
+ds_status ds_fitter_step_count(ds_fitter *f, int32_t *step) {
+    clear_error();
+    DS_REQUIRE(f && step, "NULL argument");
+    *step = f->impl.getIterCount();
+    return DS_OK;
+}
+
+ds_status ds_fitter_views(ds_fitter *f, ds_gauss3d *params, ds_splat2d *splat2d,
+                          ds_splat2d *grad_splat2d, float **grad_rgb) {
+    clear_error();
+    DS_REQUIRE(f, "NULL fitter");
+    DS_GUARD_BEGIN
+    f->impl.views(params, splat2d, grad_splat2d, grad_rgb);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+// ------------------------------------------------------------------ renderer
+
+struct ds_renderer {
+    dsh::SplatRenderer impl;
+};
+
+ds_status ds_renderer_create(int width, int height, int count, int sh_degree, int64_t isect_cap,
+                             ds_renderer **out) {
+    clear_error();
+    DS_REQUIRE(out, "NULL argument");
+    DS_REQUIRE(width > 0 && height > 0 && count >= 0, "bad renderer size");
+    DS_REQUIRE(sh_degree >= 0 && sh_degree <= 3, "sh_degree must be in [0,3]");
+    DS_GUARD_BEGIN
+    auto r = std::make_unique<ds_renderer>();
+    r->impl.init(width, height, count, sh_degree, isect_cap);
+    *out = r.release();
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_renderer_destroy(ds_renderer *r) {
+    clear_error();
+    DS_GUARD_BEGIN
+    delete r;
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_renderer_upload_params(ds_renderer *r, const float *host_slab) {
+    clear_error();
+    DS_REQUIRE(r && host_slab, "NULL argument");
+    DS_GUARD_BEGIN
+    r->impl.uploadParams(host_slab);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_renderer_render(ds_renderer *r, const float view[16], const float proj[16],
+                             const float cam_pos[3]) {
+    clear_error();
+    DS_REQUIRE(r && view && proj && cam_pos, "NULL argument");
+    DS_GUARD_BEGIN
+    r->impl.render(view, proj, cam_pos);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_renderer_download_image(ds_renderer *r, float *host_rgb) {
+    clear_error();
+    DS_REQUIRE(r && host_rgb, "NULL argument");
+    DS_TRY_HIP(hipStreamSynchronize(r->impl.getStream()));
+    const size_t bytes = (size_t)r->impl.width() * r->impl.height() * 3 * sizeof(float);
+    DS_TRY_HIP(hipMemcpy(host_rgb, r->impl.getOutput(), bytes, hipMemcpyDeviceToHost));
+    return DS_OK;
+}
+
+ds_status ds_renderer_get_raster(ds_renderer *r, ds_raster **out) {
+    clear_error();
+    DS_REQUIRE(r && out, "NULL argument");
+    *out = r->impl.raster();
+    return DS_OK;
+}
+
+ds_status ds_renderer_get_stream(ds_renderer *r, ds_stream_t *s) {
+    clear_error();
+    DS_REQUIRE(r && s, "NULL argument");
+    *s = r->impl.getStream();
+    return DS_OK;
+}
+
+ds_status ds_renderer_synchronize(ds_renderer *r) {
+    clear_error();
+    DS_REQUIRE(r, "NULL renderer");
+    DS_TRY_HIP(hipStreamSynchronize(r->impl.getStream()));
+    return DS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// Host-side internals shared by the C-ABI implementation files: error
+// reporting, launch-wrapper declarations and small RAII device buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <string>
+#include <utility>
+
+#include "../../include/ds_api.h"
+
+namespace ds {
+
+// Records a detail message for ds_last_error() and returns `st`.
+ds_status set_error(ds_status st, const std::string &msg);
+void clear_error();
+
+#define DS_TRY_HIP(expr)                                                                  \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return ::ds::set_error(_e == hipErrorOutOfMemory ? DS_ERR_OUT_OF_MEMORY        \
+                                                             : DS_ERR_HIP,                \
+                                   std::string(#expr) + ": " + hipGetErrorString(_e));    \
+    } while (0)
+
+#define DS_CHECK_LAUNCH(name)                                                             \
+    do {                                                                                  \
+        hipError_t _e = hipGetLastError();                                                \
+        if (_e != hipSuccess)                                                             \
+            return ::ds::set_error(DS_ERR_HIP,                                            \
+                                   std::string("launch ") + (name) + ": " +               \
+                                       hipGetErrorString(_e));                            \
+    } while (0)
+
+#define DS_TRY(expr)                                  \
+    do {                                              \
+        ds_status _s = (expr);                        \
+        if (_s != DS_OK) return _s;                   \
+    } while (0)
+
+#define DS_REQUIRE(cond, msg)                                             \
+    do {                                                                  \
+        if (!(cond)) return ::ds::set_error(DS_ERR_INVALID_ARG, (msg));   \
+    } while (0)
+
+inline hipStream_t as_stream(ds_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Move-only device allocation.
+template <typename T>
+struct DevBuf {
+    T *ptr = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), n(o.n) { o.ptr = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); ptr = o.ptr; n = o.n; o.ptr = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+    ds_status alloc(size_t count) {
+        if (count == n && ptr) return DS_OK;
+        release();
+        if (count == 0) return DS_OK;
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&ptr), count * sizeof(T));
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return set_error(DS_ERR_OUT_OF_MEMORY, "hipMalloc of " +
+                                                       std::to_string(count * sizeof(T)) +
+                                                       " bytes: " + hipGetErrorString(e));
+        }
+        n = count;
+        return DS_OK;
+    }
+    operator T *() const { return ptr; }
+};
+
+// ------------------------------------------------------------- launchers
+// Each returns DS_OK or the launch error; all are asynchronous on `s`.
+
+ds_status fill_u32(void *p, uint32_t v, size_t count, hipStream_t s);
+
+// activation / projection (kernels/activ_project.hip)
+struct PV {
+    float m[16];
+};
+struct Cam3 {
+    float x, y, z;
+};
+ds_status launch_activ_fwd(const ds_gauss3d &in, const ds_splat3d &out, int deg, Cam3 cam,
+                           hipStream_t s);
+ds_status launch_activ_bwd(const ds_gauss3d &in, const ds_splat3d &out, const ds_splat3d &gout,
+                           const ds_gauss3d &gin, int deg, Cam3 cam, hipStream_t s);
+ds_status launch_project_fwd(const ds_splat3d &in, const ds_splat2d &out, const PV &pv,
+                             hipStream_t s);
+ds_status launch_project_bwd(const ds_splat3d &in, const ds_splat2d &gout,
+                             const ds_splat3d &gin, const PV &pv, hipStream_t s);
+ds_status launch_activ_project_fwd(const ds_gauss3d &in, const ds_splat2d &out, int deg,
+                                   Cam3 cam, const PV &pv, hipStream_t s);
+ds_status launch_activ_project_bwd(const ds_gauss3d &in, const ds_splat2d &gout,
+                                   const ds_gauss3d &gin, int deg, Cam3 cam, const PV &pv,
+                                   hipStream_t s);
+
+// loss / optimiser (kernels/optim.hip)
+ds_status launch_mse_fwd(const float *p, const float *t, int64_t npix, float *loss,
+                         hipStream_t s);
+ds_status launch_mse_bwd(const float *p, const float *t, int64_t npix, float *grad,
+                         hipStream_t s);
+ds_status launch_adam(const ds_adam_group *groups, int n_groups, const ds_adam_hparams &hp,
+                      int32_t step, int32_t *step_dev, hipStream_t s);
+
+}  // namespace ds

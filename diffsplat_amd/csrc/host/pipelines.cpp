@@ -1,0 +1,207 @@
+// ImageFitter / SplatRenderer implementation (see pipelines.hpp).
+#include "pipelines.hpp"
+
+#include <algorithm>
+
+namespace dsh {
+
+static void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("[diffsplat] ") + what + ": " +
+                                 hipGetErrorString(e));
+}
+
+// ============================================================ ImageFitter
+
+ImageFitter::ImageFitter() = default;
+
+ImageFitter::~ImageFitter() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void ImageFitter::referenceCamera(int w, int h, float view[16], float proj[16]) {
+    const float hw = w * 0.5f, hh = h * 0.5f;
+    const float zr = std::min(hw, hh) * 0.25f;
+    for (int k = 0; k < 16; ++k) view[k] = (k % 5 == 0) ? 1.f : 0.f;
+    ds_mat_ortho(-hw, hw, -hh, hh, -zr, zr, proj);
+}
+
+void ImageFitter::init(const FitterOptions &o) {
+    if (o.width <= 0 || o.height <= 0 || o.count < 0 || o.sh_degree < 0 || o.sh_degree > 3)
+        throw std::runtime_error("[diffsplat] ImageFitter::init: bad configuration");
+    opt = o;
+    fused = o.fused;
+    if (!stream) hip_check(hipStreamCreate(&stream), "hipStreamCreate");
+    gaussian_params.allocate(o.count, o.sh_degree);
+    d_target_pixels.allocate((size_t)o.width * o.height * 3);
+    d_target_pixels.zero(stream);
+}
+
+void ImageFitter::setTarget(const float *host_rgb) {
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    d_target_pixels.upload(host_rgb, (size_t)opt.width * opt.height * 3);
+}
+
+void ImageFitter::uploadParams(const float *host_slab) {
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    gaussian_params.slab.upload(host_slab, gaussian_params.slab.size);
+    optimizer.reset(stream);
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+void ImageFitter::downloadParams(float *host_slab) const {
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    gaussian_params.slab.download(host_slab, gaussian_params.slab.size);
+}
+
+void ImageFitter::downloadGrads(float *host_slab) const {
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    const Gaussian3DGrads &g = fused ? const_cast<GaussProjectLayer &>(gp_layer).getGradInput()
+                                     : const_cast<GaussActivLayer &>(atv_layer).getGradInput();
+    g.slab.download(host_slab, g.slab.size);
+}
+
+void ImageFitter::initLayers() {
+    const int n = opt.count;
+    pipeline.clear();
+    Layer *all[5] = {&atv_layer, &psp_layer, &gp_layer, &ras_layer, &mse_layer};
+    for (Layer *l : all) l->setStream(stream);
+
+    ras_layer.allocate(opt.width, opt.height, n, opt.isect_cap, 0, opt.grad_mode);
+    ras_layer.allocateGrad(n);
+    mse_layer.allocate(opt.width, opt.height);
+    mse_layer.setTarget(d_target_pixels.ptr);
+    mse_layer.setInput(ras_layer.getOutput().ptr);
+    ras_layer.setGradOutput(&mse_layer.getGradInput());
+
+    Gaussian3DGrads *param_grads = nullptr;
+    if (fused) {
+        gp_layer.setSHDegree(opt.sh_degree);
+        gp_layer.setCamera(opt.view, opt.proj);
+        gp_layer.setCameraPosition(opt.cam_pos[0], opt.cam_pos[1], opt.cam_pos[2]);
+        gp_layer.allocate(n);
+        gp_layer.allocateGrad(n);
+        gp_layer.setInput(&gaussian_params);
+        ras_layer.setInput(&gp_layer.getOutput());
+        gp_layer.setGradOutput(&ras_layer.getGradInput());
+        mse_layer.setFusedIntoRasterizer(true);
+        ras_layer.setLossTarget(d_target_pixels.ptr, mse_layer.lossDevice(),
+                                mse_layer.getGradInput().ptr);
+        pipeline.add(&gp_layer);
+        pipeline.add(&ras_layer);
+        pipeline.add(&mse_layer);
+        param_grads = &gp_layer.getGradInput();
+    } else {
+        atv_layer.setSHDegree(opt.sh_degree);
+        atv_layer.setCameraPosition(opt.cam_pos[0], opt.cam_pos[1], opt.cam_pos[2]);
+        psp_layer.setCamera(opt.view, opt.proj);
+        atv_layer.allocate(n);
+        psp_layer.allocate(n);
+        atv_layer.allocateGrad(n);
+        psp_layer.allocateGrad(n);
+        atv_layer.setInput(&gaussian_params);
+        psp_layer.setInput(&atv_layer.getOutput());
+        ras_layer.setInput(&psp_layer.getOutput());
+        psp_layer.setGradOutput(&ras_layer.getGradInput());
+        atv_layer.setGradOutput(&psp_layer.getGradInput());
+        mse_layer.setFusedIntoRasterizer(false);
+        ras_layer.setLossTarget(nullptr, nullptr, nullptr);
+        pipeline.add(&atv_layer);
+        pipeline.add(&psp_layer);
+        pipeline.add(&ras_layer);
+        pipeline.add(&mse_layer);
+        param_grads = &atv_layer.getGradInput();
+    }
+    param_grads->zero(stream);
+    optimizer = Adam();
+    optimizer.init(opt.lr.adam);
+    registerGaussian3DGroups(optimizer, gaussian_params, *param_grads, opt.lr);
+    optimizer.reset(stream);
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+void ImageFitter::stepOnce() {
+    pipeline.zeroGrad();
+    pipeline.forward();
+    pipeline.backward();
+    optimizer.step(stream);
+}
+
+void ImageFitter::setUseGraph(bool on) { use_graph = on; }
+
+void ImageFitter::step() {
+    if (!use_graph) {
+        stepOnce();
+        return;
+    }
+    if (!graph_exec) {
+        hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal),
+                  "hipStreamBeginCapture");
+        try {
+            stepOnce();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+        hip_check(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0),
+                  "hipGraphInstantiate");
+    }
+    hip_check(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch");
+}
+
+void ImageFitter::forwardOnly() { pipeline.forward(); }
+
+void ImageFitter::views(ds_gauss3d *params, ds_splat2d *s2, ds_splat2d *gs2, float **grad_rgb) {
+    if (params) *params = gaussian_params.view();
+    if (s2) *s2 = fused ? gp_layer.getOutput().view() : psp_layer.getOutput().view();
+    if (gs2) *gs2 = ras_layer.getGradInput().view();
+    if (grad_rgb) *grad_rgb = mse_layer.getGradInput().ptr;
+}
+
+float ImageFitter::getLoss() { return mse_layer.getLoss(); }
+
+// ============================================================ SplatRenderer
+
+SplatRenderer::SplatRenderer() = default;
+
+SplatRenderer::~SplatRenderer() {
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void SplatRenderer::init(int width, int height, int count, int deg, int64_t isect_cap) {
+    if (width <= 0 || height <= 0 || count < 0 || deg < 0 || deg > 3)
+        throw std::runtime_error("[diffsplat] SplatRenderer::init: bad configuration");
+    w = width;
+    h = height;
+    sh_degree = deg;
+    if (!stream) hip_check(hipStreamCreate(&stream), "hipStreamCreate");
+    gaussian_params.allocate(count, deg);
+    gp_layer.setStream(stream);
+    ras_layer.setStream(stream);
+    gp_layer.setSHDegree(deg);
+    gp_layer.allocate(count);
+    ras_layer.allocate(width, height, count, isect_cap);
+    gp_layer.setInput(&gaussian_params);
+    ras_layer.setInput(&gp_layer.getOutput());
+    pipeline.clear();
+    pipeline.add(&gp_layer);
+    pipeline.add(&ras_layer);
+}
+
+void SplatRenderer::uploadParams(const float *host_slab) {
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    gaussian_params.slab.upload(host_slab, gaussian_params.slab.size);
+}
+
+void SplatRenderer::render(const float view[16], const float proj[16], const float cam_pos[3]) {
+    gp_layer.setCameraPosition(cam_pos[0], cam_pos[1], cam_pos[2]);
+    gp_layer.setCamera(view, proj);
+    pipeline.forward();
+}
+
+}  // namespace dsh

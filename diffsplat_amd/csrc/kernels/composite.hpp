@@ -1,0 +1,36 @@
+// Front-to-back compositing of sorted splat lists per 16x16 tile, forward and
+// backward (launchers; kernels in composite.hip).
+#pragma once
+
+#include "../ds_device.hpp"
+#include "../ds_internal.hpp"
+
+namespace ds {
+
+constexpr int kTile = 16;
+constexpr int kTilePix = kTile * kTile;
+constexpr float kMaxAlpha = 0.99f;
+constexpr float kAlphaThres = 1.0f / 255.0f;
+constexpr float kTThres = 0.0001f;
+
+// Packed per-splat render record written by tile assignment (3 x float4):
+//   [0] = {x, y, conic_a, conic_b}   [1] = {conic_c, opacity, 1/det, splat id bits}
+//   [2] = {r, g, b, 0}
+// conic = inverse 2D covariance: a = cyy/det, b = -cxy/det, c = cxx/det.
+
+struct CompositeArgs {
+    const float4 *rec;
+    const uint32_t *flat;
+    const int2 *ranges;
+    int W, H, ntx, ntiles;
+};
+
+// pix_per_thread: 1, 2 or 4 pixels per thread (256, 128 or 64 threads per tile).
+ds_status launch_composite_fwd(const CompositeArgs &a, int pix_per_thread, float *out_rgb,
+                               float *out_T, int32_t *out_last, const float *target,
+                               float *loss, float *grad, hipStream_t s);
+ds_status launch_composite_bwd(const CompositeArgs &a, int pix_per_thread, int grad_mode,
+                               const float *T, const int32_t *last, const float *grad_rgb,
+                               const ds_splat2d &grad, hipStream_t s);
+
+}  // namespace ds

@@ -1,0 +1,475 @@
+// Tile rasterizer: tile assignment, (tile | depth) radix sort, tile ranges,
+// front-to-back compositing and its analytic backward.
+//
+// Reference semantics (src/layers/tile_rasterize_layer.cu):
+//   makeKey                 30-38
+//   tileAssignKernel        52-144
+//   buildTileRangesKernel   148-162
+//   compositeFwdKernel      192-309
+//   compositeBwdKernel      356-578
+//   allocate/resize/forward/backward  591-728
+//
+// MI355X design:
+//  * Tile assignment writes a per-splat tile rectangle, pair count and a
+//    packed 48-byte render record (centre, conic, opacity, 1/det, rgb) so
+//    compositing gathers one contiguous record per pair instead of nine SoA
+//    arrays.
+//  * The 64-bit (tile << 32 | depth) sort is factored: every pair of one
+//    splat shares the low 32 key bits, so the four LSD passes over the depth
+//    bits run over N splats instead of over the pairs; pairs are then emitted
+//    in depth order at scanned offsets (deterministic, no atomic slots) and
+//    stably sorted on the tile bits only. The result equals a stable sort of
+//    the reference's 64-bit keys.
+//  * Compositing: one 256-thread workgroup (4 wave64) per 16x16 tile,
+//    256-record batches in LDS, workgroup-to-tile mapping that keeps
+//    neighbouring tiles on one XCD's L2. The MSE loss and its gradient can
+//    be fused into the forward epilogue.
+//  * Backward: wave64 reductions, then LDS float atomics across the four
+//    waves, then one global atomic per (splat, field, tile).
+#include <float.h>
+
+#include "../ds_device.hpp"
+#include "../ds_internal.hpp"
+#include "composite.hpp"
+#include "sort_scan.hpp"
+
+namespace ds {
+
+namespace {
+
+// Scalars kept on the device: indices into ds_raster::scalars.
+enum { SC_NISECT = 0, SC_WANTED = 1, SC_VISIBLE = 2, SC_COUNT = 3, SC_NUM = 8 };
+
+__device__ __forceinline__ uint32_t depth_bits(float z) {
+    uint32_t u = __float_as_uint(z);
+    return (u >> 31) ? ~u : (u ^ 0x80000000u);
+}
+
+// floor(v) converted to int with the float clamped to [-1, hi] first, so an
+// out-of-range extent saturates identically on every platform.
+__device__ __forceinline__ int tile_floor(float v, int hi) {
+    float f = floorf(v);
+    f = fmaxf(f, -1.0f);
+    f = fminf(f, (float)hi);
+    return (int)f;
+}
+
+// Start of a forward: device scalars, empty tile ranges, zero loss.
+__global__ __launch_bounds__(256) void k_begin(uint32_t *sc, uint32_t count, int2 *ranges,
+                                               int ntiles, float *loss) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < SC_NUM) sc[i] = (i == SC_COUNT) ? count : 0u;
+    if (i == 0 && loss) *loss = 0.f;
+    for (int t = i; t < ntiles; t += gridDim.x * 256) ranges[t] = make_int2(0, 0);
+}
+
+// Zeroes up to ten arrays of n floats in one launch.
+struct ZeroList {
+    float *p[10];
+};
+__global__ __launch_bounds__(256) void k_zero10(ZeroList z, int n) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+#pragma unroll
+        for (int f = 0; f < 10; ++f)
+            if (z.p[f]) z.p[f][i] = 0.f;
+    }
+}
+
+// Per splat: cull test and tile rectangle exactly as tileAssignKernel
+// (src/layers/tile_rasterize_layer.cu:76-125), with FP contraction off so
+// the decisions are reproducible on the host.
+__global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H, int ntx,
+                                              int nty, uint2 *__restrict__ rect,
+                                              uint32_t *__restrict__ ntiles,
+                                              uint32_t *__restrict__ dkey,
+                                              uint32_t *__restrict__ didx,
+                                              float4 *__restrict__ rec, uint32_t *sc) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    bool visible = false;
+    if (i < n) {
+        const float x = in.pos_x[i], y = in.pos_y[i], z = in.pos_z[i];
+        int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+        float cxx = 0.f, cxy = 0.f, cyy = 0.f, det = 0.f;
+        if (fabsf(x) <= 1.0f && fabsf(y) <= 1.0f && fabsf(z) <= 1.0f) {
+            cxx = in.cov_xx[i];
+            cxy = in.cov_xy[i];
+            cyy = in.cov_yy[i];
+            det = cxx * cyy - cxy * cxy;
+            if (det > 0.f) {
+                const float ex = 3.f * sqrtf(cxx);
+                const float ey = 3.f * sqrtf(cyy);
+                const float pixel_ndc = fmaxf(2.f / (float)W, 2.f / (float)H);
+                if (fmaxf(ex, ey) * 2.f >= pixel_ndc) {
+                    const float fx = (float)W / (float)kTile;
+                    const float fy = (float)H / (float)kTile;
+                    tx0 = max(tile_floor((x - ex + 1.f) * 0.5f * fx, ntx), 0);
+                    tx1 = min(tile_floor((x + ex + 1.f) * 0.5f * fx, ntx), ntx - 1);
+                    ty0 = max(tile_floor((1.f - (y + ey)) * 0.5f * fy, nty), 0);
+                    ty1 = min(tile_floor((1.f - (y - ey)) * 0.5f * fy, nty), nty - 1);
+                    visible = (tx0 <= tx1 && ty0 <= ty1);
+                }
+            }
+        }
+        didx[i] = (uint32_t)i;
+        if (visible) {
+            rect[i] = make_uint2((uint32_t)tx0 | ((uint32_t)tx1 << 16),
+                                 (uint32_t)ty0 | ((uint32_t)ty1 << 16));
+            ntiles[i] = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
+            dkey[i] = depth_bits(z);
+            const float idet = 1.f / det;
+            rec[3 * i + 0] = make_float4(x, y, cyy * idet, -cxy * idet);
+            rec[3 * i + 1] = make_float4(cxx * idet, in.opacity[i], idet, __uint_as_float(i));
+            rec[3 * i + 2] = make_float4(in.color_r[i], in.color_g[i], in.color_b[i], 0.f);
+        } else {
+            ntiles[i] = 0;
+            dkey[i] = 0xFFFFFFFFu;
+        }
+    }
+    const uint64_t vis = __ballot(visible);
+    if (__lane_id() == 0 && vis) atomicAdd(&sc[SC_VISIBLE], (uint32_t)__popcll(vis));
+}
+
+// Writes the pairs of each splat, taken in depth order, at its scanned
+// offset, tiles in row-major order. Pairs at or past the cap are dropped.
+__global__ __launch_bounds__(256) void k_emit(const uint32_t *__restrict__ order,
+                                              const uint32_t *__restrict__ off,
+                                              const uint32_t *__restrict__ ntiles,
+                                              const uint2 *__restrict__ rect,
+                                              const uint32_t *sc, int ntx,
+                                              uint32_t *__restrict__ tkey,
+                                              uint32_t *__restrict__ tval) {
+    const uint32_t n = sc[SC_COUNT];
+    const uint32_t cap = sc[SC_NISECT];
+    for (int64_t k = gtid(); k < n; k += gstride()) {
+        const uint32_t g = order[k];
+        if (ntiles[g] == 0) continue;
+        uint32_t slot = off[k];
+        if (slot >= cap) continue;
+        const uint2 r = rect[g];
+        const int tx0 = r.x & 0xFFFF, tx1 = r.x >> 16, ty0 = r.y & 0xFFFF, ty1 = r.y >> 16;
+        for (int ty = ty0; ty <= ty1; ++ty) {
+            for (int tx = tx0; tx <= tx1; ++tx) {
+                if (slot >= cap) return;
+                tkey[slot] = (uint32_t)(ty * ntx + tx);
+                tval[slot] = g;
+                ++slot;
+            }
+        }
+    }
+}
+
+// Tile boundaries in the sorted pair list (buildTileRangesKernel), plus the
+// optional 64-bit sorted keys.
+__global__ __launch_bounds__(256) void k_ranges(const uint32_t *__restrict__ tkey,
+                                                const uint32_t *__restrict__ tval,
+                                                const float *__restrict__ pos_z,
+                                                const uint32_t *sc, int2 *__restrict__ ranges,
+                                                uint64_t *__restrict__ keys64) {
+    const uint32_t n = sc[SC_NISECT];
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t t = tkey[i];
+        if (i == 0 || tkey[i - 1] != t) ranges[t].x = (int)i;
+        if (i == (int64_t)n - 1 || tkey[i + 1] != t) ranges[t].y = (int)(i + 1);
+        if (keys64) keys64[i] = ((uint64_t)t << 32) | depth_bits(pos_z[tval[i]]);
+    }
+}
+
+int grid_cap(uint64_t n, int per, int cap) {
+    uint64_t b = (n + per - 1) / per;
+    if (b < 1) b = 1;
+    return b > (uint64_t)cap ? cap : (int)b;
+}
+
+}  // namespace
+}  // namespace ds
+
+// ====================================================================== object
+
+struct ds_raster {
+    int W = 0, H = 0, ntx = 0, nty = 0, ntiles = 0;
+    int count_cap = 0;
+    uint32_t isect_cap = 0;
+    int keep_keys = 0;
+    int grad_mode = DS_GRAD_EXACT;
+    int pix_per_thread = 4;  // compositing: pixels per thread (1, 2 or 4)
+    int tile_bits = 1;
+    // per splat
+    ds::DevBuf<uint2> rect;
+    ds::DevBuf<uint32_t> ntiles_of, dkey_a, dkey_b, didx_a, didx_b, pair_off;
+    ds::DevBuf<float4> rec;
+    // per pair
+    ds::DevBuf<uint32_t> tkey_a, tkey_b, tval_a, tval_b;
+    ds::DevBuf<uint64_t> keys64;
+    // per tile / pixel
+    ds::DevBuf<int2> ranges;
+    ds::DevBuf<float> T;
+    ds::DevBuf<int32_t> last;
+    // scratch
+    ds::DevBuf<uint32_t> hist, part, scalars;
+    // results of the last forward
+    const uint32_t *flat = nullptr;
+    const uint32_t *tkey_sorted = nullptr;
+    int last_count = -1;
+    bool has_fwd = false;
+};
+
+namespace ds {
+
+static ds_status raster_alloc_pixels(ds_raster *r, int W, int H) {
+    DS_REQUIRE(W > 0 && H > 0 && W <= 16384 && H <= 16384, "raster size out of range");
+    r->W = W;
+    r->H = H;
+    r->ntx = (W + kTile - 1) / kTile;
+    r->nty = (H + kTile - 1) / kTile;
+    r->ntiles = r->ntx * r->nty;
+    int bits = 1;
+    while ((1 << bits) < r->ntiles) ++bits;
+    r->tile_bits = bits;
+    DS_TRY(r->ranges.alloc((size_t)r->ntiles));
+    DS_TRY(r->T.alloc((size_t)W * H));
+    DS_TRY(r->last.alloc((size_t)W * H));
+    r->has_fwd = false;
+    return DS_OK;
+}
+
+static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_rgb,
+                                const float *target, float *loss, float *grad, hipStream_t s) {
+    const int n = in.count;
+    DS_REQUIRE(n >= 0 && n <= r->count_cap, "splat count exceeds the rasterizer's count_cap");
+    DS_REQUIRE(out_rgb != nullptr, "out_rgb is NULL");
+    DS_REQUIRE((target == nullptr) == (loss == nullptr) && (target == nullptr) == (grad == nullptr),
+               "target, loss and grad must be given together");
+    uint32_t *sc = r->scalars.ptr;
+    hipLaunchKernelGGL(k_begin, dim3(grid_cap((uint64_t)r->ntiles, 256, 256)), dim3(256), 0, s,
+                       sc, (uint32_t)n, r->ranges.ptr, r->ntiles, loss);
+    DS_CHECK_LAUNCH("begin");
+    if (n > 0) {
+        hipLaunchKernelGGL(k_prep, dim3((n + 255) / 256), dim3(256), 0, s, in, n, r->W, r->H,
+                           r->ntx, r->nty, r->rect.ptr, r->ntiles_of.ptr, r->dkey_a.ptr,
+                           r->didx_a.ptr, r->rec.ptr, sc);
+        DS_CHECK_LAUNCH("prep");
+        // depth sort of splats: four 8-bit LSD passes over the 32 depth bits
+        uint32_t *ka = r->dkey_a.ptr, *kb = r->dkey_b.ptr, *va = r->didx_a.ptr,
+                 *vb = r->didx_b.ptr;
+        for (int pass = 0; pass < 4; ++pass) {
+            DS_TRY(radix_pass<uint32_t>(ka, va, kb, vb, sc + SC_COUNT, (uint32_t)n, 8 * pass,
+                                        r->hist.ptr, r->part.ptr, s));
+            std::swap(ka, kb);
+            std::swap(va, vb);
+        }
+        const uint32_t *order = va;
+        // pair offsets in depth order; total and capped count
+        DS_TRY(scan_exclusive_u32(r->ntiles_of.ptr, order, r->pair_off.ptr,
+                                  ScanLen{sc + SC_COUNT, 1, 1}, (uint32_t)n, r->part.ptr,
+                                  sc + SC_WANTED, sc + SC_NISECT, r->isect_cap, s));
+        hipLaunchKernelGGL(k_emit, dim3(grid_cap((uint64_t)n, 256, 4096)), dim3(256), 0, s, order,
+                           (const uint32_t *)r->pair_off.ptr, (const uint32_t *)r->ntiles_of.ptr,
+                           (const uint2 *)r->rect.ptr, (const uint32_t *)sc, r->ntx,
+                           r->tkey_a.ptr, r->tval_a.ptr);
+        DS_CHECK_LAUNCH("emit");
+        // stable sort of the pairs on the tile bits
+        uint32_t *tka = r->tkey_a.ptr, *tkb = r->tkey_b.ptr, *tva = r->tval_a.ptr,
+                 *tvb = r->tval_b.ptr;
+        for (int shift = 0; shift < r->tile_bits; shift += kRadixBits) {
+            DS_TRY(radix_pass<uint32_t>(tka, tva, tkb, tvb, sc + SC_NISECT, r->isect_cap, shift,
+                                        r->hist.ptr, r->part.ptr, s));
+            std::swap(tka, tkb);
+            std::swap(tva, tvb);
+        }
+        r->tkey_sorted = tka;
+        r->flat = tva;
+        hipLaunchKernelGGL(k_ranges, dim3(grid_cap(r->isect_cap, 256, 4096)), dim3(256), 0, s,
+                           (const uint32_t *)tka, (const uint32_t *)tva, (const float *)in.pos_z,
+                           (const uint32_t *)sc, r->ranges.ptr,
+                           r->keep_keys ? r->keys64.ptr : (uint64_t *)nullptr);
+        DS_CHECK_LAUNCH("ranges");
+    } else {
+        r->flat = r->tval_a.ptr;
+        r->tkey_sorted = r->tkey_a.ptr;
+    }
+    const CompositeArgs ca{(const float4 *)r->rec.ptr, r->flat, (const int2 *)r->ranges.ptr,
+                           r->W, r->H, r->ntx, r->ntiles};
+    DS_TRY(launch_composite_fwd(ca, r->pix_per_thread, out_rgb, r->T.ptr, r->last.ptr, target,
+                                loss, grad, s));
+    r->last_count = n;
+    r->has_fwd = true;
+    return DS_OK;
+}
+
+static ds_status raster_backward(ds_raster *r, const ds_splat2d &in, const float *grad_rgb,
+                                 const ds_splat2d &gin, hipStream_t s) {
+    DS_REQUIRE(r->has_fwd, "ds_raster_bwd called before ds_raster_fwd");
+    DS_REQUIRE(in.count == r->last_count, "splat count differs from the last forward");
+    DS_REQUIRE(grad_rgb != nullptr, "grad_rgb is NULL");
+    const CompositeArgs ca{(const float4 *)r->rec.ptr, r->flat, (const int2 *)r->ranges.ptr,
+                           r->W, r->H, r->ntx, r->ntiles};
+    DS_TRY(launch_composite_bwd(ca, r->pix_per_thread, r->grad_mode, r->T.ptr, r->last.ptr,
+                                grad_rgb, gin, s));
+    return DS_OK;
+}
+
+}  // namespace ds
+
+// ====================================================================== C-ABI
+
+using namespace ds;
+
+extern "C" ds_status ds_raster_create(const ds_raster_config *cfg, ds_raster **out) {
+    clear_error();
+    DS_REQUIRE(cfg && out, "NULL argument");
+    DS_REQUIRE(cfg->count_cap >= 0, "count_cap < 0");
+    int64_t cap = cfg->isect_cap > 0 ? cfg->isect_cap : (int64_t)1 << 25;
+    DS_REQUIRE(cap <= ((int64_t)1 << 30), "isect_cap above 2^30");
+    DS_REQUIRE(cfg->grad_mode == DS_GRAD_EXACT || cfg->grad_mode == DS_GRAD_REFERENCE,
+               "grad_mode must be DS_GRAD_EXACT or DS_GRAD_REFERENCE");
+    auto *r = new ds_raster();
+    r->count_cap = cfg->count_cap;
+    r->isect_cap = (uint32_t)cap;
+    r->keep_keys = cfg->keep_keys;
+    r->grad_mode = cfg->grad_mode;
+    ds_status st = raster_alloc_pixels(r, cfg->width, cfg->height);
+    const size_t nc = (size_t)(cfg->count_cap > 0 ? cfg->count_cap : 1);
+    if (st == DS_OK) st = r->rect.alloc(nc);
+    if (st == DS_OK) st = r->ntiles_of.alloc(nc);
+    if (st == DS_OK) st = r->dkey_a.alloc(nc);
+    if (st == DS_OK) st = r->dkey_b.alloc(nc);
+    if (st == DS_OK) st = r->didx_a.alloc(nc);
+    if (st == DS_OK) st = r->didx_b.alloc(nc);
+    if (st == DS_OK) st = r->pair_off.alloc(nc);
+    if (st == DS_OK) st = r->rec.alloc(3 * nc);
+    if (st == DS_OK) st = r->tkey_a.alloc(cap);
+    if (st == DS_OK) st = r->tkey_b.alloc(cap);
+    if (st == DS_OK) st = r->tval_a.alloc(cap);
+    if (st == DS_OK) st = r->tval_b.alloc(cap);
+    if (st == DS_OK && r->keep_keys) st = r->keys64.alloc(cap);
+    const uint64_t max_sort = (uint64_t)std::max<int64_t>(cap, (int64_t)nc);
+    if (st == DS_OK) st = r->hist.alloc(sort_hist_words(max_sort));
+    if (st == DS_OK) st = r->part.alloc(kSpineMax);
+    if (st == DS_OK) st = r->scalars.alloc(SC_NUM);
+    if (st == DS_OK) st = fill_u32(r->scalars.ptr, 0u, SC_NUM, nullptr);
+    if (st == DS_OK) st = fill_u32(r->ranges.ptr, 0u, (size_t)r->ntiles * 2, nullptr);
+    if (st == DS_OK && hipDeviceSynchronize() != hipSuccess)
+        st = set_error(DS_ERR_HIP, "hipDeviceSynchronize after raster create");
+    if (st != DS_OK) {
+        delete r;
+        return st;
+    }
+    *out = r;
+    return DS_OK;
+}
+
+extern "C" ds_status ds_raster_destroy(ds_raster *r) {
+    clear_error();
+    delete r;
+    return DS_OK;
+}
+
+extern "C" ds_status ds_raster_resize(ds_raster *r, int width, int height) {
+    clear_error();
+    DS_REQUIRE(r, "NULL raster");
+    if (width == r->W && height == r->H) return DS_OK;
+    return raster_alloc_pixels(r, width, height);
+}
+
+extern "C" ds_status ds_raster_fwd(ds_raster *r, const ds_splat2d *in, float *out_rgb,
+                                   const float *target_rgb, float *loss_dev, float *grad_rgb,
+                                   ds_stream_t s) {
+    clear_error();
+    DS_REQUIRE(r && in, "NULL argument");
+    return raster_forward(r, *in, out_rgb, target_rgb, loss_dev, grad_rgb, as_stream(s));
+}
+
+extern "C" ds_status ds_raster_bwd(ds_raster *r, const ds_splat2d *in, const float *grad_rgb,
+                                   const ds_splat2d *grad_in, ds_stream_t s) {
+    clear_error();
+    DS_REQUIRE(r && in && grad_in, "NULL argument");
+    return raster_backward(r, *in, grad_rgb, *grad_in, as_stream(s));
+}
+
+extern "C" ds_status ds_raster_counts(ds_raster *r, ds_stream_t s, uint32_t *n_isects,
+                                      uint32_t *visible_count, uint32_t *n_wanted) {
+    clear_error();
+    DS_REQUIRE(r, "NULL raster");
+    uint32_t h[SC_NUM];
+    DS_TRY_HIP(hipMemcpyAsync(h, r->scalars.ptr, sizeof(h), hipMemcpyDeviceToHost, as_stream(s)));
+    DS_TRY_HIP(hipStreamSynchronize(as_stream(s)));
+    if (n_isects) *n_isects = h[SC_NISECT];
+    if (visible_count) *visible_count = h[SC_VISIBLE];
+    if (n_wanted) *n_wanted = h[SC_WANTED];
+    return DS_OK;
+}
+
+extern "C" ds_status ds_raster_get_view(ds_raster *r, ds_raster_view *v) {
+    clear_error();
+    DS_REQUIRE(r && v, "NULL argument");
+    v->isect_keys_sorted = r->keep_keys ? r->keys64.ptr : nullptr;
+    v->flatten_ids = r->flat;
+    v->tile_ranges = reinterpret_cast<const int32_t *>(r->ranges.ptr);
+    v->transmittance = r->T.ptr;
+    v->last_ids = r->last.ptr;
+    v->n_isects = r->scalars.ptr + SC_NISECT;
+    v->n_isects_wanted = r->scalars.ptr + SC_WANTED;
+    v->visible_count = r->scalars.ptr + SC_VISIBLE;
+    v->num_tiles_x = r->ntx;
+    v->num_tiles_y = r->nty;
+    v->width = r->W;
+    v->height = r->H;
+    return DS_OK;
+}
+
+extern "C" ds_status ds_raster_set_tuning(ds_raster *r, int pix_per_thread) {
+    clear_error();
+    DS_REQUIRE(r, "NULL raster");
+    DS_REQUIRE(pix_per_thread == 1 || pix_per_thread == 2 || pix_per_thread == 4,
+               "pix_per_thread must be 1, 2 or 4");
+    r->pix_per_thread = pix_per_thread;
+    return DS_OK;
+}
+
+extern "C" ds_status ds_splat2d_zero(const ds_splat2d *v, ds_stream_t s) {
+    clear_error();
+    DS_REQUIRE(v, "NULL view");
+    DS_REQUIRE(v->count >= 0, "negative count");
+    if (v->count == 0) return DS_OK;
+    ZeroList z{{v->pos_x, v->pos_y, v->pos_z, v->cov_xx, v->cov_xy, v->cov_yy, v->color_r,
+                v->color_g, v->color_b, v->opacity}};
+    hipLaunchKernelGGL(k_zero10, dim3(grid_cap((uint64_t)v->count, 256, 2048)), dim3(256), 0,
+                       as_stream(s), z, v->count);
+    DS_CHECK_LAUNCH("zero10");
+    return DS_OK;
+}
+
+extern "C" ds_status ds_sort_pairs_u64(const uint64_t *keys_in, const uint32_t *vals_in,
+                                       uint64_t *keys_out, uint32_t *vals_out, int64_t n,
+                                       int key_bits, ds_stream_t s) {
+    clear_error();
+    DS_REQUIRE(n >= 0 && n <= ((int64_t)1 << 30), "n out of range");
+    DS_REQUIRE(key_bits >= 1 && key_bits <= 64, "key_bits out of range");
+    if (n == 0) return DS_OK;
+    DS_REQUIRE(keys_in && vals_in && keys_out && vals_out, "NULL argument");
+    hipStream_t st = as_stream(s);
+    DevBuf<uint64_t> kb;
+    DevBuf<uint32_t> vb, hist, part, nd;
+    DS_TRY(kb.alloc((size_t)n));
+    DS_TRY(vb.alloc((size_t)n));
+    DS_TRY(hist.alloc(sort_hist_words((uint64_t)n)));
+    DS_TRY(part.alloc(kSpineMax));
+    DS_TRY(nd.alloc(1));
+    DS_TRY(fill_u32(nd.ptr, (uint32_t)n, 1, st));
+    const int passes = (key_bits + kRadixBits - 1) / kRadixBits;
+    // ping-pong so the last pass lands in the output buffers
+    const uint64_t *ksrc = keys_in;
+    const uint32_t *vsrc = vals_in;
+    for (int p = 0; p < passes; ++p) {
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint64_t *kd = to_out ? keys_out : kb.ptr;
+        uint32_t *vd = to_out ? vals_out : vb.ptr;
+        DS_TRY(radix_pass<uint64_t>(ksrc, vsrc, kd, vd, nd.ptr, (uint32_t)n, kRadixBits * p,
+                                    hist.ptr, part.ptr, st));
+        ksrc = kd;
+        vsrc = vd;
+    }
+    DS_TRY_HIP(hipStreamSynchronize(st));
+    return DS_OK;
+}

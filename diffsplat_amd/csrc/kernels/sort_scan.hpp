@@ -1,0 +1,86 @@
+// Device-wide exclusive scan and stable LSD radix sort passes (wave64,
+// reduce-then-scan).  Element counts are read from device memory so that a
+// pass can sit in a captured graph without a host round trip; grids are
+// grid-stride over chunks.
+#pragma once
+
+#include "../ds_device.hpp"
+#include "../ds_internal.hpp"
+
+namespace ds {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanChunk = kScanThreads * kScanItems;   // 2048
+constexpr int kSpineThreads = 1024;
+constexpr int kSpineItems = 8;
+constexpr int kSpineMax = kSpineThreads * kSpineItems;  // 8192 chunks -> 16M elements
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortChunk = kSortThreads * kSortItems;   // 2048
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;                  // 256 == kSortThreads
+
+// Length of a scan: ceil(n / div) * mul with n read from device memory.
+struct ScanLen {
+    const uint32_t *n_dev;
+    uint32_t div;
+    uint32_t mul;
+};
+
+__device__ __forceinline__ uint32_t scan_len(const ScanLen &L) {
+    const uint32_t n = *L.n_dev;
+    return ((n + L.div - 1) / L.div) * L.mul;
+}
+
+// Inclusive wave64 scan.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    const int l = __lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, kWave);
+        if (l >= d) x += y;
+    }
+    return x;
+}
+
+// Exclusive scan of one value per thread across a block of NT threads;
+// returns the exclusive prefix, *total receives the block sum.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_tmp,
+                                                    uint32_t *total) {
+    constexpr int NW = NT / kWave;
+    const int w = threadIdx.x / kWave;
+    const uint32_t inc = wave_incl_scan(v);
+    if (__lane_id() == kWave - 1) s_tmp[w] = inc;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        const uint32_t s = s_tmp[k];
+        wbase += (k < w) ? s : 0u;
+        tot += s;
+    }
+    __syncthreads();
+    if (total) *total = tot;
+    return wbase + inc - v;
+}
+
+ds_status scan_exclusive_u32(const uint32_t *in, const uint32_t *gather_idx, uint32_t *out,
+                             ScanLen len, uint32_t max_len, uint32_t *part, uint32_t *total_out,
+                             uint32_t *clamped_out, uint32_t clamp_cap, hipStream_t s);
+
+// One stable LSD pass: sort (key, val) by bits [shift, shift+8) of the key.
+// n_dev: element count; hist: scratch of kRadix * ceil(max_n / kSortChunk)
+// words; part: scan scratch of kSpineMax words.
+template <typename K>
+ds_status radix_pass(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
+                     const uint32_t *n_dev, uint32_t max_n, int shift, uint32_t *hist,
+                     uint32_t *part, hipStream_t s);
+
+inline uint32_t sort_hist_words(uint64_t max_n) {
+    return (uint32_t)(kRadix * ((max_n + kSortChunk - 1) / kSortChunk));
+}
+
+}  // namespace ds

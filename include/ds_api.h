@@ -1,0 +1,410 @@
+/*
+ * ds_api.h -- C-ABI of the MI355X (gfx950) Gaussian-splatting engine.
+ *
+ * Every entry point takes plain pointers and sizes; device pointers are
+ * caller-owned unless an object (ds_raster, ds_fitter, ds_renderer) says
+ * otherwise.  Every compute call takes a HIP stream (NULL = the null stream)
+ * and is asynchronous with respect to the host unless its comment says it
+ * synchronises.  Errors are returned as ds_status; ds_last_error() gives the
+ * detail of the last failing call on the calling thread.
+ *
+ * The host-side C++ layer classes (diffsplat_amd/csrc/host/) keep the
+ * reference's class names and method semantics on top of these functions:
+ *   GaussActivLayer      src/layers/gauss_activ_layer.{h,cu}
+ *   PerspProjectLayer    src/layers/persp_project_layer.{h,cu}
+ *   TileRasterizeLayer   src/layers/tile_rasterize_layer.{h,cu}
+ *   MSELossLayer         src/layers/mse_loss_layer.{h,cu}
+ *   Adam                 src/optimizers/adam.{h,cu}
+ *   ImageFitter          src/pipelines/image_fitter.{h,cpp}
+ *   SplatRenderer        src/pipelines/splat_renderer.{h,cpp}
+ */
+#ifndef DS_API_H
+#define DS_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t, passed as an opaque pointer so that this header needs no HIP headers. */
+typedef void *ds_stream_t;
+
+typedef enum ds_status {
+    DS_OK = 0,
+    DS_ERR_INVALID_ARG = 1,   /* bad pointer, size or configuration            */
+    DS_ERR_HIP = 2,           /* a HIP runtime call failed                      */
+    DS_ERR_OUT_OF_MEMORY = 3, /* device allocation failed                       */
+    DS_ERR_CAPACITY = 4,      /* a fixed-capacity buffer would overflow         */
+    DS_ERR_STATE = 5          /* call order violated (e.g. bwd before fwd)      */
+} ds_status;
+
+/* ------------------------------------------------------------------ */
+/* Library / runtime helpers                                           */
+/* ------------------------------------------------------------------ */
+
+const char *ds_status_string(ds_status s);
+/* Detail message of the last failing call on this thread ("" if none). */
+const char *ds_last_error(void);
+/* Library version string, and the gfx target it was compiled for. */
+const char *ds_version(void);
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int ds_device_count(void);
+/* Name of device 0 ("" without a GPU). */
+ds_status ds_device_name(char *buf, size_t len);
+
+/* Device memory, for callers without their own allocator (tests, benches). */
+ds_status ds_malloc(void **dptr, size_t bytes);
+ds_status ds_free(void *dptr);
+ds_status ds_memcpy_htod(void *dst, const void *src, size_t bytes);
+ds_status ds_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+ds_status ds_memcpy_dtod(void *dst, const void *src, size_t bytes, ds_stream_t s);
+/* Fills `count` 32-bit words with `value` using a kernel (safe under graph capture). */
+ds_status ds_fill_u32(void *dptr, uint32_t value, size_t count, ds_stream_t s);
+ds_status ds_synchronize(void);
+ds_status ds_stream_create(ds_stream_t *s);
+ds_status ds_stream_destroy(ds_stream_t s);
+ds_status ds_stream_synchronize(ds_stream_t s);
+
+/* HIP events, for timing launches on a given stream. */
+typedef void *ds_event_t;
+ds_status ds_event_create(ds_event_t *e);
+ds_status ds_event_destroy(ds_event_t e);
+ds_status ds_event_record(ds_event_t e, ds_stream_t s);
+ds_status ds_event_synchronize(ds_event_t e);
+ds_status ds_event_elapsed_ms(ds_event_t start, ds_event_t end, float *ms);
+
+/* ------------------------------------------------------------------ */
+/* SoA views (one device float array per field, count elements each)  */
+/* ------------------------------------------------------------------ */
+
+/* Raw trainable parameters and their gradients.
+ * Replaces Gaussian3DParams / Gaussian3DGrads (src/types/gaussian3d.h:34-120).
+ * sh_rest_{r,g,b} are band-major: element (band, i) is at [band * count + i];
+ * they may be NULL when sh_num_bands == 0. */
+typedef struct ds_gauss3d {
+    float *pos_x, *pos_y, *pos_z;
+    float *scale_x, *scale_y, *scale_z;        /* log-scale            */
+    float *rot_w, *rot_x, *rot_y, *rot_z;      /* raw quaternion       */
+    float *sh_dc_r, *sh_dc_g, *sh_dc_b;
+    float *sh_rest_r, *sh_rest_g, *sh_rest_b;
+    float *logit_opacity;
+    int32_t count;
+    int32_t sh_num_bands;                      /* 0, 3, 8 or 15        */
+} ds_gauss3d;
+
+/* World-space activated splats. Replaces Splat3DParams / Splat3DGrads
+ * (src/types/splat3d.h:14-51). Covariance is the upper triangle. */
+typedef struct ds_splat3d {
+    float *pos_x, *pos_y, *pos_z;
+    float *cov_xx, *cov_xy, *cov_xz, *cov_yy, *cov_yz, *cov_zz;
+    float *color_r, *color_g, *color_b;
+    float *opacity;
+    int32_t count;
+} ds_splat3d;
+
+/* NDC-space splats. Replaces Splat2DParams / Splat2DGrads
+ * (src/types/splat2d.h:13-46). */
+typedef struct ds_splat2d {
+    float *pos_x, *pos_y, *pos_z;
+    float *cov_xx, *cov_xy, *cov_yy;
+    float *color_r, *color_g, *color_b;
+    float *opacity;
+    int32_t count;
+} ds_splat2d;
+
+/* ------------------------------------------------------------------ */
+/* Activation: log-scale/quaternion -> covariance, SH -> colour,       */
+/* logit -> opacity.                                                   */
+/* ------------------------------------------------------------------ */
+
+/* GaussActivLayer::forward, covForwardKernel (src/layers/gauss_activ_layer.cu:37-170, 411-440).
+ * sh_degree in [0,3]; in->sh_num_bands must cover it. cam_pos is the
+ * world-space camera position used for the SH view direction. */
+ds_status ds_activ_fwd(const ds_gauss3d *in, const ds_splat3d *out, int sh_degree,
+                       const float cam_pos[3], ds_stream_t s);
+
+/* GaussActivLayer::backward, covBackwardKernel (src/layers/gauss_activ_layer.cu:198-407, 442-476).
+ * `out` is the forward output (clamp gate and sigmoid use it); grad_out holds
+ * dL/dSplat3D with off-diagonal covariance gradients in the full-matrix
+ * convention; grad_in receives dL/dparams (overwritten, not accumulated). */
+ds_status ds_activ_bwd(const ds_gauss3d *in, const ds_splat3d *out, const ds_splat3d *grad_out,
+                       const ds_gauss3d *grad_in, int sh_degree, const float cam_pos[3],
+                       ds_stream_t s);
+
+/* ------------------------------------------------------------------ */
+/* Perspective projection world -> NDC                                 */
+/* ------------------------------------------------------------------ */
+
+/* PerspProjectLayer::setCamera + forward (src/layers/persp_project_layer.cu:40-149, 269-274, 279-292).
+ * pv is P*V, column-major (element (row, col) at pv[col*4+row]), passed by
+ * value on every call. Splats with clip_w <= 0 get pos_z = FLT_MAX and their
+ * other outputs are left unwritten, as in the reference. */
+ds_status ds_project_fwd(const ds_splat3d *in, const ds_splat2d *out, const float pv[16],
+                         ds_stream_t s);
+
+/* PerspProjectLayer::backward (src/layers/persp_project_layer.cu:173-267, 294-317). */
+ds_status ds_project_bwd(const ds_splat3d *in, const ds_splat2d *grad_out,
+                         const ds_splat3d *grad_in, const float pv[16], ds_stream_t s);
+
+/* Fused activation + projection (no Splat3D round trip through HBM).
+ * Same results as ds_activ_fwd followed by ds_project_fwd. */
+ds_status ds_activ_project_fwd(const ds_gauss3d *in, const ds_splat2d *out, int sh_degree,
+                               const float cam_pos[3], const float pv[16], ds_stream_t s);
+
+/* Fused backward: dL/dSplat2D -> dL/dparams. Same results as ds_project_bwd
+ * followed by ds_activ_bwd (the clamp gate and sigmoid derivative are
+ * recomputed from the parameters instead of read from a saved Splat3D). */
+ds_status ds_activ_project_bwd(const ds_gauss3d *in, const ds_splat2d *grad_out,
+                               const ds_gauss3d *grad_in, int sh_degree,
+                               const float cam_pos[3], const float pv[16], ds_stream_t s);
+
+/* ------------------------------------------------------------------ */
+/* Tile rasterizer                                                     */
+/* ------------------------------------------------------------------ */
+
+typedef struct ds_raster ds_raster;
+
+typedef struct ds_raster_config {
+    int32_t width, height;     /* image size in pixels                                     */
+    int32_t count_cap;         /* maximum splat count per call                             */
+    int64_t isect_cap;         /* maximum (tile, splat) pairs; 0 = 1<<25 as the reference  */
+    int32_t keep_keys;         /* 1: also materialise the sorted 64-bit (tile|depth) keys  */
+    int32_t grad_mode;         /* DS_GRAD_EXACT (0) or DS_GRAD_REFERENCE (1), see below    */
+} ds_raster_config;
+
+/* dL/dalpha in the compositing backward.
+ * DS_GRAD_EXACT: T_before * c - (colour behind) / (1 - alpha), the true
+ *   derivative of front-to-back compositing (agrees with finite differences).
+ * DS_GRAD_REFERENCE: T_before * (c - colour behind), the expression of
+ *   src/layers/tile_rasterize_layer.cu:491-495, where "colour behind" is the
+ *   absolute sum of c_i alpha_i T_i and is not divided by T after the splat. */
+#define DS_GRAD_EXACT 0
+#define DS_GRAD_REFERENCE 1
+
+/* Device pointers into the rasterizer's own buffers, valid until the next
+ * ds_raster_fwd / resize / destroy on the same object. */
+typedef struct ds_raster_view {
+    const uint64_t *isect_keys_sorted; /* [n_isects] or NULL unless keep_keys               */
+    const uint32_t *flatten_ids;       /* [n_isects] splat index per sorted pair             */
+    const int32_t *tile_ranges;        /* [num_tiles][2] {start, end}; {0,0} for empty      */
+    const float *transmittance;        /* [H*W] final T per pixel                            */
+    const int32_t *last_ids;           /* [H*W] sorted index of last contributor             */
+    const uint32_t *n_isects;          /* device scalar: pairs kept (<= isect_cap)           */
+    const uint32_t *n_isects_wanted;   /* device scalar: pairs requested before the cap      */
+    const uint32_t *visible_count;     /* device scalar                                      */
+    int32_t num_tiles_x, num_tiles_y;
+    int32_t width, height;
+} ds_raster_view;
+
+/* TileRasterizeLayer::allocate (src/layers/tile_rasterize_layer.cu:591-616). */
+ds_status ds_raster_create(const ds_raster_config *cfg, ds_raster **out);
+ds_status ds_raster_destroy(ds_raster *r);
+/* TileRasterizeLayer::resize (src/layers/tile_rasterize_layer.cu:618-633). */
+ds_status ds_raster_resize(ds_raster *r, int width, int height);
+
+/* TileRasterizeLayer::forward (src/layers/tile_rasterize_layer.cu:637-708):
+ * tile assign -> depth/tile radix sort -> tile ranges -> front-to-back
+ * compositing. out_rgb is [H*W*3] HWC. No host synchronisation.
+ * If target_rgb is non-NULL the MSE loss of the reference's MSELossLayer
+ * (sum over channels / (H*W)) is written to *loss_dev and its gradient to
+ * grad_rgb (both device pointers) in the same pass; pass NULLs otherwise. */
+ds_status ds_raster_fwd(ds_raster *r, const ds_splat2d *in, float *out_rgb,
+                        const float *target_rgb, float *loss_dev, float *grad_rgb,
+                        ds_stream_t s);
+
+/* TileRasterizeLayer::backward (src/layers/tile_rasterize_layer.cu:710-728).
+ * Gradients are ACCUMULATED into grad_in (atomics), as in the reference: the
+ * caller zeroes them first (ds_splat2d_zero). grad_in->pos_z is not written. */
+ds_status ds_raster_bwd(ds_raster *r, const ds_splat2d *in, const float *grad_rgb,
+                        const ds_splat2d *grad_in, ds_stream_t s);
+
+/* Zeroes all ten arrays of a Splat2D view with fill kernels. */
+ds_status ds_splat2d_zero(const ds_splat2d *v, ds_stream_t s);
+
+/* Reads back n_isects and visible_count (synchronises the stream). Replaces
+ * TileRasterizeLayer::getVisibleCount (src/layers/tile_rasterize_layer.cu:582-587). */
+ds_status ds_raster_counts(ds_raster *r, ds_stream_t s, uint32_t *n_isects,
+                           uint32_t *visible_count, uint32_t *n_isects_wanted);
+
+ds_status ds_raster_get_view(ds_raster *r, ds_raster_view *out);
+
+/* Launch-shape knobs of the compositing kernels (results do not depend on
+ * them): pix_per_thread in {1, 2, 4} = pixels of a 16x16 tile per thread
+ * (256, 128 or 64 threads per tile). Default 4. */
+ds_status ds_raster_set_tuning(ds_raster *r, int pix_per_thread);
+
+/* ------------------------------------------------------------------ */
+/* Stand-alone sort (the radix sort inside ds_raster_fwd)              */
+/* ------------------------------------------------------------------ */
+
+/* Stable ascending sort of n (u64 key, u32 value) pairs on the low
+ * `key_bits` bits, as cub::DeviceRadixSort::SortPairs does at
+ * src/layers/tile_rasterize_layer.cu:667-683. Allocates its own scratch
+ * (synchronous). */
+ds_status ds_sort_pairs_u64(const uint64_t *keys_in, const uint32_t *vals_in,
+                            uint64_t *keys_out, uint32_t *vals_out, int64_t n, int key_bits,
+                            ds_stream_t s);
+
+/* ------------------------------------------------------------------ */
+/* MSE loss                                                            */
+/* ------------------------------------------------------------------ */
+
+/* MSELossLayer::forward (src/layers/mse_loss_layer.cu:22-42, 84-91):
+ * *loss_dev = sum_px sum_c (p - t)^2 / num_pixels. */
+ds_status ds_mse_fwd(const float *pixels, const float *target, int64_t num_pixels,
+                     float *loss_dev, ds_stream_t s);
+/* MSELossLayer::backward (src/layers/mse_loss_layer.cu:54-67, 93-98): grad = 2 (p - t) / num_pixels. */
+ds_status ds_mse_bwd(const float *pixels, const float *target, int64_t num_pixels,
+                     float *grad, ds_stream_t s);
+
+/* ------------------------------------------------------------------ */
+/* Adam                                                                */
+/* ------------------------------------------------------------------ */
+
+#define DS_ADAM_MAX_GROUPS 32
+
+typedef struct ds_adam_group {
+    float *param;
+    const float *grad;
+    float *m;
+    float *v;
+    int64_t n;
+    float lr;
+} ds_adam_group;
+
+typedef struct ds_adam_hparams {
+    float beta1, beta2, epsilon;   /* reference defaults 0.9, 0.999, 1e-8 (src/optimizers/adam.h:5-9) */
+} ds_adam_hparams;
+
+/* Adam::step (src/optimizers/adam.cu:12-37, 59-73) for all groups in ONE
+ * launch. `step` is the 1-based step number t used for the bias
+ * corrections 1/(1-beta^t). */
+ds_status ds_adam_step(const ds_adam_group *groups, int n_groups, const ds_adam_hparams *hp,
+                       int32_t step, ds_stream_t s);
+
+/* Same, with the step counter in device memory: *step_dev is incremented
+ * on the device and the new value used, so a captured graph can replay it. */
+ds_status ds_adam_step_dev(const ds_adam_group *groups, int n_groups,
+                           const ds_adam_hparams *hp, int32_t *step_dev, ds_stream_t s);
+
+/* ------------------------------------------------------------------ */
+/* Image fitter (ImageFitter, src/pipelines/image_fitter.{h,cpp})      */
+/* ------------------------------------------------------------------ */
+
+typedef struct ds_fitter ds_fitter;
+
+typedef struct ds_fitter_config {
+    int32_t width, height;
+    int32_t count;
+    int32_t sh_degree;
+    float view[16], proj[16];      /* column-major, OpenGL conventions                */
+    float cam_pos[3];
+    float lr_pos, lr_scale, lr_rot, lr_sh_dc, lr_sh_rest, lr_opacity;
+    float beta1, beta2, epsilon;
+    int64_t isect_cap;             /* 0 = 1<<25                                        */
+    int32_t fused;                 /* 1: fused activation+projection and fused MSE     */
+    int32_t grad_mode;             /* DS_GRAD_EXACT or DS_GRAD_REFERENCE               */
+} ds_fitter_config;
+
+/* Fills cfg with the reference's defaults (src/pipelines/image_fitter.cpp:18-34,
+ * src/optimizers/gaussian3d_adam.h:11-19): ortho pixel-unit camera, 3DGS LRs. */
+void ds_fitter_default_config(int width, int height, int count, ds_fitter_config *cfg);
+
+ds_status ds_fitter_create(const ds_fitter_config *cfg, ds_fitter **out);
+ds_status ds_fitter_destroy(ds_fitter *f);
+/* Uploads the target image, host [H*W*3] HWC floats. */
+ds_status ds_fitter_set_target(ds_fitter *f, const float *host_rgb);
+/* Parameter slab layout (host, field-major, n = count, b = bands):
+ * pos_x,pos_y,pos_z, scale_x..z, rot_w..z, sh_dc_r..b  (13 x n),
+ * sh_rest_r, sh_rest_g, sh_rest_b (3 x b*n), logit_opacity (n).
+ * Uploading resets Adam state and the step counter. */
+ds_status ds_fitter_upload_params(ds_fitter *f, const float *host_slab);
+ds_status ds_fitter_download_params(ds_fitter *f, float *host_slab);
+/* Same layout, gradients of the last step. */
+ds_status ds_fitter_download_grads(ds_fitter *f, float *host_slab);
+/* ImageFitter::step (src/pipelines/image_fitter.cpp:103-112), n_steps times,
+ * asynchronously on the fitter's stream. */
+ds_status ds_fitter_step(ds_fitter *f, int n_steps);
+/* Forward only (render + loss) with the current parameters. */
+ds_status ds_fitter_forward(ds_fitter *f);
+/* Loss of the last forward (synchronises). */
+ds_status ds_fitter_loss(ds_fitter *f, float *loss);
+ds_status ds_fitter_synchronize(ds_fitter *f);
+/* Rendered image of the last forward, host [H*W*3] (synchronises). */
+ds_status ds_fitter_download_image(ds_fitter *f, float *host_rgb);
+ds_status ds_fitter_get_raster(ds_fitter *f, ds_raster **r);
+ds_status ds_fitter_get_stream(ds_fitter *f, ds_stream_t *s);
+/* 1: capture one step into a HIP graph on first use and replay it. */
+ds_status ds_fitter_set_graph(ds_fitter *f, int enable);
+ds_status ds_fitter_step_count(ds_fitter *f, int32_t *step);
+/* Device views of the fitter's own buffers (valid for the fitter's life):
+ * the parameters, the Splat2D the rasterizer consumes, its gradient, and
+ * dL/d(image) [H*W*3]. Any pointer may be NULL. */
+ds_status ds_fitter_views(ds_fitter *f, ds_gauss3d *params, ds_splat2d *splat2d,
+                          ds_splat2d *grad_splat2d, float **grad_rgb);
+
+/* ------------------------------------------------------------------ */
+/* Forward-only renderer (SplatRenderer, src/pipelines/splat_renderer.{h,cpp}) */
+/* ------------------------------------------------------------------ */
+
+typedef struct ds_renderer ds_renderer;
+
+ds_status ds_renderer_create(int width, int height, int count, int sh_degree, int64_t isect_cap,
+                             ds_renderer **out);
+ds_status ds_renderer_destroy(ds_renderer *r);
+/* Same slab layout as ds_fitter_upload_params. */
+ds_status ds_renderer_upload_params(ds_renderer *r, const float *host_slab);
+/* SplatRenderer::render (src/pipelines/splat_renderer.cpp:85-91). Async. */
+ds_status ds_renderer_render(ds_renderer *r, const float view[16], const float proj[16],
+                             const float cam_pos[3]);
+ds_status ds_renderer_download_image(ds_renderer *r, float *host_rgb);
+ds_status ds_renderer_get_raster(ds_renderer *r, ds_raster **out);
+ds_status ds_renderer_get_stream(ds_renderer *r, ds_stream_t *s);
+ds_status ds_renderer_synchronize(ds_renderer *r);
+
+/* ------------------------------------------------------------------ */
+/* Synthetic workloads and cameras (host only, deterministic)          */
+/* ------------------------------------------------------------------ */
+
+/* Target image: smooth gradient + 3 sinusoids + n_ellipses filled ellipses
+ * of uniform colour, [H*W*3] HWC in [0,1]. */
+void ds_gen_target_image(uint64_t seed, int width, int height, int n_ellipses, float *out_rgb);
+
+/* Splats inside a pinhole frustum (camera at the origin looking down -z,
+ * vertical fov fovy_deg, aspect width/height, depth in [zmin, zmax]):
+ * log-scale ~ N(ls_mean, ls_std), quaternion = normalize(N(0,1)^4),
+ * logit = logit_opacity, colour ~ U[0,1] (stored as DC SH). Writes the
+ * slab layout of ds_fitter_upload_params with sh_degree 0. */
+void ds_gen_frustum_splats(uint64_t seed, int count, float fovy_deg, float aspect, float zmin,
+                           float zmax, float ls_mean, float ls_std, float logit_opacity,
+                           float *slab);
+
+/* Clustered scene: positions from n_clusters anisotropic Gaussian clusters
+ * in the unit cube, log-scale ~ N(ls_mean, ls_std), opacity logit ~ N(0,2),
+ * SH degree sh_degree with rest coefficients ~ N(0, 0.2) and DC colour
+ * N(0.5, 0.2) (stored as DC SH). Slab layout as above. */
+void ds_gen_cluster_scene(uint64_t seed, int count, int sh_degree, int n_clusters, float ls_mean,
+                          float ls_std, float *slab);
+
+/* The reference fitter's random init semantics (src/utils/splat_utils.h:10-51)
+ * on a counter-based generator: pixel-unit box, isotropic log-scale
+ * ln2 + U[-0.5,0.5], identity rotation, random DC colour, opacity in [0.6,1]. */
+void ds_gen_fitter_init(uint64_t seed, int count, int width, int height, float *slab);
+
+/* OpenGL-convention camera matrices (column-major, clip z in [-1,1]). */
+void ds_mat_perspective(float fovy_rad, float aspect, float znear, float zfar, float out[16]);
+void ds_mat_ortho(float l, float r, float b, float t, float n, float f, float out[16]);
+void ds_mat_look_at(const float eye[3], const float center[3], const float up[3], float out[16]);
+void ds_mat_mul(const float a[16], const float b[16], float out[16]);
+
+/* Slab size in floats for count splats of sh_degree: (14 + 3*bands) * count. */
+int64_t ds_slab_floats(int count, int sh_degree);
+int ds_sh_bands(int sh_degree);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* DS_API_H */

@@ -1,0 +1,107 @@
+"""Shared scenes and pipeline compositions for the tests.
+
+The FP64 pipeline below chains the oracle's stages in the reference's step
+order (src/pipelines/image_fitter.cpp:103-112): activation -> projection ->
+tile assign/sort/ranges -> compositing -> MSE, and back.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle
+
+# The reference's finite-difference scene (tests/grad_check.cu:33-37, 72-132):
+# 8 splats on a 2x4 grid at staggered depth, anisotropic scales, a 15 degree
+# tilt about z, DC colour 0 (-> 0.5), logit 2, a 32x32 target of 0.7, the
+# fitter's ortho camera and a camera position of (0, 0, 5).
+GC_N, GC_W, GC_H = 8, 32, 32
+
+
+def ortho(l, r, b, t, n, f):
+    m = np.zeros(16, np.float64)
+    m[0] = 2 / (r - l)
+    m[5] = 2 / (t - b)
+    m[10] = -2 / (f - n)
+    m[12] = -(r + l) / (r - l)
+    m[13] = -(t + b) / (t - b)
+    m[14] = -(f + n) / (f - n)
+    m[15] = 1
+    return m
+
+
+def gradcheck_scene(deg: int = 0, seed: int = 0):
+    n, W, H = GC_N, GC_W, GC_H
+    nb = oracle.bands(deg)
+    p = np.zeros((14 + 3 * nb) * n, np.float64)
+    f = lambda k: p[k * n:(k + 1) * n]  # noqa: E731
+    i = np.arange(n)
+    f(0)[:] = (i % 4) * 4.0 - 6.0
+    f(1)[:] = (i // 4) * 6.0 - 3.0
+    f(2)[:] = i * 0.1
+    f(3)[:], f(4)[:], f(5)[:] = 1.2, 0.8, 0.5
+    f(6)[:], f(7)[:], f(8)[:], f(9)[:] = 0.991, 0.0, 0.0, 0.131
+    p[-n:] = 2.0
+    if nb:
+        rng = np.random.default_rng(seed)
+        p[13 * n:13 * n + 3 * nb * n] = rng.normal(0.0, 0.15, 3 * nb * n)
+    target = np.full(W * H * 3, 0.7, np.float64)
+    hw, hh = W * 0.5, H * 0.5
+    zr = min(hw, hh) * 0.25
+    pv = ortho(-hw, hw, -hh, hh, -zr, zr)  # identity view
+    cam = np.array([0.0, 0.0, 5.0])
+    return dict(params=p, n=n, W=W, H=H, deg=deg, target=target, pv=pv, cam=cam)
+
+
+def fp64_forward(params, n, deg, cam, pv, W, H, target, pairs=None):
+    """FP64 forward. Tile decisions come from the fp32 transcription of the
+    (rounded) Splat2D unless `pairs` (a tile_assign_sort result) is given."""
+    s3 = oracle.act_fwd(params, n, deg, cam)
+    s2 = oracle.proj_fwd(s3, n, pv)
+    ta = pairs if pairs is not None else oracle.tile_assign_sort(s2.astype(np.float32), n, W, H)
+    comp = oracle.comp_fwd(s2, n, ta["ids"], ta["ranges"], W, H)
+    loss, dLdC = oracle.mse(comp["rgb"], target, W * H)
+    return dict(s3=s3, s2=s2, ta=ta, comp=comp, loss=loss, dLdC=dLdC)
+
+
+def fp64_backward(fw, params, n, deg, cam, pv, W, H, grad_mode=0):
+    gs2, _ = oracle.comp_bwd(fw["s2"], n, fw["ta"]["ids"], fw["ta"]["ranges"], W, H,
+                             fw["comp"]["T"], fw["comp"]["last"], fw["dLdC"],
+                             grad_mode=grad_mode)
+    gs3 = oracle.proj_bwd(fw["s3"], gs2, n, pv)
+    gp = oracle.act_bwd(params, fw["s3"], gs3, n, deg, cam)
+    return dict(gs2=gs2, gs3=gs3, gparams=gp)
+
+
+def fd_grad(scene, h=1e-6, pairs=None):
+    """Central finite difference of the FP64 loss w.r.t. every parameter."""
+    p0 = scene["params"]
+    g = np.zeros_like(p0)
+    args = (scene["n"], scene["deg"], scene["cam"], scene["pv"], scene["W"], scene["H"],
+            scene["target"])
+    for k in range(len(p0)):
+        p = p0.copy()
+        p[k] += h
+        lp = fp64_forward(p, *args, pairs=pairs)["loss"]
+        p[k] -= 2 * h
+        lm = fp64_forward(p, *args, pairs=pairs)["loss"]
+        g[k] = (lp - lm) / (2 * h)
+    return g
+
+
+def field_names(deg):
+    nb = oracle.bands(deg)
+    names = ["pos_x", "pos_y", "pos_z", "scale_x", "scale_y", "scale_z", "rot_w", "rot_x",
+             "rot_y", "rot_z", "sh_dc_r", "sh_dc_g", "sh_dc_b"]
+    names += [f"sh_rest_{c}[{b}]" for c in "rgb" for b in range(nb)]
+    names += ["logit_opacity"]
+    return names
+
+
+def close(a, b, atol=1e-5, rtol=1e-4):
+    """|a - b| <= atol + rtol * |b| elementwise; returns (ok, max_abs, max_rel)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    d = np.abs(a - b)
+    ok = bool(np.all(d <= atol + rtol * np.abs(b)))
+    rel = d / np.maximum(np.abs(b), 1e-30)
+    return ok, float(d.max(initial=0.0)), float(rel[d > atol].max(initial=0.0))

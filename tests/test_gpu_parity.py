@@ -1,0 +1,302 @@
+"""GPU parity of every stage of the hot path against the oracle.
+
+Each stage gets the same inputs on the GPU (through the C-ABI) and in the
+oracle. Integer outputs (tile counts, sorted keys, flatten ids, tile ranges)
+must be bit-exact with the fp32 transcription; float outputs must satisfy
+|gpu - fp64| <= 1e-5 + 1e-4 |fp64| elementwise.
+
+Image pixels where the FP64 oracle took a discrete decision (alpha >= 1/255,
+o*g <= 0.99, T <= 1e-4) within a relative margin of 5e-5 can legitimately
+differ (fp32 may decide the other way); they are excluded from the image
+comparison, counted and bounded. Gradients need no exclusion: a flipped
+decision moves a per-splat gradient by about alpha * T * dL/dC, far below the
+1e-5 absolute tolerance.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import diffsplat_amd as ds
+import oracle
+from gpu_stages import Stages, frustum_scene, with_sh
+from helpers import close, fp64_backward, fp64_forward, gradcheck_scene
+
+pytestmark = pytest.mark.gpu
+
+ATOL, RTOL = 1e-5, 1e-4
+MARGIN = 5e-5
+REPORT = os.environ.get("DS_PARITY_REPORT")
+
+
+def report(name, **kv):
+    print(f"[parity] {name}: " + " ".join(f"{k}={v}" for k, v in kv.items()), flush=True)
+    if REPORT:
+        with open(REPORT, "a") as f:
+            f.write(json.dumps({"test": name, **kv}) + "\n")
+
+
+def check_close(name, gpu, ref, mask=None, atol=ATOL, rtol=RTOL):
+    g = np.asarray(gpu, np.float64)
+    r = np.asarray(ref, np.float64)
+    if mask is not None:
+        g, r = g[mask], r[mask]
+    ok, mabs, mrel = close(g, r, atol, rtol)
+    scale = float(np.abs(r).max(initial=0.0))
+    report(name, max_abs=f"{mabs:.3e}", max_rel_above_atol=f"{mrel:.3e}", max_ref=f"{scale:.3e}",
+           n=int(g.size))
+    assert ok, f"{name}: max_abs={mabs:.3e} max_rel={mrel:.3e}"
+    return mabs
+
+
+SCENES = {
+    # name: (seed, n, W, H, ls_mean)
+    "c0": (0, 4096, 256, 256, -3.0),
+    "ragged": (5, 1500, 200, 120, -2.6),
+}
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2, 3])
+def test_activation_projection_forward(deg):
+    seed, n, W, H, ls = SCENES["c0"]
+    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    cam = np.array([0.3, -0.2, 0.5], np.float32)
+    params = with_sh(params, n, deg, seed) if deg else params
+    st = Stages(params, n, deg, W, H, view, proj, cam)
+    s3 = st.activ_fwd()
+    s2 = st.project_fwd()
+    s2f = st.activ_project_fwd()
+    r3 = oracle.act_fwd(params, n, deg, cam)
+    r2 = oracle.proj_fwd(r3, n, st.pv)
+    live = np.repeat(r2[2 * n:3 * n] < 1e30, 10)
+    check_close(f"activ_fwd_sh{deg}", s3, r3)
+    check_close(f"project_fwd_sh{deg}", s2, r2, live)
+    check_close(f"activ_project_fwd_sh{deg}", s2f, r2, live)
+
+
+@pytest.mark.parametrize("scene", ["c0", "ragged", "c1"])
+def test_tile_assign_sort_ranges_bitexact(scene):
+    if scene == "c1":
+        seed, n, W, H, ls = 1000, 60000, 1280, 720, -3.0
+    else:
+        seed, n, W, H, ls = SCENES[scene]
+    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    st = Stages(params, n, 0, W, H, view, proj, cam)
+    s2 = st.activ_project_fwd()
+    out = st.raster_fwd()
+    ref = oracle.tile_assign_sort(s2, n, W, H)
+    report(f"tile_assign_{scene}", n_isects=out["n_isects"], visible=out["visible"])
+    assert out["n_isects"] == ref["n_isects"] and out["wanted"] == ref["wanted"]
+    assert out["visible"] == ref["visible"]
+    assert np.array_equal(out["keys"], ref["keys"])
+    assert np.array_equal(out["flatten_ids"], ref["ids"])
+    assert np.array_equal(out["tile_ranges"], ref["ranges"])
+
+
+def test_isect_cap_drops_farthest_pairs():
+    seed, n, W, H, ls = SCENES["c0"]
+    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    cap = 5000
+    st = Stages(params, n, 0, W, H, view, proj, cam, isect_cap=cap)
+    s2 = st.activ_project_fwd()
+    out = st.raster_fwd()
+    ref = oracle.tile_assign_sort(s2, n, W, H, cap=cap)
+    assert ref["wanted"] > cap and out["n_isects"] == cap == ref["n_isects"]
+    assert out["wanted"] == ref["wanted"]
+    assert np.array_equal(out["keys"], ref["keys"])
+    assert np.array_equal(out["flatten_ids"], ref["ids"])
+    assert np.array_equal(out["tile_ranges"], ref["ranges"])
+
+
+def test_all_culled_renders_black():
+    n, W, H = 300, 64, 64
+    params, view, proj, cam = frustum_scene(3, n, W, H)
+    params = params.copy()
+    params[2 * n:3 * n] = 5.0  # behind the camera (camera looks down -z)
+    st = Stages(params, n, 0, W, H, view, proj, cam)
+    st.activ_project_fwd()
+    out = st.raster_fwd()
+    assert out["n_isects"] == 0 and out["visible"] == 0
+    assert np.all(out["rgb"] == 0.0) and np.all(out["T"] == 1.0)
+    gs2 = st.raster_bwd(np.ones(3 * W * H, np.float32))
+    assert np.all(gs2 == 0.0)
+
+
+@pytest.mark.parametrize("n,bits", [(1, 64), (1000, 64), (70001, 44), (300000, 20)])
+def test_sort_pairs_u64(n, bits):
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 2 ** 63, n, dtype=np.uint64) & np.uint64((1 << bits) - 1 if bits < 64
+                                                                     else 2 ** 64 - 1)
+    keys[: n // 3] = keys[0]  # many ties to check stability
+    vals = np.arange(n, dtype=np.uint32)
+    dk = ds.DeviceArray.from_numpy(keys)
+    dv = ds.DeviceArray.from_numpy(vals)
+    ko = ds.DeviceArray(8 * n)
+    vo = ds.DeviceArray(4 * n)
+    ds.check(ds.lib().ds_sort_pairs_u64(dk.ptr, dv.ptr, ko.ptr, vo.ptr, n, bits, None), "sort")
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(ko.download(np.uint64, n), keys[order])
+    assert np.array_equal(vo.download(np.uint32, n), vals[order])
+
+
+@pytest.mark.parametrize("ppt", [1, 2, 4])
+@pytest.mark.parametrize("scene", ["c0", "ragged"])
+def test_composite_forward(scene, ppt):
+    seed, n, W, H, ls = SCENES[scene]
+    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    st = Stages(params, n, 0, W, H, view, proj, cam, pix_per_thread=ppt)
+    s2 = st.activ_project_fwd()
+    out = st.raster_fwd()
+    ref = oracle.comp_fwd(s2, n, out["flatten_ids"], out["tile_ranges"], W, H)
+    ok_px = ref["margin"] > MARGIN
+    flips = int((~ok_px).sum())
+    report(f"composite_fwd_{scene}_ppt{ppt}", ambiguous_px=flips, E_fwd=ref["E"])
+    assert flips <= max(3, W * H // 500)
+    check_close(f"composite_rgb_{scene}_ppt{ppt}", out["rgb"], ref["rgb"], np.repeat(ok_px, 3))
+    check_close(f"composite_T_{scene}_ppt{ppt}", out["T"], ref["T"], ok_px)
+    assert np.array_equal(out["last_ids"][ok_px], ref["last"][ok_px])
+
+
+@pytest.mark.parametrize("ppt", [1, 2, 4])
+@pytest.mark.parametrize("grad_mode", [0, 1])
+@pytest.mark.parametrize("scene", ["c0", "ragged"])
+def test_composite_backward(scene, grad_mode, ppt):
+    seed, n, W, H, ls = SCENES[scene]
+    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    target = ds.target_image(seed, W, H, 32)
+    st = Stages(params, n, 0, W, H, view, proj, cam, grad_mode=grad_mode, pix_per_thread=ppt)
+    s2 = st.activ_project_fwd()
+    out = st.raster_fwd(target)
+    ref = oracle.comp_fwd(s2, n, out["flatten_ids"], out["tile_ranges"], W, H)
+    # fused MSE in the compositing epilogue
+    loss_ref, dldc_ref = oracle.mse(ref["rgb"], target, W * H)
+    report(f"fused_mse_{scene}", loss_gpu=out["loss"], loss_ref=loss_ref)
+    assert abs(out["loss"] - loss_ref) <= 1e-5 + 1e-4 * loss_ref
+    ok_px = np.repeat(ref["margin"] > MARGIN, 3)
+    check_close(f"fused_mse_grad_{scene}", out["grad_rgb"], dldc_ref, ok_px)
+    # backward from the GPU's own dL/dC, oracle from its own FP64 forward state
+    gs2 = st.raster_bwd(out["grad_rgb"])
+    rg, e_bwd = oracle.comp_bwd(s2, n, out["flatten_ids"], out["tile_ranges"], W, H, ref["T"],
+                                ref["last"], out["grad_rgb"], grad_mode=grad_mode)
+    report(f"composite_bwd_{scene}_mode{grad_mode}_ppt{ppt}", E_bwd=e_bwd)
+    mask = np.ones(10 * n, bool)
+    mask[2 * n:3 * n] = False  # pos_z receives no gradient from compositing
+    check_close(f"composite_bwd_{scene}_mode{grad_mode}_ppt{ppt}", gs2, rg, mask)
+
+
+@pytest.mark.parametrize("deg", [0, 3])
+def test_projection_activation_backward(deg):
+    seed, n, W, H, ls = SCENES["c0"]
+    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    cam = np.array([0.3, -0.2, 0.5], np.float32)
+    params = with_sh(params, n, deg, seed) if deg else params
+    st = Stages(params, n, deg, W, H, view, proj, cam)
+    s3 = st.activ_fwd()
+    st.project_fwd()
+    rng = np.random.default_rng(seed + 1)
+    gs2 = rng.normal(0, 1e-3, 10 * n).astype(np.float32)
+    st.upload_gs2(gs2)
+    gs3 = st.project_bwd()
+    gp = st.activ_bwd()
+    gpf = st.activ_project_bwd()
+    r3 = oracle.act_fwd(params, n, deg, cam)
+    rgs3 = oracle.proj_bwd(r3, gs2, n, st.pv)
+    # the activation backward gates colour on the saved fp32 output; feed both the GPU's
+    rgp = oracle.act_bwd(params, s3, gs3, n, deg, cam)
+    rgp_chain = oracle.act_bwd(params, r3, rgs3, n, deg, cam)
+    check_close(f"project_bwd_sh{deg}", gs3, rgs3)
+    check_close(f"activ_bwd_sh{deg}", gp, rgp)
+    check_close(f"activ_project_bwd_sh{deg}", gpf, rgp_chain)
+
+
+def test_mse_and_adam():
+    rng = np.random.default_rng(7)
+    npix = 123457
+    p = rng.uniform(0, 1, 3 * npix).astype(np.float32)
+    t = rng.uniform(0, 1, 3 * npix).astype(np.float32)
+    dp, dt = ds.DeviceArray.from_numpy(p), ds.DeviceArray.from_numpy(t)
+    dl, dg = ds.DeviceArray(4), ds.DeviceArray(4 * 3 * npix)
+    ds.check(ds.lib().ds_mse_fwd(dp.ptr, dt.ptr, npix, dl.ptr, None), "mse_fwd")
+    ds.check(ds.lib().ds_mse_bwd(dp.ptr, dt.ptr, npix, dg.ptr, None), "mse_bwd")
+    loss_ref, g_ref = oracle.mse(p, t, npix)
+    loss = float(dl.download(np.float32, 1)[0])
+    report("mse", loss=loss, loss_ref=loss_ref)
+    assert abs(loss - loss_ref) <= 1e-5 + 1e-4 * loss_ref
+    check_close("mse_grad", dg.download(np.float32, 3 * npix), g_ref)
+    # Adam: 3 groups with different lr, 5 steps, device step counter
+    sizes, lrs = [1000, 77777, 5], [1e-2, 5e-3, 6e-4]
+    groups, host = [], []
+    for sz, lr in zip(sizes, lrs):
+        arr = [rng.normal(0, 1, sz).astype(np.float32) for _ in range(2)]
+        dev = [ds.DeviceArray.from_numpy(a) for a in arr] + [ds.DeviceArray(4 * sz)
+                                                           for _ in range(2)]
+        dev[2].zero()
+        dev[3].zero()
+        host.append((arr, dev, lr))
+        groups.append(ds.AdamGroup(dev[0].ptr, dev[1].ptr, dev[2].ptr, dev[3].ptr, sz, lr))
+    tab = (ds.AdamGroup * len(groups))(*groups)
+    hp = ds.AdamHParams(0.9, 0.999, 1e-8)
+    step = ds.DeviceArray(4)
+    step.zero()
+    for _ in range(5):
+        ds.check(ds.lib().ds_adam_step_dev(tab, len(groups), hp, step.ptr, None), "adam")
+    assert int(step.download(np.int32, 1)[0]) == 5
+    for (p0, g), dev, lr in host:
+        rp, rm, rv = p0.astype(np.float64), np.zeros(len(p0)), np.zeros(len(p0))
+        for t_ in range(1, 6):
+            rp, rm, rv = oracle.adam(rp, g, rm, rv, lr, 0.9, 0.999, 1e-8, t_)
+        check_close(f"adam_params_{len(p0)}", dev[0].download(np.float32, len(p0)), rp)
+        check_close(f"adam_m_{len(p0)}", dev[2].download(np.float32, len(p0)), rm)
+        check_close(f"adam_v_{len(p0)}", dev[3].download(np.float32, len(p0)), rv)
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_fitter_step_matches_oracle_chain(fused):
+    """One full ImageFitter step: gradients and post-Adam parameters."""
+    seed, n, W, H, ls = SCENES["c0"]
+    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    target = ds.target_image(seed, W, H, 32)
+    lr = {"lr_pos": 1e-2, "lr_sh_dc": 1e-2, "lr_scale": 5e-3}
+    fit = ds.ImageFitter(W, H, n, view=view, proj=proj, cam_pos=cam, lr=lr, fused=fused)
+    fit.set_target(target)
+    fit.upload_params(params)
+    fit.step(1)
+    g_gpu = fit.grads()
+    p_gpu = fit.params()
+    pv = ds.mat_mul(proj, view).astype(np.float64)
+    fw = fp64_forward(params.astype(np.float64), n, 0, cam, pv, W, H, target)
+    bw = fp64_backward(fw, params.astype(np.float64), n, 0, cam, pv, W, H)
+    mask = np.ones(14 * n, bool)
+    check_close(f"fitter_grads_fused{int(fused)}", g_gpu, bw["gparams"])
+    # post-Adam: first step moves each parameter by about lr * sign(g); compare where
+    # |g| is well above the fp32 noise floor of the gradient
+    lr_f = np.concatenate([np.full(3 * n, 1e-2), np.full(3 * n, 5e-3), np.full(4 * n, 6e-4),
+                           np.full(3 * n, 1e-2), np.full(n, 6e-3)])
+    g64 = bw["gparams"]
+    m = 0.1 * g64
+    v = 0.001 * g64 * g64
+    p_ref = params.astype(np.float64) - lr_f * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+    strong = mask & (np.abs(g64) > 1e-3 * np.abs(g64).max())
+    check_close(f"fitter_adam_fused{int(fused)}", p_gpu, p_ref, strong)
+
+
+def test_gradcheck_scene_against_finite_differences():
+    """The reference's grad_check scene (tests/grad_check.cu:72-132) on the GPU:
+    analytic gradients vs FP64 central differences of the oracle's loss."""
+    from helpers import fd_grad
+
+    sc = gradcheck_scene(0)
+    n, W, H = sc["n"], sc["W"], sc["H"]
+    view = np.eye(4, dtype=np.float32).reshape(-1)
+    proj = sc["pv"].astype(np.float32)  # identity view: PV = P
+    fit = ds.ImageFitter(W, H, n, view=view, proj=proj, cam_pos=sc["cam"])
+    fit.set_target(sc["target"].astype(np.float32))
+    fit.upload_params(sc["params"].astype(np.float32))
+    fit.step(1)
+    g_gpu = fit.grads()
+    fw = fp64_forward(sc["params"], n, 0, sc["cam"], sc["pv"], W, H, sc["target"])
+    fd = fd_grad(sc, pairs=fw["ta"])
+    check_close("gradcheck_scene_vs_fd", g_gpu, fd, atol=1e-5, rtol=1e-3)
