@@ -106,14 +106,14 @@ _SIGS = {
     "ds_event_elapsed_ms": (C.c_int, [_P, _P, _F]),
     "ds_activ_fwd": (C.c_int, [C.POINTER(Gauss3D), C.POINTER(Splat3D), C.c_int, _F, _P]),
     "ds_activ_bwd": (C.c_int, [C.POINTER(Gauss3D), C.POINTER(Splat3D), C.POINTER(Splat3D),
-                               C.POINTER(Gauss3D), C.c_int, _F, _P]),
+                               C.POINTER(Gauss3D), C.c_int, _F, C.c_int, _P]),
     "ds_project_fwd": (C.c_int, [C.POINTER(Splat3D), C.POINTER(Splat2D), _F, _P]),
     "ds_project_bwd": (C.c_int, [C.POINTER(Splat3D), C.POINTER(Splat2D), C.POINTER(Splat3D),
-                                 _F, _P]),
+                                 _F, C.c_int, _P]),
     "ds_activ_project_fwd": (C.c_int, [C.POINTER(Gauss3D), C.POINTER(Splat2D), C.c_int, _F,
                                        _F, _P]),
     "ds_activ_project_bwd": (C.c_int, [C.POINTER(Gauss3D), C.POINTER(Splat2D),
-                                       C.POINTER(Gauss3D), C.c_int, _F, _F, _P]),
+                                       C.POINTER(Gauss3D), C.c_int, _F, _F, C.c_int, _P]),
     "ds_raster_create": (C.c_int, [C.POINTER(RasterConfig), C.POINTER(C.c_void_p)]),
     "ds_raster_destroy": (C.c_int, [_P]),
     "ds_raster_resize": (C.c_int, [_P, C.c_int, C.c_int]),

@@ -205,6 +205,12 @@ static ds_status check_gauss(const ds_gauss3d *g, int deg, const char *what) {
     return DS_OK;
 }
 
+static ds_status check_mode(int m) {
+    DS_REQUIRE(m == DS_GRAD_EXACT || m == DS_GRAD_REFERENCE,
+               "grad_mode must be DS_GRAD_EXACT or DS_GRAD_REFERENCE");
+    return DS_OK;
+}
+
 static Cam3 cam_of(const float *c) { return c ? Cam3{c[0], c[1], c[2]} : Cam3{0.f, 0.f, 0.f}; }
 
 static PV pv_of(const float *m) {
@@ -222,12 +228,15 @@ ds_status ds_activ_fwd(const ds_gauss3d *in, const ds_splat3d *out, int deg,
 }
 
 ds_status ds_activ_bwd(const ds_gauss3d *in, const ds_splat3d *out, const ds_splat3d *gout,
-                       const ds_gauss3d *gin, int deg, const float cam_pos[3], ds_stream_t s) {
+                       const ds_gauss3d *gin, int deg, const float cam_pos[3], int grad_mode,
+                       ds_stream_t s) {
     clear_error();
     DS_TRY(check_gauss(in, deg, "ds_activ_bwd"));
     DS_TRY(check_gauss(gin, deg, "ds_activ_bwd grad_in"));
     DS_REQUIRE(out && gout, "ds_activ_bwd: NULL view");
-    return launch_activ_bwd(*in, *out, *gout, *gin, deg, cam_of(cam_pos), as_stream(s));
+    DS_TRY(check_mode(grad_mode));
+    return launch_activ_bwd(*in, *out, *gout, *gin, deg, cam_of(cam_pos), grad_mode,
+                            as_stream(s));
 }
 
 ds_status ds_project_fwd(const ds_splat3d *in, const ds_splat2d *out, const float pv[16],
@@ -238,10 +247,11 @@ ds_status ds_project_fwd(const ds_splat3d *in, const ds_splat2d *out, const floa
 }
 
 ds_status ds_project_bwd(const ds_splat3d *in, const ds_splat2d *gout, const ds_splat3d *gin,
-                         const float pv[16], ds_stream_t s) {
+                         const float pv[16], int grad_mode, ds_stream_t s) {
     clear_error();
     DS_REQUIRE(in && gout && gin && pv, "ds_project_bwd: NULL argument");
-    return launch_project_bwd(*in, *gout, *gin, pv_of(pv), as_stream(s));
+    DS_TRY(check_mode(grad_mode));
+    return launch_project_bwd(*in, *gout, *gin, pv_of(pv), grad_mode, as_stream(s));
 }
 
 ds_status ds_activ_project_fwd(const ds_gauss3d *in, const ds_splat2d *out, int deg,
@@ -254,13 +264,14 @@ ds_status ds_activ_project_fwd(const ds_gauss3d *in, const ds_splat2d *out, int 
 
 ds_status ds_activ_project_bwd(const ds_gauss3d *in, const ds_splat2d *gout,
                                const ds_gauss3d *gin, int deg, const float cam_pos[3],
-                               const float pv[16], ds_stream_t s) {
+                               const float pv[16], int grad_mode, ds_stream_t s) {
     clear_error();
+    DS_TRY(check_mode(grad_mode));
     DS_TRY(check_gauss(in, deg, "ds_activ_project_bwd"));
     DS_TRY(check_gauss(gin, deg, "ds_activ_project_bwd grad_in"));
     DS_REQUIRE(gout && pv, "ds_activ_project_bwd: NULL argument");
     return launch_activ_project_bwd(*in, *gout, *gin, deg, cam_of(cam_pos), pv_of(pv),
-                                    as_stream(s));
+                                    grad_mode, as_stream(s));
 }
 
 ds_status ds_mse_fwd(const float *pixels, const float *target, int64_t num_pixels,

@@ -99,16 +99,16 @@ struct Cam3 {
 ds_status launch_activ_fwd(const ds_gauss3d &in, const ds_splat3d &out, int deg, Cam3 cam,
                            hipStream_t s);
 ds_status launch_activ_bwd(const ds_gauss3d &in, const ds_splat3d &out, const ds_splat3d &gout,
-                           const ds_gauss3d &gin, int deg, Cam3 cam, hipStream_t s);
+                           const ds_gauss3d &gin, int deg, Cam3 cam, int mode, hipStream_t s);
 ds_status launch_project_fwd(const ds_splat3d &in, const ds_splat2d &out, const PV &pv,
                              hipStream_t s);
 ds_status launch_project_bwd(const ds_splat3d &in, const ds_splat2d &gout,
-                             const ds_splat3d &gin, const PV &pv, hipStream_t s);
+                             const ds_splat3d &gin, const PV &pv, int mode, hipStream_t s);
 ds_status launch_activ_project_fwd(const ds_gauss3d &in, const ds_splat2d &out, int deg,
                                    Cam3 cam, const PV &pv, hipStream_t s);
 ds_status launch_activ_project_bwd(const ds_gauss3d &in, const ds_splat2d &gout,
                                    const ds_gauss3d &gin, int deg, Cam3 cam, const PV &pv,
-                                   hipStream_t s);
+                                   int mode, hipStream_t s);
 
 // loss / optimiser (kernels/optim.hip)
 ds_status launch_mse_fwd(const float *p, const float *t, int64_t npix, float *loss,

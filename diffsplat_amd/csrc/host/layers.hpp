@@ -32,9 +32,12 @@ public:
     virtual void backward() = 0;
     virtual void zeroGrad() = 0;
     void setStream(hipStream_t s) { stream = s; }
+    // DS_GRAD_EXACT (true derivative) or DS_GRAD_REFERENCE (the reference's expressions).
+    void setGradMode(int m) { grad_mode = m; }
 
 protected:
     hipStream_t stream = nullptr;
+    int grad_mode = DS_GRAD_EXACT;
 };
 
 template <typename TIn, typename TOut, typename TGradIn, typename TGradOut>
@@ -79,7 +82,7 @@ public:
         ds_splat3d out = output.view();
         ds_splat3d go = grad_output->view();
         ds_gauss3d gi = grad_input.view();
-        check(ds_activ_bwd(&in, &out, &go, &gi, sh_degree, cam.data(), stream),
+        check(ds_activ_bwd(&in, &out, &go, &gi, sh_degree, cam.data(), grad_mode, stream),
               "GaussActivLayer::backward");
     }
 
@@ -107,7 +110,8 @@ public:
         ds_splat3d in = input->view();
         ds_splat2d go = grad_output->view();
         ds_splat3d gi = grad_input.view();
-        check(ds_project_bwd(&in, &go, &gi, pv, stream), "PerspProjectLayer::backward");
+        check(ds_project_bwd(&in, &go, &gi, pv, grad_mode, stream),
+              "PerspProjectLayer::backward");
     }
 
 private:
@@ -135,7 +139,7 @@ public:
         ds_gauss3d in = input->view();
         ds_splat2d go = grad_output->view();
         ds_gauss3d gi = grad_input.view();
-        check(ds_activ_project_bwd(&in, &go, &gi, sh_degree, cam.data(), pv, stream),
+        check(ds_activ_project_bwd(&in, &go, &gi, sh_degree, cam.data(), pv, grad_mode, stream),
               "GaussProjectLayer::backward");
     }
 

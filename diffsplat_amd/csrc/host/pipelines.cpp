@@ -67,7 +67,10 @@ void ImageFitter::initLayers() {
     const int n = opt.count;
     pipeline.clear();
     Layer *all[5] = {&atv_layer, &psp_layer, &gp_layer, &ras_layer, &mse_layer};
-    for (Layer *l : all) l->setStream(stream);
+    for (Layer *l : all) {
+        l->setStream(stream);
+        l->setGradMode(opt.grad_mode);
+    }
 
     ras_layer.allocate(opt.width, opt.height, n, opt.isect_cap, 0, opt.grad_mode);
     ras_layer.allocateGrad(n);

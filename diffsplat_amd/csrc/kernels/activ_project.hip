@@ -210,6 +210,107 @@ __device__ __forceinline__ Cov3 cov3_grad(const Proj &p, float a, float b, float
     return g;
 }
 
+// d Y_k / d(x, y, z) of the non-DC basis polynomials (x, y, z independent).
+template <int DEG>
+__device__ __forceinline__ void sh_basis_rest_grad(float3 v, float dY[15][3]) {
+    const float x = v.x, y = v.y, z = v.z;
+    auto set = [&](int k, float a, float b, float c) {
+        dY[k][0] = a;
+        dY[k][1] = b;
+        dY[k][2] = c;
+    };
+    if constexpr (DEG >= 1) {
+        set(0, 0.f, -SH_C1, 0.f);
+        set(1, 0.f, 0.f, SH_C1);
+        set(2, -SH_C1, 0.f, 0.f);
+    }
+    if constexpr (DEG >= 2) {
+        set(3, SH_C2_0 * y, SH_C2_0 * x, 0.f);
+        set(4, 0.f, SH_C2_1 * z, SH_C2_1 * y);
+        set(5, -2.f * SH_C2_2 * x, -2.f * SH_C2_2 * y, 4.f * SH_C2_2 * z);
+        set(6, SH_C2_3 * z, 0.f, SH_C2_3 * x);
+        set(7, 2.f * SH_C2_4 * x, -2.f * SH_C2_4 * y, 0.f);
+    }
+    if constexpr (DEG >= 3) {
+        const float xx = x * x, yy = y * y, zz = z * z;
+        set(8, 6.f * SH_C3_0 * x * y, SH_C3_0 * (3.f * xx - 3.f * yy), 0.f);
+        set(9, SH_C3_1 * y * z, SH_C3_1 * x * z, SH_C3_1 * x * y);
+        set(10, -2.f * SH_C3_2 * x * y, SH_C3_2 * (4.f * zz - xx - 3.f * yy),
+            8.f * SH_C3_2 * y * z);
+        set(11, -6.f * SH_C3_3 * x * z, -6.f * SH_C3_3 * y * z,
+            SH_C3_3 * (6.f * zz - 3.f * xx - 3.f * yy));
+        set(12, SH_C3_4 * (4.f * zz - 3.f * xx - yy), -2.f * SH_C3_4 * x * y,
+            8.f * SH_C3_4 * x * z);
+        set(13, 2.f * SH_C3_5 * x * z, -2.f * SH_C3_5 * y * z, SH_C3_5 * (xx - yy));
+        set(14, SH_C3_6 * (3.f * xx - 3.f * yy), -6.f * SH_C3_6 * x * y, 0.f);
+    }
+}
+
+// Position gradient through the SH view direction v = (cam - p)/|cam - p|,
+// added to g[3] (exact mode; the reference treats v as constant).
+template <int DEG>
+__device__ __forceinline__ void view_dir_pos_grad(const ds_gauss3d &in, int i, float px,
+                                                  float py, float pz, Cam3 cam, float dR,
+                                                  float dG, float dB, float g[3]) {
+    if constexpr (DEG > 0) {
+        constexpr int NB = sh_bands(DEG);
+        const int n = in.count;
+        const float dx = cam.x - px, dy = cam.y - py, dz = cam.z - pz;
+        const float inv = rsqrtf(fmaxf(dx * dx + dy * dy + dz * dz, 1e-12f));
+        const float3 v = make_float3(dx * inv, dy * inv, dz * inv);
+        float dY[15][3];
+        sh_basis_rest_grad<DEG>(v, dY);
+        float gv[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const float w = dR * in.sh_rest_r[k * n + i] + dG * in.sh_rest_g[k * n + i] +
+                            dB * in.sh_rest_b[k * n + i];
+            gv[0] += w * dY[k][0];
+            gv[1] += w * dY[k][1];
+            gv[2] += w * dY[k][2];
+        }
+        const float vg = v.x * gv[0] + v.y * gv[1] + v.z * gv[2];
+        g[0] -= (gv[0] - vg * v.x) * inv;
+        g[1] -= (gv[1] - vg * v.y) * inv;
+        g[2] -= (gv[2] - vg * v.z) * inv;
+    }
+}
+
+// Position gradient through J in Sigma2D = J Sigma3D J^T (exact mode; the
+// reference propagates position only through the projected mean).
+__device__ __forceinline__ void jacobian_pos_grad(const PV &pv, const Proj &p, const Cov3 &c,
+                                                  float a, float b, float cc, float g[3]) {
+    const float S[3][3] = {{c.xx, c.xy, c.xz}, {c.xy, c.yy, c.yz}, {c.xz, c.yz, c.zz}};
+    const float G2[2][2] = {{a, 0.5f * b}, {0.5f * b, cc}};
+    float JS[2][3], dJ[2][3];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            JS[r][k] = p.J[r][0] * S[0][k] + p.J[r][1] * S[1][k] + p.J[r][2] * S[2][k];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) dJ[r][k] = 2.f * (G2[r][0] * JS[0][k] + G2[r][1] * JS[1][k]);
+    const float *m = pv.m;
+    const float iw = 1.f / p.cw;
+    const float iw2 = iw * iw;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float acc = 0.f;
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int col = 0; col < 3; ++col) {
+                const float dJdp =
+                    (m[4 * col + r] * m[4 * k + 3] - m[4 * k + r] * m[4 * col + 3]) * iw2 -
+                    2.f * p.J[r][col] * m[4 * k + 3] * iw;
+                acc += dJ[r][col] * dJdp;
+            }
+        g[k] += acc;
+    }
+}
+
 // ============================================================ kernels
 
 template <int DEG>
@@ -236,14 +337,11 @@ __global__ __launch_bounds__(kBlock) void k_activ_fwd(ds_gauss3d in, ds_splat3d 
     out.opacity[i] = sigmoid(in.logit_opacity[i]);
 }
 
-template <int DEG>
+template <int DEG, int MODE>
 __global__ __launch_bounds__(kBlock) void k_activ_bwd(ds_gauss3d in, ds_splat3d out,
                                                       ds_splat3d go, ds_gauss3d gi, Cam3 cam) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= in.count) return;
-    gi.pos_x[i] = go.pos_x[i];
-    gi.pos_y[i] = go.pos_y[i];
-    gi.pos_z[i] = go.pos_z[i];
     RS o = make_rs(in.scale_x[i], in.scale_y[i], in.scale_z[i], in.rot_w[i], in.rot_x[i],
                    in.rot_y[i], in.rot_z[i]);
     Cov3 G{go.cov_xx[i], go.cov_xy[i], go.cov_xz[i], go.cov_yy[i], go.cov_yz[i], go.cov_zz[i]};
@@ -266,6 +364,11 @@ __global__ __launch_bounds__(kBlock) void k_activ_bwd(ds_gauss3d in, ds_splat3d 
         pz = in.pos_z[i];
     }
     sh_backward<DEG>(gi, i, px, py, pz, cam, dR, dG, dB);
+    float gp[3] = {go.pos_x[i], go.pos_y[i], go.pos_z[i]};
+    if constexpr (MODE == DS_GRAD_EXACT) view_dir_pos_grad<DEG>(in, i, px, py, pz, cam, dR, dG, dB, gp);
+    gi.pos_x[i] = gp[0];
+    gi.pos_y[i] = gp[1];
+    gi.pos_z[i] = gp[2];
     const float s = out.opacity[i];
     gi.logit_opacity[i] = go.opacity[i] * s * (1.f - s);
 }
@@ -293,6 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_project_fwd(ds_splat3d in, ds_splat2
     out.opacity[i] = in.opacity[i];
 }
 
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_project_bwd(ds_splat3d in, ds_splat2d go,
                                                         ds_splat3d gi, PV pv) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -314,9 +418,15 @@ __global__ __launch_bounds__(kBlock) void k_project_bwd(ds_splat3d in, ds_splat2
     gi.cov_yz[i] = g.yz;
     gi.cov_zz[i] = g.zz;
     const float gx = go.pos_x[i], gy = go.pos_y[i];
-    gi.pos_x[i] = gx * p.J[0][0] + gy * p.J[1][0];
-    gi.pos_y[i] = gx * p.J[0][1] + gy * p.J[1][1];
-    gi.pos_z[i] = gx * p.J[0][2] + gy * p.J[1][2];
+    float gp[3] = {gx * p.J[0][0] + gy * p.J[1][0], gx * p.J[0][1] + gy * p.J[1][1],
+                   gx * p.J[0][2] + gy * p.J[1][2]};
+    if constexpr (MODE == DS_GRAD_EXACT) {
+        Cov3 c{in.cov_xx[i], in.cov_xy[i], in.cov_xz[i], in.cov_yy[i], in.cov_yz[i], in.cov_zz[i]};
+        jacobian_pos_grad(pv, p, c, go.cov_xx[i], go.cov_xy[i], go.cov_yy[i], gp);
+    }
+    gi.pos_x[i] = gp[0];
+    gi.pos_y[i] = gp[1];
+    gi.pos_z[i] = gp[2];
     gi.color_r[i] = go.color_r[i];
     gi.color_g[i] = go.color_g[i];
     gi.color_b[i] = go.color_b[i];
@@ -352,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_activ_project_fwd(ds_gauss3d in, ds_
     out.opacity[i] = sigmoid(in.logit_opacity[i]);
 }
 
-template <int DEG>
+template <int DEG, int MODE>
 __global__ __launch_bounds__(kBlock) void k_activ_project_bwd(ds_gauss3d in, ds_splat2d go,
                                                               ds_gauss3d gi, Cam3 cam, PV pv) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -368,14 +478,15 @@ __global__ __launch_bounds__(kBlock) void k_activ_project_bwd(ds_gauss3d in, ds_
         return;
     }
     // projection backward
-    Cov3 G = cov3_grad(p, go.cov_xx[i], go.cov_xy[i], go.cov_yy[i]);
+    const float ga = go.cov_xx[i], gb = go.cov_xy[i], gc = go.cov_yy[i];
+    Cov3 G = cov3_grad(p, ga, gb, gc);
     const float gx = go.pos_x[i], gy = go.pos_y[i];
-    gi.pos_x[i] = gx * p.J[0][0] + gy * p.J[1][0];
-    gi.pos_y[i] = gx * p.J[0][1] + gy * p.J[1][1];
-    gi.pos_z[i] = gx * p.J[0][2] + gy * p.J[1][2];
+    float gp[3] = {gx * p.J[0][0] + gy * p.J[1][0], gx * p.J[0][1] + gy * p.J[1][1],
+                   gx * p.J[0][2] + gy * p.J[1][2]};
     // activation backward
     RS o = make_rs(in.scale_x[i], in.scale_y[i], in.scale_z[i], in.rot_w[i], in.rot_x[i],
                    in.rot_y[i], in.rot_z[i]);
+    if constexpr (MODE == DS_GRAD_EXACT) jacobian_pos_grad(pv, p, cov_of(o), ga, gb, gc, gp);
     RSGrad g = rs_backward(o, G);
     gi.scale_x[i] = g.ds[0];
     gi.scale_y[i] = g.ds[1];
@@ -390,6 +501,10 @@ __global__ __launch_bounds__(kBlock) void k_activ_project_bwd(ds_gauss3d in, ds_
     const float dG = (cg > 0.f && cg < 1.f) ? go.color_g[i] : 0.f;
     const float dB = (cb > 0.f && cb < 1.f) ? go.color_b[i] : 0.f;
     sh_backward<DEG>(gi, i, px, py, pz, cam, dR, dG, dB);
+    if constexpr (MODE == DS_GRAD_EXACT) view_dir_pos_grad<DEG>(in, i, px, py, pz, cam, dR, dG, dB, gp);
+    gi.pos_x[i] = gp[0];
+    gi.pos_y[i] = gp[1];
+    gi.pos_z[i] = gp[2];
     const float s = sigmoid(in.logit_opacity[i]);
     gi.logit_opacity[i] = go.opacity[i] * s * (1.f - s);
 }
@@ -406,6 +521,23 @@ inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
         default: hipLaunchKernelGGL(KERNEL<3>, __VA_ARGS__); break;                         \
     }
 
+#define DS_DISPATCH_DEG_MODE(deg, mode, KERNEL, ...)                                         \
+    if ((mode) == DS_GRAD_REFERENCE) {                                                      \
+        switch (deg) {                                                                      \
+            case 0: hipLaunchKernelGGL((KERNEL<0, DS_GRAD_REFERENCE>), __VA_ARGS__); break; \
+            case 1: hipLaunchKernelGGL((KERNEL<1, DS_GRAD_REFERENCE>), __VA_ARGS__); break; \
+            case 2: hipLaunchKernelGGL((KERNEL<2, DS_GRAD_REFERENCE>), __VA_ARGS__); break; \
+            default: hipLaunchKernelGGL((KERNEL<3, DS_GRAD_REFERENCE>), __VA_ARGS__); break;\
+        }                                                                                   \
+    } else {                                                                                \
+        switch (deg) {                                                                      \
+            case 0: hipLaunchKernelGGL((KERNEL<0, DS_GRAD_EXACT>), __VA_ARGS__); break;     \
+            case 1: hipLaunchKernelGGL((KERNEL<1, DS_GRAD_EXACT>), __VA_ARGS__); break;     \
+            case 2: hipLaunchKernelGGL((KERNEL<2, DS_GRAD_EXACT>), __VA_ARGS__); break;     \
+            default: hipLaunchKernelGGL((KERNEL<3, DS_GRAD_EXACT>), __VA_ARGS__); break;    \
+        }                                                                                   \
+    }
+
 ds_status launch_activ_fwd(const ds_gauss3d &in, const ds_splat3d &out, int deg, Cam3 cam,
                            hipStream_t s) {
     if (in.count <= 0) return DS_OK;
@@ -415,10 +547,10 @@ ds_status launch_activ_fwd(const ds_gauss3d &in, const ds_splat3d &out, int deg,
 }
 
 ds_status launch_activ_bwd(const ds_gauss3d &in, const ds_splat3d &out, const ds_splat3d &go,
-                           const ds_gauss3d &gi, int deg, Cam3 cam, hipStream_t s) {
+                           const ds_gauss3d &gi, int deg, Cam3 cam, int mode, hipStream_t s) {
     if (in.count <= 0) return DS_OK;
-    DS_DISPATCH_DEG(deg, k_activ_bwd, dim3(grid_for(in.count)), dim3(kBlock), 0, s, in, out, go,
-                    gi, cam);
+    DS_DISPATCH_DEG_MODE(deg, mode, k_activ_bwd, dim3(grid_for(in.count)), dim3(kBlock), 0, s,
+                         in, out, go, gi, cam);
     DS_CHECK_LAUNCH("activ_bwd");
     return DS_OK;
 }
@@ -432,10 +564,14 @@ ds_status launch_project_fwd(const ds_splat3d &in, const ds_splat2d &out, const 
 }
 
 ds_status launch_project_bwd(const ds_splat3d &in, const ds_splat2d &go, const ds_splat3d &gi,
-                             const PV &pv, hipStream_t s) {
+                             const PV &pv, int mode, hipStream_t s) {
     if (in.count <= 0) return DS_OK;
-    hipLaunchKernelGGL(k_project_bwd, dim3(grid_for(in.count)), dim3(kBlock), 0, s, in, go, gi,
-                       pv);
+    if (mode == DS_GRAD_REFERENCE)
+        hipLaunchKernelGGL(k_project_bwd<DS_GRAD_REFERENCE>, dim3(grid_for(in.count)),
+                           dim3(kBlock), 0, s, in, go, gi, pv);
+    else
+        hipLaunchKernelGGL(k_project_bwd<DS_GRAD_EXACT>, dim3(grid_for(in.count)), dim3(kBlock),
+                           0, s, in, go, gi, pv);
     DS_CHECK_LAUNCH("project_bwd");
     return DS_OK;
 }
@@ -451,10 +587,10 @@ ds_status launch_activ_project_fwd(const ds_gauss3d &in, const ds_splat2d &out, 
 
 ds_status launch_activ_project_bwd(const ds_gauss3d &in, const ds_splat2d &go,
                                    const ds_gauss3d &gi, int deg, Cam3 cam, const PV &pv,
-                                   hipStream_t s) {
+                                   int mode, hipStream_t s) {
     if (in.count <= 0) return DS_OK;
-    DS_DISPATCH_DEG(deg, k_activ_project_bwd, dim3(grid_for(in.count)), dim3(kBlock), 0, s, in,
-                    go, gi, cam, pv);
+    DS_DISPATCH_DEG_MODE(deg, mode, k_activ_project_bwd, dim3(grid_for(in.count)), dim3(kBlock),
+                         0, s, in, go, gi, cam, pv);
     DS_CHECK_LAUNCH("activ_project_bwd");
     return DS_OK;
 }

@@ -31,6 +31,22 @@ extern "C" {
 /* hipStream_t, passed as an opaque pointer so that this header needs no HIP headers. */
 typedef void *ds_stream_t;
 
+/* Gradient convention of every backward entry point.
+ * DS_GRAD_EXACT: the true derivative of the forward functions (agrees with
+ *   finite differences at every field, SH degree and camera).
+ * DS_GRAD_REFERENCE: the reference's expressions, which differ from the true
+ *   derivative in three places:
+ *   - compositing dL/dalpha = T_before * (c - colour behind) with the colour
+ *     behind not divided by T after the splat
+ *     (src/layers/tile_rasterize_layer.cu:491-495);
+ *   - the position gradient of the projection omits the dependence of the
+ *     Jacobian J in J Sigma J^T on position
+ *     (src/layers/persp_project_layer.cu:254-260);
+ *   - the SH view direction is treated as constant, so position receives no
+ *     gradient through the colour (src/layers/gauss_activ_layer.cu:182-184). */
+#define DS_GRAD_EXACT 0
+#define DS_GRAD_REFERENCE 1
+
 typedef enum ds_status {
     DS_OK = 0,
     DS_ERR_INVALID_ARG = 1,   /* bad pointer, size or configuration            */
@@ -131,7 +147,7 @@ ds_status ds_activ_fwd(const ds_gauss3d *in, const ds_splat3d *out, int sh_degre
  * convention; grad_in receives dL/dparams (overwritten, not accumulated). */
 ds_status ds_activ_bwd(const ds_gauss3d *in, const ds_splat3d *out, const ds_splat3d *grad_out,
                        const ds_gauss3d *grad_in, int sh_degree, const float cam_pos[3],
-                       ds_stream_t s);
+                       int grad_mode, ds_stream_t s);
 
 /* ------------------------------------------------------------------ */
 /* Perspective projection world -> NDC                                 */
@@ -146,7 +162,8 @@ ds_status ds_project_fwd(const ds_splat3d *in, const ds_splat2d *out, const floa
 
 /* PerspProjectLayer::backward (src/layers/persp_project_layer.cu:173-267, 294-317). */
 ds_status ds_project_bwd(const ds_splat3d *in, const ds_splat2d *grad_out,
-                         const ds_splat3d *grad_in, const float pv[16], ds_stream_t s);
+                         const ds_splat3d *grad_in, const float pv[16], int grad_mode,
+                         ds_stream_t s);
 
 /* Fused activation + projection (no Splat3D round trip through HBM).
  * Same results as ds_activ_fwd followed by ds_project_fwd. */
@@ -158,7 +175,8 @@ ds_status ds_activ_project_fwd(const ds_gauss3d *in, const ds_splat2d *out, int 
  * recomputed from the parameters instead of read from a saved Splat3D). */
 ds_status ds_activ_project_bwd(const ds_gauss3d *in, const ds_splat2d *grad_out,
                                const ds_gauss3d *grad_in, int sh_degree,
-                               const float cam_pos[3], const float pv[16], ds_stream_t s);
+                               const float cam_pos[3], const float pv[16], int grad_mode,
+                               ds_stream_t s);
 
 /* ------------------------------------------------------------------ */
 /* Tile rasterizer                                                     */
@@ -171,17 +189,8 @@ typedef struct ds_raster_config {
     int32_t count_cap;         /* maximum splat count per call                             */
     int64_t isect_cap;         /* maximum (tile, splat) pairs; 0 = 1<<25 as the reference  */
     int32_t keep_keys;         /* 1: also materialise the sorted 64-bit (tile|depth) keys  */
-    int32_t grad_mode;         /* DS_GRAD_EXACT (0) or DS_GRAD_REFERENCE (1), see below    */
+    int32_t grad_mode;         /* DS_GRAD_EXACT (0) or DS_GRAD_REFERENCE (1)               */
 } ds_raster_config;
-
-/* dL/dalpha in the compositing backward.
- * DS_GRAD_EXACT: T_before * c - (colour behind) / (1 - alpha), the true
- *   derivative of front-to-back compositing (agrees with finite differences).
- * DS_GRAD_REFERENCE: T_before * (c - colour behind), the expression of
- *   src/layers/tile_rasterize_layer.cu:491-495, where "colour behind" is the
- *   absolute sum of c_i alpha_i T_i and is not divided by T after the splat. */
-#define DS_GRAD_EXACT 0
-#define DS_GRAD_REFERENCE 1
 
 /* Device pointers into the rasterizer's own buffers, valid until the next
  * ds_raster_fwd / resize / destroy on the same object. */

@@ -34,9 +34,10 @@ def lib():
         h = C.CDLL(LIB_PATH)
         for suf, T in (("f32", C.c_float), ("f64", C.c_double)):
             getattr(h, f"or_act_fwd_{suf}").argtypes = [_P, C.c_int, C.c_int, _P, _P]
-            getattr(h, f"or_act_bwd_{suf}").argtypes = [_P, _P, _P, C.c_int, C.c_int, _P, _P]
+            getattr(h, f"or_act_bwd_{suf}").argtypes = [_P, _P, _P, C.c_int, C.c_int, _P, _P,
+                                                        C.c_int]
             getattr(h, f"or_proj_fwd_{suf}").argtypes = [_P, C.c_int, _P, _P]
-            getattr(h, f"or_proj_bwd_{suf}").argtypes = [_P, _P, C.c_int, _P, _P]
+            getattr(h, f"or_proj_bwd_{suf}").argtypes = [_P, _P, C.c_int, _P, _P, C.c_int]
             getattr(h, f"or_comp_fwd_{suf}").argtypes = [_P, C.c_int, _P, _P, C.c_int, C.c_int,
                                                          _P, _P, _P, _P, _P]
             getattr(h, f"or_comp_bwd_{suf}").argtypes = [_P, C.c_int, _P, _P, C.c_int, C.c_int,
@@ -89,11 +90,12 @@ def act_fwd(params, n, deg, cam, dtype=np.float64):
     return s3
 
 
-def act_bwd(params, s3, gs3, n, deg, cam, dtype=np.float64):
+def act_bwd(params, s3, gs3, n, deg, cam, dtype=np.float64, grad_mode=0):
     p, s, g, c = (_a(x, dtype) for x in (params, s3, gs3, cam))
     out = np.zeros((14 + 3 * bands(deg)) * n, dtype)
     getattr(lib(), f"or_act_bwd_{_suffix(dtype)}")(p.ctypes.data, s.ctypes.data, g.ctypes.data,
-                                                  n, deg, c.ctypes.data, out.ctypes.data)
+                                                  n, deg, c.ctypes.data, out.ctypes.data,
+                                                  int(grad_mode))
     return out
 
 
@@ -105,11 +107,11 @@ def proj_fwd(s3, n, pv, dtype=np.float64, init=None):
     return s2
 
 
-def proj_bwd(s3, gs2, n, pv, dtype=np.float64):
+def proj_bwd(s3, gs2, n, pv, dtype=np.float64, grad_mode=0):
     s, g, m = _a(s3, dtype), _a(gs2, dtype), _a(pv, dtype)
     out = np.zeros(13 * n, dtype)
     getattr(lib(), f"or_proj_bwd_{_suffix(dtype)}")(s.ctypes.data, g.ctypes.data, n,
-                                                   m.ctypes.data, out.ctypes.data)
+                                                   m.ctypes.data, out.ctypes.data, int(grad_mode))
     return out
 
 

@@ -112,6 +112,36 @@ void sh_rest_basis(int deg, T vx, T vy, T vz, T *Y) {
     }
 }
 
+// d Y_k / d(x, y, z) of the 15 non-DC basis polynomials, with x, y, z
+// treated as independent (the caller projects onto the unit sphere).
+template <typename T>
+void sh_rest_basis_grad(int deg, T x, T y, T z, T dY[15][3]) {
+    using C = K<T>;
+    auto set = [&](int k, T a, T b, T c) { dY[k][0] = a; dY[k][1] = b; dY[k][2] = c; };
+    if (deg >= 1) {
+        set(0, 0, -C::C1, 0);
+        set(1, 0, 0, C::C1);
+        set(2, -C::C1, 0, 0);
+    }
+    if (deg >= 2) {
+        set(3, C::C2_0 * y, C::C2_0 * x, 0);
+        set(4, 0, C::C2_1 * z, C::C2_1 * y);
+        set(5, -2 * C::C2_2 * x, -2 * C::C2_2 * y, 4 * C::C2_2 * z);
+        set(6, C::C2_3 * z, 0, C::C2_3 * x);
+        set(7, 2 * C::C2_4 * x, -2 * C::C2_4 * y, 0);
+    }
+    if (deg >= 3) {
+        const T xx = x * x, yy = y * y, zz = z * z;
+        set(8, 6 * C::C3_0 * x * y, C::C3_0 * (3 * xx - 3 * yy), 0);
+        set(9, C::C3_1 * y * z, C::C3_1 * x * z, C::C3_1 * x * y);
+        set(10, -2 * C::C3_2 * x * y, C::C3_2 * (4 * zz - xx - 3 * yy), 8 * C::C3_2 * y * z);
+        set(11, -6 * C::C3_3 * x * z, -6 * C::C3_3 * y * z, C::C3_3 * (6 * zz - 3 * xx - 3 * yy));
+        set(12, C::C3_4 * (4 * zz - 3 * xx - yy), -2 * C::C3_4 * x * y, 8 * C::C3_4 * x * z);
+        set(13, 2 * C::C3_5 * x * z, -2 * C::C3_5 * y * z, C::C3_5 * (xx - yy));
+        set(14, C::C3_6 * (3 * xx - 3 * yy), -6 * C::C3_6 * x * y, 0);
+    }
+}
+
 template <typename T>
 struct RSq {
     T q[4], inv;  // unit quaternion w,x,y,z and 1/|q_raw|
@@ -180,8 +210,10 @@ void act_fwd(const T *params, int n, int deg, const T cam[3], T *s3) {
 }
 
 template <typename T>
+// grad_mode 0 adds the position gradient through the SH view direction,
+// which the reference treats as constant (gauss_activ_layer.cu:201-203).
 void act_bwd(const T *params, const T *s3, const T *gs3, int n, int deg, const T cam[3],
-             T *gparams) {
+             T *gparams, int grad_mode) {
     const int nb = bands_of(deg);
     PSlab<T> p(const_cast<T *>(params), n, nb);
     PSlab<T> g(gparams, n, nb);
@@ -230,6 +262,19 @@ void act_bwd(const T *params, const T *s3, const T *gs3, int n, int deg, const T
             T *rest[3] = {g.rr, g.rg, g.rb};
             for (int c = 0; c < 3; ++c)
                 for (int k = 0; k < nb; ++k) rest[c][(size_t)k * n + i] = gc[c] * Y[k];
+            if (grad_mode == 0) {
+                const T v[3] = {ddx * inv, ddy * inv, ddz * inv};
+                T dY[15][3];
+                sh_rest_basis_grad<T>(deg, v[0], v[1], v[2], dY);
+                const T *coef[3] = {p.rr, p.rg, p.rb};
+                T gv[3] = {0, 0, 0};
+                for (int c = 0; c < 3; ++c)
+                    for (int k = 0; k < nb; ++k)
+                        for (int a = 0; a < 3; ++a)
+                            gv[a] += gc[c] * coef[c][(size_t)k * n + i] * dY[k][a];
+                const T vg = v[0] * gv[0] + v[1] * gv[1] + v[2] * gv[2];
+                for (int a = 0; a < 3; ++a) g.f[a][i] -= (gv[a] - vg * v[a]) * inv;
+            }
         }
         T s = s3[12 * (size_t)n + i];
         g.logit[i] = G3(12) * s * (1 - s);
@@ -278,8 +323,10 @@ void proj_fwd(const T *s3, int n, const T pv[16], T *s2) {
     }
 }
 
+// grad_mode 0 adds the position gradient through the Jacobian of the
+// projected covariance, which the reference omits (persp_project_layer.cu:254-260).
 template <typename T>
-void proj_bwd(const T *s3, const T *gs2, int n, const T pv[16], T *gs3) {
+void proj_bwd(const T *s3, const T *gs2, int n, const T pv[16], T *gs3, int grad_mode) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
         auto G = [&](int f) { return gs2[(size_t)f * n + i]; };
@@ -303,6 +350,31 @@ void proj_bwd(const T *s3, const T *gs2, int n, const T pv[16], T *gs3) {
         o[0 * (size_t)n + i] = gx * J[0][0] + gy * J[1][0];
         o[1 * (size_t)n + i] = gx * J[0][1] + gy * J[1][1];
         o[2 * (size_t)n + i] = gx * J[0][2] + gy * J[1][2];
+        if (grad_mode == 0) {
+            auto S = [&](int f) { return s3[(size_t)f * n + i]; };
+            const T C3[3][3] = {{S(3), S(4), S(5)}, {S(4), S(6), S(7)}, {S(5), S(7), S(8)}};
+            const T G2[2][2] = {{a, h}, {h, c}};
+            T JS[2][3], dJ[2][3];
+            for (int r = 0; r < 2; ++r)
+                for (int k = 0; k < 3; ++k)
+                    JS[r][k] = J[r][0] * C3[0][k] + J[r][1] * C3[1][k] + J[r][2] * C3[2][k];
+            for (int r = 0; r < 2; ++r)
+                for (int k = 0; k < 3; ++k) dJ[r][k] = 2 * (G2[r][0] * JS[0][k] + G2[r][1] * JS[1][k]);
+            const T *m = pv;
+            const T X = s3[i], Y = s3[(size_t)n + i], Z = s3[2 * (size_t)n + i];
+            const T cw = m[3] * X + m[7] * Y + m[11] * Z + m[15];
+            const T iw = 1 / cw, iw2 = iw * iw;
+            for (int k = 0; k < 3; ++k) {
+                T acc = 0;
+                for (int r = 0; r < 2; ++r)
+                    for (int col = 0; col < 3; ++col) {
+                        const T dJdp = (m[4 * col + r] * m[4 * k + 3] - m[4 * k + r] * m[4 * col + 3]) * iw2 -
+                                       2 * J[r][col] * m[4 * k + 3] * iw;
+                        acc += dJ[r][col] * dJdp;
+                    }
+                o[(size_t)k * n + i] += acc;
+            }
+        }
         for (int k = 0; k < 4; ++k) o[(size_t)(9 + k) * n + i] = G(6 + k);
     }
 }
@@ -663,14 +735,15 @@ extern "C" {
         act_fwd<T>(params, n, deg, cam, s3);                                                  \
     }                                                                                         \
     void or_act_bwd_##SUF(const T *params, const T *s3, const T *gs3, int n, int deg,         \
-                          const T *cam, T *gparams) {                                         \
-        act_bwd<T>(params, s3, gs3, n, deg, cam, gparams);                                    \
+                          const T *cam, T *gparams, int grad_mode) {                          \
+        act_bwd<T>(params, s3, gs3, n, deg, cam, gparams, grad_mode);                         \
     }                                                                                         \
     void or_proj_fwd_##SUF(const T *s3, int n, const T *pv, T *s2) {                          \
         proj_fwd<T>(s3, n, pv, s2);                                                           \
     }                                                                                         \
-    void or_proj_bwd_##SUF(const T *s3, const T *gs2, int n, const T *pv, T *gs3) {           \
-        proj_bwd<T>(s3, gs2, n, pv, gs3);                                                     \
+    void or_proj_bwd_##SUF(const T *s3, const T *gs2, int n, const T *pv, T *gs3,             \
+                           int grad_mode) {                                                   \
+        proj_bwd<T>(s3, gs2, n, pv, gs3, grad_mode);                                          \
     }                                                                                         \
     void or_comp_fwd_##SUF(const T *s2, int n, const uint32_t *flat, const int32_t *ranges,   \
                            int W, int H, T *rgb, T *Tout, int32_t *last, double *margin,      \
@@ -781,8 +854,9 @@ float or_fit_step(void *h) {
     std::fill(f.gs2.begin(), f.gs2.end(), 0.f);
     comp_bwd<float>(f.s2.data(), n, f.ids.data(), f.ranges.data(), f.W, f.H, f.Tf.data(),
                     f.last.data(), f.dLdC.data(), f.gs2.data(), nullptr, f.grad_mode);
-    proj_bwd<float>(f.s3.data(), f.gs2.data(), n, f.pv, f.gs3.data());
-    act_bwd<float>(f.params.data(), f.s3.data(), f.gs3.data(), n, f.deg, f.cam, f.grads.data());
+    proj_bwd<float>(f.s3.data(), f.gs2.data(), n, f.pv, f.gs3.data(), f.grad_mode);
+    act_bwd<float>(f.params.data(), f.s3.data(), f.gs3.data(), n, f.deg, f.cam, f.grads.data(),
+                   f.grad_mode);
     ++f.step_no;
     // groups: pos(3) lr0, scale(3) lr1, rot(4) lr2, dc(3) lr3, rest(3*nb) lr4, logit lr5
     const int64_t N = n;

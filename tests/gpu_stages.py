@@ -15,6 +15,7 @@ class Stages:
     def __init__(self, params, n, deg, W, H, view, proj, cam, grad_mode=0, isect_cap=0,
                  pix_per_thread=4):
         self.n, self.deg, self.W, self.H = n, deg, W, H
+        self.grad_mode = grad_mode
         self.cam = np.ascontiguousarray(cam, np.float32)
         self.pv = ds.mat_mul(proj, view)
         self.params = np.ascontiguousarray(params, np.float32)
@@ -83,18 +84,20 @@ class Stages:
         self.d_gs2.upload(np.ascontiguousarray(gs2, np.float32))
 
     def project_bwd(self):
-        ds.check(ds.lib().ds_project_bwd(self.s3, self.gs2, self.gs3, ds.fptr(self.pv), None),
+        ds.check(ds.lib().ds_project_bwd(self.s3, self.gs2, self.gs3, ds.fptr(self.pv),
+                                         self.grad_mode, None),
                  "ds_project_bwd")
         return self.d_gs3.download(np.float32, 13 * self.n)
 
     def activ_bwd(self):
         ds.check(ds.lib().ds_activ_bwd(self.g, self.s3, self.gs3, self.gg, self.deg,
-                                       ds.fptr(self.cam), None), "ds_activ_bwd")
+                                       ds.fptr(self.cam), self.grad_mode, None), "ds_activ_bwd")
         return self.d_gp.download(np.float32, ds.slab_floats(self.n, self.deg))
 
     def activ_project_bwd(self):
         ds.check(ds.lib().ds_activ_project_bwd(self.g, self.gs2, self.gg, self.deg,
-                                               ds.fptr(self.cam), ds.fptr(self.pv), None),
+                                               ds.fptr(self.cam), ds.fptr(self.pv),
+                                               self.grad_mode, None),
                  "ds_activ_project_bwd")
         return self.d_gp.download(np.float32, ds.slab_floats(self.n, self.deg))
 

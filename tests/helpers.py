@@ -52,6 +52,38 @@ def gradcheck_scene(deg: int = 0, seed: int = 0):
     return dict(params=p, n=n, W=W, H=H, deg=deg, target=target, pv=pv, cam=cam)
 
 
+def random_scene(seed: int = 0, deg: int = 0, n: int = 10, W: int = 40, H: int = 32):
+    """Small scene with general rotations and anisotropic scales under a
+    perspective camera at (0.2, -0.1, 3) looking at the origin."""
+    rng = np.random.default_rng(seed)
+    nb = oracle.bands(deg)
+    p = np.zeros((14 + 3 * nb) * n, np.float64)
+    P = p.reshape(-1, n)
+    P[0:2] = rng.uniform(-0.5, 0.5, (2, n))
+    P[2] = rng.uniform(-0.4, 0.4, n)
+    P[3:6] = rng.normal(-2.2, 0.3, (3, n))
+    P[6:10] = rng.normal(0, 1, (4, n))
+    P[10:13] = rng.normal(0, 0.6, (3, n))
+    if nb:
+        p[13 * n:13 * n + 3 * nb * n] = rng.normal(0, 0.2, 3 * nb * n)
+    P[-1] = rng.normal(0.5, 1.0, n)
+    eye = np.array([0.2, -0.1, 3.0])
+    f = -eye / np.linalg.norm(eye)
+    s = np.cross(f, [0, 1, 0])
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    V = np.eye(4)
+    V[0, :3], V[1, :3], V[2, :3] = s, u, -f
+    V[:3, 3] = [-s @ eye, -u @ eye, f @ eye]
+    t = np.tan(0.5 * 0.8)
+    Pm = np.zeros((4, 4))
+    Pm[0, 0], Pm[1, 1] = 1 / (t * W / H), 1 / t
+    Pm[2, 2], Pm[2, 3], Pm[3, 2] = -(20 + 0.1) / (20 - 0.1), -2 * 20 * 0.1 / (20 - 0.1), -1
+    pv = (Pm @ V).T.reshape(-1)
+    target = rng.uniform(0, 1, W * H * 3)
+    return dict(params=p, n=n, W=W, H=H, deg=deg, target=target, pv=pv, cam=eye)
+
+
 def fp64_forward(params, n, deg, cam, pv, W, H, target, pairs=None):
     """FP64 forward. Tile decisions come from the fp32 transcription of the
     (rounded) Splat2D unless `pairs` (a tile_assign_sort result) is given."""
@@ -67,8 +99,8 @@ def fp64_backward(fw, params, n, deg, cam, pv, W, H, grad_mode=0):
     gs2, _ = oracle.comp_bwd(fw["s2"], n, fw["ta"]["ids"], fw["ta"]["ranges"], W, H,
                              fw["comp"]["T"], fw["comp"]["last"], fw["dLdC"],
                              grad_mode=grad_mode)
-    gs3 = oracle.proj_bwd(fw["s3"], gs2, n, pv)
-    gp = oracle.act_bwd(params, fw["s3"], gs3, n, deg, cam)
+    gs3 = oracle.proj_bwd(fw["s3"], gs2, n, pv, grad_mode=grad_mode)
+    gp = oracle.act_bwd(params, fw["s3"], gs3, n, deg, cam, grad_mode=grad_mode)
     return dict(gs2=gs2, gs3=gs3, gparams=gp)
 
 

@@ -187,13 +187,14 @@ def test_composite_backward(scene, grad_mode, ppt):
     check_close(f"composite_bwd_{scene}_mode{grad_mode}_ppt{ppt}", gs2, rg, mask)
 
 
+@pytest.mark.parametrize("grad_mode", [0, 1])
 @pytest.mark.parametrize("deg", [0, 3])
-def test_projection_activation_backward(deg):
+def test_projection_activation_backward(deg, grad_mode):
     seed, n, W, H, ls = SCENES["c0"]
     params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
     cam = np.array([0.3, -0.2, 0.5], np.float32)
     params = with_sh(params, n, deg, seed) if deg else params
-    st = Stages(params, n, deg, W, H, view, proj, cam)
+    st = Stages(params, n, deg, W, H, view, proj, cam, grad_mode=grad_mode)
     s3 = st.activ_fwd()
     st.project_fwd()
     rng = np.random.default_rng(seed + 1)
@@ -203,13 +204,14 @@ def test_projection_activation_backward(deg):
     gp = st.activ_bwd()
     gpf = st.activ_project_bwd()
     r3 = oracle.act_fwd(params, n, deg, cam)
-    rgs3 = oracle.proj_bwd(r3, gs2, n, st.pv)
+    rgs3 = oracle.proj_bwd(r3, gs2, n, st.pv, grad_mode=grad_mode)
     # the activation backward gates colour on the saved fp32 output; feed both the GPU's
-    rgp = oracle.act_bwd(params, s3, gs3, n, deg, cam)
-    rgp_chain = oracle.act_bwd(params, r3, rgs3, n, deg, cam)
-    check_close(f"project_bwd_sh{deg}", gs3, rgs3)
-    check_close(f"activ_bwd_sh{deg}", gp, rgp)
-    check_close(f"activ_project_bwd_sh{deg}", gpf, rgp_chain)
+    rgp = oracle.act_bwd(params, s3, gs3, n, deg, cam, grad_mode=grad_mode)
+    rgp_chain = oracle.act_bwd(params, r3, rgs3, n, deg, cam, grad_mode=grad_mode)
+    tag = f"sh{deg}_mode{grad_mode}"
+    check_close(f"project_bwd_{tag}", gs3, rgs3)
+    check_close(f"activ_bwd_{tag}", gp, rgp)
+    check_close(f"activ_project_bwd_{tag}", gpf, rgp_chain)
 
 
 def test_mse_and_adam():
@@ -253,14 +255,15 @@ def test_mse_and_adam():
         check_close(f"adam_v_{len(p0)}", dev[3].download(np.float32, len(p0)), rv)
 
 
-@pytest.mark.parametrize("fused", [True, False])
-def test_fitter_step_matches_oracle_chain(fused):
+@pytest.mark.parametrize("fused,grad_mode", [(True, 0), (False, 0), (True, 1), (False, 1)])
+def test_fitter_step_matches_oracle_chain(fused, grad_mode):
     """One full ImageFitter step: gradients and post-Adam parameters."""
     seed, n, W, H, ls = SCENES["c0"]
     params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
     target = ds.target_image(seed, W, H, 32)
     lr = {"lr_pos": 1e-2, "lr_sh_dc": 1e-2, "lr_scale": 5e-3}
-    fit = ds.ImageFitter(W, H, n, view=view, proj=proj, cam_pos=cam, lr=lr, fused=fused)
+    fit = ds.ImageFitter(W, H, n, view=view, proj=proj, cam_pos=cam, lr=lr, fused=fused,
+                         grad_mode=grad_mode)
     fit.set_target(target)
     fit.upload_params(params)
     fit.step(1)
@@ -268,9 +271,9 @@ def test_fitter_step_matches_oracle_chain(fused):
     p_gpu = fit.params()
     pv = ds.mat_mul(proj, view).astype(np.float64)
     fw = fp64_forward(params.astype(np.float64), n, 0, cam, pv, W, H, target)
-    bw = fp64_backward(fw, params.astype(np.float64), n, 0, cam, pv, W, H)
+    bw = fp64_backward(fw, params.astype(np.float64), n, 0, cam, pv, W, H, grad_mode=grad_mode)
     mask = np.ones(14 * n, bool)
-    check_close(f"fitter_grads_fused{int(fused)}", g_gpu, bw["gparams"])
+    check_close(f"fitter_grads_fused{int(fused)}_mode{grad_mode}", g_gpu, bw["gparams"])
     # post-Adam: first step moves each parameter by about lr * sign(g); compare where
     # |g| is well above the fp32 noise floor of the gradient
     lr_f = np.concatenate([np.full(3 * n, 1e-2), np.full(3 * n, 5e-3), np.full(4 * n, 6e-4),
@@ -280,7 +283,7 @@ def test_fitter_step_matches_oracle_chain(fused):
     v = 0.001 * g64 * g64
     p_ref = params.astype(np.float64) - lr_f * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
     strong = mask & (np.abs(g64) > 1e-3 * np.abs(g64).max())
-    check_close(f"fitter_adam_fused{int(fused)}", p_gpu, p_ref, strong)
+    check_close(f"fitter_adam_fused{int(fused)}_mode{grad_mode}", p_gpu, p_ref, strong)
 
 
 def test_gradcheck_scene_against_finite_differences():
