@@ -51,8 +51,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--ppt", type=int, default=int(os.environ.get("DS_PPT", "4")),
-                    help="compositing pixels per thread (1, 2 or 4)")
+    ap.add_argument("--no-viewer", action="store_true")
+    ap.add_argument("--ppt", type=str, default=os.environ.get("DS_PPT", "auto"),
+                    help="compositing pixels per thread 'FWD,BWD' (1, 2 or 4) or 'auto'")
     return ap.parse_args()
 
 
@@ -100,6 +101,28 @@ def roofline_bwd(ds, oracle, fit, W, H, N, reps=20):
             "note": "fp32 vector (VALU) work priced against the 157.3 TFLOP/s fp32 peak"}
 
 
+def viewer_fps(ds, n=1_000_000, W=1920, H=1080, n_cams=120):
+    """Forward-only renderer on the clustered SH-3 scene over an orbit of
+    n_cams cameras at radius 3 (each frame: activation+projection, tile
+    assign, sort, compositing)."""
+    params = ds.cluster_scene(2000, n, 3, 64)
+    r = ds.SplatRenderer(W, H, n, 3)
+    r.upload_params(params)
+    cams = ds.orbit_cameras(n_cams, W, H)
+    r.render(*cams[0])
+    r.synchronize()
+    t0 = time.perf_counter()
+    for view, proj, eye in cams:
+        r.render(view, proj, eye)
+    r.synchronize()
+    el = time.perf_counter() - t0
+    st = r.raster_state()
+    r.close()
+    return {"workload": f"viewer {W}x{H}, {n} splats SH degree 3, {n_cams}-camera orbit",
+            "frames_per_s": round(n_cams / el, 2), "ms_per_frame": round(1e3 * el / n_cams, 3),
+            "n_isects_last_frame": int(st["n_isects"])}
+
+
 def cpu_baseline(oracle, params, target, W, H, N, pv, cam, lr6, seconds):
     threads = oracle.set_threads(0)
     fit = oracle.Fit(params, target, N, W, H, 0, pv, cam, lr6)
@@ -140,9 +163,14 @@ def main():
     fit = ds.ImageFitter(W, H, N, view=view, proj=proj, cam_pos=cam, lr=lr)
     fit.set_target(target)
     fit.upload_params(params)
-    ras = C.c_void_p()
-    ds.check(ds.lib().ds_fitter_get_raster(fit.h, C.byref(ras)), "ds_fitter_get_raster")
-    ds.check(ds.lib().ds_raster_set_tuning(ras, args.ppt), "ds_raster_set_tuning")
+    if args.ppt == "auto":
+        fit.forward()
+        tuning = fit.autotune(reps=3)
+    else:
+        f_ppt, b_ppt = (int(x) for x in args.ppt.split(","))
+        fit.set_tuning(f_ppt, b_ppt)
+        tuning = {"fwd_pix_per_thread": f_ppt, "bwd_pix_per_thread": b_ppt}
+    print(f"[bench] tuning {tuning}", file=sys.stderr, flush=True)
     fit.set_graph(bool(args.graph))
     fit.step(args.warmup)
     fit.synchronize()
@@ -158,6 +186,10 @@ def main():
 
     roof = None
     cpu = None
+    viewer = None
+    if not args.no_viewer:
+        viewer = viewer_fps(ds)
+        print(f"[bench] viewer {viewer}", file=sys.stderr, flush=True)
     if not args.no_roofline or not args.no_cpu:
         import oracle
     if not args.no_roofline:
@@ -179,11 +211,11 @@ def main():
                                f"pinhole frustum, {n_ell}-ellipse synthetic target, "
                                "fwd+bwd+Adam per step",
                    "width": W, "height": H, "splats": N, "graph": bool(args.graph),
-                   "pix_per_thread": args.ppt,
+                   "compositing_tuning": tuning,
                    "loss_after": round(loss, 6),
                    "psnr_db_after": round(ds.psnr_from_loss(loss), 3),
                    "steps_trained": args.steps + args.warmup},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "viewer": viewer,
     }
     if cpu:
         out["vs_cpu_baseline"] = round(value / cpu["value"], 2)

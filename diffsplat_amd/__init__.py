@@ -244,6 +244,20 @@ class ImageFitter:
     def set_graph(self, on: bool) -> None:
         check(lib().ds_fitter_set_graph(self.h, int(on)), "ds_fitter_set_graph")
 
+    def autotune(self, reps: int = 3) -> dict:
+        """Picks the fastest compositing launch shapes on the current state."""
+        f, b = C.c_int(), C.c_int()
+        tf, tb = C.c_float(), C.c_float()
+        check(lib().ds_fitter_autotune(self.h, reps, C.byref(f), C.byref(b), C.byref(tf),
+                                       C.byref(tb)), "ds_fitter_autotune")
+        return {"fwd_pix_per_thread": f.value, "bwd_pix_per_thread": b.value,
+                "fwd_ms": round(tf.value, 4), "bwd_ms": round(tb.value, 4)}
+
+    def set_tuning(self, fwd_ppt: int, bwd_ppt: int) -> None:
+        r = C.c_void_p()
+        check(lib().ds_fitter_get_raster(self.h, C.byref(r)), "ds_fitter_get_raster")
+        check(lib().ds_raster_set_tuning(r, fwd_ppt, bwd_ppt), "ds_raster_set_tuning")
+
     def stream(self):
         s = C.c_void_p()
         check(lib().ds_fitter_get_stream(self.h, C.byref(s)), "ds_fitter_get_stream")

@@ -123,7 +123,10 @@ _SIGS = {
     "ds_raster_counts": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                    C.POINTER(C.c_uint32)]),
     "ds_raster_get_view": (C.c_int, [_P, C.POINTER(RasterView)]),
-    "ds_raster_set_tuning": (C.c_int, [_P, C.c_int]),
+    "ds_raster_set_tuning": (C.c_int, [_P, C.c_int, C.c_int]),
+    "ds_raster_get_tuning": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "ds_fitter_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _F,
+                                     _F]),
     "ds_sort_pairs_u64": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_int, _P]),
     "ds_mse_fwd": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
     "ds_mse_bwd": (C.c_int, [_P, _P, C.c_int64, _P, _P]),

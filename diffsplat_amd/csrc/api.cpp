@@ -494,6 +494,16 @@ ds_status ds_fitter_set_graph(ds_fitter *f, int enable) {
     return DS_OK;
 }
 
+ds_status ds_fitter_autotune(ds_fitter *f, int reps, int *fwd_ppt, int *bwd_ppt, float *fwd_ms,
+                             float *bwd_ms) {
+    clear_error();
+    DS_REQUIRE(f, "NULL fitter");
+    DS_GUARD_BEGIN
+    f->impl.autotune(reps, fwd_ppt, bwd_ppt, fwd_ms, bwd_ms);
+    return DS_OK;
+    DS_GUARD_END
+}
+
 ds_status ds_fitter_step_count(ds_fitter *f, int32_t *step) {
     clear_error();
     DS_REQUIRE(f && step, "NULL argument");

@@ -159,6 +159,54 @@ void ImageFitter::step() {
 
 void ImageFitter::forwardOnly() { pipeline.forward(); }
 
+void ImageFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd,
+                           float *ms_bwd) {
+    if (reps < 1) reps = 1;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    auto timed = [&](auto &&body) {
+        body();  // warm-up
+        hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+        for (int k = 0; k < reps; ++k) body();
+        hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        return ms / reps;
+    };
+    ds_raster *r = ras_layer.raster();
+    int fwd0 = 1, bwd0 = 4;
+    check(ds_raster_get_tuning(r, &fwd0, &bwd0), "autotune");
+    const int opts[3] = {1, 2, 4};
+    int bf = fwd0, bb = bwd0;
+    float tf = 1e30f, tb = 1e30f;
+    for (int p : opts) {
+        check(ds_raster_set_tuning(r, p, bwd0), "autotune");
+        const float t = timed([&] { pipeline.forward(); });
+        if (t < tf) { tf = t; bf = p; }
+    }
+    check(ds_raster_set_tuning(r, bf, bwd0), "autotune");
+    pipeline.forward();
+    for (int p : opts) {
+        check(ds_raster_set_tuning(r, bf, p), "autotune");
+        const float t = timed([&] { ras_layer.backward(); });
+        if (t < tb) { tb = t; bb = p; }
+    }
+    check(ds_raster_set_tuning(r, bf, bb), "autotune");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // the captured step (if any) used the old launch shapes
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    if (best_fwd) *best_fwd = bf;
+    if (best_bwd) *best_bwd = bb;
+    if (ms_fwd) *ms_fwd = tf;
+    if (ms_bwd) *ms_bwd = tb;
+}
+
 void ImageFitter::views(ds_gauss3d *params, ds_splat2d *s2, ds_splat2d *gs2, float **grad_rgb) {
     if (params) *params = gaussian_params.view();
     if (s2) *s2 = fused ? gp_layer.getOutput().view() : psp_layer.getOutput().view();

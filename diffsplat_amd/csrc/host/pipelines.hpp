@@ -44,6 +44,9 @@ public:
     void step();          // zeroGrad, forward, backward, Adam
     void forwardOnly();   // forward (render + loss)
     void setUseGraph(bool on);
+    // Times the compositing launch shapes on the current parameters (forward
+    // and backward only, no Adam update) and keeps the fastest of each.
+    void autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd, float *ms_bwd);
 
     float getLoss();
     float *getOutput() { return ras_layer.getOutput().ptr; }

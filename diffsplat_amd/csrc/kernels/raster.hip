@@ -192,7 +192,8 @@ struct ds_raster {
     uint32_t isect_cap = 0;
     int keep_keys = 0;
     int grad_mode = DS_GRAD_EXACT;
-    int pix_per_thread = 4;  // compositing: pixels per thread (1, 2 or 4)
+    int fwd_ppt = 1;  // compositing forward: pixels per thread (1, 2 or 4)
+    int bwd_ppt = 4;  // compositing backward: pixels per thread
     int tile_bits = 1;
     // per splat
     ds::DevBuf<uint2> rect;
@@ -290,7 +291,7 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
     }
     const CompositeArgs ca{(const float4 *)r->rec.ptr, r->flat, (const int2 *)r->ranges.ptr,
                            r->W, r->H, r->ntx, r->ntiles};
-    DS_TRY(launch_composite_fwd(ca, r->pix_per_thread, out_rgb, r->T.ptr, r->last.ptr, target,
+    DS_TRY(launch_composite_fwd(ca, r->fwd_ppt, out_rgb, r->T.ptr, r->last.ptr, target,
                                 loss, grad, s));
     r->last_count = n;
     r->has_fwd = true;
@@ -304,7 +305,7 @@ static ds_status raster_backward(ds_raster *r, const ds_splat2d &in, const float
     DS_REQUIRE(grad_rgb != nullptr, "grad_rgb is NULL");
     const CompositeArgs ca{(const float4 *)r->rec.ptr, r->flat, (const int2 *)r->ranges.ptr,
                            r->W, r->H, r->ntx, r->ntiles};
-    DS_TRY(launch_composite_bwd(ca, r->pix_per_thread, r->grad_mode, r->T.ptr, r->last.ptr,
+    DS_TRY(launch_composite_bwd(ca, r->bwd_ppt, r->grad_mode, r->T.ptr, r->last.ptr,
                                 grad_rgb, gin, s));
     return DS_OK;
 }
@@ -418,12 +419,21 @@ extern "C" ds_status ds_raster_get_view(ds_raster *r, ds_raster_view *v) {
     return DS_OK;
 }
 
-extern "C" ds_status ds_raster_set_tuning(ds_raster *r, int pix_per_thread) {
+extern "C" ds_status ds_raster_set_tuning(ds_raster *r, int fwd_ppt, int bwd_ppt) {
     clear_error();
     DS_REQUIRE(r, "NULL raster");
-    DS_REQUIRE(pix_per_thread == 1 || pix_per_thread == 2 || pix_per_thread == 4,
-               "pix_per_thread must be 1, 2 or 4");
-    r->pix_per_thread = pix_per_thread;
+    auto ok = [](int p) { return p == 1 || p == 2 || p == 4; };
+    DS_REQUIRE(ok(fwd_ppt) && ok(bwd_ppt), "pixels per thread must be 1, 2 or 4");
+    r->fwd_ppt = fwd_ppt;
+    r->bwd_ppt = bwd_ppt;
+    return DS_OK;
+}
+
+extern "C" ds_status ds_raster_get_tuning(ds_raster *r, int *fwd_ppt, int *bwd_ppt) {
+    clear_error();
+    DS_REQUIRE(r, "NULL raster");
+    if (fwd_ppt) *fwd_ppt = r->fwd_ppt;
+    if (bwd_ppt) *bwd_ppt = r->bwd_ppt;
     return DS_OK;
 }
 

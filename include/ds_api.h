@@ -240,9 +240,11 @@ ds_status ds_raster_counts(ds_raster *r, ds_stream_t s, uint32_t *n_isects,
 ds_status ds_raster_get_view(ds_raster *r, ds_raster_view *out);
 
 /* Launch-shape knobs of the compositing kernels (results do not depend on
- * them): pix_per_thread in {1, 2, 4} = pixels of a 16x16 tile per thread
- * (256, 128 or 64 threads per tile). Default 4. */
-ds_status ds_raster_set_tuning(ds_raster *r, int pix_per_thread);
+ * them): pixels of a 16x16 tile per thread in {1, 2, 4} (256, 128 or 64
+ * threads per tile), for the forward and the backward separately.
+ * Defaults: forward 1, backward 4. */
+ds_status ds_raster_set_tuning(ds_raster *r, int fwd_pix_per_thread, int bwd_pix_per_thread);
+ds_status ds_raster_get_tuning(ds_raster *r, int *fwd_pix_per_thread, int *bwd_pix_per_thread);
 
 /* ------------------------------------------------------------------ */
 /* Stand-alone sort (the radix sort inside ds_raster_fwd)              */
@@ -347,6 +349,12 @@ ds_status ds_fitter_get_raster(ds_fitter *f, ds_raster **r);
 ds_status ds_fitter_get_stream(ds_fitter *f, ds_stream_t *s);
 /* 1: capture one step into a HIP graph on first use and replay it. */
 ds_status ds_fitter_set_graph(ds_fitter *f, int enable);
+/* Autotuner: times every compositing launch shape on the current parameters
+ * (forward and backward only; parameters and Adam state are unchanged; `reps`
+ * timed runs each) and keeps the fastest forward and backward shapes.
+ * Synchronises. Any of the outputs may be NULL. */
+ds_status ds_fitter_autotune(ds_fitter *f, int reps, int *fwd_pix_per_thread,
+                             int *bwd_pix_per_thread, float *fwd_ms, float *bwd_ms);
 ds_status ds_fitter_step_count(ds_fitter *f, int32_t *step);
 /* Device views of the fitter's own buffers (valid for the fitter's life):
  * the parameters, the Splat2D the rasterizer consumes, its gradient, and

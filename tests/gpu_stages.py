@@ -37,7 +37,8 @@ class Stages:
         self.gs2 = ds.splat2d_view(self.d_gs2, n)
         self.ras = ds.Rasterizer(W, H, n, isect_cap=isect_cap, keep_keys=True,
                                  grad_mode=grad_mode)
-        ds.check(ds.lib().ds_raster_set_tuning(self.ras.h, pix_per_thread), "set_tuning")
+        ds.check(ds.lib().ds_raster_set_tuning(self.ras.h, pix_per_thread, pix_per_thread),
+                 "set_tuning")
 
     # ---- forward
     def activ_fwd(self):

@@ -55,7 +55,7 @@ def test_invalid_arguments_are_rejected_before_any_device_call():
     cfg = _lib.RasterConfig(64, 64, 10, 0, 0, 7)  # bad grad_mode
     assert L.ds_raster_create(C.byref(cfg), C.byref(h)) == 1
     assert b"grad_mode" in L.ds_last_error()
-    assert L.ds_raster_set_tuning(None, 4) == 1
+    assert L.ds_raster_set_tuning(None, 4, 4) == 1
     g = _lib.Gauss3D()
     g.count = 5
     s3 = _lib.Splat3D()
