@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-viewer", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (counter calibration)")
     ap.add_argument("--ppt", type=str, default=os.environ.get("DS_PPT", "auto"),
                     help="compositing pixels per thread 'FWD,BWD' (1, 2 or 4) or 'auto'")
     return ap.parse_args()
@@ -152,6 +154,16 @@ def main():
         sys.exit(2)
 
     import diffsplat_amd as ds
+
+    if args.copy_probe:
+        nbytes = args.copy_probe << 20
+        a, b = ds.DeviceArray(nbytes), ds.DeviceArray(nbytes)
+        a.zero()
+        for _ in range(5):
+            ds.check(ds.lib().ds_memcpy_dtod(b.ptr, a.ptr, nbytes, None), "ds_memcpy_dtod")
+        ds.check(ds.lib().ds_synchronize(), "ds_synchronize")
+        print(json.dumps({"copy_probe_bytes": nbytes, "copies": 5}), flush=True)
+        return
 
     W, H, N, n_ell, seed = CONFIGS[args.config]
     target = ds.target_image(seed, W, H, n_ell)
