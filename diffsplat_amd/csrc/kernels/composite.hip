@@ -30,8 +30,12 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
     return x * q + min(x, r) + local;
 }
 
+// u = (Sigma^-1 d)_x, w = (Sigma^-1 d)_y for d = splat centre - pixel, so that
+// dist2 = dx u + dy w, d dist2/d(x,y) = 2 (u, w), and the conic gradients of
+// the reference (-1/4 ddx^2, -1/4 ddy^2, -2 b dist2 - 2 dx dy / det) are
+// -u^2, -w^2 and -2 u w (the last by u w = b dist2 + dx dy / det).
 struct AlphaEval {
-    float dx, dy, dist2, g, alpha;
+    float u, w, dist2, g, og, alpha;
 };
 
 // Gaussian falloff and alpha of one splat at one pixel. Forward and backward
@@ -39,13 +43,16 @@ struct AlphaEval {
 // Returns false when the splat is skipped (sigma < 0 or alpha < 1/255).
 __device__ __forceinline__ bool eval_alpha(const float4 &A, const float4 &B, float px, float py,
                                            AlphaEval &e) {
-    e.dx = A.x - px;
-    e.dy = A.y - py;
-    e.dist2 = fmaf(e.dx * e.dx, A.z, fmaf(2.f * e.dx * e.dy, A.w, e.dy * e.dy * B.x));
-    const float sigma = 0.5f * e.dist2;
-    if (sigma < 0.f) return false;
-    e.g = __expf(-sigma);
-    e.alpha = fminf(kMaxAlpha, B.y * e.g);
+    constexpr float kNegHalfLog2e = -0.72134752044448170f;  // -0.5 / ln 2
+    const float dx = A.x - px;
+    const float dy = A.y - py;
+    e.u = fmaf(A.z, dx, A.w * dy);
+    e.w = fmaf(A.w, dx, B.x * dy);
+    e.dist2 = fmaf(dx, e.u, dy * e.w);
+    if (e.dist2 < 0.f) return false;  // sigma = dist2 / 2 < 0
+    e.g = __builtin_amdgcn_exp2f(e.dist2 * kNegHalfLog2e);
+    e.og = B.y * e.g;
+    e.alpha = fminf(kMaxAlpha, e.og);
     return e.alpha >= kAlphaThres;
 }
 
@@ -56,20 +63,23 @@ __device__ __forceinline__ float pixel_ndc_y(int i, int H) {
     return 1.f - 2.f * ((float)i + 0.5f) / (float)H;
 }
 
-template <int CTRL, int ROW_MASK, int BANK_MASK>
+// DPP-permuted copy of v, all lanes enabled; a lane whose source lies outside
+// the pattern reads 0 (bound_ctrl), so `v + dpp(v)` folds into one
+// v_add_f32_dpp.
+template <int CTRL>
 __device__ __forceinline__ float dpp(float v) {
-    return __int_as_float(
-        __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, BANK_MASK, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
-// Sum over the 64 lanes of a wave, valid in lane 63. All lanes must be active.
+// Sum over the 64 lanes of a wave, valid in lane 63 only (other lanes hold
+// partial sums). All lanes must be active.
 __device__ __forceinline__ float wave_sum_to_63(float v) {
-    v += dpp<0xB1, 0xF, 0xF>(v);   // quad_perm [1,0,3,2]
-    v += dpp<0x4E, 0xF, 0xF>(v);   // quad_perm [2,3,0,1]  -> quad sums
-    v += dpp<0x114, 0xF, 0xE>(v);  // row_shr:4
-    v += dpp<0x118, 0xF, 0xC>(v);  // row_shr:8            -> row sum in lane 15 of each row
-    v += dpp<0x142, 0xA, 0xF>(v);  // row_bcast:15
-    v += dpp<0x143, 0xC, 0xF>(v);  // row_bcast:31         -> total in lane 63
+    v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]  -> every lane: its quad's sum
+    v += dpp<0x114>(v);  // row_shr:4
+    v += dpp<0x118>(v);  // row_shr:8            -> lane 15 of each row: the row's sum
+    v += dpp<0x142>(v);  // row_bcast:15         -> lane 63: rows 2 + 3
+    v += dpp<0x143>(v);  // row_bcast:31         -> lane 63: all four rows
     return v;
 }
 
@@ -190,7 +200,9 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
     const int trow = tile / a.ntx, tcol = tile % a.ntx;
     const int j = tcol * kTile + (tid & 15);
     const float px = pixel_ndc_x(j, a.W);
-    float py[P], T[P], gr[P], gg[P], gb[P], ar[P], ag[P], ab[P];
+    // Per pixel: transmittance (walked back from T_final), last contributor,
+    // dL/dC, and acc = (colour of the splats behind) . dL/dC.
+    float py[P], T[P], gr[P], gg[P], gb[P], acc[P];
     int32_t lst[P];
     int my_last = -1;
 #pragma unroll
@@ -204,7 +216,7 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
         gr[k] = inside ? gout[3 * pix + 0] : 0.f;
         gg[k] = inside ? gout[3 * pix + 1] : 0.f;
         gb[k] = inside ? gout[3 * pix + 2] : 0.f;
-        ar[k] = ag[k] = ab[k] = 0.f;
+        acc[k] = 0.f;
         my_last = max(my_last, lst[k]);
     }
     const int wave_last = wave_max(my_last);
@@ -247,37 +259,36 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
                 anyv = true;
                 const float4 C = sC[t];
                 const float ra = __builtin_amdgcn_rcpf(fmaxf(1.f - e.alpha, 1e-6f));
-                const float Tb = T[k] * ra;
+                const float Tb = T[k] * ra;  // transmittance in front of this splat
                 const float fac = e.alpha * Tb;
-                v[5] += fac * gr[k];
-                v[6] += fac * gg[k];
-                v[7] += fac * gb[k];
+                v[5] = fmaf(fac, gr[k], v[5]);
+                v[6] = fmaf(fac, gg[k], v[6]);
+                v[7] = fmaf(fac, gb[k], v[7]);
+                const float cg = fmaf(C.x, gr[k], fmaf(C.y, gg[k], C.z * gb[k]));
                 float v_alpha;
                 if constexpr (GRAD_MODE == DS_GRAD_REFERENCE)
-                    v_alpha = Tb * ((C.x - ar[k]) * gr[k] + (C.y - ag[k]) * gg[k] +
-                                    (C.z - ab[k]) * gb[k]);
+                    v_alpha = Tb * (cg - acc[k]);
                 else
-                    v_alpha = Tb * (C.x * gr[k] + C.y * gg[k] + C.z * gb[k]) -
-                              ra * (ar[k] * gr[k] + ag[k] * gg[k] + ab[k] * gb[k]);
-                if (B.y * e.g <= kMaxAlpha) {
-                    v[8] += e.g * v_alpha;
-                    const float v_d2 = -0.5f * B.y * e.g * v_alpha;
-                    const float ddx = 2.f * (e.dx * A.z + e.dy * A.w);
-                    const float ddy = 2.f * (e.dx * A.w + e.dy * B.x);
-                    v[0] += v_d2 * ddx;
-                    v[1] += v_d2 * ddy;
-                    v[2] += v_d2 * (-0.25f * ddx * ddx);
-                    v[3] += v_d2 * (-2.f * A.w * e.dist2 - 2.f * e.dx * e.dy * B.z);
-                    v[4] += v_d2 * (-0.25f * ddy * ddy);
+                    v_alpha = Tb * cg - ra * acc[k];
+                if (e.og <= kMaxAlpha) {
+                    v[8] = fmaf(e.g, v_alpha, v[8]);
+                    const float s = -0.5f * e.og * v_alpha;  // dL/d dist2
+                    const float p = s * e.u, q = s * e.w;
+                    v[0] += p;                   // x2 after the reduction
+                    v[1] += q;                   // x2 after the reduction
+                    v[2] = fmaf(-p, e.u, v[2]);
+                    v[3] = fmaf(-p, e.w, v[3]);  // x2 after the reduction
+                    v[4] = fmaf(-q, e.w, v[4]);
                 }
-                ar[k] += C.x * fac;
-                ag[k] += C.y * fac;
-                ab[k] += C.z * fac;
+                acc[k] = fmaf(fac, cg, acc[k]);
                 T[k] = Tb;
             }
             if (__ballot(anyv) == 0) continue;
 #pragma unroll
             for (int f = 0; f < 9; ++f) v[f] = wave_sum_to_63(v[f]);
+            v[0] *= 2.f;
+            v[1] *= 2.f;
+            v[3] *= 2.f;
             if (__lane_id() == kWave - 1) {
                 if constexpr (NW == 1) {
                     const uint32_t g = __float_as_uint(B.w);

@@ -62,23 +62,37 @@ def test_fused_and_layered_pipelines_agree(grad_mode):
     assert abs(a.loss() - b.loss()) <= 1e-4 * b.loss()
 
 
+CHECKPOINTS = (1, 5, 10, 20, 30, 50)
+
+
 def test_fit_200_steps_matches_cpu_fit():
-    """The 256x256 / 4096-splat workload for 200 steps: PSNR of the GPU fit and
-    of the oracle's fp32 fit of the same inputs agree within 0.3 dB (the
-    trajectories differ only through float rounding and atomic order)."""
+    """The 256x256 / 4096-splat workload against the oracle's fp32 fit of the
+    same inputs. The fit is chaotic: the oracle's own fits from starts that
+    differ by one ulp agree to 4e-5 relative in loss through step 50, then
+    spread to 6% in loss (0.45 dB PSNR over five starts) by step 200. So the
+    loss after each of the first 50 steps must agree to 5e-4 relative, and the
+    200-step PSNR to 1 dB."""
     fit, params, target, pv, cam = make(graph=True)
-    fit.forward()
-    psnr0 = ds.psnr_from_loss(fit.loss())
-    fit.step(200)
-    fit.forward()
-    psnr_gpu = ds.psnr_from_loss(fit.loss())
     lr6 = np.array([LR["lr_pos"], LR["lr_scale"], LR["lr_rot"], LR["lr_sh_dc"], 6e-3,
                     LR["lr_opacity"]], np.float32)
     cpu = oracle.Fit(params, target, N, W, H, 0, pv, cam, lr6)
-    for _ in range(200):
-        cpu.step()
-    loss_cpu = cpu.step()  # its forward renders the 200-step state
-    psnr_cpu = ds.psnr_from_loss(loss_cpu)
+    cpu_loss = [cpu.step() for _ in range(201)]  # entry k: loss after k updates
+    cpu.close()
+    fit.forward()
+    psnr0 = ds.psnr_from_loss(fit.loss())
+    done = 0
+    for k in CHECKPOINTS:
+        fit.step(k - done)
+        done = k
+        fit.forward()
+        lg = fit.loss()
+        rel = abs(lg - cpu_loss[k]) / cpu_loss[k]
+        print(f"[parity] fit_step{k}: gpu={lg:.7f} cpu={cpu_loss[k]:.7f} rel={rel:.2e}")
+        assert rel <= 5e-4, (k, lg, cpu_loss[k])
+    fit.step(200 - done)
+    fit.forward()
+    psnr_gpu = ds.psnr_from_loss(fit.loss())
+    psnr_cpu = ds.psnr_from_loss(cpu_loss[200])
     print(f"[parity] fit200: psnr_init={psnr0:.3f} gpu={psnr_gpu:.3f} cpu={psnr_cpu:.3f}")
     assert psnr_gpu > psnr0 + 8.0
-    assert abs(psnr_gpu - psnr_cpu) <= 0.3
+    assert abs(psnr_gpu - psnr_cpu) <= 1.0
