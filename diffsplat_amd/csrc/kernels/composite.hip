@@ -11,9 +11,11 @@
 // MI355X layout: a workgroup owns one tile and each thread P pixels of it
 // (P = 4: one wave64 per tile; P = 2: two waves; P = 1: four waves). Splat
 // records are staged in LDS in batches of one record per thread and read as
-// broadcasts. In the backward the nine per-splat gradient sums are reduced
-// across the wave with DPP row operations (no LDS traffic), amortised over P
-// pixels per lane, and lane 63 issues one global float atomic per field
+// broadcasts. The per-pixel work has no branches: a skipped splat is a
+// select that adds exact zeros. In the backward the nine per-splat gradient
+// sums are reduced across the wave by a transposed DPP reduction that leaves
+// one field per lane (no LDS traffic), amortised over P pixels per lane, and
+// nine lanes then issue one global float atomic instruction together
 // (through an LDS pre-reduction when the tile has more than one wave).
 // The transmittance is stored directly (not as 1 - T) so the backward's
 // division chain starts from an exact value.
@@ -38,9 +40,11 @@ struct AlphaEval {
     float u, w, dist2, g, og, alpha;
 };
 
-// Gaussian falloff and alpha of one splat at one pixel. Forward and backward
-// run this same code so their discrete decisions agree bit for bit.
-// Returns false when the splat is skipped (sigma < 0 or alpha < 1/255).
+// Gaussian falloff and alpha of one splat at one pixel, without branches.
+// Forward and backward run this same code so their discrete decisions agree
+// bit for bit. Returns false when the splat is skipped (sigma < 0 or
+// alpha < 1/255); g, og and alpha are then meaningless (g may be inf) and
+// callers select them away rather than multiplying by zero.
 __device__ __forceinline__ bool eval_alpha(const float4 &A, const float4 &B, float px, float py,
                                            AlphaEval &e) {
     constexpr float kNegHalfLog2e = -0.72134752044448170f;  // -0.5 / ln 2
@@ -49,11 +53,10 @@ __device__ __forceinline__ bool eval_alpha(const float4 &A, const float4 &B, flo
     e.u = fmaf(A.z, dx, A.w * dy);
     e.w = fmaf(A.w, dx, B.x * dy);
     e.dist2 = fmaf(dx, e.u, dy * e.w);
-    if (e.dist2 < 0.f) return false;  // sigma = dist2 / 2 < 0
     e.g = __builtin_amdgcn_exp2f(e.dist2 * kNegHalfLog2e);
     e.og = B.y * e.g;
     e.alpha = fminf(kMaxAlpha, e.og);
-    return e.alpha >= kAlphaThres;
+    return (e.dist2 >= 0.f) & (e.alpha >= kAlphaThres);  // sigma = dist2 / 2 >= 0
 }
 
 __device__ __forceinline__ float pixel_ndc_x(int j, int W) {
@@ -71,16 +74,61 @@ __device__ __forceinline__ float dpp(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
-// Sum over the 64 lanes of a wave, valid in lane 63 only (other lanes hold
-// partial sums). All lanes must be active.
-__device__ __forceinline__ float wave_sum_to_63(float v) {
-    v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
-    v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]  -> every lane: its quad's sum
-    v += dpp<0x114>(v);  // row_shr:4
-    v += dpp<0x118>(v);  // row_shr:8            -> lane 15 of each row: the row's sum
-    v += dpp<0x142>(v);  // row_bcast:15         -> lane 63: rows 2 + 3
-    v += dpp<0x143>(v);  // row_bcast:31         -> lane 63: all four rows
-    return v;
+// One halving step of a transposed reduction: lanes whose selector bit is
+// clear keep field a, the others field b, each summed with the DPP partner
+// (which has the opposite selector bit).
+template <int CTRL>
+__device__ __forceinline__ float pair_step(float a, float b, bool hi) {
+    const float sa = a + dpp<CTRL>(a);
+    const float sb = b + dpp<CTRL>(b);
+    return hi ? sb : sa;
+}
+
+// Sum over the 64 lanes of a wave of nine per-lane values v[0..8], all lanes
+// active. Four DPP steps inside each 16-lane row (partners lane^1, lane^2,
+// lane-4 and lane-8 in the row) leave one field per lane, then two
+// permlane swaps add the four rows. Result: lanes 0, 8, 4, 2, 10, 1, 9, 5, 3
+// of every row hold the sums of fields 0..8 (field_of_lane); the other lanes
+// hold garbage.
+__device__ __forceinline__ float wave_sum9_transposed(const float (&v)[9], int lane) {
+    const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8;
+    constexpr int kXor1 = 0xB1, kXor2 = 0x4E, kRor4 = 0x124, kRor8 = 0x128;
+    const float r0 = pair_step<kXor1>(v[0], v[5], b0);
+    const float r1 = pair_step<kXor1>(v[1], v[6], b0);
+    const float r2 = pair_step<kXor1>(v[2], v[7], b0);
+    const float r3 = pair_step<kXor1>(v[3], v[8], b0);
+    const float r4 = v[4] + dpp<kXor1>(v[4]);  // field 4 in even lanes
+    const float s0 = pair_step<kXor2>(r0, r3, b1);
+    const float s1 = pair_step<kXor2>(r1, r4, b1);
+    const float s2 = r2 + dpp<kXor2>(r2);      // valid where b1 == 0
+    const float t0 = pair_step<kRor4>(s0, s2, b2);
+    const float t1 = s1 + dpp<kRor4>(s1);      // valid where b2 == 0
+    float z = pair_step<kRor8>(t0, t1, b3);
+    // rows 0+1 and 2+3, then the two halves
+    const auto q = __builtin_amdgcn_permlane16_swap(__float_as_uint(z), __float_as_uint(z), false,
+                                                    false);
+    z = __uint_as_float(q[0]) + __uint_as_float(q[1]);
+    const auto h = __builtin_amdgcn_permlane32_swap(__float_as_uint(z), __float_as_uint(z), false,
+                                                    false);
+    return __uint_as_float(h[0]) + __uint_as_float(h[1]);
+}
+
+// Field whose wave sum a lane holds after wave_sum9_transposed (-1: none).
+// Every row holds the same sums; row 0 issues the atomics.
+__device__ __forceinline__ int field_of_lane(int lane) {
+    if (lane >= 16) return -1;
+    switch (lane) {
+        case 0: return 0;
+        case 8: return 1;
+        case 4: return 2;
+        case 2: return 3;
+        case 10: return 4;
+        case 1: return 5;
+        case 9: return 6;
+        case 5: return 7;
+        case 3: return 8;
+        default: return -1;
+    }
 }
 
 template <bool MSE, int P>
@@ -130,20 +178,21 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
 #pragma unroll
             for (int k = 0; k < P; ++k) any |= !done[k];
             if (!any) break;
-            const float4 A = sA[t], B = sB[t];
+            const float4 A = sA[t], B = sB[t], C = sC[t];
 #pragma unroll
             for (int k = 0; k < P; ++k) {
-                if (done[k]) continue;
+                // A skipped splat gets alpha 0, which leaves T and the colour
+                // exactly unchanged (T * (1 - 0) == T).
                 AlphaEval e;
-                if (!eval_alpha(A, B, px, py[k], e)) continue;
-                const float vis = e.alpha * T[k];
-                const float4 C = sC[t];
+                const bool ok = eval_alpha(A, B, px, py[k], e) & !done[k];
+                const float alpha = ok ? e.alpha : 0.f;
+                const float vis = alpha * T[k];
                 cr[k] += C.x * vis;
                 cg[k] += C.y * vis;
                 cb[k] += C.z * vis;
-                cur[k] = b + t;
-                T[k] = T[k] * (1.f - e.alpha);
-                if (T[k] <= kTThres) done[k] = true;
+                cur[k] = ok ? b + t : cur[k];
+                T[k] = T[k] * (1.f - alpha);
+                done[k] |= T[k] <= kTThres;
             }
         }
     }
@@ -232,6 +281,14 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
     }
     float *const dst[9] = {grad.pos_x,  grad.pos_y,   grad.cov_xx,  grad.cov_xy, grad.cov_yy,
                            grad.color_r, grad.color_g, grad.color_b, grad.opacity};
+    // The field this lane adds after the transposed reduction, its array,
+    // and the factor 2 of d dist2 / d(x, y) and of the symmetric cxy term.
+    const int lane = __lane_id();
+    const int my_field = field_of_lane(lane);
+    float *my_dst = nullptr;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) my_dst = my_field == f ? dst[f] : my_dst;
+    const float my_scale = (my_field == 0 || my_field == 1 || my_field == 3) ? 2.f : 1.f;
     const int end = min(range.y, block_last + 1);
     for (int bend = end - 1; bend >= range.x; bend -= NT) {
         const int bs = min(NT, bend + 1 - range.x);
@@ -247,20 +304,29 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
         for (int t = max(0, bend - wave_last); t < bs; ++t) {
             const int idx = bend - t;
             const float4 A = sA[t], B = sB[t];
-            float v[9];
-#pragma unroll
-            for (int f = 0; f < 9; ++f) v[f] = 0.f;
+            // Pass 1: which of this lane's pixels the splat reaches. A wave
+            // whose pixels it reaches none of skips the rest.
+            AlphaEval e[P];
+            bool ok[P];
             bool anyv = false;
 #pragma unroll
             for (int k = 0; k < P; ++k) {
-                if (idx > lst[k]) continue;
-                AlphaEval e;
-                if (!eval_alpha(A, B, px, py[k], e)) continue;
-                anyv = true;
-                const float4 C = sC[t];
-                const float ra = __builtin_amdgcn_rcpf(fmaxf(1.f - e.alpha, 1e-6f));
-                const float Tb = T[k] * ra;  // transmittance in front of this splat
-                const float fac = e.alpha * Tb;
+                ok[k] = eval_alpha(A, B, px, py[k], e[k]) & (idx <= lst[k]);
+                anyv |= ok[k];
+            }
+            if (__ballot(anyv) == 0) continue;
+            // Pass 2, without branches: a skipped pixel gets alpha 0 and gate
+            // false, which adds exact zeros and keeps T and acc unchanged.
+            const float4 C = sC[t];
+            float v[9];
+#pragma unroll
+            for (int f = 0; f < 9; ++f) v[f] = 0.f;
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                const float alpha = ok[k] ? e[k].alpha : 0.f;
+                const float ra = __builtin_amdgcn_rcpf(fmaxf(1.f - alpha, 1e-6f));
+                const float Tb = ok[k] ? T[k] * ra : T[k];  // transmittance in front of the splat
+                const float fac = alpha * Tb;
                 v[5] = fmaf(fac, gr[k], v[5]);
                 v[6] = fmaf(fac, gg[k], v[6]);
                 v[7] = fmaf(fac, gb[k], v[7]);
@@ -270,34 +336,24 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
                     v_alpha = Tb * (cg - acc[k]);
                 else
                     v_alpha = Tb * cg - ra * acc[k];
-                if (e.og <= kMaxAlpha) {
-                    v[8] = fmaf(e.g, v_alpha, v[8]);
-                    const float s = -0.5f * e.og * v_alpha;  // dL/d dist2
-                    const float p = s * e.u, q = s * e.w;
-                    v[0] += p;                   // x2 after the reduction
-                    v[1] += q;                   // x2 after the reduction
-                    v[2] = fmaf(-p, e.u, v[2]);
-                    v[3] = fmaf(-p, e.w, v[3]);  // x2 after the reduction
-                    v[4] = fmaf(-q, e.w, v[4]);
-                }
+                const bool gate = ok[k] & (e[k].og <= kMaxAlpha);  // alpha below its clamp
+                v[8] = fmaf(gate ? e[k].g : 0.f, v_alpha, v[8]);
+                const float s = gate ? -0.5f * e[k].og * v_alpha : 0.f;  // dL/d dist2
+                const float p = s * e[k].u, q = s * e[k].w;
+                v[0] += p;                      // x2 after the reduction
+                v[1] += q;                      // x2 after the reduction
+                v[2] = fmaf(-p, e[k].u, v[2]);
+                v[3] = fmaf(-p, e[k].w, v[3]);  // x2 after the reduction
+                v[4] = fmaf(-q, e[k].w, v[4]);
                 acc[k] = fmaf(fac, cg, acc[k]);
                 T[k] = Tb;
             }
-            if (__ballot(anyv) == 0) continue;
-#pragma unroll
-            for (int f = 0; f < 9; ++f) v[f] = wave_sum_to_63(v[f]);
-            v[0] *= 2.f;
-            v[1] *= 2.f;
-            v[3] *= 2.f;
-            if (__lane_id() == kWave - 1) {
+            const float sum = my_scale * wave_sum9_transposed(v, lane);
+            if (my_field >= 0) {  // nine lanes of row 0, one field each
                 if constexpr (NW == 1) {
-                    const uint32_t g = __float_as_uint(B.w);
-#pragma unroll
-                    for (int f = 0; f < 9; ++f)
-                        if (v[f] != 0.f) atomicAdd(dst[f] + g, v[f]);
+                    if (sum != 0.f) atomicAdd(my_dst + __float_as_uint(B.w), sum);
                 } else {
-#pragma unroll
-                    for (int f = 0; f < 9; ++f) atomicAdd(&sG[f][t], v[f]);
+                    atomicAdd(&sG[my_field][t], sum);
                 }
             }
         }
