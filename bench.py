@@ -112,6 +112,7 @@ def viewer_fps(ds, n=1_000_000, W=1920, H=1080, n_cams=120):
     r.upload_params(params)
     cams = ds.orbit_cameras(n_cams, W, H)
     r.render(*cams[0])
+    tuning = r.autotune(reps=3)
     r.synchronize()
     t0 = time.perf_counter()
     for view, proj, eye in cams:
@@ -122,7 +123,7 @@ def viewer_fps(ds, n=1_000_000, W=1920, H=1080, n_cams=120):
     r.close()
     return {"workload": f"viewer {W}x{H}, {n} splats SH degree 3, {n_cams}-camera orbit",
             "frames_per_s": round(n_cams / el, 2), "ms_per_frame": round(1e3 * el / n_cams, 3),
-            "n_isects_last_frame": int(st["n_isects"])}
+            "n_isects_last_frame": int(st["n_isects"]), "compositing_tuning": tuning}
 
 
 def cpu_baseline(oracle, params, target, W, H, N, pv, cam, lr6, seconds):

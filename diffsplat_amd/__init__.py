@@ -307,6 +307,15 @@ class SplatRenderer:
         c = np.ascontiguousarray(cam_pos, np.float32)
         check(lib().ds_renderer_render(self.h, fptr(v), fptr(p), fptr(c)), "ds_renderer_render")
 
+    def autotune(self, reps: int = 5) -> dict:
+        """Times the last camera's frame for each compositing launch shape and
+        keeps the fastest (needs one render first)."""
+        best = C.c_int()
+        ms = C.c_float()
+        check(lib().ds_renderer_autotune(self.h, reps, C.byref(best), C.byref(ms)),
+              "ds_renderer_autotune")
+        return {"fwd_pix_per_thread": best.value, "ms_per_frame": round(ms.value, 4)}
+
     def synchronize(self) -> None:
         check(lib().ds_renderer_synchronize(self.h), "ds_renderer_synchronize")
 

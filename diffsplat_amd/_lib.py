@@ -157,6 +157,7 @@ _SIGS = {
     "ds_renderer_destroy": (C.c_int, [_P]),
     "ds_renderer_upload_params": (C.c_int, [_P, _F]),
     "ds_renderer_render": (C.c_int, [_P, _F, _F, _F]),
+    "ds_renderer_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), _F]),
     "ds_renderer_download_image": (C.c_int, [_P, _F]),
     "ds_renderer_get_raster": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
     "ds_renderer_get_stream": (C.c_int, [_P, C.POINTER(C.c_void_p)]),

@@ -525,6 +525,7 @@ ds_status ds_fitter_views(ds_fitter *f, ds_gauss3d *params, ds_splat2d *splat2d,
 
 struct ds_renderer {
     dsh::SplatRenderer impl;
+    bool rendered = false;  // a camera has been set
 };
 
 ds_status ds_renderer_create(int width, int height, int count, int sh_degree, int64_t isect_cap,
@@ -564,6 +565,21 @@ ds_status ds_renderer_render(ds_renderer *r, const float view[16], const float p
     DS_REQUIRE(r && view && proj && cam_pos, "NULL argument");
     DS_GUARD_BEGIN
     r->impl.render(view, proj, cam_pos);
+    r->rendered = true;
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_renderer_autotune(ds_renderer *r, int reps, int *fwd_pix_per_thread,
+                               float *ms_per_frame) {
+    clear_error();
+    DS_REQUIRE(r, "NULL renderer");
+    DS_REQUIRE(r->rendered, "ds_renderer_autotune needs one ds_renderer_render first");
+    DS_GUARD_BEGIN
+    float ms = 0.f;
+    const int best = r->impl.autotune(reps, &ms);
+    if (fwd_pix_per_thread) *fwd_pix_per_thread = best;
+    if (ms_per_frame) *ms_per_frame = ms;
     return DS_OK;
     DS_GUARD_END
 }

@@ -255,4 +255,35 @@ void SplatRenderer::render(const float view[16], const float proj[16], const flo
     pipeline.forward();
 }
 
+int SplatRenderer::autotune(int reps, float *ms_best) {
+    if (reps < 1) reps = 1;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    ds_raster *r = ras_layer.raster();
+    int fwd0 = 1, bwd0 = 4;
+    check(ds_raster_get_tuning(r, &fwd0, &bwd0), "autotune");
+    int best = fwd0;
+    float tbest = 1e30f;
+    for (int p : {1, 2, 4}) {
+        check(ds_raster_set_tuning(r, p, bwd0), "autotune");
+        pipeline.forward();  // warm-up
+        hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+        for (int k = 0; k < reps; ++k) pipeline.forward();
+        hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (ms / reps < tbest) {
+            tbest = ms / reps;
+            best = p;
+        }
+    }
+    check(ds_raster_set_tuning(r, best, bwd0), "autotune");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_best) *ms_best = tbest;
+    return best;
+}
+
 }  // namespace dsh

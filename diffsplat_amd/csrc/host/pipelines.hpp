@@ -86,6 +86,9 @@ public:
     void init(int w, int h, int count, int sh_degree, int64_t isect_cap = 0);
     void uploadParams(const float *host_slab);
     void render(const float view[16], const float proj[16], const float cam_pos[3]);
+    // Times the frame at the last camera for each compositing launch shape
+    // (1, 2 or 4 pixels per thread), keeps the fastest and returns it.
+    int autotune(int reps, float *ms_best);
     float *getOutput() { return ras_layer.getOutput().ptr; }
     uint32_t getVisibleCount() { return ras_layer.getVisibleCount(); }
     void setActiveSHDegree(int d) { gp_layer.setSHDegree(d); }

@@ -173,24 +173,37 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
         }
         __syncthreads();
         const int bs = min(NT, range.y - b);
+        // Every exit below is wave-uniform (a ballot), so the loop runs on
+        // scalar branches; lanes whose pixels are done ride along adding
+        // exact zeros.
         for (int t = 0; t < bs; ++t) {
             bool any = false;
 #pragma unroll
             for (int k = 0; k < P; ++k) any |= !done[k];
-            if (!any) break;
-            const float4 A = sA[t], B = sB[t], C = sC[t];
+            if (__ballot(any) == 0) break;
+            const float4 A = sA[t], B = sB[t];
+            // Pass 1: which of this lane's pixels the splat reaches; a wave
+            // whose pixels it reaches none of skips the rest.
+            AlphaEval e[P];
+            bool ok[P];
+            bool anyv = false;
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                ok[k] = eval_alpha(A, B, px, py[k], e[k]) & !done[k];
+                anyv |= ok[k];
+            }
+            if (__ballot(anyv) == 0) continue;
+            const float4 C = sC[t];
 #pragma unroll
             for (int k = 0; k < P; ++k) {
                 // A skipped splat gets alpha 0, which leaves T and the colour
                 // exactly unchanged (T * (1 - 0) == T).
-                AlphaEval e;
-                const bool ok = eval_alpha(A, B, px, py[k], e) & !done[k];
-                const float alpha = ok ? e.alpha : 0.f;
+                const float alpha = ok[k] ? e[k].alpha : 0.f;
                 const float vis = alpha * T[k];
                 cr[k] += C.x * vis;
                 cg[k] += C.y * vis;
                 cb[k] += C.z * vis;
-                cur[k] = ok ? b + t : cur[k];
+                cur[k] = ok[k] ? b + t : cur[k];
                 T[k] = T[k] * (1.f - alpha);
                 done[k] |= T[k] <= kTThres;
             }

@@ -130,32 +130,38 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
     if (__lane_id() == 0 && vis) atomicAdd(&sc[SC_VISIBLE], (uint32_t)__popcll(vis));
 }
 
-// Writes the pairs of each splat, taken in depth order, at its scanned
-// offset, tiles in row-major order. Pairs at or past the cap are dropped.
+// Writes the pairs of the splats, taken in depth order, each splat's tiles in
+// row-major order starting at its scanned offset; pairs at or past the cap
+// are dropped. One thread per pair slot, so a splat covering thousands of
+// tiles costs no more than thousands of small ones: slot p belongs to the
+// last depth rank d with off[d] <= p (off is the exclusive scan of the tile
+// counts in depth order, so that splat's range [off[d], off[d+1]) holds p and
+// is not empty), found by binary search.
 __global__ __launch_bounds__(256) void k_emit(const uint32_t *__restrict__ order,
                                               const uint32_t *__restrict__ off,
-                                              const uint32_t *__restrict__ ntiles,
                                               const uint2 *__restrict__ rect,
                                               const uint32_t *sc, int ntx,
                                               uint32_t *__restrict__ tkey,
                                               uint32_t *__restrict__ tval) {
     const uint32_t n = sc[SC_COUNT];
-    const uint32_t cap = sc[SC_NISECT];
-    for (int64_t k = gtid(); k < n; k += gstride()) {
-        const uint32_t g = order[k];
-        if (ntiles[g] == 0) continue;
-        uint32_t slot = off[k];
-        if (slot >= cap) continue;
-        const uint2 r = rect[g];
-        const int tx0 = r.x & 0xFFFF, tx1 = r.x >> 16, ty0 = r.y & 0xFFFF, ty1 = r.y >> 16;
-        for (int ty = ty0; ty <= ty1; ++ty) {
-            for (int tx = tx0; tx <= tx1; ++tx) {
-                if (slot >= cap) return;
-                tkey[slot] = (uint32_t)(ty * ntx + tx);
-                tval[slot] = g;
-                ++slot;
-            }
+    const uint32_t m = sc[SC_NISECT];  // pairs kept: the total clamped at the cap
+    for (int64_t p = gtid(); p < m; p += gstride()) {
+        uint32_t lo = 0, hi = n;  // off[lo] <= p (off[0] == 0); the answer lies in [lo, hi)
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (off[mid] <= (uint32_t)p)
+                lo = mid;
+            else
+                hi = mid;
         }
+        const uint32_t g = order[lo];
+        const uint32_t j = (uint32_t)p - off[lo];  // index inside the splat's rectangle
+        const uint2 r = rect[g];
+        const uint32_t tx0 = r.x & 0xFFFF, tx1 = r.x >> 16, ty0 = r.y & 0xFFFF;
+        const uint32_t w = tx1 - tx0 + 1;
+        const uint32_t ty = ty0 + j / w, tx = tx0 + j % w;
+        tkey[p] = ty * (uint32_t)ntx + tx;
+        tval[p] = g;
     }
 }
 
@@ -264,8 +270,8 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
         DS_TRY(scan_exclusive_u32(r->ntiles_of.ptr, order, r->pair_off.ptr,
                                   ScanLen{sc + SC_COUNT, 1, 1}, (uint32_t)n, r->part.ptr,
                                   sc + SC_WANTED, sc + SC_NISECT, r->isect_cap, s));
-        hipLaunchKernelGGL(k_emit, dim3(grid_cap((uint64_t)n, 256, 4096)), dim3(256), 0, s, order,
-                           (const uint32_t *)r->pair_off.ptr, (const uint32_t *)r->ntiles_of.ptr,
+        hipLaunchKernelGGL(k_emit, dim3(grid_cap((uint64_t)r->isect_cap, 256, 4096)), dim3(256),
+                           0, s, order, (const uint32_t *)r->pair_off.ptr,
                            (const uint2 *)r->rect.ptr, (const uint32_t *)sc, r->ntx,
                            r->tkey_a.ptr, r->tval_a.ptr);
         DS_CHECK_LAUNCH("emit");

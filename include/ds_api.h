@@ -376,6 +376,11 @@ ds_status ds_renderer_upload_params(ds_renderer *r, const float *host_slab);
 /* SplatRenderer::render (src/pipelines/splat_renderer.cpp:85-91). Async. */
 ds_status ds_renderer_render(ds_renderer *r, const float view[16], const float proj[16],
                              const float cam_pos[3]);
+/* Autotuner (no counterpart in the reference): re-renders the last camera
+ * `reps` times per compositing launch shape, keeps the fastest. Call after
+ * one ds_renderer_render. Synchronises. Either output may be NULL. */
+ds_status ds_renderer_autotune(ds_renderer *r, int reps, int *fwd_pix_per_thread,
+                               float *ms_per_frame);
 ds_status ds_renderer_download_image(ds_renderer *r, float *host_rgb);
 ds_status ds_renderer_get_raster(ds_renderer *r, ds_raster **out);
 ds_status ds_renderer_get_stream(ds_renderer *r, ds_stream_t *s);
