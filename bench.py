@@ -33,6 +33,14 @@ PEAK_HBM_GBS = 8000.0
 # Algorithmic FP32 work per splat evaluation of the compositing backward
 # (FMA = 2 FLOP): falloff, alpha, the colour / alpha / conic gradients.
 BWD_FLOP_PER_EVAL = 60.0
+# HBM bytes per launch of the compositing backward (one pixel per thread),
+# from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over 30 steps of this
+# workload, scaled by a 256 MiB copy (fetch x2.000, write x1.000): see
+# profiles/fit_720p_60k_counters.md. Counters cannot be read live here; the
+# write bytes are the kernel's device-scope float atomics.
+BWD_TRAFFIC_READ = 27.73e6
+BWD_TRAFFIC_WRITE = 133.16e6
+BWD_TRAFFIC_STATE = "fit step 32 (counter passes over 30 steps after 2 warm-up steps)"
 
 CONFIGS = {
     # name: (W, H, N, n_ellipses, seed)
@@ -97,10 +105,14 @@ def roofline_bwd(ds, oracle, fit, W, H, N, reps=20):
     achieved = flops / avg_s / 1e12
     return {"kernel": "k_composite_bwd", "bound": "mfma", "unit": "TFLOP/s",
             "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-            "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": BWD_TRAFFIC_READ + BWD_TRAFFIC_WRITE,
+            "traffic_read_bytes": BWD_TRAFFIC_READ, "traffic_write_bytes": BWD_TRAFFIC_WRITE,
+            "traffic_state": BWD_TRAFFIC_STATE,
             "avg_launch_ms": round(avg_s * 1e3, 4), "splat_evals_per_launch": int(e_bwd),
             "flop_per_eval": BWD_FLOP_PER_EVAL, "n_isects": int(st["n_isects"]),
-            "note": "fp32 vector (VALU) work priced against the 157.3 TFLOP/s fp32 peak"}
+            "note": "fp32 vector (VALU) work priced against the 157.3 TFLOP/s fp32 peak "
+                    "(which counts packed FMA); the kernel is VALU-issue-bound"}
 
 
 def viewer_fps(ds, n=1_000_000, W=1920, H=1080, n_cams=120):
