@@ -11,6 +11,17 @@ static void hip_check(hipError_t e, const char *what) {
                                  hipGetErrorString(e));
 }
 
+// Index into {1, 2, 4} pixels per thread of the launch shape to keep, given
+// the times t[] of the three: the current shape stays unless another is at
+// least 5% faster. Shapes that tie on the tuning frame can differ a lot
+// later in a fit, so a near-tie is not worth a switch.
+static int pick_with_margin(const float t[3], int current_ppt) {
+    int best = current_ppt == 1 ? 0 : current_ppt == 2 ? 1 : 2;
+    for (int i = 0; i < 3; ++i)
+        if (t[i] < 0.95f * t[best]) best = i;
+    return best;
+}
+
 // ============================================================ ImageFitter
 
 ImageFitter::ImageFitter() = default;
@@ -176,23 +187,24 @@ void ImageFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd
         return ms / reps;
     };
     ds_raster *r = ras_layer.raster();
-    int fwd0 = 1, bwd0 = 4;
+    int fwd0 = 1, bwd0 = 1;
     check(ds_raster_get_tuning(r, &fwd0, &bwd0), "autotune");
     const int opts[3] = {1, 2, 4};
-    int bf = fwd0, bb = bwd0;
-    float tf = 1e30f, tb = 1e30f;
-    for (int p : opts) {
-        check(ds_raster_set_tuning(r, p, bwd0), "autotune");
-        const float t = timed([&] { pipeline.forward(); });
-        if (t < tf) { tf = t; bf = p; }
+    float tfw[3], tbw[3];
+    for (int i = 0; i < 3; ++i) {
+        check(ds_raster_set_tuning(r, opts[i], bwd0), "autotune");
+        tfw[i] = timed([&] { pipeline.forward(); });
     }
+    const int bf = opts[pick_with_margin(tfw, fwd0)];
     check(ds_raster_set_tuning(r, bf, bwd0), "autotune");
     pipeline.forward();
-    for (int p : opts) {
-        check(ds_raster_set_tuning(r, bf, p), "autotune");
-        const float t = timed([&] { ras_layer.backward(); });
-        if (t < tb) { tb = t; bb = p; }
+    for (int i = 0; i < 3; ++i) {
+        check(ds_raster_set_tuning(r, bf, opts[i]), "autotune");
+        tbw[i] = timed([&] { ras_layer.backward(); });
     }
+    const int ib = pick_with_margin(tbw, bwd0);
+    const int bb = opts[ib];
+    const float tf = tfw[bf == 1 ? 0 : bf == 2 ? 1 : 2], tb = tbw[ib];
     check(ds_raster_set_tuning(r, bf, bb), "autotune");
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -261,12 +273,12 @@ int SplatRenderer::autotune(int reps, float *ms_best) {
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
     ds_raster *r = ras_layer.raster();
-    int fwd0 = 1, bwd0 = 4;
+    int fwd0 = 1, bwd0 = 1;
     check(ds_raster_get_tuning(r, &fwd0, &bwd0), "autotune");
-    int best = fwd0;
-    float tbest = 1e30f;
-    for (int p : {1, 2, 4}) {
-        check(ds_raster_set_tuning(r, p, bwd0), "autotune");
+    const int opts[3] = {1, 2, 4};
+    float t[3];
+    for (int i = 0; i < 3; ++i) {
+        check(ds_raster_set_tuning(r, opts[i], bwd0), "autotune");
         pipeline.forward();  // warm-up
         hip_check(hipEventRecord(e0, stream), "hipEventRecord");
         for (int k = 0; k < reps; ++k) pipeline.forward();
@@ -274,16 +286,14 @@ int SplatRenderer::autotune(int reps, float *ms_best) {
         hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-        if (ms / reps < tbest) {
-            tbest = ms / reps;
-            best = p;
-        }
+        t[i] = ms / reps;
     }
-    check(ds_raster_set_tuning(r, best, bwd0), "autotune");
+    const int ib = pick_with_margin(t, fwd0);
+    check(ds_raster_set_tuning(r, opts[ib], bwd0), "autotune");
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (ms_best) *ms_best = tbest;
-    return best;
+    if (ms_best) *ms_best = t[ib];
+    return opts[ib];
 }
 
 }  // namespace dsh

@@ -37,8 +37,25 @@ namespace ds {
 
 namespace {
 
-// Scalars kept on the device: indices into ds_raster::scalars.
-enum { SC_NISECT = 0, SC_WANTED = 1, SC_VISIBLE = 2, SC_COUNT = 3, SC_NUM = 8 };
+// Scalars kept on the device: indices into ds_raster::scalars. Those below
+// SC_EPOCH and the tickets start each forward at zero (SC_COUNT at n).
+enum {
+    SC_NISECT = 0,
+    SC_WANTED = 1,
+    SC_VISIBLE = 2,
+    SC_COUNT = 3,
+    SC_EPOCH = 8,    // look-back epoch: +1 every forward, never 0
+    SC_ERR = 9,      // a look-back spin gave up in this forward
+    SC_TICKET = 16,  // one ticket counter per look-back slot
+    SC_NUM = 32
+};
+
+// Look-back slots (and digit-histogram rows): the four depth passes, the
+// pair-offset scan.
+enum { LB_DEPTH = 0, LB_SCAN = 4, LB_SLOTS = 8 };
+constexpr int kHistRows = 4;  // digit histograms of the depth passes
+// Largest splat count sorted with one-kernel passes (64 chunks).
+constexpr uint32_t kOneKernelMaxKeys = 64u * kSortChunk;
 
 __device__ __forceinline__ uint32_t depth_bits(float z) {
     uint32_t u = __float_as_uint(z);
@@ -54,13 +71,45 @@ __device__ __forceinline__ int tile_floor(float v, int hi) {
     return (int)f;
 }
 
-// Start of a forward: device scalars, empty tile ranges, zero loss.
+// Start of a forward: device scalars, a new look-back epoch, zero tickets
+// and digit histograms, empty tile ranges, zero loss.
 __global__ __launch_bounds__(256) void k_begin(uint32_t *sc, uint32_t count, int2 *ranges,
-                                               int ntiles, float *loss) {
+                                               int ntiles, float *loss, uint32_t *ghist) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < SC_NUM) sc[i] = (i == SC_COUNT) ? count : 0u;
+    if (i < SC_EPOCH) sc[i] = (i == SC_COUNT) ? count : 0u;
+    if (i == SC_EPOCH) {
+        const uint32_t e = sc[SC_EPOCH] + 1u;
+        sc[SC_EPOCH] = e ? e : 1u;
+    }
+    if (i == SC_ERR || (i >= SC_TICKET && i < SC_TICKET + LB_SLOTS)) sc[i] = 0u;
     if (i == 0 && loss) *loss = 0.f;
+    for (int t = i; t < kHistRows * kRadix; t += gridDim.x * 256) ghist[t] = 0u;
     for (int t = i; t < ntiles; t += gridDim.x * 256) ranges[t] = make_int2(0, 0);
+}
+
+// Adds 1 to bin d of an LDS histogram for every active lane. When all active
+// lanes share the bin (the upper bytes of the tile ids of consecutive pairs,
+// of the depths of neighbouring splats) one lane adds their count: same-bin
+// LDS atomics from a whole wave would serialise.
+__device__ __forceinline__ void lds_count(uint32_t *h, uint32_t d) {
+    const uint64_t active = __ballot(true);
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    if (__ballot(d == d0) == active) {
+        if (__lane_id() == __ffsll((unsigned long long)active) - 1)
+            atomicAdd(&h[d0], (uint32_t)__popcll(active));
+    } else {
+        atomicAdd(&h[d], 1u);
+    }
+}
+
+// Adds a 256-thread workgroup's LDS digit histograms (rows x 256 bins; row r
+// counts bits [8r, 8r + 8) of the keys) to the global rows.
+__device__ __forceinline__ void flush_histograms(const uint32_t (*s_h)[kRadix], int rows,
+                                                 uint32_t *grows) {
+    for (int r = 0; r < rows; ++r) {
+        const uint32_t c = s_h[r][threadIdx.x];
+        if (c) atomicAdd(&grows[r * kRadix + threadIdx.x], c);
+    }
 }
 
 // Zeroes up to ten arrays of n floats in one launch.
@@ -83,10 +132,16 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
                                               uint32_t *__restrict__ ntiles,
                                               uint32_t *__restrict__ dkey,
                                               uint32_t *__restrict__ didx,
-                                              float4 *__restrict__ rec, uint32_t *sc) {
+                                              float4 *__restrict__ rec, uint32_t *sc,
+                                              uint32_t *ghist) {
 #pragma clang fp contract(off)
+    __shared__ uint32_t s_h[4][kRadix];  // the four depth-byte histograms
+#pragma unroll
+    for (int r = 0; r < 4; ++r) s_h[r][threadIdx.x] = 0u;
+    __syncthreads();
     const int i = blockIdx.x * 256 + threadIdx.x;
     bool visible = false;
+    uint32_t key = 0xFFFFFFFFu;  // culled splats sort last
     if (i < n) {
         const float x = in.pos_x[i], y = in.pos_y[i], z = in.pos_z[i];
         int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
@@ -116,18 +171,22 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
             rect[i] = make_uint2((uint32_t)tx0 | ((uint32_t)tx1 << 16),
                                  (uint32_t)ty0 | ((uint32_t)ty1 << 16));
             ntiles[i] = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
-            dkey[i] = depth_bits(z);
+            key = depth_bits(z);
             const float idet = 1.f / det;
             rec[3 * i + 0] = make_float4(x, y, cyy * idet, -cxy * idet);
             rec[3 * i + 1] = make_float4(cxx * idet, in.opacity[i], idet, __uint_as_float(i));
             rec[3 * i + 2] = make_float4(in.color_r[i], in.color_g[i], in.color_b[i], 0.f);
         } else {
             ntiles[i] = 0;
-            dkey[i] = 0xFFFFFFFFu;
         }
+        dkey[i] = key;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) lds_count(s_h[r], (key >> (8 * r)) & 0xFFu);
     }
     const uint64_t vis = __ballot(visible);
     if (__lane_id() == 0 && vis) atomicAdd(&sc[SC_VISIBLE], (uint32_t)__popcll(vis));
+    __syncthreads();
+    flush_histograms(s_h, 4, ghist + LB_DEPTH * kRadix);
 }
 
 // Writes the pairs of the splats, taken in depth order, each splat's tiles in
@@ -199,8 +258,9 @@ struct ds_raster {
     int keep_keys = 0;
     int grad_mode = DS_GRAD_EXACT;
     int fwd_ppt = 1;  // compositing forward: pixels per thread (1, 2 or 4)
-    int bwd_ppt = 4;  // compositing backward: pixels per thread
+    int bwd_ppt = 1;  // compositing backward: pixels per thread
     int tile_bits = 1;
+    int tile_passes = 1;  // 8-bit radix passes over the tile ids
     // per splat
     ds::DevBuf<uint2> rect;
     ds::DevBuf<uint32_t> ntiles_of, dkey_a, dkey_b, didx_a, didx_b, pair_off;
@@ -212,8 +272,11 @@ struct ds_raster {
     ds::DevBuf<int2> ranges;
     ds::DevBuf<float> T;
     ds::DevBuf<int32_t> last;
-    // scratch
-    ds::DevBuf<uint32_t> hist, part, scalars;
+    // scratch: device scalars, depth-digit histograms, look-back status words
+    // of the depth passes and of the pair-offset scan, and the per-chunk
+    // histograms and scan partials of the tile passes
+    ds::DevBuf<uint32_t> scalars, ghist, hist, part;
+    ds::DevBuf<uint64_t> lb_depth, lb_scan;
     // results of the last forward
     const uint32_t *flat = nullptr;
     const uint32_t *tkey_sorted = nullptr;
@@ -233,6 +296,7 @@ static ds_status raster_alloc_pixels(ds_raster *r, int W, int H) {
     int bits = 1;
     while ((1 << bits) < r->ntiles) ++bits;
     r->tile_bits = bits;
+    r->tile_passes = (bits + kRadixBits - 1) / kRadixBits;
     DS_TRY(r->ranges.alloc((size_t)r->ntiles));
     DS_TRY(r->T.alloc((size_t)W * H));
     DS_TRY(r->last.alloc((size_t)W * H));
@@ -248,39 +312,61 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
     DS_REQUIRE((target == nullptr) == (loss == nullptr) && (target == nullptr) == (grad == nullptr),
                "target, loss and grad must be given together");
     uint32_t *sc = r->scalars.ptr;
+    uint32_t *gh = r->ghist.ptr;
+    // look-back slot `slot` with its status words at `status`
+    auto slot = [&](uint64_t *status, int lb) {
+        return LookbackSlot{status, sc + SC_TICKET + lb, sc + SC_EPOCH, sc + SC_ERR};
+    };
     hipLaunchKernelGGL(k_begin, dim3(grid_cap((uint64_t)r->ntiles, 256, 256)), dim3(256), 0, s,
-                       sc, (uint32_t)n, r->ranges.ptr, r->ntiles, loss);
+                       sc, (uint32_t)n, r->ranges.ptr, r->ntiles, loss, gh);
     DS_CHECK_LAUNCH("begin");
     if (n > 0) {
         hipLaunchKernelGGL(k_prep, dim3((n + 255) / 256), dim3(256), 0, s, in, n, r->W, r->H,
                            r->ntx, r->nty, r->rect.ptr, r->ntiles_of.ptr, r->dkey_a.ptr,
-                           r->didx_a.ptr, r->rec.ptr, sc);
+                           r->didx_a.ptr, r->rec.ptr, sc, gh);
         DS_CHECK_LAUNCH("prep");
-        // depth sort of splats: four 8-bit LSD passes over the 32 depth bits
+        // Depth sort of splats: four 8-bit LSD passes over the 32 depth bits,
+        // then the pair offsets in depth order (total and capped count). Up
+        // to kOneKernelMaxKeys splats each pass and the scan are one kernel
+        // with look-back (launches dominate there); above, reduce-then-scan
+        // passes win, as the look-back traffic grows with the chunk count.
         uint32_t *ka = r->dkey_a.ptr, *kb = r->dkey_b.ptr, *va = r->didx_a.ptr,
                  *vb = r->didx_b.ptr;
+        const bool one_kernel = (uint32_t)n <= kOneKernelMaxKeys;
+        const size_t dstride = lookback_words((uint64_t)r->count_cap, kRadix);
         for (int pass = 0; pass < 4; ++pass) {
-            DS_TRY(radix_pass<uint32_t>(ka, va, kb, vb, sc + SC_COUNT, (uint32_t)n, 8 * pass,
-                                        r->hist.ptr, r->part.ptr, s));
+            if (one_kernel)
+                DS_TRY(onesweep_pass(ka, va, kb, vb, sc + SC_COUNT, (uint32_t)n, 8 * pass,
+                                     gh + (LB_DEPTH + pass) * kRadix,
+                                     slot(r->lb_depth.ptr + pass * dstride, LB_DEPTH + pass), s));
+            else
+                DS_TRY(radix_pass<uint32_t>(ka, va, kb, vb, sc + SC_COUNT, (uint32_t)n, 8 * pass,
+                                            r->hist.ptr, r->part.ptr, s));
             std::swap(ka, kb);
             std::swap(va, vb);
         }
         const uint32_t *order = va;
-        // pair offsets in depth order; total and capped count
-        DS_TRY(scan_exclusive_u32(r->ntiles_of.ptr, order, r->pair_off.ptr,
-                                  ScanLen{sc + SC_COUNT, 1, 1}, (uint32_t)n, r->part.ptr,
-                                  sc + SC_WANTED, sc + SC_NISECT, r->isect_cap, s));
+        if (one_kernel)
+            DS_TRY(scan_chained(r->ntiles_of.ptr, order, r->pair_off.ptr, sc + SC_COUNT,
+                                (uint32_t)n, sc + SC_WANTED, sc + SC_NISECT, r->isect_cap,
+                                slot(r->lb_scan.ptr, LB_SCAN), s));
+        else
+            DS_TRY(scan_exclusive_u32(r->ntiles_of.ptr, order, r->pair_off.ptr,
+                                      ScanLen{sc + SC_COUNT, 1, 1}, (uint32_t)n, r->part.ptr,
+                                      sc + SC_WANTED, sc + SC_NISECT, r->isect_cap, s));
         hipLaunchKernelGGL(k_emit, dim3(grid_cap((uint64_t)r->isect_cap, 256, 4096)), dim3(256),
                            0, s, order, (const uint32_t *)r->pair_off.ptr,
                            (const uint2 *)r->rect.ptr, (const uint32_t *)sc, r->ntx,
                            r->tkey_a.ptr, r->tval_a.ptr);
         DS_CHECK_LAUNCH("emit");
-        // stable sort of the pairs on the tile bits
+        // Stable sort of the pairs on the tile bits. With hundreds of chunks
+        // the reduce-then-scan passes beat one-kernel passes, whose look-back
+        // traffic grows with the chunk count.
         uint32_t *tka = r->tkey_a.ptr, *tkb = r->tkey_b.ptr, *tva = r->tval_a.ptr,
                  *tvb = r->tval_b.ptr;
-        for (int shift = 0; shift < r->tile_bits; shift += kRadixBits) {
-            DS_TRY(radix_pass<uint32_t>(tka, tva, tkb, tvb, sc + SC_NISECT, r->isect_cap, shift,
-                                        r->hist.ptr, r->part.ptr, s));
+        for (int pass = 0; pass < r->tile_passes; ++pass) {
+            DS_TRY(radix_pass<uint32_t>(tka, tva, tkb, tvb, sc + SC_NISECT, r->isect_cap,
+                                        kRadixBits * pass, r->hist.ptr, r->part.ptr, s));
             std::swap(tka, tkb);
             std::swap(tva, tvb);
         }
@@ -326,8 +412,9 @@ extern "C" ds_status ds_raster_create(const ds_raster_config *cfg, ds_raster **o
     clear_error();
     DS_REQUIRE(cfg && out, "NULL argument");
     DS_REQUIRE(cfg->count_cap >= 0, "count_cap < 0");
+    DS_REQUIRE((uint64_t)cfg->count_cap <= kLookbackMaxCount, "count_cap above 2^30 - 1");
     int64_t cap = cfg->isect_cap > 0 ? cfg->isect_cap : (int64_t)1 << 25;
-    DS_REQUIRE(cap <= ((int64_t)1 << 30), "isect_cap above 2^30");
+    DS_REQUIRE(cap <= (int64_t)kLookbackMaxCount, "isect_cap above 2^30 - 1");
     DS_REQUIRE(cfg->grad_mode == DS_GRAD_EXACT || cfg->grad_mode == DS_GRAD_REFERENCE,
                "grad_mode must be DS_GRAD_EXACT or DS_GRAD_REFERENCE");
     auto *r = new ds_raster();
@@ -350,8 +437,15 @@ extern "C" ds_status ds_raster_create(const ds_raster_config *cfg, ds_raster **o
     if (st == DS_OK) st = r->tval_a.alloc(cap);
     if (st == DS_OK) st = r->tval_b.alloc(cap);
     if (st == DS_OK && r->keep_keys) st = r->keys64.alloc(cap);
-    const uint64_t max_sort = (uint64_t)std::max<int64_t>(cap, (int64_t)nc);
-    if (st == DS_OK) st = r->hist.alloc(sort_hist_words(max_sort));
+    // look-back status words start at zero: epoch 0 is never used
+    const size_t lbd = 4 * lookback_words(nc, kRadix), lbs = lookback_words(nc, 1);
+    if (st == DS_OK) st = r->lb_depth.alloc(lbd);
+    if (st == DS_OK) st = fill_u32((uint32_t *)r->lb_depth.ptr, 0u, 2 * lbd, nullptr);
+    if (st == DS_OK) st = r->lb_scan.alloc(lbs);
+    if (st == DS_OK) st = fill_u32((uint32_t *)r->lb_scan.ptr, 0u, 2 * lbs, nullptr);
+    if (st == DS_OK) st = r->ghist.alloc((size_t)kHistRows * kRadix);
+    if (st == DS_OK)
+        st = r->hist.alloc(sort_hist_words(std::max<uint64_t>((uint64_t)cap, (uint64_t)nc)));
     if (st == DS_OK) st = r->part.alloc(kSpineMax);
     if (st == DS_OK) st = r->scalars.alloc(SC_NUM);
     if (st == DS_OK) st = fill_u32(r->scalars.ptr, 0u, SC_NUM, nullptr);
@@ -404,6 +498,9 @@ extern "C" ds_status ds_raster_counts(ds_raster *r, ds_stream_t s, uint32_t *n_i
     if (n_isects) *n_isects = h[SC_NISECT];
     if (visible_count) *visible_count = h[SC_VISIBLE];
     if (n_wanted) *n_wanted = h[SC_WANTED];
+    if (h[SC_ERR])
+        return set_error(DS_ERR_STATE, "a sort or scan look-back gave up waiting in the last "
+                                       "forward; its results are not valid");
     return DS_OK;
 }
 

@@ -114,94 +114,292 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const K *__restrict
     }
 }
 
+// LDS of one workgroup ranking and scattering a chunk of kSortChunk keys.
+template <typename K>
+struct ChunkLds {
+    static constexpr int NW = kSortThreads / kWave;
+    K k[kSortChunk];
+    uint32_t v[kSortChunk];
+    uint32_t w[NW][kRadix];  // per-wave digit counts, then per-wave digit offsets
+    uint32_t run[kRadix];    // digit counts of the chunk
+    uint32_t start[kRadix];  // exclusive prefix of run over digits
+    uint32_t gb[kRadix];     // global position of the chunk's first key of each digit
+    uint32_t tmp[NW];
+    uint32_t ticket;
+};
+
+// Loads chunk c and ranks its keys stably within their digit: rank[j] is the
+// number of earlier keys of the chunk with the same digit. Leaves the digit
+// counts in L.run and their exclusive prefix in L.start. Ranking matches
+// the 8 digit bits across the wave with ballots, then orders the waves.
+template <typename K>
+__device__ __forceinline__ void chunk_rank(const K *__restrict__ kin,
+                                           const uint32_t *__restrict__ vin, uint32_t base,
+                                           uint32_t cnt, int shift, K (&key)[kSortItems],
+                                           uint32_t (&val)[kSortItems],
+                                           uint32_t (&rank)[kSortItems], ChunkLds<K> &L) {
+    constexpr int NW = ChunkLds<K>::NW;
+    const int tid = threadIdx.x;
+    const int w = tid / kWave;
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        const uint32_t e = j * kSortThreads + tid;
+        if (e < cnt) {
+            key[j] = kin[base + e];
+            val[j] = vin[base + e];
+        } else {
+            key[j] = 0;
+            val[j] = 0;
+        }
+    }
+    L.run[tid] = 0;
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+#pragma unroll
+        for (int q = 0; q < NW; ++q) L.w[q][tid] = 0;
+        __syncthreads();
+        const bool valid = (uint32_t)(j * kSortThreads + tid) < cnt;
+        const uint32_t d = digit_of(key[j], shift);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < kRadixBits; ++bit) {
+            const bool on = (d >> bit) & 1u;
+            const uint64_t b = __ballot(on);
+            peers &= on ? b : ~b;
+        }
+        const uint32_t r = (uint32_t)__popcll(peers & lt);
+        if (valid && r == 0) L.w[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {
+            uint32_t run = L.run[tid];
+#pragma unroll
+            for (int q = 0; q < NW; ++q) {
+                const uint32_t cq = L.w[q][tid];
+                L.w[q][tid] = run;
+                run += cq;
+            }
+            L.run[tid] = run;
+        }
+        __syncthreads();
+        rank[j] = valid ? L.w[w][d] + r : 0u;
+        __syncthreads();
+    }
+    L.start[tid] = block_excl_scan<kSortThreads>(L.run[tid], L.tmp, nullptr);
+}
+
+// Reorders the ranked chunk by digit in LDS, then writes each run of equal
+// digits contiguously at L.gb[digit] (which the caller has filled).
+template <typename K>
+__device__ __forceinline__ void chunk_scatter(const K (&key)[kSortItems],
+                                              const uint32_t (&val)[kSortItems],
+                                              const uint32_t (&rank)[kSortItems], uint32_t cnt,
+                                              int shift, ChunkLds<K> &L, K *__restrict__ kout,
+                                              uint32_t *__restrict__ vout) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+        if ((uint32_t)(j * kSortThreads + tid) < cnt) {
+            const uint32_t p = L.start[digit_of(key[j], shift)] + rank[j];
+            L.k[p] = key[j];
+            L.v[p] = val[j];
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = tid; p < cnt; p += kSortThreads) {
+        const K k = L.k[p];
+        const uint32_t d = digit_of(k, shift);
+        const uint32_t pos = L.gb[d] + (p - L.start[d]);
+        kout[pos] = k;
+        vout[pos] = L.v[p];
+    }
+    __syncthreads();
+}
+
 template <typename K>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const K *__restrict__ kin, const uint32_t *__restrict__ vin, K *__restrict__ kout,
     uint32_t *__restrict__ vout, const uint32_t *n_dev, int shift,
     const uint32_t *__restrict__ hist) {
-    constexpr int NW = kSortThreads / kWave;
-    __shared__ K s_k[kSortChunk];
-    __shared__ uint32_t s_v[kSortChunk];
-    __shared__ uint32_t s_w[NW][kRadix];
-    __shared__ uint32_t s_run[kRadix];
-    __shared__ uint32_t s_start[kRadix];
-    __shared__ uint32_t s_gb[kRadix];
-    __shared__ uint32_t s_tmp[NW];
-
+    __shared__ ChunkLds<K> L;
     const uint32_t n = *n_dev;
     const uint32_t nch = (n + kSortChunk - 1) / kSortChunk;
     const int tid = threadIdx.x;
-    const int w = tid / kWave;
-    const uint64_t lt = lanemask_lt();
-
     for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
         const uint32_t base = c * kSortChunk;
         const uint32_t cnt = min((uint32_t)kSortChunk, n - base);
         K key[kSortItems];
-        uint32_t val[kSortItems];
-        uint32_t rank[kSortItems];
+        uint32_t val[kSortItems], rank[kSortItems];
+        chunk_rank(kin, vin, base, cnt, shift, key, val, rank, L);
+        L.gb[tid] = hist[tid * nch + c];
+        __syncthreads();
+        chunk_scatter(key, val, rank, cnt, shift, L, kout, vout);
+    }
+}
+
+// ---------------------------------------------------------------- look-back
+
+enum : uint32_t { kLbAggregate = 1u, kLbPrefix = 2u };
+// Polls of one status word before giving up (each poll sleeps ~64 cycles);
+// a running predecessor publishes within microseconds.
+constexpr uint32_t kLbSpinLimit = 1u << 20;
+
+__device__ __forceinline__ void lb_publish(uint64_t *word, uint32_t epoch, uint32_t state,
+                                           uint32_t value) {
+    const uint64_t w = ((uint64_t)epoch << 32) | ((uint64_t)state << 30) | value;
+    __hip_atomic_store(word, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of the values of the chunks before chunk c in one digit column
+// (words at status[p * stride] for p < c), walking back until a chunk
+// with its inclusive prefix published. Every chunk before c has published
+// at least its aggregate, or is about to (see LookbackSlot).
+// The sum saturates at kLookbackMaxCount. The words of kLbWindow
+// predecessors are loaded at once (one memory round trip per window rather
+// than per chunk) and consumed nearest first.
+constexpr int kLbWindow = 16;
+
+__device__ __forceinline__ uint32_t lb_exclusive(const uint64_t *status, size_t stride,
+                                                 uint32_t c, uint32_t epoch, uint32_t *err) {
+    uint64_t excl = 0;
+    uint32_t spins = 0;
+    int64_t p = (int64_t)c - 1;  // nearest chunk not yet added
+    while (p >= 0) {
+        uint64_t w[kLbWindow];
 #pragma unroll
-        for (int j = 0; j < kSortItems; ++j) {
-            const uint32_t e = j * kSortThreads + tid;
-            if (e < cnt) {
-                key[j] = kin[base + e];
-                val[j] = vin[base + e];
+        for (int j = 0; j < kLbWindow; ++j)
+            w[j] = (p - j >= 0) ? __hip_atomic_load(status + (size_t)(p - j) * stride,
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                : 0ull;
+        int used = 0;  // words consumed from the window
+        bool done = false, stalled = false;
+#pragma unroll
+        for (int j = 0; j < kLbWindow; ++j) {
+            if (done || stalled || p - j < 0) continue;
+            const uint32_t state = (uint32_t)(w[j] >> 30) & 3u;
+            if ((uint32_t)(w[j] >> 32) != epoch || state == 0u) {
+                stalled = true;  // not published yet: reload from here
+                continue;
+            }
+            excl += (uint32_t)w[j] & kLookbackMaxCount;
+            ++used;
+            done = (state == kLbPrefix);
+        }
+        if (done) break;
+        p -= used;
+        if (used > 0) {
+            spins = 0;
+        } else if (++spins > kLbSpinLimit) {
+            atomicOr(err, 1u);
+            break;
+        } else {
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return (uint32_t)min(excl, (uint64_t)kLookbackMaxCount);
+}
+
+__device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
+    return (uint32_t)min((uint64_t)a + b, (uint64_t)kLookbackMaxCount);
+}
+
+// One stable radix pass in one kernel (see LookbackSlot).
+__global__ __launch_bounds__(kSortThreads) void k_onesweep(
+    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, const uint32_t *n_dev, int shift,
+    const uint32_t *__restrict__ ghist, LookbackSlot lb) {
+    __shared__ ChunkLds<uint32_t> L;
+    const uint32_t n = *n_dev;
+    const uint32_t nch = (n + kSortChunk - 1) / kSortChunk;
+    const uint32_t epoch = *lb.epoch;
+    const int tid = threadIdx.x;
+    // first position of each digit in the output
+    const uint32_t dbase = block_excl_scan<kSortThreads>(ghist[tid], L.tmp, nullptr);
+    for (;;) {
+        if (tid == 0) L.ticket = atomicAdd(lb.ticket, 1u);
+        __syncthreads();
+        const uint32_t c = L.ticket;
+        if (c >= nch) break;  // uniform
+        const uint32_t base = c * kSortChunk;
+        const uint32_t cnt = min((uint32_t)kSortChunk, n - base);
+        uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+        chunk_rank(kin, vin, base, cnt, shift, key, val, rank, L);
+        uint64_t *col = lb.status + (size_t)c * kRadix + tid;
+        const uint32_t mine = L.run[tid];
+        uint32_t excl = 0;
+        if (c == 0) {
+            lb_publish(col, epoch, kLbPrefix, mine);
+        } else {
+            lb_publish(col, epoch, kLbAggregate, mine);
+            excl = lb_exclusive(lb.status + tid, kRadix, c, epoch, lb.err);
+            lb_publish(col, epoch, kLbPrefix, excl + mine);
+        }
+        L.gb[tid] = dbase + excl;
+        __syncthreads();
+        chunk_scatter(key, val, rank, cnt, shift, L, kout, vout);
+    }
+}
+
+// Exclusive scan in one kernel: chunks of kScanChunk elements, one status
+// word per chunk. Prefixes and the total saturate at kLookbackMaxCount
+// (2^30 - 1); the chunk sums themselves must stay below 2^32.
+__global__ __launch_bounds__(kScanThreads) void k_scan_chained(
+    const uint32_t *__restrict__ in, const uint32_t *__restrict__ gidx, const uint32_t *n_dev,
+    uint32_t *__restrict__ out, uint32_t *total_out, uint32_t *clamped_out, uint32_t cap,
+    LookbackSlot lb) {
+    __shared__ uint32_t s_tmp[kScanThreads / kWave];
+    __shared__ uint32_t s_ticket, s_excl;
+    const uint32_t n = *n_dev;
+    const uint32_t nch = (n + kScanChunk - 1) / kScanChunk;
+    const uint32_t epoch = *lb.epoch;
+    const int tid = threadIdx.x;
+    if (nch == 0) {
+        if (blockIdx.x == 0 && tid == 0) {
+            if (total_out) *total_out = 0u;
+            if (clamped_out) *clamped_out = 0u;
+        }
+        return;
+    }
+    for (;;) {
+        if (tid == 0) s_ticket = atomicAdd(lb.ticket, 1u);
+        __syncthreads();
+        const uint32_t c = s_ticket;
+        if (c >= nch) break;  // uniform
+        const uint32_t base = c * kScanChunk + tid * kScanItems;
+        uint32_t v[kScanItems];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const uint32_t e = base + j;
+            v[j] = (e < n) ? (gidx ? in[gidx[e]] : in[e]) : 0u;
+            sum += v[j];
+        }
+        uint32_t tot;
+        const uint32_t run = block_excl_scan<kScanThreads>(sum, s_tmp, &tot);
+        if (tid == 0) {
+            const uint32_t agg = min(tot, kLookbackMaxCount);
+            uint32_t excl = 0;
+            if (c == 0) {
+                lb_publish(lb.status, epoch, kLbPrefix, agg);
             } else {
-                key[j] = 0;
-                val[j] = 0;
+                lb_publish(lb.status + c, epoch, kLbAggregate, agg);
+                excl = lb_exclusive(lb.status, 1, c, epoch, lb.err);
+                lb_publish(lb.status + c, epoch, kLbPrefix, sat_add(excl, agg));
             }
-        }
-        s_run[tid] = 0;
-#pragma unroll
-        for (int j = 0; j < kSortItems; ++j) {
-#pragma unroll
-            for (int q = 0; q < NW; ++q) s_w[q][tid] = 0;
-            __syncthreads();
-            const bool valid = (uint32_t)(j * kSortThreads + tid) < cnt;
-            const uint32_t d = digit_of(key[j], shift);
-            uint64_t peers = __ballot(valid);
-#pragma unroll
-            for (int bit = 0; bit < kRadixBits; ++bit) {
-                const bool on = (d >> bit) & 1u;
-                const uint64_t b = __ballot(on);
-                peers &= on ? b : ~b;
-            }
-            const uint32_t r = (uint32_t)__popcll(peers & lt);
-            if (valid && r == 0) s_w[w][d] = (uint32_t)__popcll(peers);
-            __syncthreads();
-            {
-                uint32_t run = s_run[tid];
-#pragma unroll
-                for (int q = 0; q < NW; ++q) {
-                    const uint32_t cq = s_w[q][tid];
-                    s_w[q][tid] = run;
-                    run += cq;
-                }
-                s_run[tid] = run;
-            }
-            __syncthreads();
-            rank[j] = valid ? s_w[w][d] + r : 0u;
-            __syncthreads();
-        }
-        const uint32_t start = block_excl_scan<kSortThreads>(s_run[tid], s_tmp, nullptr);
-        s_start[tid] = start;
-        s_gb[tid] = hist[tid * nch + c];
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < kSortItems; ++j) {
-            if ((uint32_t)(j * kSortThreads + tid) < cnt) {
-                const uint32_t p = s_start[digit_of(key[j], shift)] + rank[j];
-                s_k[p] = key[j];
-                s_v[p] = val[j];
+            s_excl = excl;
+            if (c == nch - 1) {
+                const uint32_t total = sat_add(excl, agg);
+                if (total_out) *total_out = total;
+                if (clamped_out) *clamped_out = min(total, cap);
             }
         }
         __syncthreads();
-        for (uint32_t p = tid; p < cnt; p += kSortThreads) {
-            const K k = s_k[p];
-            const uint32_t d = digit_of(k, shift);
-            const uint32_t pos = s_gb[d] + (p - s_start[d]);
-            kout[pos] = k;
-            vout[pos] = s_v[p];
+        uint32_t acc = sat_add(s_excl, run);
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const uint32_t e = base + j;
+            if (e < n) out[e] = acc;
+            acc = sat_add(acc, v[j]);
         }
         __syncthreads();
     }
@@ -263,5 +461,28 @@ template ds_status radix_pass<uint32_t>(const uint32_t *, const uint32_t *, uint
 template ds_status radix_pass<uint64_t>(const uint64_t *, const uint32_t *, uint64_t *,
                                         uint32_t *, const uint32_t *, uint32_t, int, uint32_t *,
                                         uint32_t *, hipStream_t);
+
+ds_status onesweep_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                        uint32_t *vout, const uint32_t *n_dev, uint32_t max_n, int shift,
+                        const uint32_t *ghist, LookbackSlot lb, hipStream_t s) {
+    if (max_n > kLookbackMaxCount)
+        return set_error(DS_ERR_CAPACITY, "radix pass longer than 2^30 - 1 keys");
+    const uint64_t max_chunks = ((uint64_t)max_n + kSortChunk - 1) / kSortChunk;
+    if (max_chunks == 0) return DS_OK;
+    hipLaunchKernelGGL(k_onesweep, dim3(clamp_grid(max_chunks, 2048)), dim3(kSortThreads), 0, s,
+                       kin, vin, kout, vout, n_dev, shift, ghist, lb);
+    DS_CHECK_LAUNCH("onesweep");
+    return DS_OK;
+}
+
+ds_status scan_chained(const uint32_t *in, const uint32_t *gidx, uint32_t *out,
+                       const uint32_t *n_dev, uint32_t max_n, uint32_t *total_out,
+                       uint32_t *clamped_out, uint32_t cap, LookbackSlot lb, hipStream_t s) {
+    const uint64_t max_chunks = ((uint64_t)max_n + kScanChunk - 1) / kScanChunk;
+    hipLaunchKernelGGL(k_scan_chained, dim3(clamp_grid(max_chunks, 1024)), dim3(kScanThreads), 0,
+                       s, in, gidx, n_dev, out, total_out, clamped_out, cap, lb);
+    DS_CHECK_LAUNCH("scan_chained");
+    return DS_OK;
+}
 
 }  // namespace ds

@@ -83,4 +83,43 @@ inline uint32_t sort_hist_words(uint64_t max_n) {
     return (uint32_t)(kRadix * ((max_n + kSortChunk - 1) / kSortChunk));
 }
 
+// ------------------------------------------------------------------
+// One-kernel passes with decoupled look-back.
+//
+// A radix pass or a scan is one kernel: each workgroup draws the next chunk
+// index from a ticket counter, publishes its chunk's per-digit counts, and
+// looks back over earlier chunks for its exclusive prefix. A workgroup only
+// waits on chunks with lower tickets, whose workgroups are running and
+// publish their counts before they wait themselves, so the chain always
+// advances. Status words are {epoch, state | value}: a word written in an
+// earlier forward carries an older epoch and reads as "not ready", so the
+// status memory is never cleared (it is zero at allocation; epochs start at
+// 1). The spin is bounded: if it ever gives up it sets *err and the result
+// is wrong but the kernel ends.
+struct LookbackSlot {
+    uint64_t *status;       // kRadix (or 1 for a scan) words per chunk
+    uint32_t *ticket;       // zero before the pass
+    const uint32_t *epoch;  // current forward's epoch, never 0
+    uint32_t *err;
+};
+
+// Values in status words are below 2^30.
+constexpr uint32_t kLookbackMaxCount = (1u << 30) - 1;
+
+inline size_t lookback_words(uint64_t max_n, int digits) {
+    return (size_t)digits * (size_t)((max_n + kSortChunk - 1) / kSortChunk);
+}
+
+// Stable pass on bits [shift, shift + 8) of 32-bit keys; ghist holds the
+// pass's 256-bin digit histogram over all n keys.
+ds_status onesweep_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                        uint32_t *vout, const uint32_t *n_dev, uint32_t max_n, int shift,
+                        const uint32_t *ghist, LookbackSlot lb, hipStream_t s);
+
+// Exclusive scan of in[gidx[e]] (or in[e]) for e < n_dev[0], in one kernel;
+// writes the total and min(total, cap).
+ds_status scan_chained(const uint32_t *in, const uint32_t *gidx, uint32_t *out,
+                       const uint32_t *n_dev, uint32_t max_n, uint32_t *total_out,
+                       uint32_t *clamped_out, uint32_t cap, LookbackSlot lb, hipStream_t s);
+
 }  // namespace ds

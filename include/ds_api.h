@@ -186,8 +186,9 @@ typedef struct ds_raster ds_raster;
 
 typedef struct ds_raster_config {
     int32_t width, height;     /* image size in pixels                                     */
-    int32_t count_cap;         /* maximum splat count per call                             */
-    int64_t isect_cap;         /* maximum (tile, splat) pairs; 0 = 1<<25 as the reference  */
+    int32_t count_cap;         /* maximum splat count per call (< 2^30)                    */
+    int64_t isect_cap;         /* maximum (tile, splat) pairs (< 2^30); 0 = 1<<25 as the
+                                  reference                                                */
     int32_t keep_keys;         /* 1: also materialise the sorted 64-bit (tile|depth) keys  */
     int32_t grad_mode;         /* DS_GRAD_EXACT (0) or DS_GRAD_REFERENCE (1)               */
 } ds_raster_config;
@@ -233,7 +234,9 @@ ds_status ds_raster_bwd(ds_raster *r, const ds_splat2d *in, const float *grad_rg
 ds_status ds_splat2d_zero(const ds_splat2d *v, ds_stream_t s);
 
 /* Reads back n_isects and visible_count (synchronises the stream). Replaces
- * TileRasterizeLayer::getVisibleCount (src/layers/tile_rasterize_layer.cu:582-587). */
+ * TileRasterizeLayer::getVisibleCount (src/layers/tile_rasterize_layer.cu:582-587).
+ * n_isects_wanted saturates at 2^30 - 1. Returns DS_ERR_STATE if the last
+ * forward's sort could not complete (its output is then not valid). */
 ds_status ds_raster_counts(ds_raster *r, ds_stream_t s, uint32_t *n_isects,
                            uint32_t *visible_count, uint32_t *n_isects_wanted);
 
@@ -242,7 +245,7 @@ ds_status ds_raster_get_view(ds_raster *r, ds_raster_view *out);
 /* Launch-shape knobs of the compositing kernels (results do not depend on
  * them): pixels of a 16x16 tile per thread in {1, 2, 4} (256, 128 or 64
  * threads per tile), for the forward and the backward separately.
- * Defaults: forward 1, backward 4. */
+ * Defaults: forward 1, backward 1. */
 ds_status ds_raster_set_tuning(ds_raster *r, int fwd_pix_per_thread, int bwd_pix_per_thread);
 ds_status ds_raster_get_tuning(ds_raster *r, int *fwd_pix_per_thread, int *bwd_pix_per_thread);
 
@@ -351,8 +354,9 @@ ds_status ds_fitter_get_stream(ds_fitter *f, ds_stream_t *s);
 ds_status ds_fitter_set_graph(ds_fitter *f, int enable);
 /* Autotuner: times every compositing launch shape on the current parameters
  * (forward and backward only; parameters and Adam state are unchanged; `reps`
- * timed runs each) and keeps the fastest forward and backward shapes.
- * Synchronises. Any of the outputs may be NULL. */
+ * timed runs each) and switches to the fastest forward and backward shapes
+ * when they beat the current ones by at least 5%. Synchronises. Any of the
+ * outputs may be NULL. */
 ds_status ds_fitter_autotune(ds_fitter *f, int reps, int *fwd_pix_per_thread,
                              int *bwd_pix_per_thread, float *fwd_ms, float *bwd_ms);
 ds_status ds_fitter_step_count(ds_fitter *f, int32_t *step);
@@ -377,8 +381,9 @@ ds_status ds_renderer_upload_params(ds_renderer *r, const float *host_slab);
 ds_status ds_renderer_render(ds_renderer *r, const float view[16], const float proj[16],
                              const float cam_pos[3]);
 /* Autotuner (no counterpart in the reference): re-renders the last camera
- * `reps` times per compositing launch shape, keeps the fastest. Call after
- * one ds_renderer_render. Synchronises. Either output may be NULL. */
+ * `reps` times per compositing launch shape, switches to the fastest when it
+ * beats the current one by at least 5%. Call after one ds_renderer_render.
+ * Synchronises. Either output may be NULL. */
 ds_status ds_renderer_autotune(ds_renderer *r, int reps, int *fwd_pix_per_thread,
                                float *ms_per_frame);
 ds_status ds_renderer_download_image(ds_renderer *r, float *host_rgb);
