@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-viewer", action="store_true")
+    ap.add_argument("--no-time-to-psnr", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (counter calibration)")
     ap.add_argument("--ppt", type=str, default=os.environ.get("DS_PPT", "auto"),
@@ -113,6 +115,89 @@ def roofline_bwd(ds, oracle, fit, W, H, N, reps=20):
             "flop_per_eval": BWD_FLOP_PER_EVAL, "n_isects": int(st["n_isects"]),
             "note": "fp32 vector (VALU) work priced against the 157.3 TFLOP/s fp32 peak "
                     "(which counts packed FMA); the kernel is VALU-issue-bound"}
+
+
+def time_to_psnr(ds, target_db=30.0, W=256, H=256, N=4096, n_ell=32, seed=0, max_steps=6000,
+                 check_every=10):
+    """Wall time for the plain fitter (random init in the pinhole frustum) to
+    reach target_db PSNR on the 256x256 / 4096-splat workload, HIP graph on
+    (its capture on the first step is inside the timed region). The loss of
+    the last step's forward is read every check_every steps (one host
+    synchronisation)."""
+    target = ds.target_image(seed, W, H, n_ell)
+    params = ds.frustum_splats(seed, N, aspect=W / H)
+    view, proj, cam = ds.fitter_camera(W, H)
+    lr = {"lr_pos": 1e-2, "lr_sh_dc": 1e-2, "lr_scale": 5e-3, "lr_rot": 6e-4,
+          "lr_opacity": 6e-3}
+    fit = ds.ImageFitter(W, H, N, view=view, proj=proj, cam_pos=cam, lr=lr)
+    fit.set_target(target)
+    fit.upload_params(params)
+    fit.set_graph(True)
+    fit.synchronize()
+    t0 = time.perf_counter()
+    steps, psnr = 0, 0.0
+    while steps < max_steps:
+        fit.step(check_every)
+        steps += check_every
+        psnr = ds.psnr_from_loss(fit.loss())  # loss of the last step's forward
+        if psnr >= target_db:
+            break
+    el = time.perf_counter() - t0
+    fit.close()
+    return {"workload": f"plain fitter {W}x{H}, {N} random-init splats", "target_db": target_db,
+            "reached": psnr >= target_db, "steps": steps, "seconds": round(el, 4),
+            "psnr_db": round(psnr, 3), "init": "plain (random splats in the frustum)"}
+
+
+def batched_fit(ds, count=64, steps=500, seq_steps=10):
+    """Batched fitting: `count` independent fits of uneven sizes (areas
+    log-uniform 128^2..1024x576, splat counts log-uniform 2k..100k, the
+    synthetic target and random frustum init of each) stepped together for
+    `steps` steps after the autotuner. Also times `seq_steps` batch steps with
+    one stream (the instances one after another) for comparison."""
+    lr = {"lr_pos": 1e-2, "lr_sh_dc": 1e-2, "lr_scale": 5e-3, "lr_rot": 6e-4,
+          "lr_opacity": 6e-3}
+    sizes = ds.batch_instances(count)
+    fits = []
+    for w, h, n, seed in sizes:
+        view, proj, cam = ds.fitter_camera(w, h)
+        f = ds.ImageFitter(w, h, n, view=view, proj=proj, cam_pos=cam, lr=lr,
+                           isect_cap=min(1 << 25, max(1 << 20, 64 * n)))
+        f.set_target(ds.target_image(seed, w, h, 32))
+        f.upload_params(ds.frustum_splats(seed, n, aspect=w / h))
+        fits.append(f)
+    batch = ds.BatchFitter(fits)
+    tuned = batch.autotune(reps=3)
+    streams = tuned["streams"]
+    batch.set_streams(1)
+    batch.step(1)
+    batch.synchronize()
+    t0 = time.perf_counter()
+    batch.step(seq_steps)
+    batch.synchronize()
+    seq_ms = 1e3 * (time.perf_counter() - t0) / seq_steps
+    batch.set_streams(streams)
+    batch.step(1)
+    batch.synchronize()
+    t0 = time.perf_counter()
+    batch.step(steps)
+    batch.synchronize()
+    el = time.perf_counter() - t0
+    psnr = []
+    for f in fits:
+        f.forward()
+        psnr.append(ds.psnr_from_loss(f.loss()))
+    batch.close()
+    for f in fits:
+        f.close()
+    return {"workload": f"{count} uneven fits (128^2..1024x576 pixels, 2k..100k splats), "
+                        f"{steps} steps each", "instances": count, "steps": steps,
+            "streams": streams, "seconds": round(el, 4),
+            "instance_steps_per_s": round(count * steps / el, 1),
+            "ms_per_batch_step": round(1e3 * el / steps, 4),
+            "one_stream_ms_per_batch_step": round(seq_ms, 4),
+            "psnr_db_mean": round(float(np.mean(psnr)), 3),
+            "psnr_db_min": round(float(np.min(psnr)), 3)}
 
 
 def viewer_fps(ds, n=1_000_000, W=1920, H=1080, n_cams=120):
@@ -212,9 +297,17 @@ def main():
     roof = None
     cpu = None
     viewer = None
+    ttp = None
     if not args.no_viewer:
         viewer = viewer_fps(ds)
         print(f"[bench] viewer {viewer}", file=sys.stderr, flush=True)
+    if not args.no_time_to_psnr:
+        ttp = time_to_psnr(ds)
+        print(f"[bench] time to psnr {ttp}", file=sys.stderr, flush=True)
+    batched = None
+    if not args.no_batch:
+        batched = batched_fit(ds)
+        print(f"[bench] batched {batched}", file=sys.stderr, flush=True)
     if not args.no_roofline or not args.no_cpu:
         import oracle
     if not args.no_roofline:
@@ -240,7 +333,8 @@ def main():
                    "loss_after": round(loss, 6),
                    "psnr_db_after": round(ds.psnr_from_loss(loss), 3),
                    "steps_trained": args.steps + args.warmup},
-        "roofline": roof, "cpu_baseline": cpu, "viewer": viewer,
+        "roofline": roof, "cpu_baseline": cpu, "viewer": viewer, "time_to_30db": ttp,
+        "batched": batched,
     }
     if cpu:
         out["vs_cpu_baseline"] = round(value / cpu["value"], 2)

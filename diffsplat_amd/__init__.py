@@ -285,6 +285,62 @@ class ImageFitter:
             pass
 
 
+class BatchFitter:
+    """ds_batch: many ImageFitters stepped together, each replaying its own
+    captured step, spread over a few streams. Keeps the fitters alive."""
+
+    def __init__(self, fitters):
+        self.fitters = list(fitters)
+        arr = (C.c_void_p * len(self.fitters))(*[f.h for f in self.fitters])
+        h = C.c_void_p()
+        check(lib().ds_batch_create(arr, len(self.fitters), C.byref(h)), "ds_batch_create")
+        self.h = h.value
+
+    def step(self, n: int = 1) -> None:
+        check(lib().ds_batch_step(self.h, n), "ds_batch_step")
+
+    def synchronize(self) -> None:
+        check(lib().ds_batch_synchronize(self.h), "ds_batch_synchronize")
+
+    def set_streams(self, n: int) -> None:
+        check(lib().ds_batch_set_streams(self.h, n), "ds_batch_set_streams")
+
+    def autotune(self, reps: int = 3) -> dict:
+        ns = C.c_int()
+        ms = C.c_float()
+        check(lib().ds_batch_autotune(self.h, reps, C.byref(ns), C.byref(ms)), "ds_batch_autotune")
+        return {"streams": ns.value, "ms_per_batch_step": round(ms.value, 4)}
+
+    def close(self):
+        if self.h:
+            lib().ds_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_instances(count: int = 64, seed: int = 3000, min_area: int = 128 * 128,
+                    max_area: int = 1024 * 576, min_splats: int = 2000, max_splats: int = 100_000):
+    """Sizes of the batched-fitting workload: per instance an image area
+    log-uniform in [min_area, max_area] with an aspect ratio uniform in
+    [1, 16/9], and a splat count log-uniform in [min_splats, max_splats].
+    Returns (width, height, splats, seed) per instance."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        area = math.exp(rng.uniform(math.log(min_area), math.log(max_area)))
+        aspect = rng.uniform(1.0, 16.0 / 9.0)
+        w = max(16, int(round(math.sqrt(area * aspect))))
+        h = max(16, int(round(area / w)))
+        n = int(round(math.exp(rng.uniform(math.log(min_splats), math.log(max_splats)))))
+        out.append((w, h, n, seed + i))
+    return out
+
+
 class SplatRenderer:
     """ds_renderer: the reference's SplatRenderer (forward only, SH degree 0-3)."""
 

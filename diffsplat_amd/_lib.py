@@ -156,6 +156,12 @@ _SIGS = {
                                      C.POINTER(C.c_void_p)]),
     "ds_renderer_destroy": (C.c_int, [_P]),
     "ds_renderer_upload_params": (C.c_int, [_P, _F]),
+    "ds_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+    "ds_batch_destroy": (C.c_int, [_P]),
+    "ds_batch_step": (C.c_int, [_P, C.c_int]),
+    "ds_batch_synchronize": (C.c_int, [_P]),
+    "ds_batch_set_streams": (C.c_int, [_P, C.c_int]),
+    "ds_batch_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), _F]),
     "ds_renderer_render": (C.c_int, [_P, _F, _F, _F]),
     "ds_renderer_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), _F]),
     "ds_renderer_download_image": (C.c_int, [_P, _F]),

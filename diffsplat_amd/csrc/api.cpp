@@ -521,6 +521,76 @@ ds_status ds_fitter_views(ds_fitter *f, ds_gauss3d *params, ds_splat2d *splat2d,
     DS_GUARD_END
 }
 
+// ------------------------------------------------------------------ batch
+
+struct ds_batch {
+    dsh::BatchFitter impl;
+};
+
+ds_status ds_batch_create(ds_fitter *const *fitters, int n, ds_batch **out) {
+    clear_error();
+    DS_REQUIRE(fitters && out, "NULL argument");
+    DS_REQUIRE(n >= 1, "a batch needs at least one fitter");
+    for (int i = 0; i < n; ++i) DS_REQUIRE(fitters[i], "NULL fitter in the batch");
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) DS_REQUIRE(fitters[i] != fitters[j], "fitter listed twice");
+    DS_GUARD_BEGIN
+    auto b = std::make_unique<ds_batch>();
+    std::vector<dsh::ImageFitter *> v((size_t)n);
+    for (int i = 0; i < n; ++i) v[(size_t)i] = &fitters[i]->impl;
+    b->impl.init(std::move(v));
+    *out = b.release();
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_batch_destroy(ds_batch *b) {
+    clear_error();
+    DS_GUARD_BEGIN
+    delete b;
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_batch_step(ds_batch *b, int n_steps) {
+    clear_error();
+    DS_REQUIRE(b, "NULL batch");
+    DS_REQUIRE(n_steps >= 0, "n_steps < 0");
+    DS_GUARD_BEGIN
+    b->impl.step(n_steps);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_batch_synchronize(ds_batch *b) {
+    clear_error();
+    DS_REQUIRE(b, "NULL batch");
+    DS_GUARD_BEGIN
+    b->impl.synchronize();
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_batch_set_streams(ds_batch *b, int n_streams) {
+    clear_error();
+    DS_REQUIRE(b, "NULL batch");
+    DS_REQUIRE(n_streams >= 1 && n_streams <= 64, "n_streams must be in [1, 64]");
+    DS_GUARD_BEGIN
+    b->impl.setStreams(n_streams);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_batch_autotune(ds_batch *b, int reps, int *n_streams, float *ms_per_batch_step) {
+    clear_error();
+    DS_REQUIRE(b, "NULL batch");
+    DS_GUARD_BEGIN
+    const int ns = b->impl.autotune(reps, ms_per_batch_step);
+    if (n_streams) *n_streams = ns;
+    return DS_OK;
+    DS_GUARD_END
+}
+
 // ------------------------------------------------------------------ renderer
 
 struct ds_renderer {

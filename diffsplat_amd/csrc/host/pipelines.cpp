@@ -1,7 +1,8 @@
-// ImageFitter / SplatRenderer implementation (see pipelines.hpp).
+// ImageFitter / BatchFitter / SplatRenderer implementation (see pipelines.hpp).
 #include "pipelines.hpp"
 
 #include <algorithm>
+#include <chrono>
 
 namespace dsh {
 
@@ -27,9 +28,24 @@ static int pick_with_margin(const float t[3], int current_ppt) {
 ImageFitter::ImageFitter() = default;
 
 ImageFitter::~ImageFitter() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    dropGraph();
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+}
+
+void ImageFitter::dropGraph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
-    if (stream) (void)hipStreamDestroy(stream);
+    graph_exec = nullptr;
+    graph = nullptr;
+}
+
+void ImageFitter::bindStream(hipStream_t s) {
+    if (stream) hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    stream = s ? s : own_stream;
+    Layer *all[5] = {&atv_layer, &psp_layer, &gp_layer, &ras_layer, &mse_layer};
+    for (Layer *l : all) l->setStream(stream);
+    dropGraph();  // the captured step launches on the old stream
 }
 
 void ImageFitter::referenceCamera(int w, int h, float view[16], float proj[16]) {
@@ -44,7 +60,8 @@ void ImageFitter::init(const FitterOptions &o) {
         throw std::runtime_error("[diffsplat] ImageFitter::init: bad configuration");
     opt = o;
     fused = o.fused;
-    if (!stream) hip_check(hipStreamCreate(&stream), "hipStreamCreate");
+    if (!own_stream) hip_check(hipStreamCreate(&own_stream), "hipStreamCreate");
+    if (!stream) stream = own_stream;
     gaussian_params.allocate(o.count, o.sh_degree);
     d_target_pixels.allocate((size_t)o.width * o.height * 3);
     d_target_pixels.zero(stream);
@@ -208,11 +225,7 @@ void ImageFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd
     check(ds_raster_set_tuning(r, bf, bb), "autotune");
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    // the captured step (if any) used the old launch shapes
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    graph_exec = nullptr;
-    graph = nullptr;
+    dropGraph();  // the captured step (if any) used the old launch shapes
     if (best_fwd) *best_fwd = bf;
     if (best_bwd) *best_bwd = bb;
     if (ms_fwd) *ms_fwd = tf;
@@ -227,6 +240,98 @@ void ImageFitter::views(ds_gauss3d *params, ds_splat2d *s2, ds_splat2d *gs2, flo
 }
 
 float ImageFitter::getLoss() { return mse_layer.getLoss(); }
+
+// ============================================================ BatchFitter
+
+BatchFitter::~BatchFitter() { releaseStreams(); }
+
+void BatchFitter::releaseStreams() {
+    for (ImageFitter *f : fit) f->bindStream(nullptr);  // synchronises the shared streams
+    for (hipStream_t s : streams) (void)hipStreamDestroy(s);
+    streams.clear();
+}
+
+void BatchFitter::init(std::vector<ImageFitter *> fitters) {
+    releaseStreams();
+    fit = std::move(fitters);
+    step_ms.assign(fit.size(), 1.f);
+    for (ImageFitter *f : fit) f->setUseGraph(true);
+    setStreams(1);
+}
+
+void BatchFitter::setStreams(int n_streams) {
+    if (n_streams < 1) n_streams = 1;
+    if (n_streams > (int)fit.size()) n_streams = std::max(1, (int)fit.size());
+    releaseStreams();
+    streams.resize((size_t)n_streams);
+    for (hipStream_t &s : streams) hip_check(hipStreamCreate(&s), "hipStreamCreate");
+    // longest step first, each to the stream with the least work so far
+    std::vector<int> order(fit.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return step_ms[(size_t)a] > step_ms[(size_t)b]; });
+    std::vector<float> load((size_t)n_streams, 0.f);
+    for (int i : order) {
+        const size_t s = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[s] += step_ms[(size_t)i];
+        fit[(size_t)i]->bindStream(streams[s]);
+    }
+}
+
+void BatchFitter::step(int n) {
+    for (int k = 0; k < n; ++k)
+        for (ImageFitter *f : fit) f->step();
+}
+
+void BatchFitter::synchronize() {
+    for (hipStream_t s : streams) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+int BatchFitter::autotune(int reps, float *ms_per_batch_step) {
+    if (reps < 1) reps = 1;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    // each instance alone on its own stream: launch shapes, then its step time
+    for (size_t i = 0; i < fit.size(); ++i) {
+        ImageFitter *f = fit[i];
+        f->bindStream(nullptr);
+        f->forwardOnly();
+        f->autotune(reps, nullptr, nullptr, nullptr, nullptr);
+        f->step();  // capture
+        hip_check(hipEventRecord(e0, f->getStream()), "hipEventRecord");
+        for (int k = 0; k < reps; ++k) f->step();
+        hip_check(hipEventRecord(e1, f->getStream()), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        step_ms[i] = ms / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // stream count: wall time of `reps` batch steps
+    int best = 1;
+    float tbest = 1e30f;
+    for (int ns : {1, 2, 4, 8}) {
+        if (ns > 1 && ns > (int)fit.size()) break;
+        setStreams(ns);
+        step(1);  // captures on the new streams
+        synchronize();
+        const auto t0 = std::chrono::steady_clock::now();
+        step(reps);
+        synchronize();
+        const float ms =
+            std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+            reps;
+        if (ms < 0.95f * tbest) {
+            tbest = ms;
+            best = ns;
+        }
+    }
+    setStreams(best);
+    if (ms_per_batch_step) *ms_per_batch_step = tbest;
+    return best;
+}
 
 // ============================================================ SplatRenderer
 

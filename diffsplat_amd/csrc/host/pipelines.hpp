@@ -44,6 +44,10 @@ public:
     void step();          // zeroGrad, forward, backward, Adam
     void forwardOnly();   // forward (render + loss)
     void setUseGraph(bool on);
+    // Runs the fitter's work on stream s from now on (nullptr: its own
+    // stream). Synchronises the old stream; a captured step is dropped and
+    // captured again on the next step.
+    void bindStream(hipStream_t s);
     // Times the compositing launch shapes on the current parameters (forward
     // and backward only, no Adam update) and keeps the fastest of each.
     void autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd, float *ms_bwd);
@@ -59,9 +63,11 @@ public:
 
 private:
     void stepOnce();
+    void dropGraph();
     FitterOptions opt;
     bool fused = true;
-    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;  // created by init, destroyed with the fitter
+    hipStream_t stream = nullptr;      // the stream work goes to (own_stream unless bound)
     Gaussian3DParams gaussian_params;
     Pipeline pipeline;
     GaussActivLayer atv_layer;
@@ -74,6 +80,39 @@ private:
     bool use_graph = false;
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
+};
+
+// Many independent fits stepped together (no counterpart in the reference):
+// each instance replays its own captured step, the instances are spread over
+// a few streams (largest measured step time first, each to the least-loaded
+// stream) so that small fits overlap on the GPU's hardware queues. The
+// fitters are borrowed and must outlive the batch.
+class BatchFitter {
+public:
+    BatchFitter() = default;
+    ~BatchFitter();
+    BatchFitter(const BatchFitter &) = delete;
+    BatchFitter &operator=(const BatchFitter &) = delete;
+
+    void init(std::vector<ImageFitter *> fitters);
+    // Spreads the instances over n_streams streams using the last measured
+    // step times (equal weights before any measurement).
+    void setStreams(int n_streams);
+    int numStreams() const { return (int)streams.size(); }
+    void step(int n);   // n steps of every instance (asynchronous)
+    void synchronize();
+    // Tunes each instance's compositing launch shapes, measures each
+    // instance's step time, then times `reps` batch steps for 1, 2, 4 and 8
+    // streams and keeps the fastest. Takes real optimisation steps.
+    int autotune(int reps, float *ms_per_batch_step);
+    int size() const { return (int)fit.size(); }
+    ImageFitter *fitter(int i) { return fit[(size_t)i]; }
+
+private:
+    void releaseStreams();
+    std::vector<ImageFitter *> fit;
+    std::vector<hipStream_t> streams;
+    std::vector<float> step_ms;  // measured step time per instance
 };
 
 class SplatRenderer {

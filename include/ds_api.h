@@ -367,6 +367,30 @@ ds_status ds_fitter_views(ds_fitter *f, ds_gauss3d *params, ds_splat2d *splat2d,
                           ds_splat2d *grad_splat2d, float **grad_rgb);
 
 /* ------------------------------------------------------------------ */
+/* Batched fitting: many independent fitters stepped together (no        */
+/* counterpart in the reference, which fits one image per process)       */
+/* ------------------------------------------------------------------ */
+
+/* Each instance replays its own captured step (graphs are switched on);
+ * instances are spread over a few streams, largest measured step first, each
+ * to the least-loaded stream, so that small fits overlap on the GPU. The
+ * fitters are borrowed: destroy the batch before them. While in a batch a
+ * fitter's other calls still work (they run on its batch stream). */
+typedef struct ds_batch ds_batch;
+
+ds_status ds_batch_create(ds_fitter *const *fitters, int n, ds_batch **out);
+ds_status ds_batch_destroy(ds_batch *b);
+/* n_steps steps of every instance. Asynchronous. */
+ds_status ds_batch_step(ds_batch *b, int n_steps);
+ds_status ds_batch_synchronize(ds_batch *b);
+ds_status ds_batch_set_streams(ds_batch *b, int n_streams);
+/* Tunes every instance's compositing launch shapes, measures every
+ * instance's step time, then times `reps` batch steps with 1, 2, 4 and 8
+ * streams and keeps the fastest (it takes real optimisation steps).
+ * Synchronises. Either output may be NULL. */
+ds_status ds_batch_autotune(ds_batch *b, int reps, int *n_streams, float *ms_per_batch_step);
+
+/* ------------------------------------------------------------------ */
 /* Forward-only renderer (SplatRenderer, src/pipelines/splat_renderer.{h,cpp}) */
 /* ------------------------------------------------------------------ */
 

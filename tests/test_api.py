@@ -72,6 +72,14 @@ def test_invalid_arguments_are_rejected_before_any_device_call():
     L.ds_fitter_default_config(32, 32, 4, C.byref(fc))
     fc.sh_degree = 5
     assert L.ds_fitter_create(C.byref(fc), C.byref(h)) == 1
+    fake = (C.c_void_p * 2)(0x1000, 0x1000)  # never dereferenced: rejected first
+    assert L.ds_batch_create(fake, 0, C.byref(h)) == 1
+    assert L.ds_batch_create(fake, 2, C.byref(h)) == 1
+    assert b"twice" in L.ds_last_error()
+    nulls = (C.c_void_p * 1)(None)
+    assert L.ds_batch_create(nulls, 1, C.byref(h)) == 1
+    assert L.ds_batch_step(None, 1) == 1
+    assert L.ds_renderer_autotune(None, 3, None, None) == 1
     with pytest.raises(ds.DiffsplatError):
         ds.check(L.ds_raster_create(None, C.byref(h)), "ds_raster_create")
 
