@@ -117,18 +117,22 @@ def roofline_bwd(ds, oracle, fit, W, H, N, reps=20):
                     "(which counts packed FMA); the kernel is VALU-issue-bound"}
 
 
-def time_to_psnr(ds, target_db=30.0, W=256, H=256, N=4096, n_ell=32, seed=0, max_steps=6000,
+TTP_POS_DECAY = 0.998  # position lr schedule of the time-to-PSNR fits
+
+
+def time_to_psnr(ds, target_db=30.0, W=256, H=256, N=4096, n_ell=32, seed=0, max_steps=3000,
                  check_every=10):
     """Wall time for the plain fitter (random init in the pinhole frustum) to
     reach target_db PSNR on the 256x256 / 4096-splat workload, HIP graph on
     (its capture on the first step is inside the timed region). The loss of
     the last step's forward is read every check_every steps (one host
-    synchronisation)."""
+    synchronisation). The position learning rate decays by TTP_POS_DECAY per
+    step: at a constant rate the fit oscillates below 30 dB."""
     target = ds.target_image(seed, W, H, n_ell)
     params = ds.frustum_splats(seed, N, aspect=W / H)
     view, proj, cam = ds.fitter_camera(W, H)
     lr = {"lr_pos": 1e-2, "lr_sh_dc": 1e-2, "lr_scale": 5e-3, "lr_rot": 6e-4,
-          "lr_opacity": 6e-3}
+          "lr_opacity": 6e-3, "lr_pos_decay": TTP_POS_DECAY}
     fit = ds.ImageFitter(W, H, N, view=view, proj=proj, cam_pos=cam, lr=lr)
     fit.set_target(target)
     fit.upload_params(params)
@@ -146,7 +150,8 @@ def time_to_psnr(ds, target_db=30.0, W=256, H=256, N=4096, n_ell=32, seed=0, max
     fit.close()
     return {"workload": f"plain fitter {W}x{H}, {N} random-init splats", "target_db": target_db,
             "reached": psnr >= target_db, "steps": steps, "seconds": round(el, 4),
-            "psnr_db": round(psnr, 3), "init": "plain (random splats in the frustum)"}
+            "psnr_db": round(psnr, 3), "init": "plain (random splats in the frustum)",
+            "lr_pos_decay": TTP_POS_DECAY}
 
 
 def batched_fit(ds, count=64, steps=500, seq_steps=10):

@@ -64,7 +64,8 @@ class RasterView(C.Structure):
 
 class AdamGroup(C.Structure):
     _fields_ = [("param", C.c_void_p), ("grad", C.c_void_p), ("m", C.c_void_p),
-                ("v", C.c_void_p), ("n", C.c_int64), ("lr", C.c_float)]
+                ("v", C.c_void_p), ("n", C.c_int64), ("lr", C.c_float),
+                ("decay", C.c_float)]
 
 
 class AdamHParams(C.Structure):
@@ -78,7 +79,7 @@ class FitterConfig(C.Structure):
                 ("lr_rot", C.c_float), ("lr_sh_dc", C.c_float), ("lr_sh_rest", C.c_float),
                 ("lr_opacity", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
                 ("epsilon", C.c_float), ("isect_cap", C.c_int64), ("fused", C.c_int32),
-                ("grad_mode", C.c_int32)]
+                ("grad_mode", C.c_int32), ("lr_pos_decay", C.c_float)]
 
 
 _P = C.c_void_p

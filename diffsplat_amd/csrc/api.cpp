@@ -348,6 +348,7 @@ void ds_fitter_default_config(int width, int height, int count, ds_fitter_config
     cfg->isect_cap = 0;
     cfg->fused = 1;
     cfg->grad_mode = DS_GRAD_EXACT;
+    cfg->lr_pos_decay = 1.f;
 }
 
 ds_status ds_fitter_create(const ds_fitter_config *cfg, ds_fitter **out) {
@@ -371,6 +372,7 @@ ds_status ds_fitter_create(const ds_fitter_config *cfg, ds_fitter **out) {
     o.lr.lr_sh_dc = cfg->lr_sh_dc;
     o.lr.lr_sh_rest = cfg->lr_sh_rest;
     o.lr.lr_opacity = cfg->lr_opacity;
+    o.lr.lr_pos_decay = cfg->lr_pos_decay > 0.f ? cfg->lr_pos_decay : 1.f;
     o.lr.adam.beta1 = cfg->beta1;
     o.lr.adam.beta2 = cfg->beta2;
     o.lr.adam.epsilon = cfg->epsilon;

@@ -287,6 +287,7 @@ struct AdamGroup {
     float lr;
     float *m;  // views into the optimizer's moment slabs
     float *v;
+    float decay;  // lr * decay^(t-1); 1: constant (the reference)
 };
 
 // One launch per step over every group; the step counter lives on the
@@ -294,11 +295,12 @@ struct AdamGroup {
 class Adam {
 public:
     void init(const AdamParams &p = {}) { hp = p; }
-    void addGroup(float *param, const float *grad, int64_t n, float lr) {
-        groups.push_back({param, grad, n, lr, nullptr, nullptr});
+    void addGroup(float *param, const float *grad, int64_t n, float lr, float decay = 1.f) {
+        groups.push_back({param, grad, n, lr, nullptr, nullptr, decay});
         built = false;
     }
     void setLr(int idx, float lr) { groups[idx].lr = lr; }
+    void setDecay(int idx, float decay) { groups[idx].decay = decay; }
     // Allocates the moment slabs and zeroes them and the step counter.
     void reset(hipStream_t s = nullptr) {
         int64_t total = 0;
@@ -322,7 +324,7 @@ public:
         if (!built) reset(s);
         std::vector<ds_adam_group> tab;
         tab.reserve(groups.size());
-        for (auto &g : groups) tab.push_back({g.param, g.grad, g.m, g.v, g.n, g.lr});
+        for (auto &g : groups) tab.push_back({g.param, g.grad, g.m, g.v, g.n, g.lr, g.decay});
         ds_adam_hparams h{hp.beta1, hp.beta2, hp.epsilon};
         check(ds_adam_step_dev(tab.data(), (int)tab.size(), &h, step_dev.ptr, s), "Adam::step");
         ++step_count;
@@ -347,15 +349,16 @@ struct Gaussian3DAdamConfig {
     float lr_sh_dc = 6.0e-3f;
     float lr_sh_rest = 6.0e-3f;
     float lr_opacity = 6.0e-3f;
+    float lr_pos_decay = 1.f;  // per-step position lr schedule (not in the reference)
     AdamParams adam = {};
 };
 
 inline void registerGaussian3DGroups(Adam &adam, Gaussian3DParams &p, Gaussian3DGrads &g,
                                      const Gaussian3DAdamConfig &cfg = {}) {
     const int64_t n = p.count;
-    adam.addGroup(p.pos_x, g.pos_x, n, cfg.lr_pos);
-    adam.addGroup(p.pos_y, g.pos_y, n, cfg.lr_pos);
-    adam.addGroup(p.pos_z, g.pos_z, n, cfg.lr_pos);
+    adam.addGroup(p.pos_x, g.pos_x, n, cfg.lr_pos, cfg.lr_pos_decay);
+    adam.addGroup(p.pos_y, g.pos_y, n, cfg.lr_pos, cfg.lr_pos_decay);
+    adam.addGroup(p.pos_z, g.pos_z, n, cfg.lr_pos, cfg.lr_pos_decay);
     adam.addGroup(p.scale_x, g.scale_x, n, cfg.lr_scale);
     adam.addGroup(p.scale_y, g.scale_y, n, cfg.lr_scale);
     adam.addGroup(p.scale_z, g.scale_z, n, cfg.lr_scale);

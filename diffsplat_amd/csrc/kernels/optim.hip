@@ -55,6 +55,7 @@ struct AdamTable {
     float *v[DS_ADAM_MAX_GROUPS];
     int64_t n[DS_ADAM_MAX_GROUPS];
     float lr[DS_ADAM_MAX_GROUPS];
+    float decay[DS_ADAM_MAX_GROUPS];  // lr * decay^(t-1); 1: constant
     int count;
 };
 
@@ -74,7 +75,8 @@ __global__ __launch_bounds__(kBlock) void k_adam(AdamTable tab, float beta1, flo
         const float *__restrict__ G = tab.grad[k];
         float *__restrict__ M = tab.m[k];
         float *__restrict__ V = tab.v[k];
-        const float lr = tab.lr[k];
+        const float d = tab.decay[k];
+        const float lr = d == 1.f ? tab.lr[k] : tab.lr[k] * powf(d, (float)(t - 1));
         const int64_t n = tab.n[k];
         for (int64_t i = gtid(); i < n; i += gstride()) {
             const float g = G[i];
@@ -132,6 +134,7 @@ ds_status launch_adam(const ds_adam_group *groups, int n_groups, const ds_adam_h
         tab.v[k] = groups[k].v;
         tab.n[k] = groups[k].n;
         tab.lr[k] = groups[k].lr;
+        tab.decay[k] = (groups[k].decay > 0.f) ? groups[k].decay : 1.f;
         total += groups[k].n;
         if (groups[k].n > largest) largest = groups[k].n;
     }

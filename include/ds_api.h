@@ -286,6 +286,9 @@ typedef struct ds_adam_group {
     float *v;
     int64_t n;
     float lr;
+    /* Exponential schedule (not in the reference): step t uses
+     * lr * decay^(t-1). 0 or 1 keeps lr constant, as the reference does. */
+    float decay;
 } ds_adam_group;
 
 typedef struct ds_adam_hparams {
@@ -320,6 +323,8 @@ typedef struct ds_fitter_config {
     int64_t isect_cap;             /* 0 = 1<<25                                        */
     int32_t fused;                 /* 1: fused activation+projection and fused MSE     */
     int32_t grad_mode;             /* DS_GRAD_EXACT or DS_GRAD_REFERENCE               */
+    float lr_pos_decay;            /* position lr schedule per step (ds_adam_group.decay);
+                                      1 = constant as the reference                    */
 } ds_fitter_config;
 
 /* Fills cfg with the reference's defaults (src/pipelines/image_fitter.cpp:18-34,

@@ -97,7 +97,7 @@ def test_fitter_defaults_are_the_references():
     zr = 360 * 0.25
     want = np.array(ds.ortho(-640, 640, -360, 360, -zr, zr))
     np.testing.assert_allclose(np.array(fc.proj), want)
-    assert fc.fused == 1 and fc.grad_mode == 0
+    assert fc.fused == 1 and fc.grad_mode == 0 and fc.lr_pos_decay == 1.0
 
 
 def test_camera_matrices_follow_opengl_conventions():

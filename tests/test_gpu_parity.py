@@ -228,17 +228,18 @@ def test_mse_and_adam():
     report("mse", loss=loss, loss_ref=loss_ref)
     assert abs(loss - loss_ref) <= 1e-5 + 1e-4 * loss_ref
     check_close("mse_grad", dg.download(np.float32, 3 * npix), g_ref)
-    # Adam: 3 groups with different lr, 5 steps, device step counter
-    sizes, lrs = [1000, 77777, 5], [1e-2, 5e-3, 6e-4]
+    # Adam: 3 groups with different lr and lr schedules (decay 0 = constant),
+    # 5 steps, device step counter
+    sizes, lrs, decays = [1000, 77777, 5], [1e-2, 5e-3, 6e-4], [1.0, 0.9, 0.0]
     groups, host = [], []
-    for sz, lr in zip(sizes, lrs):
+    for sz, lr, dec in zip(sizes, lrs, decays):
         arr = [rng.normal(0, 1, sz).astype(np.float32) for _ in range(2)]
         dev = [ds.DeviceArray.from_numpy(a) for a in arr] + [ds.DeviceArray(4 * sz)
                                                            for _ in range(2)]
         dev[2].zero()
         dev[3].zero()
-        host.append((arr, dev, lr))
-        groups.append(ds.AdamGroup(dev[0].ptr, dev[1].ptr, dev[2].ptr, dev[3].ptr, sz, lr))
+        host.append((arr, dev, lr, dec))
+        groups.append(ds.AdamGroup(dev[0].ptr, dev[1].ptr, dev[2].ptr, dev[3].ptr, sz, lr, dec))
     tab = (ds.AdamGroup * len(groups))(*groups)
     hp = ds.AdamHParams(0.9, 0.999, 1e-8)
     step = ds.DeviceArray(4)
@@ -246,10 +247,11 @@ def test_mse_and_adam():
     for _ in range(5):
         ds.check(ds.lib().ds_adam_step_dev(tab, len(groups), hp, step.ptr, None), "adam")
     assert int(step.download(np.int32, 1)[0]) == 5
-    for (p0, g), dev, lr in host:
+    for (p0, g), dev, lr, dec in host:
         rp, rm, rv = p0.astype(np.float64), np.zeros(len(p0)), np.zeros(len(p0))
         for t_ in range(1, 6):
-            rp, rm, rv = oracle.adam(rp, g, rm, rv, lr, 0.9, 0.999, 1e-8, t_)
+            lr_t = lr * (dec ** (t_ - 1) if dec > 0 else 1.0)
+            rp, rm, rv = oracle.adam(rp, g, rm, rv, lr_t, 0.9, 0.999, 1e-8, t_)
         check_close(f"adam_params_{len(p0)}", dev[0].download(np.float32, len(p0)), rp)
         check_close(f"adam_m_{len(p0)}", dev[2].download(np.float32, len(p0)), rm)
         check_close(f"adam_v_{len(p0)}", dev[3].download(np.float32, len(p0)), rv)
