@@ -241,6 +241,21 @@ class ImageFitter:
         check(lib().ds_fitter_download_image(self.h, fptr(out)), "ds_fitter_download_image")
         return out
 
+    def param_slabs(self):
+        """Device pointers (parameter slab, parameter-gradient slab)."""
+        p, g = C.c_void_p(), C.c_void_p()
+        check(lib().ds_fitter_param_slabs(self.h, C.byref(p), C.byref(g)), "ds_fitter_param_slabs")
+        return p.value, g.value
+
+    def set_target_device(self, ptr) -> None:
+        check(lib().ds_fitter_set_target_device(self.h, ptr), "ds_fitter_set_target_device")
+
+    def reset_optimizer(self) -> None:
+        check(lib().ds_fitter_reset_optimizer(self.h), "ds_fitter_reset_optimizer")
+
+    def forward_backward(self) -> None:
+        check(lib().ds_fitter_forward_backward(self.h), "ds_fitter_forward_backward")
+
     def set_graph(self, on: bool) -> None:
         check(lib().ds_fitter_set_graph(self.h, int(on)), "ds_fitter_set_graph")
 
@@ -276,6 +291,62 @@ class ImageFitter:
     def close(self):
         if self.h:
             lib().ds_fitter_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class InitNet:
+    """ds_initnet: the learned initializer (a bf16 MFMA CNN producing one SH-0
+    splat per 4x4-pixel cell), trained end to end through the renderer."""
+
+    def __init__(self, width: int, height: int, fovy: float = math.radians(60.0), seed: int = 0):
+        h = C.c_void_p()
+        check(lib().ds_initnet_create(width, height, fovy, seed, C.byref(h)), "ds_initnet_create")
+        self.h = h.value
+        self.width, self.height = width, height
+        npar, nspl = C.c_int64(), C.c_int32()
+        check(lib().ds_initnet_info(self.h, C.byref(npar), C.byref(nspl)), "ds_initnet_info")
+        self.num_params, self.num_splats = npar.value, nspl.value
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.num_params, np.float32)
+        check(lib().ds_initnet_get_params(self.h, fptr(out)), "ds_initnet_get_params")
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32).reshape(-1)
+        assert p.size == self.num_params
+        check(lib().ds_initnet_set_params(self.h, fptr(p)), "ds_initnet_set_params")
+
+    def infer(self, image, slab, stream=None) -> None:
+        """image, slab: DeviceArray or device pointers."""
+        ip = getattr(image, "ptr", image)
+        sp = getattr(slab, "ptr", slab)
+        check(lib().ds_initnet_infer(self.h, ip, sp, stream), "ds_initnet_infer")
+
+    def head(self) -> np.ndarray:
+        out = np.empty(self.num_splats * 16, np.float32)
+        check(lib().ds_initnet_head(self.h, fptr(out)), "ds_initnet_head")
+        return out
+
+    def grads(self, image: "DeviceArray", grad_slab: "DeviceArray") -> np.ndarray:
+        out = np.empty(self.num_params, np.float32)
+        check(lib().ds_initnet_grads(self.h, image.ptr, grad_slab.ptr, fptr(out)),
+              "ds_initnet_grads")
+        return out
+
+    def train_step(self, fitter: "ImageFitter", image: "DeviceArray", lr: float) -> None:
+        check(lib().ds_initnet_train_step(self.h, fitter.h, image.ptr, lr),
+              "ds_initnet_train_step")
+
+    def close(self):
+        if self.h:
+            lib().ds_initnet_destroy(self.h)
             self.h = None
 
     def __del__(self):

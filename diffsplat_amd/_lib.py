@@ -157,6 +157,20 @@ _SIGS = {
                                      C.POINTER(C.c_void_p)]),
     "ds_renderer_destroy": (C.c_int, [_P]),
     "ds_renderer_upload_params": (C.c_int, [_P, _F]),
+    "ds_initnet_create": (C.c_int, [C.c_int, C.c_int, C.c_float, C.c_uint64,
+                                    C.POINTER(C.c_void_p)]),
+    "ds_initnet_destroy": (C.c_int, [_P]),
+    "ds_initnet_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "ds_initnet_get_params": (C.c_int, [_P, _F]),
+    "ds_initnet_set_params": (C.c_int, [_P, _F]),
+    "ds_initnet_infer": (C.c_int, [_P, _P, _P, _P]),
+    "ds_initnet_head": (C.c_int, [_P, _F]),
+    "ds_initnet_grads": (C.c_int, [_P, _P, _P, _F]),
+    "ds_initnet_train_step": (C.c_int, [_P, _P, _P, C.c_float]),
+    "ds_fitter_forward_backward": (C.c_int, [_P]),
+    "ds_fitter_param_slabs": (C.c_int, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "ds_fitter_set_target_device": (C.c_int, [_P, _P]),
+    "ds_fitter_reset_optimizer": (C.c_int, [_P]),
     "ds_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
     "ds_batch_destroy": (C.c_int, [_P]),
     "ds_batch_step": (C.c_int, [_P, C.c_int]),

@@ -523,6 +523,43 @@ ds_status ds_fitter_views(ds_fitter *f, ds_gauss3d *params, ds_splat2d *splat2d,
     DS_GUARD_END
 }
 
+ds_status ds_fitter_forward_backward(ds_fitter *f) {
+    clear_error();
+    DS_REQUIRE(f, "NULL fitter");
+    DS_GUARD_BEGIN
+    f->impl.forwardBackward();
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_param_slabs(ds_fitter *f, float **params_dev, const float **grads_dev) {
+    clear_error();
+    DS_REQUIRE(f, "NULL fitter");
+    DS_GUARD_BEGIN
+    if (params_dev) *params_dev = f->impl.paramSlab();
+    if (grads_dev) *grads_dev = f->impl.gradSlab();
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_set_target_device(ds_fitter *f, const float *dev_rgb) {
+    clear_error();
+    DS_REQUIRE(f && dev_rgb, "NULL argument");
+    DS_GUARD_BEGIN
+    f->impl.setTargetDevice(dev_rgb);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_reset_optimizer(ds_fitter *f) {
+    clear_error();
+    DS_REQUIRE(f, "NULL fitter");
+    DS_GUARD_BEGIN
+    f->impl.resetOptimizer();
+    return DS_OK;
+    DS_GUARD_END
+}
+
 // ------------------------------------------------------------------ batch
 
 struct ds_batch {

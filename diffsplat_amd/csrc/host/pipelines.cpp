@@ -187,6 +187,28 @@ void ImageFitter::step() {
 
 void ImageFitter::forwardOnly() { pipeline.forward(); }
 
+void ImageFitter::forwardBackward() {
+    pipeline.zeroGrad();
+    pipeline.forward();
+    pipeline.backward();
+}
+
+void ImageFitter::setTargetDevice(const float *dev_rgb) {
+    hip_check(hipMemcpyAsync(d_target_pixels.ptr, dev_rgb,
+                             sizeof(float) * (size_t)opt.width * opt.height * 3,
+                             hipMemcpyDeviceToDevice, stream),
+              "hipMemcpyAsync");
+}
+
+void ImageFitter::resetOptimizer() {
+    optimizer.reset(stream);
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+const float *ImageFitter::gradSlab() {
+    return fused ? gp_layer.getGradInput().slab.ptr : atv_layer.getGradInput().slab.ptr;
+}
+
 void ImageFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd,
                            float *ms_bwd) {
     if (reps < 1) reps = 1;

@@ -43,6 +43,11 @@ public:
 
     void step();          // zeroGrad, forward, backward, Adam
     void forwardOnly();   // forward (render + loss)
+    void forwardBackward();  // zeroGrad, forward, backward (no Adam update)
+    void setTargetDevice(const float *dev_rgb);  // device-to-device, on the fitter's stream
+    void resetOptimizer();  // Adam moments and step counter to zero, parameters kept
+    float *paramSlab() { return gaussian_params.slab.ptr; }
+    const float *gradSlab();
     void setUseGraph(bool on);
     // Runs the fitter's work on stream s from now on (nullptr: its own
     // stream). Synchronises the old stream; a captured step is dropped and

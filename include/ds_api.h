@@ -371,6 +371,63 @@ ds_status ds_fitter_step_count(ds_fitter *f, int32_t *step);
 ds_status ds_fitter_views(ds_fitter *f, ds_gauss3d *params, ds_splat2d *splat2d,
                           ds_splat2d *grad_splat2d, float **grad_rgb);
 
+/* Forward and backward of one step without the Adam update (zeroGrad,
+ * forward, backward), asynchronous: the parameter gradients are then in the
+ * gradient slab. For callers that own the optimisation, such as the learned
+ * initializer's training. No counterpart in the reference. */
+ds_status ds_fitter_forward_backward(ds_fitter *f);
+/* Device pointers to the parameter slab and the parameter-gradient slab
+ * (layout of ds_fitter_upload_params), valid for the fitter's life. Writes
+ * to the parameter slab must be ordered with the fitter's stream. */
+ds_status ds_fitter_param_slabs(ds_fitter *f, float **params_dev, const float **grads_dev);
+/* Target image from device memory [H*W*3], copied on the fitter's stream. */
+ds_status ds_fitter_set_target_device(ds_fitter *f, const float *dev_rgb);
+/* Adam moments and step counter to zero; the parameters are kept
+ * (synchronises). */
+ds_status ds_fitter_reset_optimizer(ds_fitter *f);
+
+/* ------------------------------------------------------------------ */
+/* Learned initializer (no counterpart in the reference, which starts    */
+/* every fit from random splats, src/utils/splat_utils.h:10-51)           */
+/* ------------------------------------------------------------------ */
+
+/* A small CNN on bf16 MFMA (conv3x3 32 -> conv3x3/2 32 -> conv3x3/2 64 ->
+ * conv3x3 64 -> conv1x1 16, ReLU between) maps a W x H target image to one
+ * SH-0 splat per 4x4-pixel cell ((W/4)*(H/4) splats) for the pinhole camera
+ * at the origin looking down -z with vertical field of view `fovy`
+ * (radians): each splat sits on the plane z = -4 within 4 pixels of its
+ * cell, with learned anisotropic scale, in-plane rotation, colour and
+ * opacity. It is trained end to end through the differentiable renderer
+ * (loss: the fitter's MSE of its own render against the image). */
+typedef struct ds_initnet ds_initnet;
+
+/* W and H multiples of 4 in [16, 4096]; weights drawn from `seed`. */
+ds_status ds_initnet_create(int width, int height, float fovy, uint64_t seed, ds_initnet **out);
+ds_status ds_initnet_destroy(ds_initnet *n);
+/* Number of weights and biases (the ds_initnet_get_params layout) and of
+ * splats the net produces. Either output may be NULL. */
+ds_status ds_initnet_info(ds_initnet *n, int64_t *num_params, int32_t *num_splats);
+/* Weights and biases, host floats: per layer W[co][kh][kw][ci] then b[co].
+ * Setting them resets the net's optimizer. Both synchronise. */
+ds_status ds_initnet_get_params(ds_initnet *n, float *host);
+ds_status ds_initnet_set_params(ds_initnet *n, const float *host);
+/* Image (device, [H*W*3] in [0,1]) -> SH-0 parameter slab (device, the
+ * ds_fitter_upload_params layout for num_splats splats). Asynchronous. */
+ds_status ds_initnet_infer(ds_initnet *n, const float *dev_rgb, float *dev_slab, ds_stream_t s);
+/* Head values [num_splats][16] of the last forward, host (synchronises). */
+ds_status ds_initnet_head(ds_initnet *n, float *host_h);
+/* Gradients of all weights and biases (ds_initnet_get_params layout, host)
+ * for the image and a given gradient of the output slab (device).
+ * Synchronises; leaves the weights unchanged. */
+ds_status ds_initnet_grads(ds_initnet *n, const float *dev_rgb, const float *dev_grad_slab,
+                           float *host_grads);
+/* One training step on one image: inference into the fitter's parameters,
+ * the fitter's forward and backward against the same image, backpropagation
+ * into the net, one Adam step (lr) on its weights. The fitter (count =
+ * num_splats, SH degree 0, same camera and size) is only used to render; its
+ * own optimizer is not stepped. Asynchronous on the fitter's stream. */
+ds_status ds_initnet_train_step(ds_initnet *n, ds_fitter *f, const float *dev_rgb, float lr);
+
 /* ------------------------------------------------------------------ */
 /* Batched fitting: many independent fitters stepped together (no        */
 /* counterpart in the reference, which fits one image per process)       */
