@@ -10,7 +10,10 @@ Prints ONE JSON line. `roofline` is for the compositing backward (the
 dominant kernel), timed with HIP events on the fitter's stream at the state
 reached after the timed steps; its algorithmic work (splat evaluations) is
 counted by the oracle on the same state. `cpu_baseline` is the oracle's fp32
-OpenMP transcription of the same step on this host's cores.
+OpenMP transcription of the same step on this host's cores. Beside the line:
+`time_to_30db` (the metric's second half: wall time to 30 dB PSNR at
+256x256 / 4096 splats from the learned initializer vs from random splats),
+`viewer` (1M-splat 1080p frames/s) and `batched` (64 uneven fits at once).
 """
 from __future__ import annotations
 
@@ -61,6 +64,8 @@ def parse():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-viewer", action="store_true")
     ap.add_argument("--no-time-to-psnr", action="store_true")
+    ap.add_argument("--eval-images", type=int, default=64,
+                    help="held-out images of the time-to-30dB comparison")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (counter calibration)")
@@ -120,38 +125,100 @@ def roofline_bwd(ds, oracle, fit, W, H, N, reps=20):
 TTP_POS_DECAY = 0.998  # position lr schedule of the time-to-PSNR fits
 
 
-def time_to_psnr(ds, target_db=30.0, W=256, H=256, N=4096, n_ell=32, seed=0, max_steps=3000,
-                 check_every=10):
-    """Wall time for the plain fitter (random init in the pinhole frustum) to
-    reach target_db PSNR on the 256x256 / 4096-splat workload, HIP graph on
-    (its capture on the first step is inside the timed region). The loss of
-    the last step's forward is read every check_every steps (one host
-    synchronisation). The position learning rate decays by TTP_POS_DECAY per
-    step: at a constant rate the fit oscillates below 30 dB."""
-    target = ds.target_image(seed, W, H, n_ell)
-    params = ds.frustum_splats(seed, N, aspect=W / H)
+def fit_to_psnr(ds, net, image, d_image, seed, W, H, N, target_db=30.0, max_steps=3000,
+                check_every=10):
+    """Wall time for one fit of `image` to reach target_db PSNR: from the
+    random frustum init (net None) or from the learned initializer's output
+    (its inference is inside the timed region). HIP graph on (its capture on
+    the first step is inside the timed region too); the loss of the last
+    step's forward is read every check_every steps (one host sync). The
+    position learning rate decays by TTP_POS_DECAY per step: at a constant
+    rate the fit oscillates below 30 dB."""
     view, proj, cam = ds.fitter_camera(W, H)
     lr = {"lr_pos": 1e-2, "lr_sh_dc": 1e-2, "lr_scale": 5e-3, "lr_rot": 6e-4,
           "lr_opacity": 6e-3, "lr_pos_decay": TTP_POS_DECAY}
     fit = ds.ImageFitter(W, H, N, view=view, proj=proj, cam_pos=cam, lr=lr)
-    fit.set_target(target)
-    fit.upload_params(params)
+    fit.set_target(image)
+    fit.upload_params(ds.frustum_splats(seed, N, aspect=W / H))
     fit.set_graph(True)
     fit.synchronize()
     t0 = time.perf_counter()
+    if net is not None:
+        pslab, _ = fit.param_slabs()
+        net.infer(d_image, pslab, fit.stream())
+        fit.reset_optimizer()
     steps, psnr = 0, 0.0
     while steps < max_steps:
         fit.step(check_every)
         steps += check_every
-        psnr = ds.psnr_from_loss(fit.loss())  # loss of the last step's forward
+        psnr = ds.psnr_from_loss(fit.loss())
         if psnr >= target_db:
             break
     el = time.perf_counter() - t0
     fit.close()
-    return {"workload": f"plain fitter {W}x{H}, {N} random-init splats", "target_db": target_db,
-            "reached": psnr >= target_db, "steps": steps, "seconds": round(el, 4),
-            "psnr_db": round(psnr, 3), "init": "plain (random splats in the frustum)",
-            "lr_pos_decay": TTP_POS_DECAY}
+    return steps, el, psnr
+
+
+def learned_vs_plain(ds, n_train=10000, train_steps=10000, n_eval=64, W=256, H=256,
+                     target_db=30.0):
+    """Time to target_db on held-out synthetic images (seeds 90000+i), from
+    random init and from the learned initializer, both with the same fitter
+    settings. The initializer is first trained end to end through the
+    renderer on n_train synthetic images (seeds 4000+i), one image per step,
+    with a halving-every-3000-steps Adam rate from 2e-3."""
+    from concurrent.futures import ThreadPoolExecutor
+    N = (W // 4) * (H // 4)
+    img_floats = W * H * 3
+    t0 = time.perf_counter()
+    d_train = ds.DeviceArray(4 * img_floats * n_train)
+    chunk = 1024
+    with ThreadPoolExecutor(16) as ex:
+        for c0 in range(0, n_train, chunk):
+            seeds = range(4000 + c0, 4000 + min(n_train, c0 + chunk))
+            block = np.stack(list(ex.map(lambda s: ds.target_image(s, W, H, 32), seeds)))
+            ds.check(ds.lib().ds_memcpy_htod(C.c_void_p(d_train.ptr + 4 * img_floats * c0),
+                                             ds.fptr(block), block.nbytes), "upload")
+        evals = list(ex.map(lambda s: ds.target_image(s, W, H, 32),
+                            range(90000, 90000 + n_eval)))
+    gen_s = time.perf_counter() - t0
+    view, proj, cam = ds.fitter_camera(W, H)
+    net = ds.InitNet(W, H, seed=0)
+    trainer = ds.ImageFitter(W, H, N, view=view, proj=proj, cam_pos=cam)
+    t0 = time.perf_counter()
+    for k in range(train_steps):
+        img = C.c_void_p(d_train.ptr + 4 * img_floats * (k % n_train))
+        ds.check(ds.lib().ds_initnet_train_step(net.h, trainer.h, img,
+                                                2e-3 * 0.5 ** (k / 3000.0)), "train")
+    trainer.synchronize()
+    train_s = time.perf_counter() - t0
+    del d_train
+    d_evals = [ds.DeviceArray.from_numpy(e) for e in evals]
+    # PSNR of the initializer's output alone
+    pslab, _ = trainer.param_slabs()
+    init_psnr = []
+    for d in d_evals:
+        net.infer(d, pslab, trainer.stream())
+        trainer.set_target_device(d.ptr)
+        trainer.forward()
+        init_psnr.append(ds.psnr_from_loss(trainer.loss()))
+    trainer.close()
+    out = {}
+    for name, use in (("plain", None), ("learned", net)):
+        r = [fit_to_psnr(ds, use, evals[i], d_evals[i], 90000 + i, W, H, N, target_db)
+             for i in range(n_eval)]
+        out[name] = {"mean_seconds": round(float(np.mean([x[1] for x in r])), 5),
+                     "mean_steps": round(float(np.mean([x[0] for x in r])), 1),
+                     "reached": int(sum(x[2] >= target_db for x in r)), "fits": n_eval}
+    net.close()
+    return {"workload": f"{W}x{H} synthetic targets, {N} splats, {n_eval} held-out seeds",
+            "target_db": target_db, "plain": out["plain"], "learned": out["learned"],
+            "speedup": round(out["plain"]["mean_seconds"] / out["learned"]["mean_seconds"], 2),
+            "learned_init_psnr_db": round(float(np.mean(init_psnr)), 3),
+            "train_images": n_train, "train_steps": train_steps,
+            "train_seconds": round(train_s, 3), "generate_seconds": round(gen_s, 3),
+            "lr_pos_decay": TTP_POS_DECAY,
+            "note": "learned time includes the initializer's inference; training is a one-off "
+                    "cost reported separately"}
 
 
 def batched_fit(ds, count=64, steps=500, seq_steps=10):
@@ -307,7 +374,7 @@ def main():
         viewer = viewer_fps(ds)
         print(f"[bench] viewer {viewer}", file=sys.stderr, flush=True)
     if not args.no_time_to_psnr:
-        ttp = time_to_psnr(ds)
+        ttp = learned_vs_plain(ds, n_eval=args.eval_images)
         print(f"[bench] time to psnr {ttp}", file=sys.stderr, flush=True)
     batched = None
     if not args.no_batch:
