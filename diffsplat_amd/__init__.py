@@ -111,6 +111,30 @@ def orbit_cameras(n: int, width: int, height: int, radius: float = 3.0,
     return cams
 
 
+def load_ply(path: str):
+    """3DGS PLY -> (slab, count, sh_degree), the slab in this library's
+    field-major layout and the OpenGL convention (see ds_ply_load)."""
+    n, d = C.c_int64(), C.c_int32()
+    p = str(path).encode()
+    check(lib().ds_ply_info(p, C.byref(n), C.byref(d)), "ds_ply_info")
+    slab = np.empty(slab_floats(n.value, d.value), np.float32)
+    check(lib().ds_ply_load(p, fptr(slab), n.value, d.value), "ds_ply_load")
+    return slab, n.value, d.value
+
+
+def save_ply(path: str, slab: np.ndarray, count: int, sh_degree: int) -> None:
+    slab = np.ascontiguousarray(slab, np.float32).reshape(-1)
+    assert slab.size == slab_floats(count, sh_degree)
+    check(lib().ds_ply_save(str(path).encode(), fptr(slab), count, sh_degree), "ds_ply_save")
+
+
+def normalize_scene(slab: np.ndarray, count: int, scene_scale: float = 1.0) -> np.ndarray:
+    """SplatUtils::normalizeScene on a host slab (in place; also returned)."""
+    assert slab.dtype == np.float32 and slab.flags.c_contiguous
+    check(lib().ds_normalize_scene(fptr(slab), count, scene_scale), "ds_normalize_scene")
+    return slab
+
+
 def psnr_from_loss(loss: float) -> float:
     """PSNR of the reference's loss (sum over channels / pixels): 10 log10(3 / loss)."""
     return 10.0 * math.log10(3.0 / max(loss, 1e-30))
@@ -240,6 +264,10 @@ class ImageFitter:
         out = np.empty(self.width * self.height * 3, np.float32)
         check(lib().ds_fitter_download_image(self.h, fptr(out)), "ds_fitter_download_image")
         return out
+
+    def save_ply(self, path: str) -> None:
+        """ImageFitter::savePLY."""
+        check(lib().ds_fitter_save_ply(self.h, str(path).encode()), "ds_fitter_save_ply")
 
     def param_slabs(self):
         """Device pointers (parameter slab, parameter-gradient slab)."""
@@ -422,6 +450,19 @@ class SplatRenderer:
               "ds_renderer_create")
         self.h = h.value
         self.width, self.height, self.count, self.sh_degree = width, height, count, sh_degree
+
+    @classmethod
+    def from_ply(cls, path: str, width: int, height: int, scene_scale: float = 1.0,
+                 isect_cap: int = 0) -> "SplatRenderer":
+        """SplatRenderer::loadPLY: the scene of a 3DGS PLY, normalised."""
+        h, n, d = C.c_void_p(), C.c_int64(), C.c_int32()
+        check(lib().ds_renderer_load_ply(str(path).encode(), width, height, scene_scale,
+                                         isect_cap, C.byref(h), C.byref(n), C.byref(d)),
+              "ds_renderer_load_ply")
+        r = cls.__new__(cls)
+        r.h = h.value
+        r.width, r.height, r.count, r.sh_degree = width, height, n.value, d.value
+        return r
 
     def upload_params(self, slab: np.ndarray) -> None:
         slab = np.ascontiguousarray(slab, np.float32).reshape(-1)

@@ -157,6 +157,14 @@ _SIGS = {
                                      C.POINTER(C.c_void_p)]),
     "ds_renderer_destroy": (C.c_int, [_P]),
     "ds_renderer_upload_params": (C.c_int, [_P, _F]),
+    "ds_ply_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "ds_ply_load": (C.c_int, [C.c_char_p, _F, C.c_int64, C.c_int32]),
+    "ds_ply_save": (C.c_int, [C.c_char_p, _F, C.c_int64, C.c_int32]),
+    "ds_normalize_scene": (C.c_int, [_F, C.c_int64, C.c_float]),
+    "ds_renderer_load_ply": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_float, C.c_int64,
+                                       C.POINTER(C.c_void_p), C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int32)]),
+    "ds_fitter_save_ply": (C.c_int, [_P, C.c_char_p]),
     "ds_initnet_create": (C.c_int, [C.c_int, C.c_int, C.c_float, C.c_uint64,
                                     C.POINTER(C.c_void_p)]),
     "ds_initnet_destroy": (C.c_int, [_P]),

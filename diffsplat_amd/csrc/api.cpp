@@ -1,9 +1,11 @@
 // C-ABI entry points: runtime helpers, layer kernels, MSE, Adam, and the
 // fitter / renderer objects (wrapping the host C++ classes).
+#include <cstdint>
 #include <cstring>
 #include <exception>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "ds_internal.hpp"
 #include "host/pipelines.hpp"
@@ -523,6 +525,18 @@ ds_status ds_fitter_views(ds_fitter *f, ds_gauss3d *params, ds_splat2d *splat2d,
     DS_GUARD_END
 }
 
+ds_status ds_fitter_save_ply(ds_fitter *f, const char *path) {
+    clear_error();
+    DS_REQUIRE(f && path, "NULL argument");
+    DS_GUARD_BEGIN
+    const int n = f->impl.count(), deg = f->impl.shDegree();
+    DS_REQUIRE(n > 0, "the fitter holds no splats");
+    std::vector<float> slab((size_t)ds_slab_floats(n, deg));
+    f->impl.downloadParams(slab.data());
+    return ds_ply_save(path, slab.data(), n, deg);
+    DS_GUARD_END
+}
+
 ds_status ds_fitter_forward_backward(ds_fitter *f) {
     clear_error();
     DS_REQUIRE(f, "NULL fitter");
@@ -647,6 +661,29 @@ ds_status ds_renderer_create(int width, int height, int count, int sh_degree, in
     auto r = std::make_unique<ds_renderer>();
     r->impl.init(width, height, count, sh_degree, isect_cap);
     *out = r.release();
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_renderer_load_ply(const char *path, int width, int height, float scene_scale,
+                               int64_t isect_cap, ds_renderer **out, int64_t *count,
+                               int32_t *sh_degree) {
+    clear_error();
+    DS_REQUIRE(path && out, "NULL argument");
+    int64_t n = 0;
+    int32_t deg = 0;
+    DS_TRY(ds_ply_info(path, &n, &deg));
+    DS_REQUIRE(n <= (int64_t)INT32_MAX, "PLY has more splats than a renderer holds");
+    DS_GUARD_BEGIN
+    std::vector<float> slab((size_t)ds_slab_floats((int)n, deg));
+    DS_TRY(ds_ply_load(path, slab.data(), n, deg));
+    DS_TRY(ds_normalize_scene(slab.data(), n, scene_scale));
+    auto r = std::make_unique<ds_renderer>();
+    r->impl.init(width, height, (int)n, deg, isect_cap);
+    r->impl.uploadParams(slab.data());
+    *out = r.release();
+    if (count) *count = n;
+    if (sh_degree) *sh_degree = deg;
     return DS_OK;
     DS_GUARD_END
 }

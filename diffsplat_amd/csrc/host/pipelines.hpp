@@ -60,6 +60,8 @@ public:
     float getLoss();
     float *getOutput() { return ras_layer.getOutput().ptr; }
     int getIterCount() const { return optimizer.getStepCount(); }
+    int count() const { return opt.count; }
+    int shDegree() const { return opt.sh_degree; }
     hipStream_t getStream() const { return stream; }
     ds_raster *raster() const { return ras_layer.raster(); }
     int width() const { return opt.width; }

@@ -478,6 +478,38 @@ ds_status ds_renderer_get_stream(ds_renderer *r, ds_stream_t *s);
 ds_status ds_renderer_synchronize(ds_renderer *r);
 
 /* ------------------------------------------------------------------ */
+/* Scene files (host only)                                              */
+/* ------------------------------------------------------------------ */
+
+/* 3DGS PLY: binary little-endian, float vertex properties by name (x y z,
+ * scale_0..2, rot_0..3 = w x y z, f_dc_0..2, f_rest_* channel-first, opacity;
+ * others are ignored). The file is in the OpenCV convention (y down, z
+ * forward), slabs in OpenGL (y up, z backward): y, z and the quaternion's
+ * y, z change sign on load and save. SH degree from the f_rest count as in
+ * PLYLoader::load (src/io/ply_io.cpp:15-160). ASCII files and non-float
+ * vertex properties are rejected (the reference would misread them). */
+ds_status ds_ply_info(const char *path, int64_t *count, int32_t *sh_degree);
+/* Into a host slab of ds_slab_floats(count, sh_degree) floats (count and
+ * sh_degree as ds_ply_info reports). */
+ds_status ds_ply_load(const char *path, float *host_slab, int64_t count, int32_t sh_degree);
+/* PLYSaver::save (src/io/ply_io.cpp:164-227); property order x y z f_dc
+ * f_rest opacity scale rot. */
+ds_status ds_ply_save(const char *path, const float *host_slab, int64_t count,
+                      int32_t sh_degree);
+/* SplatUtils::normalizeScene (src/utils/splat_utils.h:53-93), bit-identical:
+ * positions centred on their float centroid and scaled so that the largest
+ * |coordinate| is scene_scale; log-scales shifted by log of that factor. */
+ds_status ds_normalize_scene(float *host_slab, int64_t count, float scene_scale);
+/* SplatRenderer::loadPLY (src/pipelines/splat_renderer.cpp:27-35): load,
+ * normalise with scene_scale, create a renderer holding the scene. count and
+ * sh_degree (either may be NULL) receive the scene's. */
+ds_status ds_renderer_load_ply(const char *path, int width, int height, float scene_scale,
+                               int64_t isect_cap, ds_renderer **out, int64_t *count,
+                               int32_t *sh_degree);
+/* ImageFitter::savePLY (src/pipelines/image_fitter.cpp:114-118). Synchronises. */
+ds_status ds_fitter_save_ply(ds_fitter *f, const char *path);
+
+/* ------------------------------------------------------------------ */
 /* Synthetic workloads and cameras (host only, deterministic)          */
 /* ------------------------------------------------------------------ */
 

@@ -54,6 +54,47 @@ def test_viewer_render_matches_oracle(cam_index):
     assert ok
 
 
+def test_renderer_from_ply_matches_uploaded_scene(tmp_path):
+    """SplatRenderer::loadPLY: a scene saved as PLY (un-normalised coordinates)
+    renders bit-identically to the same file loaded, normalised on the host
+    and uploaded; the render of an uploaded slab is pinned against the oracle
+    by the parity test above."""
+    n, W, H = 20000, 320, 180
+    params = ds.cluster_scene(77, n, 3, 16)
+    params[0:3 * n] = params[0:3 * n] * np.float32(5.0) + np.float32(2.0)
+    path = tmp_path / "scene.ply"
+    ds.save_ply(path, params, n, 3)
+    r = ds.SplatRenderer.from_ply(path, W, H, scene_scale=1.0)
+    assert (r.count, r.sh_degree) == (n, 3)
+    slab, _, _ = ds.load_ply(path)
+    ds.normalize_scene(slab, n, 1.0)
+    u = ds.SplatRenderer(W, H, n, 3)
+    u.upload_params(slab)
+    for k in (5, 60):
+        view, proj, eye = ds.orbit_cameras(120, W, H)[k]
+        r.render(view, proj, eye)
+        u.render(view, proj, eye)
+        a, b = r.image(), u.image()
+        assert a.max() > 0.05 and np.array_equal(a, b)
+    r.close()
+    u.close()
+
+
+def test_fitter_save_ply_round_trips(tmp_path):
+    W, H, n = 96, 64, 800
+    view, proj, cam = ds.fitter_camera(W, H)
+    fit = ds.ImageFitter(W, H, n, view=view, proj=proj, cam_pos=cam)
+    fit.set_target(ds.target_image(3, W, H, 8))
+    fit.upload_params(ds.frustum_splats(3, n, aspect=W / H))
+    fit.step(5)
+    path = tmp_path / "fit.ply"
+    fit.save_ply(path)
+    slab, count, deg = ds.load_ply(path)
+    assert (count, deg) == (n, 0)
+    np.testing.assert_array_equal(slab, fit.params())
+    fit.close()
+
+
 def test_viewer_full_size_orbit_invariants():
     """1M splats, SH 3, 1920x1080: colours lie in [0, 1 - T] for every pixel
     of several orbit frames, and re-rendering a frame is bit-identical."""
