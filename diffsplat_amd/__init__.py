@@ -111,6 +111,24 @@ def orbit_cameras(n: int, width: int, height: int, radius: float = 3.0,
     return cams
 
 
+def load_image(path: str, width: int, height: int, padding: int = 0):
+    """ImageLoader::load: PNG / binary PPM fitted into a width x height canvas
+    (aspect kept, centred, black elsewhere) -> (HWC floats in [0,1], src_w, src_h)."""
+    out = np.empty(width * height * 3, np.float32)
+    sw, sh = C.c_int32(), C.c_int32()
+    check(lib().ds_image_load(str(path).encode(), width, height, padding, fptr(out),
+                              C.byref(sw), C.byref(sh)), "ds_image_load")
+    return out, sw.value, sh.value
+
+
+def save_png(path: str, rgb: np.ndarray, width: int, height: int) -> None:
+    """ImageSaver::saveAsPNG of HWC floats in [0,1]."""
+    rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1)
+    assert rgb.size == width * height * 3
+    check(lib().ds_image_save_png(str(path).encode(), fptr(rgb), width, height),
+          "ds_image_save_png")
+
+
 def load_ply(path: str):
     """3DGS PLY -> (slab, count, sh_degree), the slab in this library's
     field-major layout and the OpenGL convention (see ds_ply_load)."""

@@ -157,6 +157,11 @@ _SIGS = {
                                      C.POINTER(C.c_void_p)]),
     "ds_renderer_destroy": (C.c_int, [_P]),
     "ds_renderer_upload_params": (C.c_int, [_P, _F]),
+    "ds_image_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "ds_image_load": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, _F,
+                                C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "ds_image_save_png": (C.c_int, [C.c_char_p, _F, C.c_int, C.c_int]),
+    "ds_image_save_png_device": (C.c_int, [C.c_char_p, _P, C.c_int, C.c_int]),
     "ds_ply_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "ds_ply_load": (C.c_int, [C.c_char_p, _F, C.c_int64, C.c_int32]),
     "ds_ply_save": (C.c_int, [C.c_char_p, _F, C.c_int64, C.c_int32]),

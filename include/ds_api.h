@@ -509,6 +509,26 @@ ds_status ds_renderer_load_ply(const char *path, int width, int height, float sc
 /* ImageFitter::savePLY (src/pipelines/image_fitter.cpp:114-118). Synchronises. */
 ds_status ds_fitter_save_ply(ds_fitter *f, const char *path);
 
+/* Images: PNG (8 or 16 bit, grey / RGB / palette / grey+alpha / RGBA, not
+ * interlaced; alpha dropped) or binary PPM (P6, maxval 255). JPEG, which the
+ * reference also reads through stb_image, is not supported. */
+ds_status ds_image_info(const char *path, int32_t *width, int32_t *height);
+/* ImageLoader::load (src/io/image_io.cpp:23-78): the image scaled to fit
+ * inside the canvas minus `padding` on every side (scale = min((W-2p)/w,
+ * (H-2p)/h), fit size (int)(w*scale) x (int)(h*scale)), centred, black
+ * elsewhere, HWC floats value/255 into host_rgb [H*W*3]. The resampling is
+ * this library's (area average when shrinking, linear between pixel centres
+ * when growing, on 8-bit values), not stb_image_resize2's. src_w, src_h (may
+ * be NULL) receive the file's size. */
+ds_status ds_image_load(const char *path, int target_w, int target_h, int padding,
+                        float *host_rgb, int32_t *src_w, int32_t *src_h);
+/* ImageSaver::saveAsPNG (src/io/image_io.cpp:82-93): 8-bit RGB PNG of HWC
+ * floats, each byte (uint8)min(max(v * 255, 0), 255). */
+ds_status ds_image_save_png(const char *path, const float *host_rgb, int width, int height);
+/* Same from device memory (synchronises), as the reference's saveAsPNG takes. */
+ds_status ds_image_save_png_device(const char *path, const float *dev_rgb, int width,
+                                   int height);
+
 /* ------------------------------------------------------------------ */
 /* Synthetic workloads and cameras (host only, deterministic)          */
 /* ------------------------------------------------------------------ */

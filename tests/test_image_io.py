@@ -1,0 +1,175 @@
+"""Target images from files and PNG output (host code, CPU tests).
+
+PNGs are written and read here by an independent zlib-based codec, with every
+row filter and colour type the loader accepts. Placement is checked against
+ImageLoader::load's formulas (src/io/image_io.cpp:23-78), resampling against
+what the library documents (area average when shrinking: an integer factor
+gives block means; linear when growing: constants stay constant), and PNG
+output against ImageSaver::saveAsPNG's byte formula (:82-93)."""
+from __future__ import annotations
+
+import struct
+import zlib
+
+import numpy as np
+import pytest
+
+import diffsplat_amd as ds
+
+
+def png_chunk(kind: bytes, data: bytes) -> bytes:
+    return struct.pack(">I", len(data)) + kind + data + struct.pack(
+        ">I", zlib.crc32(kind + data) & 0xFFFFFFFF)
+
+
+def paeth(a, b, c):
+    p = a + b - c
+    pa, pb, pc = abs(p - a), abs(p - b), abs(p - c)
+    return a if (pa <= pb and pa <= pc) else (b if pb <= pc else c)
+
+
+def encode_png(samples: np.ndarray, ctype: int, depth: int = 8, filters=(0,), palette=None):
+    """samples: (h, w, chans) uint8 or uint16; rows use filters[y % len(filters)]."""
+    h, w = samples.shape[:2]
+    raw = samples.astype(">u2").tobytes() if depth == 16 else samples.astype(np.uint8).tobytes()
+    stride = len(raw) // h
+    bpp = samples.shape[2] * depth // 8
+    out = bytearray()
+    prev = bytes(stride)
+    for y in range(h):
+        row = raw[y * stride:(y + 1) * stride]
+        ft = filters[y % len(filters)]
+        enc = bytearray(stride)
+        for i in range(stride):
+            a = row[i - bpp] if i >= bpp else 0
+            b = prev[i]
+            c = prev[i - bpp] if i >= bpp else 0
+            pred = [0, a, b, (a + b) // 2, paeth(a, b, c)][ft]
+            enc[i] = (row[i] - pred) & 0xFF
+        out += bytes([ft]) + enc
+        prev = row
+    ihdr = struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, 0)
+    png = b"\x89PNG\r\n\x1a\n" + png_chunk(b"IHDR", ihdr)
+    if palette is not None:
+        png += png_chunk(b"PLTE", palette.astype(np.uint8).tobytes())
+    return png + png_chunk(b"IDAT", zlib.compress(bytes(out))) + png_chunk(b"IEND", b"")
+
+
+def decode_rgb8_png(data: bytes):
+    """Reads the 8-bit RGB, filter-0 PNGs that save_png writes."""
+    o, idat, w = 8, b"", 0
+    while o < len(data):
+        n = struct.unpack(">I", data[o:o + 4])[0]
+        kind, body = data[o + 4:o + 8], data[o + 8:o + 8 + n]
+        if kind == b"IHDR":
+            w, h, depth, ctype = struct.unpack(">IIBB", body[:10])
+            assert (depth, ctype) == (8, 2)
+        elif kind == b"IDAT":
+            idat += body
+        o += 12 + n
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 1 + 3 * w)
+    assert np.all(raw[:, 0] == 0)
+    return raw[:, 1:].reshape(h, w, 3)
+
+
+def native(tmp_path, png: bytes, w, h):
+    p = tmp_path / "img.png"
+    p.write_bytes(png)
+    img, sw, sh = ds.load_image(p, w, h)
+    assert (sw, sh) == (w, h)
+    return img.reshape(h, w, 3)
+
+
+@pytest.mark.parametrize("filters", [(0,), (1,), (2,), (3,), (4,), (0, 1, 2, 3, 4)])
+def test_rgb_png_every_row_filter(tmp_path, filters):
+    rng = np.random.default_rng(len(filters) * 10 + filters[0])
+    px = rng.integers(0, 256, (13, 17, 3), dtype=np.uint8)
+    got = native(tmp_path, encode_png(px, 2, filters=filters), 17, 13)
+    np.testing.assert_array_equal(got, px.astype(np.float32) / np.float32(255))
+
+
+def test_other_colour_types_and_depths(tmp_path):
+    rng = np.random.default_rng(5)
+    rgba = rng.integers(0, 256, (9, 11, 4), dtype=np.uint8)
+    got = native(tmp_path, encode_png(rgba, 6, filters=(4, 1)), 11, 9)
+    np.testing.assert_array_equal(got, rgba[..., :3] / np.float32(255))  # alpha dropped
+    grey = rng.integers(0, 256, (9, 11, 1), dtype=np.uint8)
+    got = native(tmp_path, encode_png(grey, 0, filters=(3,)), 11, 9)
+    np.testing.assert_array_equal(got, np.repeat(grey, 3, 2) / np.float32(255))
+    ga = rng.integers(0, 256, (9, 11, 2), dtype=np.uint8)
+    got = native(tmp_path, encode_png(ga, 4, filters=(2,)), 11, 9)
+    np.testing.assert_array_equal(got, np.repeat(ga[..., :1], 3, 2) / np.float32(255))
+    pal = rng.integers(0, 256, (7, 3), dtype=np.uint8)
+    idx = rng.integers(0, 7, (9, 11, 1), dtype=np.uint8)
+    got = native(tmp_path, encode_png(idx, 3, palette=pal), 11, 9)
+    np.testing.assert_array_equal(got, pal[idx[..., 0]] / np.float32(255))
+    deep = rng.integers(0, 65536, (9, 11, 3), dtype=np.uint16)
+    got = native(tmp_path, encode_png(deep, 2, depth=16, filters=(1, 4)), 11, 9)
+    np.testing.assert_array_equal(got, (deep >> 8).astype(np.float32) / np.float32(255))
+
+
+def test_binary_ppm(tmp_path):
+    px = np.random.default_rng(6).integers(0, 256, (5, 8, 3), dtype=np.uint8)
+    p = tmp_path / "img.ppm"
+    p.write_bytes(b"P6\n# comment\n8 5\n255\n" + px.tobytes())
+    img, sw, sh = ds.load_image(p, 8, 5)
+    assert (sw, sh) == (8, 5)
+    np.testing.assert_array_equal(img.reshape(5, 8, 3), px / np.float32(255))
+
+
+def test_fit_is_centred_with_black_borders(tmp_path):
+    w, h, W, H, pad = 40, 20, 64, 64, 4
+    px = np.full((h, w, 3), 200, np.uint8)
+    p = tmp_path / "img.png"
+    p.write_bytes(encode_png(px, 2))
+    img, _, _ = ds.load_image(p, W, H, pad)
+    img = img.reshape(H, W, 3)
+    scale = min(np.float32(W - 2 * pad) / w, np.float32(H - 2 * pad) / h)
+    fw, fh = int(w * scale), int(h * scale)
+    ox, oy = (W - fw) // 2, (H - fh) // 2
+    inside = np.zeros((H, W), bool)
+    inside[oy:oy + fh, ox:ox + fw] = True
+    assert (fw, fh, ox, oy) == (56, 28, 4, 18)
+    assert np.all(img[inside] == np.float32(200) / np.float32(255))  # constant stays constant
+    assert np.all(img[~inside] == 0)
+
+
+def test_shrinking_by_an_integer_factor_gives_block_means(tmp_path):
+    px = np.random.default_rng(7).integers(0, 256, (16, 24, 3), dtype=np.uint8)
+    p = tmp_path / "img.png"
+    p.write_bytes(encode_png(px, 2))
+    img, _, _ = ds.load_image(p, 12, 8)
+    means = px.reshape(8, 2, 12, 2, 3).astype(np.float64).mean(axis=(1, 3))
+    want = np.floor(means + 0.5) / 255
+    np.testing.assert_allclose(img.reshape(8, 12, 3), want, atol=1e-7)
+
+
+def test_save_png_bytes_follow_the_reference_formula(tmp_path):
+    rng = np.random.default_rng(8)
+    rgb = rng.uniform(-0.2, 1.2, (6, 9, 3)).astype(np.float32)
+    p = tmp_path / "out.png"
+    ds.save_png(p, rgb, 9, 6)
+    got = decode_rgb8_png(p.read_bytes())
+    want = np.minimum(np.maximum(rgb * np.float32(255), 0), 255).astype(np.uint8)
+    np.testing.assert_array_equal(got, want)
+    back, _, _ = ds.load_image(p, 9, 6)  # and the loader reads it back
+    np.testing.assert_array_equal(back.reshape(6, 9, 3), want / np.float32(255))
+
+
+def test_rejected_files(tmp_path):
+    p = tmp_path / "bad.png"
+    p.write_bytes(b"GIF89a....")
+    with pytest.raises(ds.DiffsplatError, match="unsupported image format"):
+        ds.load_image(p, 8, 8)
+    good = encode_png(np.zeros((4, 4, 3), np.uint8), 2)
+    broken = bytearray(good)
+    broken[30] ^= 0xFF  # inside IHDR: CRC no longer matches
+    p.write_bytes(bytes(broken))
+    with pytest.raises(ds.DiffsplatError, match="CRC"):
+        ds.load_image(p, 8, 8)
+    ihdr = struct.pack(">IIBBBBB", 4, 4, 8, 2, 0, 0, 1)  # Adam7
+    inter = b"\x89PNG\r\n\x1a\n" + png_chunk(b"IHDR", ihdr) + png_chunk(
+        b"IDAT", zlib.compress(bytes(4 * 13))) + png_chunk(b"IEND", b"")
+    p.write_bytes(inter)
+    with pytest.raises(ds.DiffsplatError, match="interlaced"):
+        ds.load_image(p, 8, 8)
