@@ -12,7 +12,8 @@ import math
 
 import numpy as np
 
-from ._lib import (AdamGroup, AdamHParams, DeviceArray, DiffsplatError, FitterConfig,
+from ._lib import (AdamGroup, AdamHParams, CameraInput, CameraSpeeds, DeviceArray,
+                   DiffsplatError, FitterConfig,
                    RasterConfig, RasterView, check, download_ptr, farr, fptr, gauss_view,
                    has_gpu, lib, splat2d_view, splat3d_view)
 
@@ -21,7 +22,7 @@ __all__ = [
     "splat2d_view", "sh_bands", "slab_floats", "target_image", "frustum_splats",
     "cluster_scene", "fitter_init", "perspective", "ortho", "look_at", "mat_mul",
     "fitter_camera", "orbit_cameras", "ImageFitter", "SplatRenderer", "Rasterizer",
-    "psnr_from_loss",
+    "psnr_from_loss", "Input", "ArcballCamera", "FlyCamera",
 ]
 
 SH_C0 = 0.28209479177387814
@@ -109,6 +110,118 @@ def orbit_cameras(n: int, width: int, height: int, radius: float = 3.0,
                        np.float32)
         cams.append((look_at(eye, (0, 0, 0), (0, 1, 0)), proj, eye))
     return cams
+
+
+KEY_BITS = {"shift": 1 << 0, "ctrl": 1 << 1, "w": 1 << 2, "a": 1 << 3, "s": 1 << 4,
+            "d": 1 << 5, "r": 1 << 6, "f": 1 << 7, "q": 1 << 8, "e": 1 << 9}
+
+
+class Input:
+    """One frame of user input (src/input/input.h:12-16): cursor movement in
+    pixels (screen y down), wheel delta, left button, and the held keys by
+    name ("shift", "ctrl", "w", "a", "s", "d", "r", "f", "q", "e")."""
+
+    def __init__(self, mouse_delta=(0.0, 0.0), scroll_delta: float = 0.0,
+                 mouse_left_held: bool = False, keys=()):
+        self.mouse_delta = mouse_delta
+        self.scroll_delta = scroll_delta
+        self.mouse_left_held = mouse_left_held
+        self.keys = set(keys)
+
+    def to_c(self) -> CameraInput:
+        mask = 0
+        for k in self.keys:
+            mask |= KEY_BITS[k.lower()]
+        return CameraInput(float(self.mouse_delta[0]), float(self.mouse_delta[1]),
+                           float(self.scroll_delta), int(bool(self.mouse_left_held)), mask)
+
+
+class _Camera:
+    """Camera (src/camera/camera.h:5-16) over ds_camera_*: update() each
+    frame, then view_matrix / projection_matrix / position for the renderer."""
+    _kind = -1
+    _speed_names: dict = {}
+
+    def __init__(self, aspect: float, fov_degrees: float = 40.0, near_plane: float = 0.1,
+                 far_plane: float = 100.0):
+        h = C.c_void_p()
+        check(lib().ds_camera_create(self._kind, aspect, fov_degrees, near_plane, far_plane,
+                                     C.byref(h)), "ds_camera_create")
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().ds_camera_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def update(self, inp: Input, dt: float) -> bool:
+        changed = C.c_int()
+        c_in = inp.to_c()
+        check(lib().ds_camera_update(self._h, C.byref(c_in), dt, C.byref(changed)),
+              "ds_camera_update")
+        return bool(changed.value)
+
+    def set_aspect(self, aspect: float) -> None:
+        check(lib().ds_camera_set_aspect(self._h, aspect), "ds_camera_set_aspect")
+
+    def set_ortho_mode(self, enable: bool) -> None:
+        check(lib().ds_camera_set_ortho(self._h, int(bool(enable))), "ds_camera_set_ortho")
+
+    def matrices(self):
+        """(view, proj, eye): column-major 4x4s as 16 floats, eye position."""
+        v, p, e = np.empty(16, np.float32), np.empty(16, np.float32), np.empty(3, np.float32)
+        check(lib().ds_camera_matrices(self._h, fptr(v), fptr(p), fptr(e)),
+              "ds_camera_matrices")
+        return v, p, e
+
+    @property
+    def view_matrix(self) -> np.ndarray:
+        return self.matrices()[0]
+
+    @property
+    def projection_matrix(self) -> np.ndarray:
+        return self.matrices()[1]
+
+    @property
+    def position(self) -> np.ndarray:
+        return self.matrices()[2]
+
+    def _speeds(self) -> CameraSpeeds:
+        s = CameraSpeeds()
+        check(lib().ds_camera_get_speeds(self._h, C.byref(s)), "ds_camera_get_speeds")
+        return s
+
+    def __getattr__(self, name):
+        field = type(self)._speed_names.get(name)
+        if field is None:
+            raise AttributeError(name)
+        return getattr(self._speeds(), field)
+
+    def __setattr__(self, name, value):
+        field = type(self)._speed_names.get(name)
+        if field is None:
+            object.__setattr__(self, name, value)
+            return
+        s = self._speeds()
+        setattr(s, field, float(value))
+        check(lib().ds_camera_set_speeds(self._h, C.byref(s)), "ds_camera_set_speeds")
+
+
+class ArcballCamera(_Camera):
+    """Orbit / shift-pan / scroll-zoom camera (src/camera/arcball_camera.h:20-71)."""
+    _kind = 0
+    _speed_names = {"speed_rotate": "rotate", "speed_pan": "pan", "speed_zoom": "zoom",
+                    "zoom_min_dist": "zoom_min_dist", "zoom_max_dist": "zoom_max_dist"}
+
+
+class FlyCamera(_Camera):
+    """WASD/RF/QE + drag-look camera (src/camera/fly_camera.h:30-76)."""
+    _kind = 1
+    _speed_names = {"speed_move": "move", "speed_look": "look", "speed_roll": "roll",
+                    "speed_mult_shift": "mult_shift", "speed_mult_ctrl": "mult_ctrl"}
 
 
 def load_image(path: str, width: int, height: int, padding: int = 0):

@@ -82,6 +82,17 @@ class FitterConfig(C.Structure):
                 ("grad_mode", C.c_int32), ("lr_pos_decay", C.c_float)]
 
 
+class CameraInput(C.Structure):
+    _fields_ = [("mouse_dx", C.c_float), ("mouse_dy", C.c_float), ("scroll", C.c_float),
+                ("mouse_left", C.c_int32), ("keys", C.c_uint32)]
+
+
+class CameraSpeeds(C.Structure):
+    _fields_ = [(f, C.c_float) for f in ("rotate", "pan", "zoom", "zoom_min_dist",
+                                         "zoom_max_dist", "move", "look", "roll",
+                                         "mult_shift", "mult_ctrl")]
+
+
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
 _SIGS = {
@@ -206,6 +217,17 @@ _SIGS = {
     "ds_mat_ortho": (None, [C.c_float] * 6 + [_F]),
     "ds_mat_look_at": (None, [_F, _F, _F, _F]),
     "ds_mat_mul": (None, [_F, _F, _F]),
+    "ds_camera_default_speeds": (None, [C.POINTER(CameraSpeeds)]),
+    "ds_camera_create": (C.c_int, [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                                   C.POINTER(_P)]),
+    "ds_camera_destroy": (None, [_P]),
+    "ds_camera_update": (C.c_int, [_P, C.POINTER(CameraInput), C.c_float,
+                                   C.POINTER(C.c_int)]),
+    "ds_camera_set_aspect": (C.c_int, [_P, C.c_float]),
+    "ds_camera_set_ortho": (C.c_int, [_P, C.c_int]),
+    "ds_camera_get_speeds": (C.c_int, [_P, C.POINTER(CameraSpeeds)]),
+    "ds_camera_set_speeds": (C.c_int, [_P, C.POINTER(CameraSpeeds)]),
+    "ds_camera_matrices": (C.c_int, [_P, _F, _F, _F]),
     "ds_slab_floats": (C.c_int64, [C.c_int, C.c_int]),
     "ds_sh_bands": (C.c_int, [C.c_int]),
 }

@@ -564,6 +564,80 @@ void ds_mat_ortho(float l, float r, float b, float t, float n, float f, float ou
 void ds_mat_look_at(const float eye[3], const float center[3], const float up[3], float out[16]);
 void ds_mat_mul(const float a[16], const float b[16], float out[16]);
 
+/* ------------------------------------------------------------------------
+ * Interactive cameras: per-frame input -> view/projection for the renderer.
+ * Replace ArcballCamera (src/camera/arcball_camera.h:20-71, .cpp:11-144) and
+ * FlyCamera (src/camera/fly_camera.h:30-76, .cpp:13-125) behind the Camera
+ * interface (src/camera/camera.h:5-16). Host-only; no GPU involved.
+ *
+ * Both start at eye (0, 0, 3); the arcball looks at the origin, the fly
+ * camera down -z. Arcball: left drag orbits (yaw/pitch, pitch within
+ * +-(pi/2 - 0.01)), shift + left drag pans eye and target together, scroll
+ * dollies along the view direction (distance clamped to [zoom_min_dist,
+ * zoom_max_dist]) or, in ortho mode, scales the frustum half-height (>= 0.01).
+ * Fly: W/S A/D R/F move along the local forward/right/up axes, Q/E roll, left
+ * drag yaws about the local up axis and pitches (clamped as above); shift and
+ * ctrl multiply every speed. Mouse deltas are clamped to +-50 pixels, screen y
+ * grows downwards, speeds are per second and scale with dt.
+ * ---------------------------------------------------------------------- */
+#define DS_CAMERA_ARCBALL 0
+#define DS_CAMERA_FLY 1
+
+/* Keys the cameras read (the reference's GLFW key queries). */
+#define DS_KEY_SHIFT (1u << 0)
+#define DS_KEY_CTRL (1u << 1)
+#define DS_KEY_W (1u << 2)
+#define DS_KEY_A (1u << 3)
+#define DS_KEY_S (1u << 4)
+#define DS_KEY_D (1u << 5)
+#define DS_KEY_R (1u << 6)
+#define DS_KEY_F (1u << 7)
+#define DS_KEY_Q (1u << 8)
+#define DS_KEY_E (1u << 9)
+
+/* One frame of input (src/input/input.h:12-16 and its key set). */
+typedef struct ds_camera_input {
+    float mouse_dx, mouse_dy; /* cursor movement since the last frame, pixels */
+    float scroll;             /* wheel delta since the last frame             */
+    int32_t mouse_left;       /* left button held                             */
+    uint32_t keys;            /* DS_KEY_* held                                */
+} ds_camera_input;
+
+/* Speeds (defaults: arcball_camera.h:40-44, fly_camera.h:44-48). The first
+ * five drive the arcball, the rest the fly camera. */
+typedef struct ds_camera_speeds {
+    float rotate;        /* rad per pixel per second            0.2   */
+    float pan;           /* distance fraction per pixel          0.005 */
+    float zoom;          /* distance fraction per scroll unit    0.1   */
+    float zoom_min_dist; /*                                      0.5   */
+    float zoom_max_dist; /*                                      50    */
+    float move;          /* world units per second               5     */
+    float look;          /* rad per pixel per second             0.2   */
+    float roll;          /* rad per second                       0.2   */
+    float mult_shift;    /*                                      4     */
+    float mult_ctrl;     /*                                      0.2   */
+} ds_camera_speeds;
+
+typedef struct ds_camera ds_camera;
+
+void ds_camera_default_speeds(ds_camera_speeds *out);
+/* ArcballCamera/FlyCamera(aspect, fovDegrees, nearPlane, farPlane); the
+ * reference's defaults are 40 degrees, 0.1, 100. */
+ds_status ds_camera_create(int kind, float aspect, float fovy_deg, float znear, float zfar,
+                           ds_camera **out);
+void ds_camera_destroy(ds_camera *cam);
+/* Camera::update (camera.h:10): *changed = 1 if the matrices changed. */
+ds_status ds_camera_update(ds_camera *cam, const ds_camera_input *in, float dt, int *changed);
+/* Camera::setAspect (camera.h:11). */
+ds_status ds_camera_set_aspect(ds_camera *cam, float aspect);
+/* Camera::setOrthoMode (camera.h:12): arcball only, a no-op for the fly camera. */
+ds_status ds_camera_set_ortho(ds_camera *cam, int enable);
+ds_status ds_camera_get_speeds(const ds_camera *cam, ds_camera_speeds *out);
+ds_status ds_camera_set_speeds(ds_camera *cam, const ds_camera_speeds *speeds);
+/* getViewMatrix / getProjectionMatrix / getPosition (camera.h:14-16); any
+ * output may be null. */
+ds_status ds_camera_matrices(const ds_camera *cam, float view[16], float proj[16], float eye[3]);
+
 /* Slab size in floats for count splats of sh_degree: (14 + 3*bands) * count. */
 int64_t ds_slab_floats(int count, int sh_degree);
 int ds_sh_bands(int sh_degree);
