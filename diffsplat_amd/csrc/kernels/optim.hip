@@ -61,31 +61,33 @@ struct AdamTable {
 
 __global__ void k_adam_tick(int32_t *step) { *step += 1; }
 
-// All groups in one launch: every thread walks the groups in order and
-// grid-strides inside each, so control flow stays uniform per wave.
+// All groups in one launch: grid row blockIdx.y updates group blockIdx.y,
+// grid-striding inside it along x. The groups' memory round trips overlap
+// (a thread that walked the groups in turn would wait on each: stores to one
+// group may alias the next group's loads).
 __global__ __launch_bounds__(kBlock) void k_adam(AdamTable tab, float beta1, float beta2,
                                                  float eps, int32_t step_host,
                                                  const int32_t *step_dev) {
+    const int k = blockIdx.y;
     const int32_t t = step_dev ? *step_dev : step_host;
     const float bc1 = 1.f / (1.f - powf(beta1, (float)t));
     const float bc2 = 1.f / (1.f - powf(beta2, (float)t));
     const float ob1 = 1.f - beta1, ob2 = 1.f - beta2;
-    for (int k = 0; k < tab.count; ++k) {
-        float *__restrict__ P = tab.param[k];
-        const float *__restrict__ G = tab.grad[k];
-        float *__restrict__ M = tab.m[k];
-        float *__restrict__ V = tab.v[k];
-        const float d = tab.decay[k];
-        const float lr = d == 1.f ? tab.lr[k] : tab.lr[k] * powf(d, (float)(t - 1));
-        const int64_t n = tab.n[k];
-        for (int64_t i = gtid(); i < n; i += gstride()) {
-            const float g = G[i];
-            const float m = beta1 * M[i] + ob1 * g;
-            const float v = beta2 * V[i] + ob2 * g * g;
-            M[i] = m;
-            V[i] = v;
-            P[i] -= lr * (m * bc1) / (sqrtf(v * bc2) + eps);
-        }
+    float *__restrict__ P = tab.param[k];
+    const float *__restrict__ G = tab.grad[k];
+    float *__restrict__ M = tab.m[k];
+    float *__restrict__ V = tab.v[k];
+    const float d = tab.decay[k];
+    const float lr = d == 1.f ? tab.lr[k] : tab.lr[k] * powf(d, (float)(t - 1));
+    const int64_t n = tab.n[k];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float g = G[i];
+        const float m = beta1 * M[i] + ob1 * g;
+        const float v = beta2 * V[i] + ob2 * g * g;
+        M[i] = m;
+        V[i] = v;
+        P[i] -= lr * (m * bc1) / (sqrtf(v * bc2) + eps);
     }
 }
 
@@ -144,8 +146,8 @@ ds_status launch_adam(const ds_adam_group *groups, int n_groups, const ds_adam_h
         DS_CHECK_LAUNCH("adam_tick");
     }
     if (total == 0) return DS_OK;
-    hipLaunchKernelGGL(k_adam, dim3(grid_cap(largest, 2048)), dim3(kBlock), 0, s, tab, hp.beta1,
-                       hp.beta2, hp.epsilon, step, (const int32_t *)step_dev);
+    hipLaunchKernelGGL(k_adam, dim3(grid_cap(largest, 2048), n_groups), dim3(kBlock), 0, s, tab,
+                       hp.beta1, hp.beta2, hp.epsilon, step, (const int32_t *)step_dev);
     DS_CHECK_LAUNCH("adam");
     return DS_OK;
 }

@@ -57,6 +57,18 @@ constexpr int kHistRows = 4;  // digit histograms of the depth passes
 // Largest splat count sorted with one-kernel passes (64 chunks).
 constexpr uint32_t kOneKernelMaxKeys = 64u * kSortChunk;
 
+// Device-scope atomics on one cache line serialise (about 11-19 ns each on
+// MI355X), so per-workgroup counts go to replicas that readers sum: the
+// digit histograms (kHistReps copies of kHistRows x 256 bins) and the
+// visible-splat count (kVisReps words, one per 128-byte line), all in the
+// ghist buffer.
+constexpr int kHistReps = 8;
+constexpr int kHistRepWords = kHistRows * kRadix;
+constexpr int kVisReps = 16;
+constexpr int kVisStride = 32;
+constexpr int kVisOffset = kHistReps * kHistRepWords;
+constexpr int kGhistWords = kVisOffset + kVisReps * kVisStride;
+
 __device__ __forceinline__ uint32_t depth_bits(float z) {
     uint32_t u = __float_as_uint(z);
     return (u >> 31) ? ~u : (u ^ 0x80000000u);
@@ -71,10 +83,12 @@ __device__ __forceinline__ int tile_floor(float v, int hi) {
     return (int)f;
 }
 
-// Start of a forward: device scalars, a new look-back epoch, zero tickets
-// and digit histograms, empty tile ranges, zero loss.
+// Start of a forward: device scalars, a new look-back epoch, zero tickets,
+// digit histograms and visible-count replicas, empty tile ranges, zero loss,
+// zero segment sums of the tile sort (zero_words words at zero).
 __global__ __launch_bounds__(256) void k_begin(uint32_t *sc, uint32_t count, int2 *ranges,
-                                               int ntiles, float *loss, uint32_t *ghist) {
+                                               int ntiles, float *loss, uint32_t *ghist,
+                                               uint32_t *zero, int zero_words) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < SC_EPOCH) sc[i] = (i == SC_COUNT) ? count : 0u;
     if (i == SC_EPOCH) {
@@ -83,8 +97,9 @@ __global__ __launch_bounds__(256) void k_begin(uint32_t *sc, uint32_t count, int
     }
     if (i == SC_ERR || (i >= SC_TICKET && i < SC_TICKET + LB_SLOTS)) sc[i] = 0u;
     if (i == 0 && loss) *loss = 0.f;
-    for (int t = i; t < kHistRows * kRadix; t += gridDim.x * 256) ghist[t] = 0u;
+    for (int t = i; t < kGhistWords; t += gridDim.x * 256) ghist[t] = 0u;
     for (int t = i; t < ntiles; t += gridDim.x * 256) ranges[t] = make_int2(0, 0);
+    for (int t = i; t < zero_words; t += gridDim.x * 256) zero[t] = 0u;
 }
 
 // Adds 1 to bin d of an LDS histogram for every active lane. When all active
@@ -126,18 +141,25 @@ __global__ __launch_bounds__(256) void k_zero10(ZeroList z, int n) {
 
 // Per splat: cull test and tile rectangle exactly as tileAssignKernel
 // (src/layers/tile_rasterize_layer.cu:76-125), with FP contraction off so
-// the decisions are reproducible on the host.
+// the decisions are reproducible on the host. With `hist` the four depth-byte
+// histograms go to replica blockIdx % kHistReps of ghist (the one-kernel
+// depth passes read them); the visible count goes to one of the kVisReps
+// words, once per workgroup.
 __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H, int ntx,
                                               int nty, uint2 *__restrict__ rect,
                                               uint32_t *__restrict__ ntiles,
                                               uint32_t *__restrict__ dkey,
                                               uint32_t *__restrict__ didx,
-                                              float4 *__restrict__ rec, uint32_t *sc,
-                                              uint32_t *ghist) {
+                                              float4 *__restrict__ rec, uint32_t *ghist,
+                                              int hist) {
 #pragma clang fp contract(off)
     __shared__ uint32_t s_h[4][kRadix];  // the four depth-byte histograms
+    __shared__ uint32_t s_vis;
+    if (hist) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) s_h[r][threadIdx.x] = 0u;
+        for (int r = 0; r < 4; ++r) s_h[r][threadIdx.x] = 0u;
+    }
+    if (threadIdx.x == 0) s_vis = 0u;
     __syncthreads();
     const int i = blockIdx.x * 256 + threadIdx.x;
     bool visible = false;
@@ -180,13 +202,17 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
             ntiles[i] = 0;
         }
         dkey[i] = key;
+        if (hist) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) lds_count(s_h[r], (key >> (8 * r)) & 0xFFu);
+            for (int r = 0; r < 4; ++r) lds_count(s_h[r], (key >> (8 * r)) & 0xFFu);
+        }
     }
     const uint64_t vis = __ballot(visible);
-    if (__lane_id() == 0 && vis) atomicAdd(&sc[SC_VISIBLE], (uint32_t)__popcll(vis));
+    if (__lane_id() == 0 && vis) atomicAdd(&s_vis, (uint32_t)__popcll(vis));
     __syncthreads();
-    flush_histograms(s_h, 4, ghist + LB_DEPTH * kRadix);
+    if (threadIdx.x == 0 && s_vis)
+        atomicAdd(&ghist[kVisOffset + (blockIdx.x % kVisReps) * kVisStride], s_vis);
+    if (hist) flush_histograms(s_h, 4, ghist + (blockIdx.x % kHistReps) * kHistRepWords);
 }
 
 // Writes the pairs of the splats, taken in depth order, each splat's tiles in
@@ -195,15 +221,22 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
 // tiles costs no more than thousands of small ones: slot p belongs to the
 // last depth rank d with off[d] <= p (off is the exclusive scan of the tile
 // counts in depth order, so that splat's range [off[d], off[d+1]) holds p and
-// is not empty), found by binary search.
+// is not empty), found by binary search. The first thread also totals the
+// visible-count replicas into the device scalar.
 __global__ __launch_bounds__(256) void k_emit(const uint32_t *__restrict__ order,
                                               const uint32_t *__restrict__ off,
-                                              const uint2 *__restrict__ rect,
-                                              const uint32_t *sc, int ntx,
+                                              const uint2 *__restrict__ rect, uint32_t *sc,
+                                              const uint32_t *__restrict__ ghist, int ntx,
                                               uint32_t *__restrict__ tkey,
                                               uint32_t *__restrict__ tval) {
     const uint32_t n = sc[SC_COUNT];
     const uint32_t m = sc[SC_NISECT];  // pairs kept: the total clamped at the cap
+    if (gtid() == 0) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int r = 0; r < kVisReps; ++r) v += ghist[kVisOffset + r * kVisStride];
+        sc[SC_VISIBLE] = v;
+    }
     for (int64_t p = gtid(); p < m; p += gstride()) {
         uint32_t lo = 0, hi = n;  // off[lo] <= p (off[0] == 0); the answer lies in [lo, hi)
         while (hi - lo > 1) {
@@ -240,6 +273,265 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t *__restrict__ tke
     }
 }
 
+// ---------------------------------------------------------------- tile sort
+// Stable counting sort of the depth-ordered pairs on their tile id in one
+// pass, for up to kTileBinsMax tiles (instead of 8-bit radix passes). The
+// pair slots are cut into chunks of kTileChunk; the pair-offset scan records
+// the splat owning each chunk's first slot.
+//   k_tile_emit     per chunk: emits the chunk's pairs (as k_emit), ranks
+//                   each among the chunk's pairs of the same tile in slot
+//                   order, writes (tile | rank << kTileIdBits) and the splat,
+//                   the chunk's per-tile counts (row c of a chunk-major count
+//                   matrix) and adds them to the sums of its segment (one of
+//                   kTileSegs runs of consecutive chunks)
+//   k_tile_segscan  per tile: exclusive scan over the segments, tile total
+//   k_tile_offsets  per tile and segment: the tile's first output slot
+//                   (exclusive scan of the totals), its range, and the count
+//                   matrix replaced in place by each (chunk, tile)'s first slot
+//   k_tile_scatter  per pair: output slot = (chunk, tile) slot + rank
+// Within a tile the pairs keep their emission order, i.e. depth order.
+constexpr int kTileChunk = 4096;
+constexpr int kTileThreads = 256;
+constexpr int kTileWaves = kTileThreads / kWave;         // 4
+constexpr int kTileWaveSlots = kTileChunk / kTileWaves;  // consecutive slots per wave
+constexpr int kTileGroups = kTileWaveSlots / kWave;      // 64-slot groups per wave
+constexpr int kTileItems = kTileChunk / kTileThreads;    // slots per thread in the head scan
+constexpr int kTileIdBits = 12;
+constexpr int kTileBinsMax = 1 << kTileIdBits;
+constexpr int kTileSegs = 32;
+
+// Chunks per segment for nch chunks.
+__device__ __forceinline__ uint32_t tile_seg_len(uint32_t nch) {
+    return (nch + kTileSegs - 1) / kTileSegs;
+}
+
+// LDS: per-wave tile counters (32 KB at kTileBinsMax), the owner of each slot
+// and each owner's first slot, 16-bit as both stay below kTileChunk.
+__global__ __launch_bounds__(kTileThreads) void k_tile_emit(
+    const uint32_t *__restrict__ order, const uint32_t *__restrict__ off,
+    const uint32_t *__restrict__ chunk_first, const uint2 *__restrict__ rect, uint32_t *sc,
+    const uint32_t *__restrict__ ghist, int ntx, int ntiles, int tile_bits,
+    uint32_t *__restrict__ pk, uint32_t *__restrict__ val, uint32_t *__restrict__ cnt,
+    uint32_t *seg) {
+    __shared__ uint16_t s_w[kTileWaves][kTileBinsMax];
+    __shared__ uint16_t s_head[kTileChunk];
+    __shared__ uint16_t s_first[kTileChunk];
+    __shared__ uint32_t s_red[kTileWaves];
+    __shared__ uint32_t s_first0;
+    const int tid = threadIdx.x, w = tid / kWave, lane = __lane_id();
+    const uint32_t n = sc[SC_COUNT];
+    const uint32_t m = sc[SC_NISECT];  // pairs kept: the total clamped at the cap
+    if (gtid() == 0) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int r = 0; r < kVisReps; ++r) v += ghist[kVisOffset + r * kVisStride];
+        sc[SC_VISIBLE] = v;
+    }
+    const uint32_t nch = (m + kTileChunk - 1) / kTileChunk;
+    const uint32_t G = tile_seg_len(nch);
+    const uint64_t lt = lanemask_lt();
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const uint32_t cs = c * kTileChunk;
+        const uint32_t cc = min((uint32_t)kTileChunk, m - cs);
+        for (int t = tid; t < ntiles; t += kTileThreads) {
+#pragma unroll
+            for (int q = 0; q < kTileWaves; ++q) s_w[q][t] = 0;
+        }
+        for (int i = tid; i < kTileChunk; i += kTileThreads) s_head[i] = 0;
+        // The splat owning the chunk's first slot (depth rank).
+        const uint32_t d0 = min(chunk_first[c], n - 1);
+        // Later splats starting inside the chunk: every visible splat has at
+        // least one pair, so they are d0 + k for k < cc, at strictly rising
+        // slots past cs; culled splats (depth-sorted last) start at or after
+        // m and are skipped.
+        if (tid == 0) s_first0 = off[d0];
+        __syncthreads();  // s_head zeroed
+        for (int k = 1 + tid; k < kTileChunk; k += kTileThreads) {
+            const uint32_t d = d0 + (uint32_t)k;
+            if (d < n) {
+                const uint32_t o = off[d];
+                if (o > cs && o < cs + cc) {
+                    s_head[o - cs] = (uint16_t)k;
+                    s_first[k] = (uint16_t)(o - cs);
+                }
+            }
+        }
+        __syncthreads();
+        // Inclusive max-scan of the heads: the owner of every slot.
+        {
+            uint32_t v[kTileItems];
+            uint32_t mx = 0;
+#pragma unroll
+            for (int j = 0; j < kTileItems; ++j) {
+                v[j] = s_head[tid * kTileItems + j];
+                mx = max(mx, v[j]);
+            }
+            uint32_t incl = mx;
+#pragma unroll
+            for (int dd = 1; dd < kWave; dd <<= 1) {
+                const uint32_t y = __shfl_up(incl, dd, kWave);
+                if (lane >= dd) incl = max(incl, y);
+            }
+            if (lane == kWave - 1) s_red[w] = incl;
+            __syncthreads();
+            uint32_t run = __shfl_up(incl, 1, kWave);
+            if (lane == 0) run = 0;
+            for (int q = 0; q < w; ++q) run = max(run, s_red[q]);
+#pragma unroll
+            for (int j = 0; j < kTileItems; ++j) {
+                run = max(run, v[j]);
+                s_head[tid * kTileItems + j] = (uint16_t)run;
+            }
+        }
+        __syncthreads();
+        // Wave w takes slots [w * kTileWaveSlots, (w + 1) * kTileWaveSlots)
+        // in groups of 64, in order.
+        uint32_t tile[kTileGroups], gid[kTileGroups], rnk[kTileGroups];
+#pragma unroll
+        for (int j = 0; j < kTileGroups; ++j) {
+            const uint32_t q = (uint32_t)(w * kTileWaveSlots + j * kWave + lane);
+            gid[j] = (q < cc) ? order[d0 + s_head[q]] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < kTileGroups; ++j) {
+            const uint32_t q = (uint32_t)(w * kTileWaveSlots + j * kWave + lane);
+            tile[j] = 0;
+            if (q < cc) {
+                const uint32_t k = s_head[q];
+                const uint32_t jj = (k == 0) ? cs + q - s_first0 : q - s_first[k];
+                const uint2 r = rect[gid[j]];
+                const uint32_t tx0 = r.x & 0xFFFF, tx1 = r.x >> 16, ty0 = r.y & 0xFFFF;
+                const uint32_t wd = tx1 - tx0 + 1;
+                tile[j] = (ty0 + jj / wd) * (uint32_t)ntx + tx0 + jj % wd;
+            }
+        }
+        // Rank among the wave's earlier slots of the same tile: lanes of one
+        // tile found by matching the tile bits with ballots; the group's
+        // first such lane advances the wave's counter.
+#pragma unroll
+        for (int j = 0; j < kTileGroups; ++j) {
+            const uint32_t q = (uint32_t)(w * kTileWaveSlots + j * kWave + lane);
+            const bool valid = q < cc;
+            const uint32_t t = tile[j];
+            uint64_t peers = __ballot(valid);
+            for (int bit = 0; bit < tile_bits; ++bit) {
+                const bool on = (t >> bit) & 1u;
+                const uint64_t bb = __ballot(on);
+                peers &= on ? bb : ~bb;
+            }
+            const uint32_t r = (uint32_t)__popcll(peers & lt);
+            uint32_t old = 0;
+            if (valid) old = s_w[w][t];
+            if (valid && r == 0) s_w[w][t] = (uint16_t)(old + (uint32_t)__popcll(peers));
+            rnk[j] = old + r;
+        }
+        __syncthreads();
+        // Per tile: each wave's first rank, the chunk's count, its segment sum.
+        for (int t = tid; t < ntiles; t += kTileThreads) {
+            uint32_t run = 0;
+#pragma unroll
+            for (int q = 0; q < kTileWaves; ++q) {
+                const uint32_t v = s_w[q][t];
+                s_w[q][t] = (uint16_t)run;
+                run += v;
+            }
+            cnt[(size_t)c * ntiles + t] = run;
+            if (run) atomicAdd(&seg[(size_t)(c / G) * ntiles + t], run);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kTileGroups; ++j) {
+            const uint32_t q = (uint32_t)(w * kTileWaveSlots + j * kWave + lane);
+            if (q < cc) {
+                const uint32_t t = tile[j];
+                pk[cs + q] = t | ((s_w[w][t] + rnk[j]) << kTileIdBits);
+                val[cs + q] = gid[j];
+            }
+        }
+        __syncthreads();  // the LDS is reused by the next chunk
+    }
+}
+
+// Per tile: segment sums -> their exclusive scan (in place), and the total.
+__global__ __launch_bounds__(256) void k_tile_segscan(uint32_t *__restrict__ seg,
+                                                      uint32_t *__restrict__ tot, int ntiles) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= ntiles) return;
+    uint32_t v[kTileSegs];
+#pragma unroll
+    for (int s = 0; s < kTileSegs; ++s) v[s] = seg[(size_t)s * ntiles + t];
+    uint32_t run = 0;
+#pragma unroll
+    for (int s = 0; s < kTileSegs; ++s) {
+        seg[(size_t)s * ntiles + t] = run;
+        run += v[s];
+    }
+    tot[t] = run;
+}
+
+// Workgroup (x, s): tiles [256 x, 256 x + 256) in segment s. The tile's first
+// output slot is the exclusive scan of the totals; segment 0 writes the
+// ranges of the non-empty tiles (empty ones stay (0, 0)); the counts of the
+// segment's chunks become first slots.
+__global__ __launch_bounds__(256) void k_tile_offsets(uint32_t *__restrict__ cnt,
+                                                      const uint32_t *__restrict__ seg,
+                                                      const uint32_t *__restrict__ tot,
+                                                      const uint32_t *sc, int ntiles,
+                                                      int2 *__restrict__ ranges) {
+    __shared__ uint32_t s_tmp[256 / kWave];
+    const uint32_t m = sc[SC_NISECT];
+    const uint32_t nch = (m + kTileChunk - 1) / kTileChunk;
+    const uint32_t G = tile_seg_len(nch);
+    const uint32_t s = blockIdx.y;
+    const int t0 = blockIdx.x * 256;
+    const int t = t0 + threadIdx.x;
+    uint32_t before = 0;
+    for (int i = threadIdx.x; i < t0; i += 256) before += tot[i];
+    uint32_t base;
+    block_excl_scan<256>(before, s_tmp, &base);
+    const uint32_t mine = (t < ntiles) ? tot[t] : 0u;
+    const uint32_t start = base + block_excl_scan<256>(mine, s_tmp, nullptr);
+    if (t >= ntiles) return;
+    if (s == 0 && mine) ranges[t] = make_int2((int)start, (int)(start + mine));
+    uint32_t run = start + seg[(size_t)s * ntiles + t];
+    const uint32_t c1 = min(s * G + G, nch);
+    for (uint32_t c = s * G; c < c1; c += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = (c + j < c1) ? cnt[(size_t)(c + j) * ntiles + t] : 0u;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (c + j < c1) {
+                cnt[(size_t)(c + j) * ntiles + t] = run;
+                run += v[j];
+            }
+        }
+    }
+}
+
+// Per pair: its sorted position; optionally the 64-bit (tile | depth) key.
+__global__ __launch_bounds__(256) void k_tile_scatter(const uint32_t *__restrict__ pk,
+                                                      const uint32_t *__restrict__ val,
+                                                      const uint32_t *__restrict__ cnt,
+                                                      const uint32_t *sc, int ntiles,
+                                                      const float *__restrict__ pos_z,
+                                                      uint32_t *__restrict__ flat,
+                                                      uint64_t *__restrict__ keys64) {
+    const uint32_t m = sc[SC_NISECT];
+    for (int64_t p = gtid(); p < m; p += gstride()) {
+        const uint32_t x = pk[p];
+        const uint32_t t = x & (uint32_t)(kTileBinsMax - 1);
+        const uint32_t c = (uint32_t)p / kTileChunk;
+        if (t >= (uint32_t)ntiles) continue;  // never, for consistent counts
+        const uint32_t dst = cnt[(size_t)c * ntiles + t] + (x >> kTileIdBits);
+        const uint32_t g = val[p];
+        if (dst < m) {  // always, for consistent counts
+            flat[dst] = g;
+            if (keys64) keys64[dst] = ((uint64_t)t << 32) | depth_bits(pos_z[g]);
+        }
+    }
+}
+
 int grid_cap(uint64_t n, int per, int cap) {
     uint64_t b = (n + per - 1) / per;
     if (b < 1) b = 1;
@@ -268,6 +560,9 @@ struct ds_raster {
     // per pair
     ds::DevBuf<uint32_t> tkey_a, tkey_b, tval_a, tval_b;
     ds::DevBuf<uint64_t> keys64;
+    // one-pass tile sort (up to kTileBinsMax tiles): chunk x tile counts,
+    // segment x tile sums, tile totals, the splat owning each chunk's first slot
+    ds::DevBuf<uint32_t> tcnt, tseg, ttot, tfirst;
     // per tile / pixel
     ds::DevBuf<int2> ranges;
     ds::DevBuf<float> T;
@@ -279,7 +574,6 @@ struct ds_raster {
     ds::DevBuf<uint64_t> lb_depth, lb_scan;
     // results of the last forward
     const uint32_t *flat = nullptr;
-    const uint32_t *tkey_sorted = nullptr;
     int last_count = -1;
     bool has_fwd = false;
 };
@@ -297,6 +591,18 @@ static ds_status raster_alloc_pixels(ds_raster *r, int W, int H) {
     while ((1 << bits) < r->ntiles) ++bits;
     r->tile_bits = bits;
     r->tile_passes = (bits + kRadixBits - 1) / kRadixBits;
+    if (r->ntiles <= kTileBinsMax) {
+        const size_t max_chunks = ((size_t)r->isect_cap + kTileChunk - 1) / kTileChunk;
+        DS_TRY(r->tcnt.alloc(max_chunks * (size_t)r->ntiles));
+        DS_TRY(r->tseg.alloc((size_t)kTileSegs * r->ntiles));
+        DS_TRY(r->ttot.alloc((size_t)r->ntiles));
+        DS_TRY(r->tfirst.alloc(max_chunks));
+    } else {
+        r->tcnt.release();
+        r->tseg.release();
+        r->ttot.release();
+        r->tfirst.release();
+    }
     DS_TRY(r->ranges.alloc((size_t)r->ntiles));
     DS_TRY(r->T.alloc((size_t)W * H));
     DS_TRY(r->last.alloc((size_t)W * H));
@@ -317,13 +623,18 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
     auto slot = [&](uint64_t *status, int lb) {
         return LookbackSlot{status, sc + SC_TICKET + lb, sc + SC_EPOCH, sc + SC_ERR};
     };
-    hipLaunchKernelGGL(k_begin, dim3(grid_cap((uint64_t)r->ntiles, 256, 256)), dim3(256), 0, s,
-                       sc, (uint32_t)n, r->ranges.ptr, r->ntiles, loss, gh);
+    const bool one_pass_tiles = r->ntiles <= kTileBinsMax;
+    const int zero_words = one_pass_tiles ? kTileSegs * r->ntiles : 0;
+    const uint64_t begin_items =
+        std::max<uint64_t>({(uint64_t)r->ntiles, (uint64_t)zero_words / 4, kGhistWords / 4});
+    hipLaunchKernelGGL(k_begin, dim3(grid_cap(begin_items, 256, 1024)), dim3(256), 0, s, sc,
+                       (uint32_t)n, r->ranges.ptr, r->ntiles, loss, gh, r->tseg.ptr, zero_words);
     DS_CHECK_LAUNCH("begin");
     if (n > 0) {
+        const bool one_kernel = (uint32_t)n <= kOneKernelMaxKeys;
         hipLaunchKernelGGL(k_prep, dim3((n + 255) / 256), dim3(256), 0, s, in, n, r->W, r->H,
                            r->ntx, r->nty, r->rect.ptr, r->ntiles_of.ptr, r->dkey_a.ptr,
-                           r->didx_a.ptr, r->rec.ptr, sc, gh);
+                           r->didx_a.ptr, r->rec.ptr, gh, one_kernel ? 1 : 0);
         DS_CHECK_LAUNCH("prep");
         // Depth sort of splats: four 8-bit LSD passes over the 32 depth bits,
         // then the pair offsets in depth order (total and capped count). Up
@@ -332,12 +643,12 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
         // passes win, as the look-back traffic grows with the chunk count.
         uint32_t *ka = r->dkey_a.ptr, *kb = r->dkey_b.ptr, *va = r->didx_a.ptr,
                  *vb = r->didx_b.ptr;
-        const bool one_kernel = (uint32_t)n <= kOneKernelMaxKeys;
         const size_t dstride = lookback_words((uint64_t)r->count_cap, kRadix);
         for (int pass = 0; pass < 4; ++pass) {
             if (one_kernel)
                 DS_TRY(onesweep_pass(ka, va, kb, vb, sc + SC_COUNT, (uint32_t)n, 8 * pass,
-                                     gh + (LB_DEPTH + pass) * kRadix,
+                                     gh + (LB_DEPTH + pass) * kRadix, kHistReps,
+                                     kHistRepWords,
                                      slot(r->lb_depth.ptr + pass * dstride, LB_DEPTH + pass), s));
             else
                 DS_TRY(radix_pass<uint32_t>(ka, va, kb, vb, sc + SC_COUNT, (uint32_t)n, 8 * pass,
@@ -346,40 +657,69 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
             std::swap(va, vb);
         }
         const uint32_t *order = va;
+        const uint64_t max_chunks = ((uint64_t)r->isect_cap + kTileChunk - 1) / kTileChunk;
+        static_assert(kTileChunk == 1 << 12, "chunk heads use shift 12");
+        const ScanHeads heads = one_pass_tiles
+                                    ? ScanHeads{r->tfirst.ptr, 12u, (uint32_t)max_chunks}
+                                    : ScanHeads{nullptr, 0u, 0u};
         if (one_kernel)
             DS_TRY(scan_chained(r->ntiles_of.ptr, order, r->pair_off.ptr, sc + SC_COUNT,
                                 (uint32_t)n, sc + SC_WANTED, sc + SC_NISECT, r->isect_cap,
-                                slot(r->lb_scan.ptr, LB_SCAN), s));
+                                slot(r->lb_scan.ptr, LB_SCAN), s, heads));
         else
             DS_TRY(scan_exclusive_u32(r->ntiles_of.ptr, order, r->pair_off.ptr,
                                       ScanLen{sc + SC_COUNT, 1, 1}, (uint32_t)n, r->part.ptr,
-                                      sc + SC_WANTED, sc + SC_NISECT, r->isect_cap, s));
-        hipLaunchKernelGGL(k_emit, dim3(grid_cap((uint64_t)r->isect_cap, 256, 4096)), dim3(256),
-                           0, s, order, (const uint32_t *)r->pair_off.ptr,
-                           (const uint2 *)r->rect.ptr, (const uint32_t *)sc, r->ntx,
-                           r->tkey_a.ptr, r->tval_a.ptr);
-        DS_CHECK_LAUNCH("emit");
-        // Stable sort of the pairs on the tile bits. With hundreds of chunks
-        // the reduce-then-scan passes beat one-kernel passes, whose look-back
-        // traffic grows with the chunk count.
-        uint32_t *tka = r->tkey_a.ptr, *tkb = r->tkey_b.ptr, *tva = r->tval_a.ptr,
-                 *tvb = r->tval_b.ptr;
-        for (int pass = 0; pass < r->tile_passes; ++pass) {
-            DS_TRY(radix_pass<uint32_t>(tka, tva, tkb, tvb, sc + SC_NISECT, r->isect_cap,
-                                        kRadixBits * pass, r->hist.ptr, r->part.ptr, s));
-            std::swap(tka, tkb);
-            std::swap(tva, tvb);
+                                      sc + SC_WANTED, sc + SC_NISECT, r->isect_cap, s, heads));
+        if (one_pass_tiles) {
+            hipLaunchKernelGGL(k_tile_emit, dim3(grid_cap(max_chunks, 1, 1024)),
+                               dim3(kTileThreads), 0, s, order,
+                               (const uint32_t *)r->pair_off.ptr,
+                               (const uint32_t *)r->tfirst.ptr, (const uint2 *)r->rect.ptr, sc,
+                               (const uint32_t *)gh, r->ntx, r->ntiles, r->tile_bits,
+                               r->tkey_a.ptr, r->tval_a.ptr, r->tcnt.ptr, r->tseg.ptr);
+            DS_CHECK_LAUNCH("tile_emit");
+            const int tblocks = (r->ntiles + 255) / 256;
+            hipLaunchKernelGGL(k_tile_segscan, dim3(tblocks), dim3(256), 0, s, r->tseg.ptr,
+                               r->ttot.ptr, r->ntiles);
+            DS_CHECK_LAUNCH("tile_segscan");
+            hipLaunchKernelGGL(k_tile_offsets, dim3(tblocks, kTileSegs), dim3(256), 0, s,
+                               r->tcnt.ptr, (const uint32_t *)r->tseg.ptr,
+                               (const uint32_t *)r->ttot.ptr, (const uint32_t *)sc, r->ntiles,
+                               r->ranges.ptr);
+            DS_CHECK_LAUNCH("tile_offsets");
+            hipLaunchKernelGGL(k_tile_scatter, dim3(grid_cap(r->isect_cap, 256, 4096)),
+                               dim3(256), 0, s, (const uint32_t *)r->tkey_a.ptr,
+                               (const uint32_t *)r->tval_a.ptr, (const uint32_t *)r->tcnt.ptr,
+                               (const uint32_t *)sc, r->ntiles, (const float *)in.pos_z,
+                               r->tval_b.ptr, r->keep_keys ? r->keys64.ptr : (uint64_t *)nullptr);
+            DS_CHECK_LAUNCH("tile_scatter");
+            r->flat = r->tval_b.ptr;
+        } else {
+            hipLaunchKernelGGL(k_emit, dim3(grid_cap((uint64_t)r->isect_cap, 256, 4096)),
+                               dim3(256), 0, s, order, (const uint32_t *)r->pair_off.ptr,
+                               (const uint2 *)r->rect.ptr, sc, (const uint32_t *)gh, r->ntx,
+                               r->tkey_a.ptr, r->tval_a.ptr);
+            DS_CHECK_LAUNCH("emit");
+            // Stable sort of the pairs on the tile bits. With hundreds of
+            // chunks the reduce-then-scan passes beat one-kernel passes,
+            // whose look-back traffic grows with the chunk count.
+            uint32_t *tka = r->tkey_a.ptr, *tkb = r->tkey_b.ptr, *tva = r->tval_a.ptr,
+                     *tvb = r->tval_b.ptr;
+            for (int pass = 0; pass < r->tile_passes; ++pass) {
+                DS_TRY(radix_pass<uint32_t>(tka, tva, tkb, tvb, sc + SC_NISECT, r->isect_cap,
+                                            kRadixBits * pass, r->hist.ptr, r->part.ptr, s));
+                std::swap(tka, tkb);
+                std::swap(tva, tvb);
+            }
+            r->flat = tva;
+            hipLaunchKernelGGL(k_ranges, dim3(grid_cap(r->isect_cap, 256, 4096)), dim3(256), 0,
+                               s, (const uint32_t *)tka, (const uint32_t *)tva,
+                               (const float *)in.pos_z, (const uint32_t *)sc, r->ranges.ptr,
+                               r->keep_keys ? r->keys64.ptr : (uint64_t *)nullptr);
+            DS_CHECK_LAUNCH("ranges");
         }
-        r->tkey_sorted = tka;
-        r->flat = tva;
-        hipLaunchKernelGGL(k_ranges, dim3(grid_cap(r->isect_cap, 256, 4096)), dim3(256), 0, s,
-                           (const uint32_t *)tka, (const uint32_t *)tva, (const float *)in.pos_z,
-                           (const uint32_t *)sc, r->ranges.ptr,
-                           r->keep_keys ? r->keys64.ptr : (uint64_t *)nullptr);
-        DS_CHECK_LAUNCH("ranges");
     } else {
         r->flat = r->tval_a.ptr;
-        r->tkey_sorted = r->tkey_a.ptr;
     }
     const CompositeArgs ca{(const float4 *)r->rec.ptr, r->flat, (const int2 *)r->ranges.ptr,
                            r->W, r->H, r->ntx, r->ntiles};
@@ -443,7 +783,7 @@ extern "C" ds_status ds_raster_create(const ds_raster_config *cfg, ds_raster **o
     if (st == DS_OK) st = fill_u32((uint32_t *)r->lb_depth.ptr, 0u, 2 * lbd, nullptr);
     if (st == DS_OK) st = r->lb_scan.alloc(lbs);
     if (st == DS_OK) st = fill_u32((uint32_t *)r->lb_scan.ptr, 0u, 2 * lbs, nullptr);
-    if (st == DS_OK) st = r->ghist.alloc((size_t)kHistRows * kRadix);
+    if (st == DS_OK) st = r->ghist.alloc((size_t)kGhistWords);
     if (st == DS_OK)
         st = r->hist.alloc(sort_hist_words(std::max<uint64_t>((uint64_t)cap, (uint64_t)nc)));
     if (st == DS_OK) st = r->part.alloc(kSpineMax);

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *in,
                                                             const uint32_t *__restrict__ gidx,
                                                             ScanLen L,
                                                             const uint32_t *__restrict__ part,
-                                                            uint32_t *out) {
+                                                            uint32_t *out, ScanHeads H) {
     __shared__ uint32_t s_tmp[kScanThreads / kWave];
     const uint32_t len = scan_len(L);
     const uint32_t nblk = (len + kScanChunk - 1) / kScanChunk;
@@ -83,6 +83,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *in,
         for (int j = 0; j < kScanItems; ++j) {
             const uint32_t e = base + j;
             if (e < len) out[e] = run;
+            mark_heads(H, e, run, v[j]);
             run += v[j];
         }
     }
@@ -307,17 +308,26 @@ __device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
 __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, const uint32_t *n_dev, int shift,
-    const uint32_t *__restrict__ ghist, LookbackSlot lb) {
+    const uint32_t *__restrict__ ghist, int hist_reps, uint32_t hist_rep_stride,
+    LookbackSlot lb) {
     __shared__ ChunkLds<uint32_t> L;
     const uint32_t n = *n_dev;
     const uint32_t nch = (n + kSortChunk - 1) / kSortChunk;
     const uint32_t epoch = *lb.epoch;
     const int tid = threadIdx.x;
+    // The digit's count over all keys; its loads and the first ticket share
+    // one memory wait.
+    uint32_t count = 0;
+    for (int r = 0; r < hist_reps; ++r) count += ghist[(size_t)r * hist_rep_stride + tid];
+    if (tid == 0) L.ticket = atomicAdd(lb.ticket, 1u);
+    __syncthreads();
     // first position of each digit in the output
-    const uint32_t dbase = block_excl_scan<kSortThreads>(ghist[tid], L.tmp, nullptr);
-    for (;;) {
-        if (tid == 0) L.ticket = atomicAdd(lb.ticket, 1u);
-        __syncthreads();
+    const uint32_t dbase = block_excl_scan<kSortThreads>(count, L.tmp, nullptr);
+    for (bool first = true;; first = false) {
+        if (!first) {
+            if (tid == 0) L.ticket = atomicAdd(lb.ticket, 1u);
+            __syncthreads();
+        }
         const uint32_t c = L.ticket;
         if (c >= nch) break;  // uniform
         const uint32_t base = c * kSortChunk;
@@ -346,7 +356,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
 __global__ __launch_bounds__(kScanThreads) void k_scan_chained(
     const uint32_t *__restrict__ in, const uint32_t *__restrict__ gidx, const uint32_t *n_dev,
     uint32_t *__restrict__ out, uint32_t *total_out, uint32_t *clamped_out, uint32_t cap,
-    LookbackSlot lb) {
+    LookbackSlot lb, ScanHeads H) {
     __shared__ uint32_t s_tmp[kScanThreads / kWave];
     __shared__ uint32_t s_ticket, s_excl;
     const uint32_t n = *n_dev;
@@ -399,6 +409,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_chained(
         for (int j = 0; j < kScanItems; ++j) {
             const uint32_t e = base + j;
             if (e < n) out[e] = acc;
+            mark_heads(H, e, acc, v[j]);
             acc = sat_add(acc, v[j]);
         }
         __syncthreads();
@@ -414,7 +425,8 @@ int clamp_grid(uint64_t blocks, int cap) {
 
 ds_status scan_exclusive_u32(const uint32_t *in, const uint32_t *gather_idx, uint32_t *out,
                              ScanLen len, uint32_t max_len, uint32_t *part, uint32_t *total_out,
-                             uint32_t *clamped_out, uint32_t clamp_cap, hipStream_t s) {
+                             uint32_t *clamped_out, uint32_t clamp_cap, hipStream_t s,
+                             ScanHeads heads) {
     const uint64_t max_blocks = ((uint64_t)max_len + kScanChunk - 1) / kScanChunk;
     if (max_blocks > (uint64_t)kSpineMax)
         return set_error(DS_ERR_CAPACITY, "scan longer than " +
@@ -430,7 +442,7 @@ ds_status scan_exclusive_u32(const uint32_t *in, const uint32_t *gather_idx, uin
     DS_CHECK_LAUNCH("scan_spine");
     if (max_blocks > 0) {
         hipLaunchKernelGGL(k_scan_down, dim3(grid), dim3(kScanThreads), 0, s, in, gather_idx,
-                           len, (const uint32_t *)part, out);
+                           len, (const uint32_t *)part, out, heads);
         DS_CHECK_LAUNCH("scan_down");
     }
     return DS_OK;
@@ -464,23 +476,26 @@ template ds_status radix_pass<uint64_t>(const uint64_t *, const uint32_t *, uint
 
 ds_status onesweep_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                         uint32_t *vout, const uint32_t *n_dev, uint32_t max_n, int shift,
-                        const uint32_t *ghist, LookbackSlot lb, hipStream_t s) {
+                        const uint32_t *ghist, int hist_reps, uint32_t hist_rep_stride,
+                        LookbackSlot lb, hipStream_t s) {
     if (max_n > kLookbackMaxCount)
         return set_error(DS_ERR_CAPACITY, "radix pass longer than 2^30 - 1 keys");
     const uint64_t max_chunks = ((uint64_t)max_n + kSortChunk - 1) / kSortChunk;
     if (max_chunks == 0) return DS_OK;
     hipLaunchKernelGGL(k_onesweep, dim3(clamp_grid(max_chunks, 2048)), dim3(kSortThreads), 0, s,
-                       kin, vin, kout, vout, n_dev, shift, ghist, lb);
+                       kin, vin, kout, vout, n_dev, shift, ghist, hist_reps, hist_rep_stride,
+                       lb);
     DS_CHECK_LAUNCH("onesweep");
     return DS_OK;
 }
 
 ds_status scan_chained(const uint32_t *in, const uint32_t *gidx, uint32_t *out,
                        const uint32_t *n_dev, uint32_t max_n, uint32_t *total_out,
-                       uint32_t *clamped_out, uint32_t cap, LookbackSlot lb, hipStream_t s) {
+                       uint32_t *clamped_out, uint32_t cap, LookbackSlot lb, hipStream_t s,
+                       ScanHeads heads) {
     const uint64_t max_chunks = ((uint64_t)max_n + kScanChunk - 1) / kScanChunk;
     hipLaunchKernelGGL(k_scan_chained, dim3(clamp_grid(max_chunks, 1024)), dim3(kScanThreads), 0,
-                       s, in, gidx, n_dev, out, total_out, clamped_out, cap, lb);
+                       s, in, gidx, n_dev, out, total_out, clamped_out, cap, lb, heads);
     DS_CHECK_LAUNCH("scan_chained");
     return DS_OK;
 }

@@ -67,9 +67,28 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_tmp,
     return wbase + inc - v;
 }
 
+// Optional scan side output: for every multiple b * 2^shift (b < count) of
+// the output positions that falls in an element's range [out[e], out[e] +
+// in[e]), first[b] = e. With first == nullptr nothing is written.
+struct ScanHeads {
+    uint32_t *first;
+    uint32_t shift;
+    uint32_t count;
+};
+
+__device__ __forceinline__ void mark_heads(const ScanHeads &H, uint32_t e, uint32_t start,
+                                           uint32_t len) {
+    if (!H.first || len == 0) return;
+    const uint64_t end = (uint64_t)start + len;
+    for (uint64_t b = ((uint64_t)start + (1u << H.shift) - 1) >> H.shift;
+         b < H.count && (b << H.shift) < end; ++b)
+        H.first[b] = e;
+}
+
 ds_status scan_exclusive_u32(const uint32_t *in, const uint32_t *gather_idx, uint32_t *out,
                              ScanLen len, uint32_t max_len, uint32_t *part, uint32_t *total_out,
-                             uint32_t *clamped_out, uint32_t clamp_cap, hipStream_t s);
+                             uint32_t *clamped_out, uint32_t clamp_cap, hipStream_t s,
+                             ScanHeads heads = ScanHeads{nullptr, 0, 0});
 
 // One stable LSD pass: sort (key, val) by bits [shift, shift+8) of the key.
 // n_dev: element count; hist: scratch of kRadix * ceil(max_n / kSortChunk)
@@ -110,16 +129,19 @@ inline size_t lookback_words(uint64_t max_n, int digits) {
     return (size_t)digits * (size_t)((max_n + kSortChunk - 1) / kSortChunk);
 }
 
-// Stable pass on bits [shift, shift + 8) of 32-bit keys; ghist holds the
-// pass's 256-bin digit histogram over all n keys.
+// Stable pass on bits [shift, shift + 8) of 32-bit keys. The pass's 256-bin
+// digit histogram over all n keys is the sum of hist_reps partial copies,
+// copy r at ghist + r * hist_rep_stride.
 ds_status onesweep_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                         uint32_t *vout, const uint32_t *n_dev, uint32_t max_n, int shift,
-                        const uint32_t *ghist, LookbackSlot lb, hipStream_t s);
+                        const uint32_t *ghist, int hist_reps, uint32_t hist_rep_stride,
+                        LookbackSlot lb, hipStream_t s);
 
 // Exclusive scan of in[gidx[e]] (or in[e]) for e < n_dev[0], in one kernel;
 // writes the total and min(total, cap).
 ds_status scan_chained(const uint32_t *in, const uint32_t *gidx, uint32_t *out,
                        const uint32_t *n_dev, uint32_t max_n, uint32_t *total_out,
-                       uint32_t *clamped_out, uint32_t cap, LookbackSlot lb, hipStream_t s);
+                       uint32_t *clamped_out, uint32_t cap, LookbackSlot lb, hipStream_t s,
+                       ScanHeads heads = ScanHeads{nullptr, 0, 0});
 
 }  // namespace ds

@@ -77,12 +77,14 @@ def test_activation_projection_forward(deg):
     check_close(f"activ_project_fwd_sh{deg}", s2f, r2, live)
 
 
-@pytest.mark.parametrize("scene", ["c0", "ragged", "c1"])
+# c1: the fit workload (one-kernel depth passes); big: above 131072 splats,
+# where the depth passes are reduce-then-scan kernels.
+BIG_SCENES = {"c1": (1000, 60000, 1280, 720, -3.0), "big": (1001, 200000, 640, 360, -3.5)}
+
+
+@pytest.mark.parametrize("scene", ["c0", "ragged", "c1", "big"])
 def test_tile_assign_sort_ranges_bitexact(scene):
-    if scene == "c1":
-        seed, n, W, H, ls = 1000, 60000, 1280, 720, -3.0
-    else:
-        seed, n, W, H, ls = SCENES[scene]
+    seed, n, W, H, ls = BIG_SCENES[scene] if scene in BIG_SCENES else SCENES[scene]
     params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
     st = Stages(params, n, 0, W, H, view, proj, cam)
     s2 = st.activ_project_fwd()
