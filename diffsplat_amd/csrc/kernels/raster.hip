@@ -306,7 +306,9 @@ __device__ __forceinline__ uint32_t tile_seg_len(uint32_t nch) {
 }
 
 // LDS: per-wave tile counters (32 KB at kTileBinsMax), the owner of each slot
-// and each owner's first slot, 16-bit as both stay below kTileChunk.
+// and each owner's first slot (16-bit: both stay below kTileChunk), and per
+// splat of the chunk its id and tile box. Three memory round trips per chunk:
+// the chunk's first and last splats, their offsets and ids, their boxes.
 __global__ __launch_bounds__(kTileThreads) void k_tile_emit(
     const uint32_t *__restrict__ order, const uint32_t *__restrict__ off,
     const uint32_t *__restrict__ chunk_first, const uint2 *__restrict__ rect, uint32_t *sc,
@@ -316,6 +318,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_emit(
     __shared__ uint16_t s_w[kTileWaves][kTileBinsMax];
     __shared__ uint16_t s_head[kTileChunk];
     __shared__ uint16_t s_first[kTileChunk];
+    __shared__ uint32_t s_gid[kTileChunk];
+    __shared__ uint32_t s_box[kTileChunk];  // tx0 | ty0 << 10 | (tx1 - tx0) << 20
     __shared__ uint32_t s_red[kTileWaves];
     __shared__ uint32_t s_first0;
     const int tid = threadIdx.x, w = tid / kWave, lane = __lane_id();
@@ -338,21 +342,39 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_emit(
             for (int q = 0; q < kTileWaves; ++q) s_w[q][t] = 0;
         }
         for (int i = tid; i < kTileChunk; i += kTileThreads) s_head[i] = 0;
-        // The splat owning the chunk's first slot (depth rank).
+        // The chunk's splats are the depth ranks d0..dn: the owners of its
+        // first slot and of the next chunk's (the last chunk: up to n - 1).
         const uint32_t d0 = min(chunk_first[c], n - 1);
-        // Later splats starting inside the chunk: every visible splat has at
-        // least one pair, so they are d0 + k for k < cc, at strictly rising
-        // slots past cs; culled splats (depth-sorted last) start at or after
-        // m and are skipped.
-        if (tid == 0) s_first0 = off[d0];
+        const uint32_t dn = (c + 1 < nch) ? min(chunk_first[c + 1], n - 1) : n - 1;
+        const uint32_t K = (dn >= d0) ? min((uint32_t)kTileChunk, dn - d0 + 1) : 1u;
+        uint32_t o[kTileItems], g[kTileItems];
+#pragma unroll
+        for (int i = 0; i < kTileItems; ++i) {
+            const uint32_t k = (uint32_t)(tid + i * kTileThreads);
+            o[i] = g[i] = 0;
+            if (k < K) {
+                o[i] = off[d0 + k];
+                g[i] = order[d0 + k];
+            }
+        }
         __syncthreads();  // s_head zeroed
-        for (int k = 1 + tid; k < kTileChunk; k += kTileThreads) {
-            const uint32_t d = d0 + (uint32_t)k;
-            if (d < n) {
-                const uint32_t o = off[d];
-                if (o > cs && o < cs + cc) {
-                    s_head[o - cs] = (uint16_t)k;
-                    s_first[k] = (uint16_t)(o - cs);
+        // Every visible splat has at least one pair, so the splats after d0
+        // start at strictly rising slots past cs; those starting at or after
+        // the chunk's end (culled splats are depth-sorted last and start at
+        // or after m) are not heads.
+#pragma unroll
+        for (int i = 0; i < kTileItems; ++i) {
+            const uint32_t k = (uint32_t)(tid + i * kTileThreads);
+            if (k < K) {
+                const uint2 r = rect[g[i]];
+                const uint32_t tx0 = r.x & 0xFFFF, tx1 = r.x >> 16, ty0 = r.y & 0xFFFF;
+                s_gid[k] = g[i];
+                s_box[k] = tx0 | (ty0 << 10) | ((tx1 - tx0) << 20);
+                if (k == 0) {
+                    s_first0 = o[i];
+                } else if (o[i] > cs && o[i] < cs + cc) {
+                    s_head[o[i] - cs] = (uint16_t)k;
+                    s_first[k] = (uint16_t)(o[i] - cs);
                 }
             }
         }
@@ -390,18 +412,13 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_emit(
 #pragma unroll
         for (int j = 0; j < kTileGroups; ++j) {
             const uint32_t q = (uint32_t)(w * kTileWaveSlots + j * kWave + lane);
-            gid[j] = (q < cc) ? order[d0 + s_head[q]] : 0u;
-        }
-#pragma unroll
-        for (int j = 0; j < kTileGroups; ++j) {
-            const uint32_t q = (uint32_t)(w * kTileWaveSlots + j * kWave + lane);
-            tile[j] = 0;
+            tile[j] = gid[j] = 0;
             if (q < cc) {
                 const uint32_t k = s_head[q];
                 const uint32_t jj = (k == 0) ? cs + q - s_first0 : q - s_first[k];
-                const uint2 r = rect[gid[j]];
-                const uint32_t tx0 = r.x & 0xFFFF, tx1 = r.x >> 16, ty0 = r.y & 0xFFFF;
-                const uint32_t wd = tx1 - tx0 + 1;
+                const uint32_t b = s_box[k];
+                const uint32_t tx0 = b & 1023u, ty0 = (b >> 10) & 1023u, wd = (b >> 20) + 1;
+                gid[j] = s_gid[k];
                 tile[j] = (ty0 + jj / wd) * (uint32_t)ntx + tx0 + jj % wd;
             }
         }

@@ -145,6 +145,13 @@ def test_normalize_scene_is_bit_identical_to_the_reference_order():
     np.testing.assert_array_equal(got, want)
     assert abs(np.abs(got[0:3 * n]).max() - 0.8) <= 1e-6
     np.testing.assert_array_equal(got[6 * n:], slab[6 * n:])  # rotations, colours untouched
+    # What the float64 mathematics fixes: centred positions, one uniform scale
+    # factor 0.8 / extent, and log-scales shifted by its log.
+    pos = slab[0:3 * n].astype(np.float64).reshape(3, n)
+    centred = pos - pos.mean(axis=1, keepdims=True)
+    k = 0.8 / np.abs(centred).max()
+    np.testing.assert_allclose(got[0:3 * n].reshape(3, n), centred * k, atol=2e-6)
+    np.testing.assert_allclose(got[3 * n:6 * n] - slab[3 * n:6 * n], math.log(k), atol=2e-6)
 
 
 def test_rejects_what_the_reference_would_misread(tmp_path):
