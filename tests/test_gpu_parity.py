@@ -98,6 +98,26 @@ def test_tile_assign_sort_ranges_bitexact(scene):
     assert np.array_equal(out["tile_ranges"], ref["ranges"])
 
 
+@pytest.mark.parametrize("W,H,cap", [(1024, 1024, 0), (1024, 1024, 1_001_234), (1040, 1024, 0)])
+def test_tile_sort_with_screen_sized_splats(W, H, cap):
+    """Splats covering up to every tile: one splat owns whole 4096-slot chunks of
+    pairs. 1024x1024 has exactly 4096 tiles (the one-pass tile sort's limit),
+    1040x1024 has 4160 (radix passes); the cap ends the pairs mid-chunk."""
+    n = 3000
+    params, view, proj, cam = frustum_scene(77, n, W, H, -0.8)
+    st = Stages(params, n, 0, W, H, view, proj, cam, isect_cap=cap)
+    s2 = st.activ_project_fwd()
+    out = st.raster_fwd()
+    ref = oracle.tile_assign_sort(s2, n, W, H, cap=cap) if cap else oracle.tile_assign_sort(
+        s2, n, W, H)
+    report(f"tile_sort_big_splats_{W}x{H}_cap{cap}", n_isects=out["n_isects"])
+    assert out["n_isects"] == ref["n_isects"] and out["wanted"] == ref["wanted"]
+    assert out["visible"] == ref["visible"]
+    assert np.array_equal(out["keys"], ref["keys"])
+    assert np.array_equal(out["flatten_ids"], ref["ids"])
+    assert np.array_equal(out["tile_ranges"], ref["ranges"])
+
+
 def test_isect_cap_drops_farthest_pairs():
     seed, n, W, H, ls = SCENES["c0"]
     params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
