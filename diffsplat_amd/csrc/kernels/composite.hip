@@ -172,15 +172,16 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
             sC[tid] = r[2];
         }
         __syncthreads();
-        const int bs = min(NT, range.y - b);
         // Every exit below is wave-uniform (a ballot), so the loop runs on
         // scalar branches; lanes whose pixels are done ride along adding
-        // exact zeros.
-        for (int t = 0; t < bs; ++t) {
-            bool any = false;
+        // exact zeros. A wave whose pixels are all done skips the batch, and
+        // leaves it (t = bs) once the last of them finishes: only a splat
+        // that reaches a pixel can finish it.
+        bool live_w = false;
 #pragma unroll
-            for (int k = 0; k < P; ++k) any |= !done[k];
-            if (__ballot(any) == 0) break;
+        for (int k = 0; k < P; ++k) live_w |= !done[k];
+        const int bs = __ballot(live_w) ? min(NT, range.y - b) : 0;
+        for (int t = 0; t < bs; ++t) {
             const float4 A = sA[t], B = sB[t];
             // Pass 1: which of this lane's pixels the splat reaches; a wave
             // whose pixels it reaches none of skips the rest.
@@ -194,6 +195,7 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
             }
             if (__ballot(anyv) == 0) continue;
             const float4 C = sC[t];
+            bool any = false;
 #pragma unroll
             for (int k = 0; k < P; ++k) {
                 // A skipped splat gets alpha 0, which leaves T and the colour
@@ -206,7 +208,9 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
                 cur[k] = ok[k] ? b + t : cur[k];
                 T[k] = T[k] * (1.f - alpha);
                 done[k] |= T[k] <= kTThres;
+                any |= !done[k];
             }
+            if (__ballot(any) == 0) t = bs;
         }
     }
     float l = 0.f;
