@@ -13,15 +13,16 @@ import math
 import numpy as np
 
 from ._lib import (AdamGroup, AdamHParams, CameraInput, CameraSpeeds, DeviceArray,
-                   DiffsplatError, FitterConfig,
-                   RasterConfig, RasterView, check, download_ptr, farr, fptr, gauss_view,
-                   has_gpu, lib, splat2d_view, splat3d_view)
+                   DiffsplatError, FitterConfig, MultiFitInstance,
+                   RasterConfig, RasterInstance, RasterView, check, download_ptr, farr, fptr,
+                   gauss_view, has_gpu, lib, splat2d_view, splat3d_view)
 
 __all__ = [
     "lib", "has_gpu", "check", "DiffsplatError", "DeviceArray", "gauss_view", "splat3d_view",
     "splat2d_view", "sh_bands", "slab_floats", "target_image", "frustum_splats",
     "cluster_scene", "fitter_init", "perspective", "ortho", "look_at", "mat_mul",
     "fitter_camera", "orbit_cameras", "ImageFitter", "SplatRenderer", "Rasterizer",
+    "MultiRasterizer", "MultiFitter",
     "psnr_from_loss", "Input", "ArcballCamera", "FlyCamera",
 ]
 
@@ -331,6 +332,54 @@ class Rasterizer:
             pass
 
 
+class MultiRasterizer(Rasterizer):
+    """ds_raster over several images at once: instance k owns the k-th run of
+    `count` splats and its own width x height image; tile ids, pixel arrays
+    and the sorted pair list are global (instances one after another)."""
+
+    def __init__(self, sizes, isect_cap: int = 0, keep_keys: bool = False, grad_mode: int = 0):
+        self.sizes = [(int(w), int(h), int(n)) for w, h, n in sizes]
+        arr = (RasterInstance * len(self.sizes))(*[RasterInstance(w, h, n)
+                                                    for w, h, n in self.sizes])
+        hd = C.c_void_p()
+        check(lib().ds_raster_create_multi(arr, len(self.sizes), isect_cap, int(keep_keys),
+                                           int(grad_mode), C.byref(hd)), "ds_raster_create_multi")
+        self.h = hd.value
+        self.width = self.height = 0
+        self.tile_base, self.pix_base, self.splat_base = [], [], []
+        t = p = q = 0
+        for w, h, n in self.sizes:
+            self.tile_base.append(t)
+            self.pix_base.append(p)
+            self.splat_base.append(q)
+            t += ((w + 15) // 16) * ((h + 15) // 16)
+            p += w * h
+            q += n
+        self.num_tiles, self.num_pixels, self.num_splats = t, p, q
+
+    def splat_instances(self) -> int:
+        ptr = C.c_void_p()
+        check(lib().ds_raster_splat_instances(self.h, C.byref(ptr)), "ds_raster_splat_instances")
+        return ptr.value
+
+    def state(self):
+        """Forward state over all instances (synchronises)."""
+        check(lib().ds_synchronize(), "ds_synchronize")
+        v = RasterView()
+        check(lib().ds_raster_get_view(self.h, C.byref(v)), "ds_raster_get_view")
+        n_isects, visible, wanted = self.counts()
+        st = {
+            "n_isects": n_isects, "visible": visible, "wanted": wanted,
+            "flatten_ids": download_ptr(v.flatten_ids, np.uint32, n_isects),
+            "tile_ranges": download_ptr(v.tile_ranges, np.int32, 2 * self.num_tiles),
+            "T": download_ptr(v.transmittance, np.float32, self.num_pixels),
+            "last_ids": download_ptr(v.last_ids, np.int32, self.num_pixels),
+        }
+        if v.isect_keys_sorted:
+            st["keys"] = download_ptr(v.isect_keys_sorted, np.uint64, n_isects)
+        return st
+
+
 class ImageFitter:
     """ds_fitter: the reference's ImageFitter (activ -> proj -> raster -> MSE -> Adam)."""
 
@@ -544,6 +593,127 @@ class BatchFitter:
     def close(self):
         if self.h:
             lib().ds_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MultiFitter:
+    """ds_multifit: many fits as one, every phase of a step one launch over
+    all instances (and the whole step one HIP graph when graphs are on).
+    instances: (width, height, count, view, proj, cam_pos) per fit."""
+
+    def __init__(self, instances, *, sh_degree: int = 0, lr=None, isect_cap: int = 0,
+                 grad_mode: int = 0):
+        self.inst = [(int(w), int(h), int(n)) for w, h, n, *_ in instances]
+        arr = (MultiFitInstance * len(instances))()
+        for k, (w, h, n, view, proj, cam) in enumerate(instances):
+            q = arr[k]
+            q.width, q.height, q.count = int(w), int(h), int(n)
+            q.view[:] = [float(x) for x in np.asarray(view, np.float32).reshape(-1)]
+            q.proj[:] = [float(x) for x in np.asarray(proj, np.float32).reshape(-1)]
+            q.cam_pos[:] = [float(x) for x in cam]
+        cfg = FitterConfig()
+        lib().ds_fitter_default_config(1, 1, 0, C.byref(cfg))
+        cfg.sh_degree = sh_degree
+        cfg.grad_mode = int(grad_mode)
+        for k, v in (lr or {}).items():
+            setattr(cfg, k, float(v))
+        hd = C.c_void_p()
+        check(lib().ds_multifit_create(arr, len(instances), C.byref(cfg), isect_cap,
+                                       C.byref(hd)), "ds_multifit_create")
+        self.h = hd.value
+        self.sh_degree = sh_degree
+
+    def __len__(self) -> int:
+        return len(self.inst)
+
+    def set_target(self, k: int, rgb: np.ndarray) -> None:
+        w, h, _ = self.inst[k]
+        rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1)
+        assert rgb.size == 3 * w * h
+        check(lib().ds_multifit_set_target(self.h, k, fptr(rgb)), "ds_multifit_set_target")
+
+    def upload_params(self, k: int, slab: np.ndarray) -> None:
+        slab = np.ascontiguousarray(slab, np.float32).reshape(-1)
+        assert slab.size == slab_floats(self.inst[k][2], self.sh_degree)
+        check(lib().ds_multifit_upload_params(self.h, k, fptr(slab)), "ds_multifit_upload_params")
+
+    def params(self, k: int) -> np.ndarray:
+        out = np.empty(slab_floats(self.inst[k][2], self.sh_degree), np.float32)
+        check(lib().ds_multifit_download_params(self.h, k, fptr(out)),
+              "ds_multifit_download_params")
+        return out
+
+    def grads(self, k: int) -> np.ndarray:
+        out = np.empty(slab_floats(self.inst[k][2], self.sh_degree), np.float32)
+        check(lib().ds_multifit_download_grads(self.h, k, fptr(out)), "ds_multifit_download_grads")
+        return out
+
+    def image(self, k: int) -> np.ndarray:
+        w, h, _ = self.inst[k]
+        out = np.empty(3 * w * h, np.float32)
+        check(lib().ds_multifit_download_image(self.h, k, fptr(out)), "ds_multifit_download_image")
+        return out
+
+    def step(self, n: int = 1) -> None:
+        check(lib().ds_multifit_step(self.h, n), "ds_multifit_step")
+
+    def forward(self) -> None:
+        check(lib().ds_multifit_forward(self.h), "ds_multifit_forward")
+
+    def forward_backward(self) -> None:
+        check(lib().ds_multifit_forward_backward(self.h), "ds_multifit_forward_backward")
+
+    def losses(self) -> np.ndarray:
+        out = np.empty(len(self.inst), np.float32)
+        check(lib().ds_multifit_losses(self.h, fptr(out)), "ds_multifit_losses")
+        return out
+
+    def synchronize(self) -> None:
+        check(lib().ds_multifit_synchronize(self.h), "ds_multifit_synchronize")
+
+    def set_graph(self, on: bool) -> None:
+        check(lib().ds_multifit_set_graph(self.h, int(on)), "ds_multifit_set_graph")
+
+    def autotune(self, reps: int = 3) -> dict:
+        f, b = C.c_int(), C.c_int()
+        tf, tb = C.c_float(), C.c_float()
+        check(lib().ds_multifit_autotune(self.h, reps, C.byref(f), C.byref(b), C.byref(tf),
+                                         C.byref(tb)), "ds_multifit_autotune")
+        return {"fwd_pix_per_thread": f.value, "bwd_pix_per_thread": b.value,
+                "fwd_ms": round(tf.value, 4), "bwd_ms": round(tb.value, 4)}
+
+    def set_tuning(self, fwd_ppt: int, bwd_ppt: int) -> None:
+        r = C.c_void_p()
+        check(lib().ds_multifit_get_raster(self.h, C.byref(r)), "ds_multifit_get_raster")
+        check(lib().ds_raster_set_tuning(r, fwd_ppt, bwd_ppt), "ds_raster_set_tuning")
+
+    def step_count(self) -> int:
+        s = C.c_int32()
+        check(lib().ds_multifit_step_count(self.h, C.byref(s)), "ds_multifit_step_count")
+        return s.value
+
+    def stream(self):
+        s = C.c_void_p()
+        check(lib().ds_multifit_get_stream(self.h, C.byref(s)), "ds_multifit_get_stream")
+        return s.value
+
+    def raster_counts(self):
+        r = C.c_void_p()
+        check(lib().ds_multifit_get_raster(self.h, C.byref(r)), "ds_multifit_get_raster")
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().ds_raster_counts(r, self.stream(), C.byref(a), C.byref(b), C.byref(c)),
+              "ds_raster_counts")
+        return a.value, b.value, c.value
+
+    def close(self):
+        if self.h:
+            lib().ds_multifit_destroy(self.h)
             self.h = None
 
     def __del__(self):

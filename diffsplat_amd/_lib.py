@@ -82,6 +82,15 @@ class FitterConfig(C.Structure):
                 ("grad_mode", C.c_int32), ("lr_pos_decay", C.c_float)]
 
 
+class RasterInstance(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("count", C.c_int32)]
+
+
+class MultiFitInstance(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("count", C.c_int32),
+                ("view", C.c_float * 16), ("proj", C.c_float * 16), ("cam_pos", C.c_float * 3)]
+
+
 class CameraInput(C.Structure):
     _fields_ = [("mouse_dx", C.c_float), ("mouse_dy", C.c_float), ("scroll", C.c_float),
                 ("mouse_left", C.c_int32), ("keys", C.c_uint32)]
@@ -126,7 +135,15 @@ _SIGS = {
                                        _F, _P]),
     "ds_activ_project_bwd": (C.c_int, [C.POINTER(Gauss3D), C.POINTER(Splat2D),
                                        C.POINTER(Gauss3D), C.c_int, _F, _F, C.c_int, _P]),
+    "ds_activ_project_fwd_multi": (C.c_int, [C.POINTER(Gauss3D), C.POINTER(Splat2D), C.c_int,
+                                             _P, _P, _P]),
+    "ds_activ_project_bwd_multi": (C.c_int, [C.POINTER(Gauss3D), C.POINTER(Splat2D),
+                                             C.POINTER(Gauss3D), C.c_int, _P, _P, C.c_int,
+                                             _P]),
     "ds_raster_create": (C.c_int, [C.POINTER(RasterConfig), C.POINTER(C.c_void_p)]),
+    "ds_raster_create_multi": (C.c_int, [C.POINTER(RasterInstance), C.c_int, C.c_int64,
+                                         C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "ds_raster_splat_instances": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
     "ds_raster_destroy": (C.c_int, [_P]),
     "ds_raster_resize": (C.c_int, [_P, C.c_int, C.c_int]),
     "ds_raster_fwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, _P, _P, _P, _P]),
@@ -201,6 +218,26 @@ _SIGS = {
     "ds_batch_synchronize": (C.c_int, [_P]),
     "ds_batch_set_streams": (C.c_int, [_P, C.c_int]),
     "ds_batch_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), _F]),
+    "ds_multifit_create": (C.c_int, [C.POINTER(MultiFitInstance), C.c_int,
+                                     C.POINTER(FitterConfig), C.c_int64,
+                                     C.POINTER(C.c_void_p)]),
+    "ds_multifit_destroy": (C.c_int, [_P]),
+    "ds_multifit_set_target": (C.c_int, [_P, C.c_int, _F]),
+    "ds_multifit_upload_params": (C.c_int, [_P, C.c_int, _F]),
+    "ds_multifit_download_params": (C.c_int, [_P, C.c_int, _F]),
+    "ds_multifit_download_grads": (C.c_int, [_P, C.c_int, _F]),
+    "ds_multifit_download_image": (C.c_int, [_P, C.c_int, _F]),
+    "ds_multifit_step": (C.c_int, [_P, C.c_int]),
+    "ds_multifit_forward": (C.c_int, [_P]),
+    "ds_multifit_forward_backward": (C.c_int, [_P]),
+    "ds_multifit_losses": (C.c_int, [_P, _F]),
+    "ds_multifit_synchronize": (C.c_int, [_P]),
+    "ds_multifit_set_graph": (C.c_int, [_P, C.c_int]),
+    "ds_multifit_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       _F, _F]),
+    "ds_multifit_step_count": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "ds_multifit_get_raster": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
+    "ds_multifit_get_stream": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
     "ds_renderer_render": (C.c_int, [_P, _F, _F, _F]),
     "ds_renderer_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), _F]),
     "ds_renderer_download_image": (C.c_int, [_P, _F]),

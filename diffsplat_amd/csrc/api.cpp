@@ -276,6 +276,32 @@ ds_status ds_activ_project_bwd(const ds_gauss3d *in, const ds_splat2d *gout,
                                     grad_mode, as_stream(s));
 }
 
+ds_status ds_activ_project_fwd_multi(const ds_gauss3d *in, const ds_splat2d *out, int deg,
+                                     const float *cam_tab, const uint16_t *splat_inst,
+                                     ds_stream_t s) {
+    clear_error();
+    DS_TRY(check_gauss(in, deg, "ds_activ_project_fwd_multi"));
+    DS_REQUIRE(out && cam_tab && (splat_inst || in->count == 0),
+               "ds_activ_project_fwd_multi: NULL argument");
+    return launch_activ_project_fwd_multi(*in, *out, deg,
+                                          reinterpret_cast<const float4 *>(cam_tab), splat_inst,
+                                          as_stream(s));
+}
+
+ds_status ds_activ_project_bwd_multi(const ds_gauss3d *in, const ds_splat2d *gout,
+                                     const ds_gauss3d *gin, int deg, const float *cam_tab,
+                                     const uint16_t *splat_inst, int grad_mode, ds_stream_t s) {
+    clear_error();
+    DS_TRY(check_mode(grad_mode));
+    DS_TRY(check_gauss(in, deg, "ds_activ_project_bwd_multi"));
+    DS_TRY(check_gauss(gin, deg, "ds_activ_project_bwd_multi grad_in"));
+    DS_REQUIRE(gout && cam_tab && (splat_inst || in->count == 0),
+               "ds_activ_project_bwd_multi: NULL argument");
+    return launch_activ_project_bwd_multi(*in, *gout, *gin, deg,
+                                          reinterpret_cast<const float4 *>(cam_tab), splat_inst,
+                                          grad_mode, as_stream(s));
+}
+
 ds_status ds_mse_fwd(const float *pixels, const float *target, int64_t num_pixels,
                      float *loss_dev, ds_stream_t s) {
     clear_error();
@@ -575,6 +601,190 @@ ds_status ds_fitter_reset_optimizer(ds_fitter *f) {
 }
 
 // ------------------------------------------------------------------ batch
+
+// ------------------------------------------------------------------ multifit
+
+struct ds_multifit {
+    dsh::MultiFitter impl;
+};
+
+ds_status ds_multifit_create(const ds_multifit_instance *inst, int n_inst,
+                             const ds_fitter_config *cfg, int64_t isect_cap, ds_multifit **out) {
+    clear_error();
+    DS_REQUIRE(inst && cfg && out, "NULL argument");
+    DS_REQUIRE(n_inst >= 1 && n_inst <= 65535, "n_inst must be in [1, 65535]");
+    DS_REQUIRE(cfg->sh_degree >= 0 && cfg->sh_degree <= 3, "sh_degree must be in [0,3]");
+    DS_REQUIRE(cfg->grad_mode == DS_GRAD_EXACT || cfg->grad_mode == DS_GRAD_REFERENCE,
+               "grad_mode must be DS_GRAD_EXACT or DS_GRAD_REFERENCE");
+    for (int k = 0; k < n_inst; ++k)
+        DS_REQUIRE(inst[k].width > 0 && inst[k].height > 0 && inst[k].count >= 0,
+                   "bad instance size");
+    DS_GUARD_BEGIN
+    auto m = std::make_unique<ds_multifit>();
+    std::vector<dsh::MultiFitInstance> v((size_t)n_inst);
+    for (int k = 0; k < n_inst; ++k) {
+        dsh::MultiFitInstance &q = v[(size_t)k];
+        q.width = inst[k].width;
+        q.height = inst[k].height;
+        q.count = inst[k].count;
+        std::memcpy(q.view, inst[k].view, sizeof(q.view));
+        std::memcpy(q.proj, inst[k].proj, sizeof(q.proj));
+        std::memcpy(q.cam_pos, inst[k].cam_pos, sizeof(q.cam_pos));
+    }
+    dsh::Gaussian3DAdamConfig lr;
+    lr.lr_pos = cfg->lr_pos;
+    lr.lr_scale = cfg->lr_scale;
+    lr.lr_rot = cfg->lr_rot;
+    lr.lr_sh_dc = cfg->lr_sh_dc;
+    lr.lr_sh_rest = cfg->lr_sh_rest;
+    lr.lr_opacity = cfg->lr_opacity;
+    lr.lr_pos_decay = cfg->lr_pos_decay > 0.f ? cfg->lr_pos_decay : 1.f;
+    lr.adam.beta1 = cfg->beta1;
+    lr.adam.beta2 = cfg->beta2;
+    lr.adam.epsilon = cfg->epsilon;
+    m->impl.init(v, cfg->sh_degree, lr, isect_cap, cfg->grad_mode);
+    *out = m.release();
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_multifit_destroy(ds_multifit *m) {
+    clear_error();
+    DS_GUARD_BEGIN
+    delete m;
+    return DS_OK;
+    DS_GUARD_END
+}
+
+#define DS_MF_INST(m, k, p)                                                   \
+    DS_REQUIRE((m) && (p), "NULL argument");                                  \
+    DS_REQUIRE((k) >= 0 && (k) < (m)->impl.size(), "instance index out of range")
+
+ds_status ds_multifit_set_target(ds_multifit *m, int k, const float *host_rgb) {
+    clear_error();
+    DS_MF_INST(m, k, host_rgb);
+    DS_GUARD_BEGIN
+    m->impl.setTarget(k, host_rgb);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_multifit_upload_params(ds_multifit *m, int k, const float *host_slab) {
+    clear_error();
+    DS_MF_INST(m, k, host_slab);
+    DS_GUARD_BEGIN
+    m->impl.uploadParams(k, host_slab);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_multifit_download_params(ds_multifit *m, int k, float *host_slab) {
+    clear_error();
+    DS_MF_INST(m, k, host_slab);
+    DS_GUARD_BEGIN
+    m->impl.downloadParams(k, host_slab);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_multifit_download_grads(ds_multifit *m, int k, float *host_slab) {
+    clear_error();
+    DS_MF_INST(m, k, host_slab);
+    DS_GUARD_BEGIN
+    m->impl.downloadGrads(k, host_slab);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_multifit_download_image(ds_multifit *m, int k, float *host_rgb) {
+    clear_error();
+    DS_MF_INST(m, k, host_rgb);
+    DS_GUARD_BEGIN
+    m->impl.downloadImage(k, host_rgb);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_multifit_step(ds_multifit *m, int n_steps) {
+    clear_error();
+    DS_REQUIRE(m && n_steps >= 0, "bad argument");
+    DS_GUARD_BEGIN
+    for (int k = 0; k < n_steps; ++k) m->impl.step();
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_multifit_forward(ds_multifit *m) {
+    clear_error();
+    DS_REQUIRE(m, "NULL multifit");
+    DS_GUARD_BEGIN
+    m->impl.forwardOnly();
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_multifit_forward_backward(ds_multifit *m) {
+    clear_error();
+    DS_REQUIRE(m, "NULL multifit");
+    DS_GUARD_BEGIN
+    m->impl.forwardBackward();
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_multifit_losses(ds_multifit *m, float *losses) {
+    clear_error();
+    DS_REQUIRE(m && losses, "NULL argument");
+    DS_GUARD_BEGIN
+    m->impl.losses(losses);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_multifit_synchronize(ds_multifit *m) {
+    clear_error();
+    DS_REQUIRE(m, "NULL multifit");
+    DS_TRY_HIP(hipStreamSynchronize(m->impl.getStream()));
+    return DS_OK;
+}
+
+ds_status ds_multifit_set_graph(ds_multifit *m, int enable) {
+    clear_error();
+    DS_REQUIRE(m, "NULL multifit");
+    m->impl.setUseGraph(enable != 0);
+    return DS_OK;
+}
+
+ds_status ds_multifit_autotune(ds_multifit *m, int reps, int *fwd_ppt, int *bwd_ppt,
+                               float *fwd_ms, float *bwd_ms) {
+    clear_error();
+    DS_REQUIRE(m, "NULL multifit");
+    DS_GUARD_BEGIN
+    m->impl.autotune(reps, fwd_ppt, bwd_ppt, fwd_ms, bwd_ms);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_multifit_step_count(ds_multifit *m, int32_t *step) {
+    clear_error();
+    DS_REQUIRE(m && step, "NULL argument");
+    *step = m->impl.getIterCount();
+    return DS_OK;
+}
+
+ds_status ds_multifit_get_raster(ds_multifit *m, ds_raster **r) {
+    clear_error();
+    DS_REQUIRE(m && r, "NULL argument");
+    *r = m->impl.raster();
+    return DS_OK;
+}
+
+ds_status ds_multifit_get_stream(ds_multifit *m, ds_stream_t *s) {
+    clear_error();
+    DS_REQUIRE(m && s, "NULL argument");
+    *s = m->impl.getStream();
+    return DS_OK;
+}
 
 struct ds_batch {
     dsh::BatchFitter impl;

@@ -109,6 +109,16 @@ ds_status launch_activ_project_fwd(const ds_gauss3d &in, const ds_splat2d &out, 
 ds_status launch_activ_project_bwd(const ds_gauss3d &in, const ds_splat2d &gout,
                                    const ds_gauss3d &gin, int deg, Cam3 cam, const PV &pv,
                                    int mode, hipStream_t s);
+// Many fits at once: splat i uses the camera of instance splat_inst[i] in
+// cam_tab (kCamRecF4 float4 per instance: the 16 PV floats column-major, then
+// the camera position in x, y, z).
+constexpr int kCamRecF4 = 5;
+ds_status launch_activ_project_fwd_multi(const ds_gauss3d &in, const ds_splat2d &out, int deg,
+                                         const float4 *cam_tab, const uint16_t *splat_inst,
+                                         hipStream_t s);
+ds_status launch_activ_project_bwd_multi(const ds_gauss3d &in, const ds_splat2d &gout,
+                                         const ds_gauss3d &gin, int deg, const float4 *cam_tab,
+                                         const uint16_t *splat_inst, int mode, hipStream_t s);
 
 // loss / optimiser (kernels/optim.hip)
 ds_status launch_mse_fwd(const float *p, const float *t, int64_t npix, float *loss,

@@ -76,6 +76,7 @@ void ImageFitter::uploadParams(const float *host_slab) {
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     gaussian_params.slab.upload(host_slab, gaussian_params.slab.size);
     optimizer.reset(stream);
+    iter_count = 0;
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
@@ -150,6 +151,7 @@ void ImageFitter::initLayers() {
     optimizer.init(opt.lr.adam);
     registerGaussian3DGroups(optimizer, gaussian_params, *param_grads, opt.lr);
     optimizer.reset(stream);
+    iter_count = 0;
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
@@ -163,6 +165,7 @@ void ImageFitter::stepOnce() {
 void ImageFitter::setUseGraph(bool on) { use_graph = on; }
 
 void ImageFitter::step() {
+    ++iter_count;
     if (!use_graph) {
         stepOnce();
         return;
@@ -202,6 +205,7 @@ void ImageFitter::setTargetDevice(const float *dev_rgb) {
 
 void ImageFitter::resetOptimizer() {
     optimizer.reset(stream);
+    iter_count = 0;
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
@@ -353,6 +357,238 @@ int BatchFitter::autotune(int reps, float *ms_per_batch_step) {
     setStreams(best);
     if (ms_per_batch_step) *ms_per_batch_step = tbest;
     return best;
+}
+
+// ============================================================ MultiFitter
+
+MultiFitter::~MultiFitter() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    dropGraph();
+    if (ras) ds_raster_destroy(ras);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void MultiFitter::dropGraph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+}
+
+void MultiFitter::init(const std::vector<MultiFitInstance> &instances, int deg,
+                       const Gaussian3DAdamConfig &lr, int64_t isect_cap, int mode) {
+    if (instances.empty() || deg < 0 || deg > 3)
+        throw std::runtime_error("[diffsplat] MultiFitter::init: bad configuration");
+    inst = instances;
+    sh_degree = deg;
+    grad_mode = mode;
+    splat_base.clear();
+    pix_base.clear();
+    n_total = pix_total = 0;
+    std::vector<ds_raster_instance> ri;
+    std::vector<float> cam_tab;
+    for (const MultiFitInstance &q : inst) {
+        if (q.width <= 0 || q.height <= 0 || q.count < 0)
+            throw std::runtime_error("[diffsplat] MultiFitter::init: bad instance");
+        splat_base.push_back(n_total);
+        pix_base.push_back(pix_total);
+        n_total += q.count;
+        pix_total += (int64_t)q.width * q.height;
+        ri.push_back({q.width, q.height, q.count});
+        float pv[16];
+        ds_mat_mul(q.proj, q.view, pv);
+        cam_tab.insert(cam_tab.end(), pv, pv + 16);
+        cam_tab.insert(cam_tab.end(), q.cam_pos, q.cam_pos + 3);
+        cam_tab.push_back(0.f);
+    }
+    if (n_total > (1 << 30) - 1)
+        throw std::runtime_error("[diffsplat] MultiFitter::init: too many splats");
+    if (!stream) hip_check(hipStreamCreate(&stream), "hipStreamCreate");
+    dropGraph();
+    if (ras) ds_raster_destroy(ras);
+    ras = nullptr;
+    check(ds_raster_create_multi(ri.data(), (int)ri.size(), isect_cap, 0, grad_mode, &ras),
+          "MultiFitter::init");
+    check(ds_raster_splat_instances(ras, &splat_inst), "MultiFitter::init");
+    const int n = (int)n_total;
+    params.allocate(n, deg);
+    grads.allocate(n, deg);
+    s2.allocate(n);
+    gs2.allocate(n);
+    params.zero(stream);
+    grads.zero(stream);
+    cams.allocate(cam_tab.size());
+    cams.upload(cam_tab.data(), cam_tab.size());
+    target.allocate((size_t)pix_total * 3);
+    image.allocate((size_t)pix_total * 3);
+    grad_rgb.allocate((size_t)pix_total * 3);
+    loss.allocate(inst.size());
+    target.zero(stream);
+    loss.zero(stream);
+    optimizer = Adam();
+    optimizer.init(lr.adam);
+    registerGaussian3DGroups(optimizer, params, grads, lr);
+    optimizer.reset(stream);
+    iter_count = 0;
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+// Instance k's rows (14 + 3 b of count floats each, the ds_fitter slab
+// layout) to or from the combined slab, whose rows hold n_total floats.
+void MultiFitter::rowsCopy(int k, float *dev_slab, float *host, bool to_device) {
+    if (k < 0 || k >= size()) throw std::runtime_error("[diffsplat] MultiFitter: bad instance");
+    const size_t nk = (size_t)inst[(size_t)k].count;
+    if (nk == 0) return;
+    const size_t rows = (size_t)ds_slab_floats(1, sh_degree);
+    float *dev = dev_slab + splat_base[(size_t)k];
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if (to_device)
+        hip_check(hipMemcpy2D(dev, (size_t)n_total * 4, host, nk * 4, nk * 4, rows,
+                              hipMemcpyHostToDevice),
+                  "hipMemcpy2D");
+    else
+        hip_check(hipMemcpy2D(host, nk * 4, dev, (size_t)n_total * 4, nk * 4, rows,
+                              hipMemcpyDeviceToHost),
+                  "hipMemcpy2D");
+}
+
+void MultiFitter::setTarget(int k, const float *host_rgb) {
+    if (k < 0 || k >= size()) throw std::runtime_error("[diffsplat] MultiFitter: bad instance");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    const MultiFitInstance &q = inst[(size_t)k];
+    hip_check(hipMemcpy(target.ptr + 3 * pix_base[(size_t)k], host_rgb,
+                        sizeof(float) * 3 * (size_t)q.width * q.height, hipMemcpyHostToDevice),
+              "hipMemcpy");
+}
+
+void MultiFitter::uploadParams(int k, const float *host_slab) {
+    rowsCopy(k, params.slab.ptr, const_cast<float *>(host_slab), true);
+    resetOptimizer();
+}
+
+void MultiFitter::downloadParams(int k, float *host_slab) {
+    rowsCopy(k, params.slab.ptr, host_slab, false);
+}
+
+void MultiFitter::downloadGrads(int k, float *host_slab) {
+    rowsCopy(k, grads.slab.ptr, host_slab, false);
+}
+
+void MultiFitter::downloadImage(int k, float *host_rgb) {
+    if (k < 0 || k >= size()) throw std::runtime_error("[diffsplat] MultiFitter: bad instance");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    const MultiFitInstance &q = inst[(size_t)k];
+    hip_check(hipMemcpy(host_rgb, image.ptr + 3 * pix_base[(size_t)k],
+                        sizeof(float) * 3 * (size_t)q.width * q.height, hipMemcpyDeviceToHost),
+              "hipMemcpy");
+}
+
+void MultiFitter::resetOptimizer() {
+    optimizer.reset(stream);
+    iter_count = 0;
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+void MultiFitter::forwardOnly() {
+    ds_gauss3d pv = params.view();
+    ds_splat2d sv = s2.view();
+    check(ds_activ_project_fwd_multi(&pv, &sv, sh_degree, cams.ptr, splat_inst, stream),
+          "MultiFitter::forward");
+    check(ds_raster_fwd(ras, &sv, image.ptr, target.ptr, loss.ptr, grad_rgb.ptr, stream),
+          "MultiFitter::forward");
+}
+
+void MultiFitter::forwardBackward() {
+    ds_splat2d gv = gs2.view();
+    check(ds_splat2d_zero(&gv, stream), "MultiFitter::zeroGrad");
+    forwardOnly();
+    ds_gauss3d pv = params.view();
+    ds_gauss3d gp = grads.view();
+    ds_splat2d sv = s2.view();
+    check(ds_raster_bwd(ras, &sv, grad_rgb.ptr, &gv, stream), "MultiFitter::backward");
+    check(ds_activ_project_bwd_multi(&pv, &gv, &gp, sh_degree, cams.ptr, splat_inst, grad_mode,
+                                     stream),
+          "MultiFitter::backward");
+}
+
+void MultiFitter::stepOnce() {
+    forwardBackward();
+    optimizer.step(stream);
+}
+
+void MultiFitter::step() {
+    ++iter_count;
+    if (!use_graph) {
+        stepOnce();
+        return;
+    }
+    if (!graph_exec) {
+        hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal),
+                  "hipStreamBeginCapture");
+        try {
+            stepOnce();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+        hip_check(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0),
+                  "hipGraphInstantiate");
+    }
+    hip_check(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch");
+}
+
+void MultiFitter::losses(float *host) {
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    hip_check(hipMemcpy(host, loss.ptr, sizeof(float) * inst.size(), hipMemcpyDeviceToHost),
+              "hipMemcpy");
+}
+
+void MultiFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd,
+                           float *ms_bwd) {
+    if (reps < 1) reps = 1;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    auto timed = [&](auto &&body) {
+        body();  // warm-up
+        hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+        for (int k = 0; k < reps; ++k) body();
+        hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        return ms / reps;
+    };
+    int fwd0 = 1, bwd0 = 1;
+    check(ds_raster_get_tuning(ras, &fwd0, &bwd0), "autotune");
+    const int opts[3] = {1, 2, 4};
+    float tfw[3], tbw[3];
+    for (int i = 0; i < 3; ++i) {
+        check(ds_raster_set_tuning(ras, opts[i], bwd0), "autotune");
+        tfw[i] = timed([&] { forwardOnly(); });
+    }
+    const int bf = opts[pick_with_margin(tfw, fwd0)];
+    check(ds_raster_set_tuning(ras, bf, bwd0), "autotune");
+    forwardOnly();
+    ds_splat2d sv = s2.view(), gv = gs2.view();
+    for (int i = 0; i < 3; ++i) {
+        check(ds_raster_set_tuning(ras, bf, opts[i]), "autotune");
+        tbw[i] = timed([&] { check(ds_raster_bwd(ras, &sv, grad_rgb.ptr, &gv, stream), "bwd"); });
+    }
+    const int ib = pick_with_margin(tbw, bwd0);
+    const int bb = opts[ib];
+    const float tf = tfw[bf == 1 ? 0 : bf == 2 ? 1 : 2], tb = tbw[ib];
+    check(ds_raster_set_tuning(ras, bf, bb), "autotune");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dropGraph();
+    if (best_fwd) *best_fwd = bf;
+    if (best_bwd) *best_bwd = bb;
+    if (ms_fwd) *ms_fwd = tf;
+    if (ms_bwd) *ms_bwd = tb;
 }
 
 // ============================================================ SplatRenderer

@@ -59,7 +59,7 @@ public:
 
     float getLoss();
     float *getOutput() { return ras_layer.getOutput().ptr; }
-    int getIterCount() const { return optimizer.getStepCount(); }
+    int getIterCount() const { return iter_count; }
     int count() const { return opt.count; }
     int shDegree() const { return opt.sh_degree; }
     hipStream_t getStream() const { return stream; }
@@ -84,6 +84,7 @@ private:
     MSELossLayer mse_layer;
     Adam optimizer;
     DeviceBuffer<float> d_target_pixels;
+    int iter_count = 0;  // steps since the optimizer was last reset (graph replays too)
     bool use_graph = false;
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -120,6 +121,74 @@ private:
     std::vector<ImageFitter *> fit;
     std::vector<hipStream_t> streams;
     std::vector<float> step_ms;  // measured step time per instance
+};
+
+// Many independent fits as one (no counterpart in the reference): every
+// phase of a step (zero grads, activation + projection, tile assign, sorts,
+// compositing with MSE, compositing backward, projection/activation
+// backward, Adam) is one launch over all instances. The instances' splats
+// are runs of one parameter slab, their cameras a device table, their tiles
+// runs of one global tile space and their images runs of one pixel array
+// (a multi-instance ds_raster). Per instance the step equals ImageFitter's
+// with the same configuration. The step can be captured into one HIP graph.
+struct MultiFitInstance {
+    int width = 0, height = 0, count = 0;
+    float view[16], proj[16];
+    float cam_pos[3] = {0.f, 0.f, 0.f};
+};
+
+class MultiFitter {
+public:
+    MultiFitter() = default;
+    ~MultiFitter();
+    MultiFitter(const MultiFitter &) = delete;
+    MultiFitter &operator=(const MultiFitter &) = delete;
+
+    void init(const std::vector<MultiFitInstance> &instances, int sh_degree,
+              const Gaussian3DAdamConfig &lr, int64_t isect_cap, int grad_mode);
+    int size() const { return (int)inst.size(); }
+    const MultiFitInstance &instance(int k) const { return inst[(size_t)k]; }
+    void setTarget(int k, const float *host_rgb);
+    // Instance k's slab in the ds_fitter_upload_params layout; resets the
+    // optimizer of every instance (moments and the shared step counter).
+    void uploadParams(int k, const float *host_slab);
+    void downloadParams(int k, float *host_slab);
+    void downloadGrads(int k, float *host_slab);
+    void downloadImage(int k, float *host_rgb);
+    void step();
+    void forwardOnly();
+    void forwardBackward();
+    void resetOptimizer();
+    void losses(float *host);  // one per instance, of the last forward (synchronises)
+    void setUseGraph(bool on) { use_graph = on; }
+    void autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd, float *ms_bwd);
+    int getIterCount() const { return iter_count; }
+    hipStream_t getStream() const { return stream; }
+    ds_raster *raster() const { return ras; }
+    int64_t totalSplats() const { return n_total; }
+
+private:
+    void stepOnce();
+    void dropGraph();
+    void rowsCopy(int k, float *dev_slab, float *host, bool to_device);
+    std::vector<MultiFitInstance> inst;
+    std::vector<int64_t> splat_base, pix_base;
+    int64_t n_total = 0, pix_total = 0;
+    int sh_degree = 0;
+    int grad_mode = DS_GRAD_EXACT;
+    hipStream_t stream = nullptr;
+    ds_raster *ras = nullptr;
+    const uint16_t *splat_inst = nullptr;  // owned by ras
+    Gaussian3DParams params;
+    Gaussian3DGrads grads;
+    Splat2DParams s2;
+    Splat2DGrads gs2;
+    DeviceBuffer<float> cams, target, image, grad_rgb, loss;
+    Adam optimizer;
+    int iter_count = 0;
+    bool use_graph = false;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
 };
 
 class SplatRenderer {

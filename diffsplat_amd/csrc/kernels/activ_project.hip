@@ -433,11 +433,42 @@ __global__ __launch_bounds__(kBlock) void k_project_bwd(ds_splat3d in, ds_splat2
     gi.opacity[i] = go.opacity[i];
 }
 
-template <int DEG>
+// Camera of splat i for the fused kernels: one camera for all splats, or a
+// per-instance table indexed by each splat's instance (many fits at once).
+struct CamUniform {
+    Cam3 cam;
+    PV pv;
+    __device__ __forceinline__ void load(int, Cam3 &c, PV &p) const {
+        c = cam;
+        p = pv;
+    }
+};
+struct CamTable {
+    const float4 *tab;      // per instance: PV columns 0-3, then camera position
+    const uint16_t *inst;   // per splat
+    __device__ __forceinline__ void load(int i, Cam3 &c, PV &p) const {
+        const float4 *r = tab + kCamRecF4 * inst[i];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 v = r[q];
+            p.m[4 * q + 0] = v.x;
+            p.m[4 * q + 1] = v.y;
+            p.m[4 * q + 2] = v.z;
+            p.m[4 * q + 3] = v.w;
+        }
+        const float4 e = r[4];
+        c = Cam3{e.x, e.y, e.z};
+    }
+};
+
+template <int DEG, class CS>
 __global__ __launch_bounds__(kBlock) void k_activ_project_fwd(ds_gauss3d in, ds_splat2d out,
-                                                              Cam3 cam, PV pv) {
+                                                              CS cs) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= in.count) return;
+    Cam3 cam;
+    PV pv;
+    cs.load(i, cam, pv);
     const float px = in.pos_x[i], py = in.pos_y[i], pz = in.pos_z[i];
     Proj p;
     if (!project_point(pv, px, py, pz, p)) {
@@ -462,11 +493,14 @@ __global__ __launch_bounds__(kBlock) void k_activ_project_fwd(ds_gauss3d in, ds_
     out.opacity[i] = sigmoid(in.logit_opacity[i]);
 }
 
-template <int DEG, int MODE>
+template <int DEG, int MODE, class CS>
 __global__ __launch_bounds__(kBlock) void k_activ_project_bwd(ds_gauss3d in, ds_splat2d go,
-                                                              ds_gauss3d gi, Cam3 cam, PV pv) {
+                                                              ds_gauss3d gi, CS cs) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= in.count) return;
+    Cam3 cam;
+    PV pv;
+    cs.load(i, cam, pv);
     const float px = in.pos_x[i], py = in.pos_y[i], pz = in.pos_z[i];
     Proj p;
     if (!project_point(pv, px, py, pz, p)) {
@@ -576,23 +610,67 @@ ds_status launch_project_bwd(const ds_splat3d &in, const ds_splat2d &go, const d
     return DS_OK;
 }
 
-ds_status launch_activ_project_fwd(const ds_gauss3d &in, const ds_splat2d &out, int deg,
-                                   Cam3 cam, const PV &pv, hipStream_t s) {
+template <class CS>
+static ds_status activ_project_fwd(const ds_gauss3d &in, const ds_splat2d &out, int deg,
+                                   const CS &cs, hipStream_t s) {
     if (in.count <= 0) return DS_OK;
-    DS_DISPATCH_DEG(deg, k_activ_project_fwd, dim3(grid_for(in.count)), dim3(kBlock), 0, s, in,
-                    out, cam, pv);
+    const dim3 grid(grid_for(in.count)), block(kBlock);
+    switch (deg) {
+        case 0: hipLaunchKernelGGL((k_activ_project_fwd<0, CS>), grid, block, 0, s, in, out, cs); break;
+        case 1: hipLaunchKernelGGL((k_activ_project_fwd<1, CS>), grid, block, 0, s, in, out, cs); break;
+        case 2: hipLaunchKernelGGL((k_activ_project_fwd<2, CS>), grid, block, 0, s, in, out, cs); break;
+        default: hipLaunchKernelGGL((k_activ_project_fwd<3, CS>), grid, block, 0, s, in, out, cs); break;
+    }
     DS_CHECK_LAUNCH("activ_project_fwd");
     return DS_OK;
+}
+
+template <int MODE, class CS>
+static void activ_project_bwd_mode(const ds_gauss3d &in, const ds_splat2d &go,
+                                   const ds_gauss3d &gi, int deg, const CS &cs, hipStream_t s) {
+    const dim3 grid(grid_for(in.count)), block(kBlock);
+    switch (deg) {
+        case 0: hipLaunchKernelGGL((k_activ_project_bwd<0, MODE, CS>), grid, block, 0, s, in, go, gi, cs); break;
+        case 1: hipLaunchKernelGGL((k_activ_project_bwd<1, MODE, CS>), grid, block, 0, s, in, go, gi, cs); break;
+        case 2: hipLaunchKernelGGL((k_activ_project_bwd<2, MODE, CS>), grid, block, 0, s, in, go, gi, cs); break;
+        default: hipLaunchKernelGGL((k_activ_project_bwd<3, MODE, CS>), grid, block, 0, s, in, go, gi, cs); break;
+    }
+}
+
+template <class CS>
+static ds_status activ_project_bwd(const ds_gauss3d &in, const ds_splat2d &go,
+                                   const ds_gauss3d &gi, int deg, const CS &cs, int mode,
+                                   hipStream_t s) {
+    if (in.count <= 0) return DS_OK;
+    if (mode == DS_GRAD_REFERENCE)
+        activ_project_bwd_mode<DS_GRAD_REFERENCE>(in, go, gi, deg, cs, s);
+    else
+        activ_project_bwd_mode<DS_GRAD_EXACT>(in, go, gi, deg, cs, s);
+    DS_CHECK_LAUNCH("activ_project_bwd");
+    return DS_OK;
+}
+
+ds_status launch_activ_project_fwd(const ds_gauss3d &in, const ds_splat2d &out, int deg,
+                                   Cam3 cam, const PV &pv, hipStream_t s) {
+    return activ_project_fwd(in, out, deg, CamUniform{cam, pv}, s);
 }
 
 ds_status launch_activ_project_bwd(const ds_gauss3d &in, const ds_splat2d &go,
                                    const ds_gauss3d &gi, int deg, Cam3 cam, const PV &pv,
                                    int mode, hipStream_t s) {
-    if (in.count <= 0) return DS_OK;
-    DS_DISPATCH_DEG_MODE(deg, mode, k_activ_project_bwd, dim3(grid_for(in.count)), dim3(kBlock),
-                         0, s, in, go, gi, cam, pv);
-    DS_CHECK_LAUNCH("activ_project_bwd");
-    return DS_OK;
+    return activ_project_bwd(in, go, gi, deg, CamUniform{cam, pv}, mode, s);
+}
+
+ds_status launch_activ_project_fwd_multi(const ds_gauss3d &in, const ds_splat2d &out, int deg,
+                                         const float4 *cam_tab, const uint16_t *splat_inst,
+                                         hipStream_t s) {
+    return activ_project_fwd(in, out, deg, CamTable{cam_tab, splat_inst}, s);
+}
+
+ds_status launch_activ_project_bwd_multi(const ds_gauss3d &in, const ds_splat2d &go,
+                                         const ds_gauss3d &gi, int deg, const float4 *cam_tab,
+                                         const uint16_t *splat_inst, int mode, hipStream_t s) {
+    return activ_project_bwd(in, go, gi, deg, CamTable{cam_tab, splat_inst}, mode, s);
 }
 
 }  // namespace ds

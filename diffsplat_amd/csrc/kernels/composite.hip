@@ -131,29 +131,51 @@ __device__ __forceinline__ int field_of_lane(int lane) {
     }
 }
 
-template <bool MSE, int P>
+// The image a tile belongs to: its size, tiles per row, the tile's index in
+// it, where its pixels start in the (concatenated) pixel arrays, and which
+// instance it is (0 for one instance).
+struct TileImage {
+    int W, H, ntx, local, pix_base, inst;
+    float inv_npix;
+};
+
+template <bool MULTI>
+__device__ __forceinline__ TileImage tile_image(const CompositeArgs &a, int tile,
+                                                float inv_npix) {
+    if constexpr (MULTI) {
+        const int k = a.tinst[tile];
+        const MultiInst I = a.mi[k];
+        return TileImage{I.W, I.H, I.ntx, tile - I.tile_base, I.pix_base, k, I.inv_npix};
+    } else {
+        return TileImage{a.W, a.H, a.ntx, tile, 0, 0, inv_npix};
+    }
+}
+
+template <bool MSE, int P, bool MULTI>
 __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
     CompositeArgs a, float *__restrict__ out_rgb, float *__restrict__ out_T,
     int32_t *__restrict__ out_last, const float *__restrict__ target, float *loss,
-    float *__restrict__ grad, float inv_npix) {
+    float *__restrict__ grad, float inv_npix_one) {
     constexpr int NT = kTilePix / P;
     constexpr int NW = NT / kWave;
     constexpr int RSTEP = kTile / P;  // rows between a thread's pixels
     __shared__ float4 sA[NT], sB[NT], sC[NT];
     __shared__ float s_red[NW];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const TileImage im = tile_image<MULTI>(a, tile, inv_npix_one);
+    const float inv_npix = im.inv_npix;
     const int tid = threadIdx.x;
-    const int trow = tile / a.ntx, tcol = tile % a.ntx;
+    const int trow = im.local / im.ntx, tcol = im.local % im.ntx;
     const int j = tcol * kTile + (tid & 15);
-    const float px = pixel_ndc_x(j, a.W);
+    const float px = pixel_ndc_x(j, im.W);
     float py[P], T[P], cr[P], cg[P], cb[P];
     int32_t cur[P];
     bool done[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         const int i = trow * kTile + (tid >> 4) + k * RSTEP;
-        py[k] = pixel_ndc_y(i, a.H);
-        done[k] = !(i < a.H && j < a.W);
+        py[k] = pixel_ndc_y(i, im.H);
+        done[k] = !(i < im.H && j < im.W);
         T[k] = 1.f;
         cr[k] = cg[k] = cb[k] = 0.f;
         cur[k] = 0;
@@ -217,8 +239,8 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         const int i = trow * kTile + (tid >> 4) + k * RSTEP;
-        if (!(i < a.H && j < a.W)) continue;
-        const int pix = i * a.W + j;
+        if (!(i < im.H && j < im.W)) continue;
+        const int pix = im.pix_base + i * im.W + j;
         out_rgb[3 * pix + 0] = cr[k];
         out_rgb[3 * pix + 1] = cg[k];
         out_rgb[3 * pix + 2] = cb[k];
@@ -243,12 +265,12 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
             float s = 0.f;
 #pragma unroll
             for (int w = 0; w < NW; ++w) s += s_red[w];
-            atomicAdd(loss, s);
+            atomicAdd(loss + im.inst, s);
         }
     }
 }
 
-template <int GRAD_MODE, int P>
+template <int GRAD_MODE, int P, bool MULTI>
 __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
     CompositeArgs a, const float *__restrict__ Tfin, const int32_t *__restrict__ last,
     const float *__restrict__ gout, ds_splat2d grad) {
@@ -262,10 +284,11 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int2 range = a.ranges[tile];
     if (range.y <= range.x) return;
+    const TileImage im = tile_image<MULTI>(a, tile, 0.f);
     const int tid = threadIdx.x;
-    const int trow = tile / a.ntx, tcol = tile % a.ntx;
+    const int trow = im.local / im.ntx, tcol = im.local % im.ntx;
     const int j = tcol * kTile + (tid & 15);
-    const float px = pixel_ndc_x(j, a.W);
+    const float px = pixel_ndc_x(j, im.W);
     // Per pixel: transmittance (walked back from T_final), last contributor,
     // dL/dC, and acc = (colour of the splats behind) . dL/dC.
     float py[P], T[P], gr[P], gg[P], gb[P], acc[P];
@@ -274,9 +297,9 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         const int i = trow * kTile + (tid >> 4) + k * RSTEP;
-        py[k] = pixel_ndc_y(i, a.H);
-        const bool inside = (i < a.H && j < a.W);
-        const int pix = inside ? i * a.W + j : 0;
+        py[k] = pixel_ndc_y(i, im.H);
+        const bool inside = (i < im.H && j < im.W);
+        const int pix = inside ? im.pix_base + i * im.W + j : 0;
         T[k] = inside ? Tfin[pix] : 1.f;
         lst[k] = inside ? last[pix] : -1;
         gr[k] = inside ? gout[3 * pix + 0] : 0.f;
@@ -400,22 +423,40 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
         default: { constexpr int PP = 4; __VA_ARGS__; } break; \
     }
 
+template <bool MULTI>
+static void composite_fwd(const CompositeArgs &a, int P, float *out_rgb, float *out_T,
+                          int32_t *out_last, const float *target, float *loss, float *grad,
+                          hipStream_t s) {
+    const float inv_npix = 1.f / ((float)a.W * (float)a.H);
+    if (target) {
+        DS_PIX_DISPATCH(P, hipLaunchKernelGGL((k_composite_fwd<true, PP, MULTI>),
+                                              dim3(a.ntiles), dim3(kTilePix / PP), 0, s, a,
+                                              out_rgb, out_T, out_last, target, loss, grad,
+                                              inv_npix));
+    } else {
+        DS_PIX_DISPATCH(P, hipLaunchKernelGGL((k_composite_fwd<false, PP, MULTI>),
+                                              dim3(a.ntiles), dim3(kTilePix / PP), 0, s, a,
+                                              out_rgb, out_T, out_last, (const float *)nullptr,
+                                              (float *)nullptr, (float *)nullptr, inv_npix));
+    }
+}
+
+template <int MODE, bool MULTI>
+static void composite_bwd(const CompositeArgs &a, int P, const float *T, const int32_t *last,
+                          const float *grad_rgb, const ds_splat2d &grad, hipStream_t s) {
+    DS_PIX_DISPATCH(P, hipLaunchKernelGGL((k_composite_bwd<MODE, PP, MULTI>), dim3(a.ntiles),
+                                          dim3(kTilePix / PP), 0, s, a, T, last, grad_rgb,
+                                          grad));
+}
+
 ds_status launch_composite_fwd(const CompositeArgs &a, int pix_per_thread, float *out_rgb,
                                float *out_T, int32_t *out_last, const float *target,
                                float *loss, float *grad, hipStream_t s) {
     if (a.ntiles <= 0) return DS_OK;
-    const float inv_npix = 1.f / ((float)a.W * (float)a.H);
-    const int P = pix_per_thread;
-    if (target) {
-        DS_PIX_DISPATCH(P, hipLaunchKernelGGL((k_composite_fwd<true, PP>), dim3(a.ntiles),
-                                              dim3(kTilePix / PP), 0, s, a, out_rgb, out_T,
-                                              out_last, target, loss, grad, inv_npix));
-    } else {
-        DS_PIX_DISPATCH(P, hipLaunchKernelGGL((k_composite_fwd<false, PP>), dim3(a.ntiles),
-                                              dim3(kTilePix / PP), 0, s, a, out_rgb, out_T,
-                                              out_last, (const float *)nullptr, (float *)nullptr,
-                                              (float *)nullptr, inv_npix));
-    }
+    if (a.mi)
+        composite_fwd<true>(a, pix_per_thread, out_rgb, out_T, out_last, target, loss, grad, s);
+    else
+        composite_fwd<false>(a, pix_per_thread, out_rgb, out_T, out_last, target, loss, grad, s);
     DS_CHECK_LAUNCH("composite_fwd");
     return DS_OK;
 }
@@ -426,13 +467,15 @@ ds_status launch_composite_bwd(const CompositeArgs &a, int pix_per_thread, int g
     if (a.ntiles <= 0) return DS_OK;
     const int P = pix_per_thread;
     if (grad_mode == DS_GRAD_REFERENCE) {
-        DS_PIX_DISPATCH(P, hipLaunchKernelGGL((k_composite_bwd<DS_GRAD_REFERENCE, PP>),
-                                              dim3(a.ntiles), dim3(kTilePix / PP), 0, s, a, T,
-                                              last, grad_rgb, grad));
+        if (a.mi)
+            composite_bwd<DS_GRAD_REFERENCE, true>(a, P, T, last, grad_rgb, grad, s);
+        else
+            composite_bwd<DS_GRAD_REFERENCE, false>(a, P, T, last, grad_rgb, grad, s);
     } else {
-        DS_PIX_DISPATCH(P, hipLaunchKernelGGL((k_composite_bwd<DS_GRAD_EXACT, PP>),
-                                              dim3(a.ntiles), dim3(kTilePix / PP), 0, s, a, T,
-                                              last, grad_rgb, grad));
+        if (a.mi)
+            composite_bwd<DS_GRAD_EXACT, true>(a, P, T, last, grad_rgb, grad, s);
+        else
+            composite_bwd<DS_GRAD_EXACT, false>(a, P, T, last, grad_rgb, grad, s);
     }
     DS_CHECK_LAUNCH("composite_bwd");
     return DS_OK;

@@ -18,11 +18,24 @@ constexpr float kTThres = 0.0001f;
 //   [2] = {r, g, b, 0}
 // conic = inverse 2D covariance: a = cyy/det, b = -cxy/det, c = cxx/det.
 
+// One fit instance of a multi-instance rasterizer: its image, its run of the
+// global tile ids and of the concatenated pixel arrays, 1 / (W H) for its loss.
+struct MultiInst {
+    int W, H, ntx, nty;
+    int tile_base, pix_base;
+    float inv_npix;
+    int pad;
+};
+
+// mi / tinst are NULL for one instance (W, H, ntx describe it); otherwise
+// tile t belongs to instance tinst[t] with the layout mi[tinst[t]].
 struct CompositeArgs {
     const float4 *rec;
     const uint32_t *flat;
     const int2 *ranges;
     int W, H, ntx, ntiles;
+    const MultiInst *mi;
+    const uint16_t *tinst;
 };
 
 // pix_per_thread: 1, 2 or 4 pixels per thread (256, 128 or 64 threads per tile).

@@ -28,6 +28,9 @@
 //    waves, then one global atomic per (splat, field, tile).
 #include <float.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "../ds_device.hpp"
 #include "../ds_internal.hpp"
 #include "composite.hpp"
@@ -84,11 +87,13 @@ __device__ __forceinline__ int tile_floor(float v, int hi) {
 }
 
 // Start of a forward: device scalars, a new look-back epoch, zero tickets,
-// digit histograms and visible-count replicas, empty tile ranges, zero loss,
-// zero segment sums of the tile sort (zero_words words at zero).
+// digit histograms and visible-count replicas, empty tile ranges, zero
+// losses (one per instance), zero segment sums of the tile sort (zero_words
+// words at zero).
 __global__ __launch_bounds__(256) void k_begin(uint32_t *sc, uint32_t count, int2 *ranges,
-                                               int ntiles, float *loss, uint32_t *ghist,
-                                               uint32_t *zero, int zero_words) {
+                                               int ntiles, float *loss, int nloss,
+                                               uint32_t *ghist, uint32_t *zero,
+                                               int zero_words) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < SC_EPOCH) sc[i] = (i == SC_COUNT) ? count : 0u;
     if (i == SC_EPOCH) {
@@ -96,7 +101,8 @@ __global__ __launch_bounds__(256) void k_begin(uint32_t *sc, uint32_t count, int
         sc[SC_EPOCH] = e ? e : 1u;
     }
     if (i == SC_ERR || (i >= SC_TICKET && i < SC_TICKET + LB_SLOTS)) sc[i] = 0u;
-    if (i == 0 && loss) *loss = 0.f;
+    if (loss)
+        for (int t = i; t < nloss; t += gridDim.x * 256) loss[t] = 0.f;
     for (int t = i; t < kGhistWords; t += gridDim.x * 256) ghist[t] = 0u;
     for (int t = i; t < ntiles; t += gridDim.x * 256) ranges[t] = make_int2(0, 0);
     for (int t = i; t < zero_words; t += gridDim.x * 256) zero[t] = 0u;
@@ -145,13 +151,17 @@ __global__ __launch_bounds__(256) void k_zero10(ZeroList z, int n) {
 // histograms go to replica blockIdx % kHistReps of ghist (the one-kernel
 // depth passes read them); the visible count goes to one of the kVisReps
 // words, once per workgroup.
+// With MULTI the image of splat i is that of its instance, mi[sinst[i]], and
+// the rectangle is in that image's tiles.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H, int ntx,
                                               int nty, uint2 *__restrict__ rect,
                                               uint32_t *__restrict__ ntiles,
                                               uint32_t *__restrict__ dkey,
                                               uint32_t *__restrict__ didx,
                                               float4 *__restrict__ rec, uint32_t *ghist,
-                                              int hist) {
+                                              int hist, const MultiInst *__restrict__ mi,
+                                              const uint16_t *__restrict__ sinst) {
 #pragma clang fp contract(off)
     __shared__ uint32_t s_h[4][kRadix];  // the four depth-byte histograms
     __shared__ uint32_t s_vis;
@@ -165,6 +175,13 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
     bool visible = false;
     uint32_t key = 0xFFFFFFFFu;  // culled splats sort last
     if (i < n) {
+        if constexpr (MULTI) {
+            const MultiInst I = mi[sinst[i]];
+            W = I.W;
+            H = I.H;
+            ntx = I.ntx;
+            nty = I.nty;
+        }
         const float x = in.pos_x[i], y = in.pos_y[i], z = in.pos_z[i];
         int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
         float cxx = 0.f, cxy = 0.f, cyy = 0.f, det = 0.f;
@@ -222,13 +239,17 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
 // last depth rank d with off[d] <= p (off is the exclusive scan of the tile
 // counts in depth order, so that splat's range [off[d], off[d+1]) holds p and
 // is not empty), found by binary search. The first thread also totals the
-// visible-count replicas into the device scalar.
+// visible-count replicas into the device scalar. With MULTI the tile id is
+// global: the splat's instance's first tile plus the tile in its image.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_emit(const uint32_t *__restrict__ order,
                                               const uint32_t *__restrict__ off,
                                               const uint2 *__restrict__ rect, uint32_t *sc,
                                               const uint32_t *__restrict__ ghist, int ntx,
                                               uint32_t *__restrict__ tkey,
-                                              uint32_t *__restrict__ tval) {
+                                              uint32_t *__restrict__ tval,
+                                              const MultiInst *__restrict__ mi,
+                                              const uint16_t *__restrict__ sinst) {
     const uint32_t n = sc[SC_COUNT];
     const uint32_t m = sc[SC_NISECT];  // pairs kept: the total clamped at the cap
     if (gtid() == 0) {
@@ -252,7 +273,12 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t *__restrict__ order
         const uint32_t tx0 = r.x & 0xFFFF, tx1 = r.x >> 16, ty0 = r.y & 0xFFFF;
         const uint32_t w = tx1 - tx0 + 1;
         const uint32_t ty = ty0 + j / w, tx = tx0 + j % w;
-        tkey[p] = ty * (uint32_t)ntx + tx;
+        if constexpr (MULTI) {
+            const MultiInst I = mi[sinst[g]];
+            tkey[p] = (uint32_t)I.tile_base + ty * (uint32_t)I.ntx + tx;
+        } else {
+            tkey[p] = ty * (uint32_t)ntx + tx;
+        }
         tval[p] = g;
     }
 }
@@ -570,6 +596,13 @@ struct ds_raster {
     int bwd_ppt = 1;  // compositing backward: pixels per thread
     int tile_bits = 1;
     int tile_passes = 1;  // 8-bit radix passes over the tile ids
+    // Several fit instances in one rasterizer (ninst > 0): splats, tiles and
+    // pixels of instance k are contiguous runs of the global index spaces;
+    // per instance its layout, per splat and per tile its instance.
+    int ninst = 0;
+    int64_t npix = 0;  // pixels of all instances
+    ds::DevBuf<ds::MultiInst> mtab;
+    ds::DevBuf<uint16_t> sinst, tinst;
     // per splat
     ds::DevBuf<uint2> rect;
     ds::DevBuf<uint32_t> ntiles_of, dkey_a, dkey_b, didx_a, didx_b, pair_off;
@@ -597,6 +630,13 @@ struct ds_raster {
 
 namespace ds {
 
+// The one-pass tile sort handles up to kTileBinsMax tiles of one instance.
+static bool one_pass_tile_sort(const ds_raster *r) {
+    return r->ninst == 0 && r->ntiles <= kTileBinsMax;
+}
+
+static ds_status raster_alloc_tiles(ds_raster *r);
+
 static ds_status raster_alloc_pixels(ds_raster *r, int W, int H) {
     DS_REQUIRE(W > 0 && H > 0 && W <= 16384 && H <= 16384, "raster size out of range");
     r->W = W;
@@ -604,11 +644,18 @@ static ds_status raster_alloc_pixels(ds_raster *r, int W, int H) {
     r->ntx = (W + kTile - 1) / kTile;
     r->nty = (H + kTile - 1) / kTile;
     r->ntiles = r->ntx * r->nty;
+    r->npix = (int64_t)W * H;
+    return raster_alloc_tiles(r);
+}
+
+// Tile-sort and per-tile / per-pixel buffers for r->ntiles tiles and r->npix
+// pixels.
+static ds_status raster_alloc_tiles(ds_raster *r) {
     int bits = 1;
     while ((1 << bits) < r->ntiles) ++bits;
     r->tile_bits = bits;
     r->tile_passes = (bits + kRadixBits - 1) / kRadixBits;
-    if (r->ntiles <= kTileBinsMax) {
+    if (one_pass_tile_sort(r)) {
         const size_t max_chunks = ((size_t)r->isect_cap + kTileChunk - 1) / kTileChunk;
         DS_TRY(r->tcnt.alloc(max_chunks * (size_t)r->ntiles));
         DS_TRY(r->tseg.alloc((size_t)kTileSegs * r->ntiles));
@@ -621,16 +668,26 @@ static ds_status raster_alloc_pixels(ds_raster *r, int W, int H) {
         r->tfirst.release();
     }
     DS_TRY(r->ranges.alloc((size_t)r->ntiles));
-    DS_TRY(r->T.alloc((size_t)W * H));
-    DS_TRY(r->last.alloc((size_t)W * H));
+    DS_TRY(r->T.alloc((size_t)r->npix));
+    DS_TRY(r->last.alloc((size_t)r->npix));
     r->has_fwd = false;
     return DS_OK;
+}
+
+static CompositeArgs composite_args(const ds_raster *r) {
+    const bool multi = r->ninst > 0;
+    return CompositeArgs{(const float4 *)r->rec.ptr, r->flat, (const int2 *)r->ranges.ptr,
+                         r->W, r->H, r->ntx, r->ntiles,
+                         multi ? (const MultiInst *)r->mtab.ptr : nullptr,
+                         multi ? (const uint16_t *)r->tinst.ptr : nullptr};
 }
 
 static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_rgb,
                                 const float *target, float *loss, float *grad, hipStream_t s) {
     const int n = in.count;
     DS_REQUIRE(n >= 0 && n <= r->count_cap, "splat count exceeds the rasterizer's count_cap");
+    DS_REQUIRE(r->ninst == 0 || n == r->count_cap,
+               "a multi-instance rasterizer takes exactly the splats of all its instances");
     DS_REQUIRE(out_rgb != nullptr, "out_rgb is NULL");
     DS_REQUIRE((target == nullptr) == (loss == nullptr) && (target == nullptr) == (grad == nullptr),
                "target, loss and grad must be given together");
@@ -640,18 +697,30 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
     auto slot = [&](uint64_t *status, int lb) {
         return LookbackSlot{status, sc + SC_TICKET + lb, sc + SC_EPOCH, sc + SC_ERR};
     };
-    const bool one_pass_tiles = r->ntiles <= kTileBinsMax;
+    const bool one_pass_tiles = one_pass_tile_sort(r);
+    const bool multi = r->ninst > 0;
+    const MultiInst *mi = multi ? r->mtab.ptr : nullptr;
+    const uint16_t *si = multi ? r->sinst.ptr : nullptr;
     const int zero_words = one_pass_tiles ? kTileSegs * r->ntiles : 0;
     const uint64_t begin_items =
         std::max<uint64_t>({(uint64_t)r->ntiles, (uint64_t)zero_words / 4, kGhistWords / 4});
+    const int nloss = multi ? r->ninst : 1;
     hipLaunchKernelGGL(k_begin, dim3(grid_cap(begin_items, 256, 1024)), dim3(256), 0, s, sc,
-                       (uint32_t)n, r->ranges.ptr, r->ntiles, loss, gh, r->tseg.ptr, zero_words);
+                       (uint32_t)n, r->ranges.ptr, r->ntiles, loss, nloss, gh, r->tseg.ptr,
+                       zero_words);
     DS_CHECK_LAUNCH("begin");
     if (n > 0) {
         const bool one_kernel = (uint32_t)n <= kOneKernelMaxKeys;
-        hipLaunchKernelGGL(k_prep, dim3((n + 255) / 256), dim3(256), 0, s, in, n, r->W, r->H,
-                           r->ntx, r->nty, r->rect.ptr, r->ntiles_of.ptr, r->dkey_a.ptr,
-                           r->didx_a.ptr, r->rec.ptr, gh, one_kernel ? 1 : 0);
+        if (multi)
+            hipLaunchKernelGGL(k_prep<true>, dim3((n + 255) / 256), dim3(256), 0, s, in, n, r->W,
+                               r->H, r->ntx, r->nty, r->rect.ptr, r->ntiles_of.ptr,
+                               r->dkey_a.ptr, r->didx_a.ptr, r->rec.ptr, gh, one_kernel ? 1 : 0,
+                               mi, si);
+        else
+            hipLaunchKernelGGL(k_prep<false>, dim3((n + 255) / 256), dim3(256), 0, s, in, n,
+                               r->W, r->H, r->ntx, r->nty, r->rect.ptr, r->ntiles_of.ptr,
+                               r->dkey_a.ptr, r->didx_a.ptr, r->rec.ptr, gh, one_kernel ? 1 : 0,
+                               mi, si);
         DS_CHECK_LAUNCH("prep");
         // Depth sort of splats: four 8-bit LSD passes over the 32 depth bits,
         // then the pair offsets in depth order (total and capped count). Up
@@ -712,10 +781,17 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
             DS_CHECK_LAUNCH("tile_scatter");
             r->flat = r->tval_b.ptr;
         } else {
-            hipLaunchKernelGGL(k_emit, dim3(grid_cap((uint64_t)r->isect_cap, 256, 4096)),
-                               dim3(256), 0, s, order, (const uint32_t *)r->pair_off.ptr,
-                               (const uint2 *)r->rect.ptr, sc, (const uint32_t *)gh, r->ntx,
-                               r->tkey_a.ptr, r->tval_a.ptr);
+            const dim3 egrid(grid_cap((uint64_t)r->isect_cap, 256, 4096));
+            if (multi)
+                hipLaunchKernelGGL(k_emit<true>, egrid, dim3(256), 0, s, order,
+                                   (const uint32_t *)r->pair_off.ptr, (const uint2 *)r->rect.ptr,
+                                   sc, (const uint32_t *)gh, r->ntx, r->tkey_a.ptr,
+                                   r->tval_a.ptr, mi, si);
+            else
+                hipLaunchKernelGGL(k_emit<false>, egrid, dim3(256), 0, s, order,
+                                   (const uint32_t *)r->pair_off.ptr, (const uint2 *)r->rect.ptr,
+                                   sc, (const uint32_t *)gh, r->ntx, r->tkey_a.ptr,
+                                   r->tval_a.ptr, mi, si);
             DS_CHECK_LAUNCH("emit");
             // Stable sort of the pairs on the tile bits. With hundreds of
             // chunks the reduce-then-scan passes beat one-kernel passes,
@@ -738,8 +814,7 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
     } else {
         r->flat = r->tval_a.ptr;
     }
-    const CompositeArgs ca{(const float4 *)r->rec.ptr, r->flat, (const int2 *)r->ranges.ptr,
-                           r->W, r->H, r->ntx, r->ntiles};
+    const CompositeArgs ca = composite_args(r);
     DS_TRY(launch_composite_fwd(ca, r->fwd_ppt, out_rgb, r->T.ptr, r->last.ptr, target,
                                 loss, grad, s));
     r->last_count = n;
@@ -752,8 +827,7 @@ static ds_status raster_backward(ds_raster *r, const ds_splat2d &in, const float
     DS_REQUIRE(r->has_fwd, "ds_raster_bwd called before ds_raster_fwd");
     DS_REQUIRE(in.count == r->last_count, "splat count differs from the last forward");
     DS_REQUIRE(grad_rgb != nullptr, "grad_rgb is NULL");
-    const CompositeArgs ca{(const float4 *)r->rec.ptr, r->flat, (const int2 *)r->ranges.ptr,
-                           r->W, r->H, r->ntx, r->ntiles};
+    const CompositeArgs ca = composite_args(r);
     DS_TRY(launch_composite_bwd(ca, r->bwd_ppt, r->grad_mode, r->T.ptr, r->last.ptr,
                                 grad_rgb, gin, s));
     return DS_OK;
@@ -765,22 +839,11 @@ static ds_status raster_backward(ds_raster *r, const ds_splat2d &in, const float
 
 using namespace ds;
 
-extern "C" ds_status ds_raster_create(const ds_raster_config *cfg, ds_raster **out) {
-    clear_error();
-    DS_REQUIRE(cfg && out, "NULL argument");
-    DS_REQUIRE(cfg->count_cap >= 0, "count_cap < 0");
-    DS_REQUIRE((uint64_t)cfg->count_cap <= kLookbackMaxCount, "count_cap above 2^30 - 1");
-    int64_t cap = cfg->isect_cap > 0 ? cfg->isect_cap : (int64_t)1 << 25;
-    DS_REQUIRE(cap <= (int64_t)kLookbackMaxCount, "isect_cap above 2^30 - 1");
-    DS_REQUIRE(cfg->grad_mode == DS_GRAD_EXACT || cfg->grad_mode == DS_GRAD_REFERENCE,
-               "grad_mode must be DS_GRAD_EXACT or DS_GRAD_REFERENCE");
-    auto *r = new ds_raster();
-    r->count_cap = cfg->count_cap;
-    r->isect_cap = (uint32_t)cap;
-    r->keep_keys = cfg->keep_keys;
-    r->grad_mode = cfg->grad_mode;
-    ds_status st = raster_alloc_pixels(r, cfg->width, cfg->height);
-    const size_t nc = (size_t)(cfg->count_cap > 0 ? cfg->count_cap : 1);
+// Per-splat and per-pair buffers, scratch and device scalars of a raster
+// whose pixel/tile buffers are allocated; frees r on failure.
+static ds_status raster_finish_create(ds_raster *r, ds_status st, ds_raster **out) {
+    const size_t nc = (size_t)(r->count_cap > 0 ? r->count_cap : 1);
+    const int64_t cap = r->isect_cap;
     if (st == DS_OK) st = r->rect.alloc(nc);
     if (st == DS_OK) st = r->ntiles_of.alloc(nc);
     if (st == DS_OK) st = r->dkey_a.alloc(nc);
@@ -817,6 +880,95 @@ extern "C" ds_status ds_raster_create(const ds_raster_config *cfg, ds_raster **o
     return DS_OK;
 }
 
+extern "C" ds_status ds_raster_create(const ds_raster_config *cfg, ds_raster **out) {
+    clear_error();
+    DS_REQUIRE(cfg && out, "NULL argument");
+    DS_REQUIRE(cfg->count_cap >= 0, "count_cap < 0");
+    DS_REQUIRE((uint64_t)cfg->count_cap <= kLookbackMaxCount, "count_cap above 2^30 - 1");
+    int64_t cap = cfg->isect_cap > 0 ? cfg->isect_cap : (int64_t)1 << 25;
+    DS_REQUIRE(cap <= (int64_t)kLookbackMaxCount, "isect_cap above 2^30 - 1");
+    DS_REQUIRE(cfg->grad_mode == DS_GRAD_EXACT || cfg->grad_mode == DS_GRAD_REFERENCE,
+               "grad_mode must be DS_GRAD_EXACT or DS_GRAD_REFERENCE");
+    auto *r = new ds_raster();
+    r->count_cap = cfg->count_cap;
+    r->isect_cap = (uint32_t)cap;
+    r->keep_keys = cfg->keep_keys;
+    r->grad_mode = cfg->grad_mode;
+    return raster_finish_create(r, raster_alloc_pixels(r, cfg->width, cfg->height), out);
+}
+
+extern "C" ds_status ds_raster_create_multi(const ds_raster_instance *inst, int n_inst,
+                                            int64_t isect_cap, int keep_keys, int grad_mode,
+                                            ds_raster **out) {
+    clear_error();
+    DS_REQUIRE(inst && out, "NULL argument");
+    DS_REQUIRE(n_inst >= 1 && n_inst <= 65535, "n_inst must be in [1, 65535]");
+    DS_REQUIRE(grad_mode == DS_GRAD_EXACT || grad_mode == DS_GRAD_REFERENCE,
+               "grad_mode must be DS_GRAD_EXACT or DS_GRAD_REFERENCE");
+    int64_t cap = isect_cap > 0 ? isect_cap : (int64_t)1 << 25;
+    DS_REQUIRE(cap <= (int64_t)kLookbackMaxCount, "isect_cap above 2^30 - 1");
+    std::vector<MultiInst> tab((size_t)n_inst);
+    int64_t nsplat = 0, ntile = 0, npix = 0;
+    for (int k = 0; k < n_inst; ++k) {
+        const ds_raster_instance &q = inst[k];
+        DS_REQUIRE(q.width > 0 && q.height > 0 && q.width <= 16384 && q.height <= 16384,
+                   "instance size out of range");
+        DS_REQUIRE(q.count >= 0, "instance splat count < 0");
+        MultiInst &m = tab[(size_t)k];
+        m.W = q.width;
+        m.H = q.height;
+        m.ntx = (q.width + kTile - 1) / kTile;
+        m.nty = (q.height + kTile - 1) / kTile;
+        m.tile_base = (int)ntile;
+        m.pix_base = (int)npix;
+        m.inv_npix = 1.f / ((float)q.width * (float)q.height);
+        m.pad = 0;
+        nsplat += q.count;
+        ntile += (int64_t)m.ntx * m.nty;
+        npix += (int64_t)q.width * q.height;
+        DS_REQUIRE(nsplat <= (int64_t)kLookbackMaxCount, "total splat count above 2^30 - 1");
+        DS_REQUIRE(ntile < ((int64_t)1 << 30) && npix < ((int64_t)1 << 31) / 3,
+                   "instances too large for one rasterizer");
+    }
+    std::vector<uint16_t> sinst((size_t)std::max<int64_t>(nsplat, 1), 0),
+        tinst((size_t)ntile, 0);
+    for (int k = 0, s0 = 0; k < n_inst; s0 += inst[k].count, ++k) {
+        std::fill(sinst.begin() + s0, sinst.begin() + s0 + inst[k].count, (uint16_t)k);
+        const MultiInst &m = tab[(size_t)k];
+        std::fill(tinst.begin() + m.tile_base, tinst.begin() + m.tile_base + m.ntx * m.nty,
+                  (uint16_t)k);
+    }
+    auto *r = new ds_raster();
+    r->count_cap = (int)nsplat;
+    r->isect_cap = (uint32_t)cap;
+    r->keep_keys = keep_keys;
+    r->grad_mode = grad_mode;
+    r->ninst = n_inst;
+    r->ntiles = (int)ntile;
+    r->npix = npix;
+    ds_status st = raster_alloc_tiles(r);
+    if (st == DS_OK) st = r->mtab.alloc(tab.size());
+    if (st == DS_OK) st = r->sinst.alloc(sinst.size());
+    if (st == DS_OK) st = r->tinst.alloc(tinst.size());
+    if (st == DS_OK &&
+        (hipMemcpy(r->mtab.ptr, tab.data(), tab.size() * sizeof(MultiInst),
+                   hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(r->sinst.ptr, sinst.data(), sinst.size() * 2, hipMemcpyHostToDevice) !=
+             hipSuccess ||
+         hipMemcpy(r->tinst.ptr, tinst.data(), tinst.size() * 2, hipMemcpyHostToDevice) !=
+             hipSuccess))
+        st = set_error(DS_ERR_HIP, "upload of the instance tables");
+    return raster_finish_create(r, st, out);
+}
+
+extern "C" ds_status ds_raster_splat_instances(ds_raster *r, const uint16_t **dev) {
+    clear_error();
+    DS_REQUIRE(r && dev, "NULL argument");
+    DS_REQUIRE(r->ninst > 0, "not a multi-instance rasterizer");
+    *dev = r->sinst.ptr;
+    return DS_OK;
+}
+
 extern "C" ds_status ds_raster_destroy(ds_raster *r) {
     clear_error();
     delete r;
@@ -826,6 +978,7 @@ extern "C" ds_status ds_raster_destroy(ds_raster *r) {
 extern "C" ds_status ds_raster_resize(ds_raster *r, int width, int height) {
     clear_error();
     DS_REQUIRE(r, "NULL raster");
+    DS_REQUIRE(r->ninst == 0, "a multi-instance rasterizer cannot be resized");
     if (width == r->W && height == r->H) return DS_OK;
     return raster_alloc_pixels(r, width, height);
 }

@@ -178,6 +178,20 @@ ds_status ds_activ_project_bwd(const ds_gauss3d *in, const ds_splat2d *grad_out,
                                const float cam_pos[3], const float pv[16], int grad_mode,
                                ds_stream_t s);
 
+/* Fused forward / backward for many fit instances in one launch (no
+ * counterpart in the reference): splat i is seen by the camera of instance
+ * splat_inst[i] (device uint16 array, e.g. ds_raster_splat_instances), whose
+ * record in cam_tab (device, 20 floats per instance) is the 16 PV floats
+ * (column-major, as pv above) then cam_pos x, y, z and one unused float.
+ * Per splat the results equal ds_activ_project_fwd / _bwd with that camera. */
+ds_status ds_activ_project_fwd_multi(const ds_gauss3d *in, const ds_splat2d *out,
+                                     int sh_degree, const float *cam_tab,
+                                     const uint16_t *splat_inst, ds_stream_t s);
+ds_status ds_activ_project_bwd_multi(const ds_gauss3d *in, const ds_splat2d *grad_out,
+                                     const ds_gauss3d *grad_in, int sh_degree,
+                                     const float *cam_tab, const uint16_t *splat_inst,
+                                     int grad_mode, ds_stream_t s);
+
 /* ------------------------------------------------------------------ */
 /* Tile rasterizer                                                     */
 /* ------------------------------------------------------------------ */
@@ -213,6 +227,29 @@ ds_status ds_raster_create(const ds_raster_config *cfg, ds_raster **out);
 ds_status ds_raster_destroy(ds_raster *r);
 /* TileRasterizeLayer::resize (src/layers/tile_rasterize_layer.cu:618-633). */
 ds_status ds_raster_resize(ds_raster *r, int width, int height);
+
+/* Several independent images in one rasterizer (no counterpart in the
+ * reference, which rasterizes one image per TileRasterizeLayer): instance k
+ * owns the k-th run of `count` splats of the Splat2D input, its own
+ * width x height image and 16x16 tiles. One ds_raster_fwd / ds_raster_bwd
+ * then runs every phase once for all instances: tile ids are global (the
+ * instances' tiles one after another, each row-major), the sorted pairs and
+ * tile ranges cover all instances, and the pixel arrays (image, target,
+ * dL/dimage, transmittance, last ids) are the instances' HWC images one after
+ * another. Within an instance every result equals that of a one-instance
+ * rasterizer on its splats (sorted splat ids offset by the instance's first
+ * splat). With a target, loss_dev receives one loss per instance. The pair
+ * cap is shared: pairs beyond it are dropped farthest depth first over all
+ * instances. The one-pass tile sort is not used. Not resizable. */
+typedef struct ds_raster_instance {
+    int32_t width, height;
+    int32_t count;
+} ds_raster_instance;
+ds_status ds_raster_create_multi(const ds_raster_instance *inst, int n_inst, int64_t isect_cap,
+                                 int keep_keys, int grad_mode, ds_raster **out);
+/* Device array [total splats] of each splat's instance (uint16), valid for the
+ * rasterizer's life. */
+ds_status ds_raster_splat_instances(ds_raster *r, const uint16_t **dev);
 
 /* TileRasterizeLayer::forward (src/layers/tile_rasterize_layer.cu:637-708):
  * tile assign -> depth/tile radix sort -> tile ranges -> front-to-back
@@ -451,6 +488,58 @@ ds_status ds_batch_set_streams(ds_batch *b, int n_streams);
  * streams and keeps the fastest (it takes real optimisation steps).
  * Synchronises. Either output may be NULL. */
 ds_status ds_batch_autotune(ds_batch *b, int reps, int *n_streams, float *ms_per_batch_step);
+
+/* Many fits as one (no counterpart in the reference): every phase of a step
+ * is ONE launch over all instances (activation + projection with a per-
+ * instance camera table, a multi-instance ds_raster for tile assign, sorts,
+ * compositing with the per-instance MSE and its backward, one Adam launch
+ * over one parameter slab), and the whole step can be captured into one HIP
+ * graph. Per instance a step equals ds_fitter_step with the same camera,
+ * learning rates and grad mode, up to the order of float atomics. All
+ * instances share the SH degree, the learning rates, the Adam step counter
+ * and the pair cap (isect_cap, 0 = 1<<25). */
+typedef struct ds_multifit ds_multifit;
+
+typedef struct ds_multifit_instance {
+    int32_t width, height;
+    int32_t count;
+    float view[16], proj[16];      /* column-major, OpenGL conventions */
+    float cam_pos[3];
+} ds_multifit_instance;
+
+/* cfg supplies sh_degree, the learning rates, lr_pos_decay, beta1/beta2/
+ * epsilon and grad_mode (its size, camera, isect_cap and fused fields are
+ * not used). */
+ds_status ds_multifit_create(const ds_multifit_instance *inst, int n_inst,
+                             const ds_fitter_config *cfg, int64_t isect_cap, ds_multifit **out);
+ds_status ds_multifit_destroy(ds_multifit *m);
+/* Instance k's target [H*W*3] HWC (host). */
+ds_status ds_multifit_set_target(ds_multifit *m, int k, const float *host_rgb);
+/* Instance k's parameters in the ds_fitter_upload_params layout. Uploading
+ * resets the Adam state of every instance and the shared step counter. */
+ds_status ds_multifit_upload_params(ds_multifit *m, int k, const float *host_slab);
+ds_status ds_multifit_download_params(ds_multifit *m, int k, float *host_slab);
+/* Gradients of the last step (same layout). Synchronises. */
+ds_status ds_multifit_download_grads(ds_multifit *m, int k, float *host_slab);
+/* Rendered image of instance k from the last forward. Synchronises. */
+ds_status ds_multifit_download_image(ds_multifit *m, int k, float *host_rgb);
+/* n_steps steps of every instance, asynchronous on the multifit's stream. */
+ds_status ds_multifit_step(ds_multifit *m, int n_steps);
+/* Forward only (render + loss of every instance). */
+ds_status ds_multifit_forward(ds_multifit *m);
+/* zero grads, forward, backward (no Adam update). */
+ds_status ds_multifit_forward_backward(ds_multifit *m);
+/* Loss of every instance from the last forward, host [n_inst]. Synchronises. */
+ds_status ds_multifit_losses(ds_multifit *m, float *losses);
+ds_status ds_multifit_synchronize(ds_multifit *m);
+/* 1: capture one step into a HIP graph on first use and replay it. */
+ds_status ds_multifit_set_graph(ds_multifit *m, int enable);
+/* As ds_fitter_autotune, over the compositing launches of all instances. */
+ds_status ds_multifit_autotune(ds_multifit *m, int reps, int *fwd_pix_per_thread,
+                               int *bwd_pix_per_thread, float *fwd_ms, float *bwd_ms);
+ds_status ds_multifit_step_count(ds_multifit *m, int32_t *step);
+ds_status ds_multifit_get_raster(ds_multifit *m, ds_raster **r);
+ds_status ds_multifit_get_stream(ds_multifit *m, ds_stream_t *s);
 
 /* ------------------------------------------------------------------ */
 /* Forward-only renderer (SplatRenderer, src/pipelines/splat_renderer.{h,cpp}) */
