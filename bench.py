@@ -224,12 +224,37 @@ def learned_vs_plain(ds, n_train=10000, train_steps=10000, n_eval=64, W=256, H=2
 def batched_fit(ds, count=64, steps=500, seq_steps=10):
     """Batched fitting: `count` independent fits of uneven sizes (areas
     log-uniform 128^2..1024x576, splat counts log-uniform 2k..100k, the
-    synthetic target and random frustum init of each) stepped together for
-    `steps` steps after the autotuner. Also times `seq_steps` batch steps with
-    one stream (the instances one after another) for comparison."""
+    synthetic target and random frustum init of each), `steps` steps each.
+    Primary: ds_multifit, every phase of a step one launch over all instances
+    and the step one HIP graph. Compared with: the instances' own captured
+    steps spread over autotuned streams (ds_batch), and the instances one
+    after another on one stream (`seq_steps` batch steps)."""
     lr = {"lr_pos": 1e-2, "lr_sh_dc": 1e-2, "lr_scale": 5e-3, "lr_rot": 6e-4,
           "lr_opacity": 6e-3}
     sizes = ds.batch_instances(count)
+    inst = []
+    for w, h, n, seed in sizes:
+        view, proj, cam = ds.fitter_camera(w, h)
+        inst.append((w, h, n, view, proj, cam))
+    mf = ds.MultiFitter(inst, lr=lr, isect_cap=1 << 27)
+    for k, (w, h, n, seed) in enumerate(sizes):
+        mf.set_target(k, ds.target_image(seed, w, h, 32))
+        mf.upload_params(k, ds.frustum_splats(seed, n, aspect=w / h))
+    mf.forward()
+    pairs0 = mf.raster_counts()[0]
+    mtune = mf.autotune(reps=3)
+    mf.set_graph(True)
+    mf.step(1)
+    mf.synchronize()
+    t0 = time.perf_counter()
+    mf.step(steps)
+    mf.synchronize()
+    m_el = time.perf_counter() - t0
+    mf.forward()
+    m_psnr = [ds.psnr_from_loss(float(x)) for x in mf.losses()]
+    pairs1 = mf.raster_counts()[0]
+    mf.close()
+
     fits = []
     for w, h, n, seed in sizes:
         view, proj, cam = ds.fitter_camera(w, h)
@@ -262,14 +287,27 @@ def batched_fit(ds, count=64, steps=500, seq_steps=10):
     batch.close()
     for f in fits:
         f.close()
+    m_ms = 1e3 * m_el / steps
     return {"workload": f"{count} uneven fits (128^2..1024x576 pixels, 2k..100k splats), "
-                        f"{steps} steps each", "instances": count, "steps": steps,
-            "streams": streams, "seconds": round(el, 4),
-            "instance_steps_per_s": round(count * steps / el, 1),
-            "ms_per_batch_step": round(1e3 * el / steps, 4),
-            "one_stream_ms_per_batch_step": round(seq_ms, 4),
-            "psnr_db_mean": round(float(np.mean(psnr)), 3),
-            "psnr_db_min": round(float(np.min(psnr)), 3)}
+                        f"{steps} steps each",
+            "instances": count, "steps": steps,
+            "pixels": int(sum(w * h for w, h, _, _ in sizes)),
+            "splats": int(sum(n for _, _, n, _ in sizes)),
+            "instance_steps_per_s": round(count * steps / m_el, 1),
+            "ms_per_batch_step": round(m_ms, 4),
+            "method": "ds_multifit: one launch per phase over all instances, one HIP graph "
+                      "per step",
+            "compositing_tuning": mtune, "pairs_first_step": int(pairs0),
+            "pairs_last_step": int(pairs1),
+            "psnr_db_mean": round(float(np.mean(m_psnr)), 3),
+            "psnr_db_min": round(float(np.min(m_psnr)), 3),
+            "streams_batch": {"streams": streams, "seconds": round(el, 4),
+                              "instance_steps_per_s": round(count * steps / el, 1),
+                              "ms_per_batch_step": round(1e3 * el / steps, 4),
+                              "psnr_db_mean": round(float(np.mean(psnr)), 3)},
+            "sequential_ms_per_batch_step": round(seq_ms, 4),
+            "speedup_vs_sequential": round(seq_ms / m_ms, 2),
+            "speedup_vs_streams": round((1e3 * el / steps) / m_ms, 2)}
 
 
 def viewer_fps(ds, n=1_000_000, W=1920, H=1080, n_cams=120):

@@ -273,6 +273,16 @@ def psnr_from_loss(loss: float) -> float:
 
 
 # ---------------------------------------------------------------- objects
+TILE_ORDERS = {0: "rows", 1: "heavy_first"}
+
+
+def tile_order_name(raster_handle) -> str:
+    """Compositing dispatch order of a ds_raster ("rows" or "heavy_first")."""
+    o = C.c_int()
+    check(lib().ds_raster_get_tile_order(raster_handle, C.byref(o)), "ds_raster_get_tile_order")
+    return TILE_ORDERS[o.value]
+
+
 class Rasterizer:
     """ds_raster: tile assign + sort + compositing, forward and backward."""
 
@@ -473,8 +483,11 @@ class ImageFitter:
         tf, tb = C.c_float(), C.c_float()
         check(lib().ds_fitter_autotune(self.h, reps, C.byref(f), C.byref(b), C.byref(tf),
                                        C.byref(tb)), "ds_fitter_autotune")
+        r = C.c_void_p()
+        check(lib().ds_fitter_get_raster(self.h, C.byref(r)), "ds_fitter_get_raster")
         return {"fwd_pix_per_thread": f.value, "bwd_pix_per_thread": b.value,
-                "fwd_ms": round(tf.value, 4), "bwd_ms": round(tb.value, 4)}
+                "fwd_ms": round(tf.value, 4), "bwd_ms": round(tb.value, 4),
+                "tile_order": tile_order_name(r)}
 
     def set_tuning(self, fwd_ppt: int, bwd_ppt: int) -> None:
         r = C.c_void_p()
@@ -685,8 +698,11 @@ class MultiFitter:
         tf, tb = C.c_float(), C.c_float()
         check(lib().ds_multifit_autotune(self.h, reps, C.byref(f), C.byref(b), C.byref(tf),
                                          C.byref(tb)), "ds_multifit_autotune")
+        r = C.c_void_p()
+        check(lib().ds_multifit_get_raster(self.h, C.byref(r)), "ds_multifit_get_raster")
         return {"fwd_pix_per_thread": f.value, "bwd_pix_per_thread": b.value,
-                "fwd_ms": round(tf.value, 4), "bwd_ms": round(tb.value, 4)}
+                "fwd_ms": round(tf.value, 4), "bwd_ms": round(tb.value, 4),
+                "tile_order": tile_order_name(r)}
 
     def set_tuning(self, fwd_ppt: int, bwd_ppt: int) -> None:
         r = C.c_void_p()
@@ -783,7 +799,10 @@ class SplatRenderer:
         ms = C.c_float()
         check(lib().ds_renderer_autotune(self.h, reps, C.byref(best), C.byref(ms)),
               "ds_renderer_autotune")
-        return {"fwd_pix_per_thread": best.value, "ms_per_frame": round(ms.value, 4)}
+        r = C.c_void_p()
+        check(lib().ds_renderer_get_raster(self.h, C.byref(r)), "ds_renderer_get_raster")
+        return {"fwd_pix_per_thread": best.value, "ms_per_frame": round(ms.value, 4),
+                "tile_order": tile_order_name(r)}
 
     def synchronize(self) -> None:
         check(lib().ds_renderer_synchronize(self.h), "ds_renderer_synchronize")

@@ -154,6 +154,8 @@ _SIGS = {
     "ds_raster_get_view": (C.c_int, [_P, C.POINTER(RasterView)]),
     "ds_raster_set_tuning": (C.c_int, [_P, C.c_int, C.c_int]),
     "ds_raster_get_tuning": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "ds_raster_set_tile_order": (C.c_int, [_P, C.c_int]),
+    "ds_raster_get_tile_order": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "ds_fitter_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _F,
                                      _F]),
     "ds_sort_pairs_u64": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_int, _P]),

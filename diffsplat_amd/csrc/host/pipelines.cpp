@@ -23,6 +23,21 @@ static int pick_with_margin(const float t[3], int current_ppt) {
     return best;
 }
 
+// Times `timed()` (ms) with each compositing dispatch order of r and keeps
+// the faster (the current order unless the other is at least 5% faster).
+template <class F>
+static void tune_tile_order(ds_raster *r, F &&timed) {
+    int cur = DS_TILE_ORDER_HEAVY_FIRST;
+    check(ds_raster_get_tile_order(r, &cur), "autotune");
+    float t[2];
+    for (int o = 0; o < 2; ++o) {
+        check(ds_raster_set_tile_order(r, o), "autotune");
+        t[o] = timed();
+    }
+    const int other = 1 - cur;
+    check(ds_raster_set_tile_order(r, t[other] < 0.95f * t[cur] ? other : cur), "autotune");
+}
+
 // ============================================================ ImageFitter
 
 ImageFitter::ImageFitter() = default;
@@ -249,6 +264,12 @@ void ImageFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd
     const int bb = opts[ib];
     const float tf = tfw[bf == 1 ? 0 : bf == 2 ? 1 : 2], tb = tbw[ib];
     check(ds_raster_set_tuning(r, bf, bb), "autotune");
+    tune_tile_order(r, [&] {
+        return timed([&] {
+            pipeline.forward();
+            ras_layer.backward();
+        });
+    });
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     dropGraph();  // the captured step (if any) used the old launch shapes
@@ -582,6 +603,12 @@ void MultiFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd
     const int bb = opts[ib];
     const float tf = tfw[bf == 1 ? 0 : bf == 2 ? 1 : 2], tb = tbw[ib];
     check(ds_raster_set_tuning(ras, bf, bb), "autotune");
+    tune_tile_order(ras, [&] {
+        return timed([&] {
+            forwardOnly();
+            check(ds_raster_bwd(ras, &sv, grad_rgb.ptr, &gv, stream), "bwd");
+        });
+    });
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     dropGraph();
@@ -653,6 +680,16 @@ int SplatRenderer::autotune(int reps, float *ms_best) {
     }
     const int ib = pick_with_margin(t, fwd0);
     check(ds_raster_set_tuning(r, opts[ib], bwd0), "autotune");
+    tune_tile_order(r, [&] {
+        pipeline.forward();  // warm-up
+        hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+        for (int k = 0; k < reps; ++k) pipeline.forward();
+        hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        return ms / reps;
+    });
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (ms_best) *ms_best = t[ib];

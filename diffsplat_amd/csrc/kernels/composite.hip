@@ -8,8 +8,9 @@
 //                                T_before = T / max(1 - alpha, 1e-6), conic
 //                                gradients via the reformulated chain rule
 //
-// MI355X layout: a workgroup owns one tile and each thread P pixels of it
-// (P = 4: one wave64 per tile; P = 2: two waves; P = 1: four waves). Splat
+// MI355X layout: a workgroup owns one tile and each thread P vertically
+// adjacent pixels of it, so a wave covers a contiguous 16 x 4P band (P = 4:
+// one wave64 per tile; P = 2: two waves; P = 1: four waves). Splat
 // records are staged in LDS in batches of one record per thread and read as
 // broadcasts. The per-pixel work has no branches: a skipped splat is a
 // select that adds exact zeros. In the backward the nine per-splat gradient
@@ -24,12 +25,11 @@
 namespace ds {
 namespace {
 
-// Workgroup -> tile so that the tiles of one XCD (dispatch is round-robin
-// over 8 XCDs) form one contiguous row-major band sharing that XCD's L2.
-__device__ __forceinline__ int xcd_tile(int bid, int nblk) {
-    const int q = nblk / 8, r = nblk % 8;
-    const int x = bid % 8, local = bid / 8;
-    return x * q + min(x, r) + local;
+// Tile of this workgroup: the dispatch order puts the tiles with the most
+// pairs first (the longest jobs start first and the grid ends evenly), or
+// is row-major.
+__device__ __forceinline__ int tile_of_block(const CompositeArgs &a) {
+    return a.order ? (int)a.order[blockIdx.x] : (int)blockIdx.x;
 }
 
 // u = (Sigma^-1 d)_x, w = (Sigma^-1 d)_y for d = splat centre - pixel, so that
@@ -158,10 +158,9 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
     float *__restrict__ grad, float inv_npix_one) {
     constexpr int NT = kTilePix / P;
     constexpr int NW = NT / kWave;
-    constexpr int RSTEP = kTile / P;  // rows between a thread's pixels
     __shared__ float4 sA[NT], sB[NT], sC[NT];
     __shared__ float s_red[NW];
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tile = tile_of_block(a);
     const TileImage im = tile_image<MULTI>(a, tile, inv_npix_one);
     const float inv_npix = im.inv_npix;
     const int tid = threadIdx.x;
@@ -173,7 +172,7 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
     bool done[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        const int i = trow * kTile + (tid >> 4) + k * RSTEP;
+        const int i = trow * kTile + (tid >> 4) * P + k;
         py[k] = pixel_ndc_y(i, im.H);
         done[k] = !(i < im.H && j < im.W);
         T[k] = 1.f;
@@ -238,7 +237,7 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
     float l = 0.f;
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        const int i = trow * kTile + (tid >> 4) + k * RSTEP;
+        const int i = trow * kTile + (tid >> 4) * P + k;
         if (!(i < im.H && j < im.W)) continue;
         const int pix = im.pix_base + i * im.W + j;
         out_rgb[3 * pix + 0] = cr[k];
@@ -276,12 +275,11 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
     const float *__restrict__ gout, ds_splat2d grad) {
     constexpr int NT = kTilePix / P;
     constexpr int NW = NT / kWave;
-    constexpr int RSTEP = kTile / P;
     constexpr int GW = NW > 1 ? NT : 1;
     __shared__ float4 sA[NT], sB[NT], sC[NT];
     __shared__ float sG[9][GW];
     __shared__ int s_last;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tile = tile_of_block(a);
     const int2 range = a.ranges[tile];
     if (range.y <= range.x) return;
     const TileImage im = tile_image<MULTI>(a, tile, 0.f);
@@ -296,7 +294,7 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
     int my_last = -1;
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        const int i = trow * kTile + (tid >> 4) + k * RSTEP;
+        const int i = trow * kTile + (tid >> 4) * P + k;
         py[k] = pixel_ndc_y(i, im.H);
         const bool inside = (i < im.H && j < im.W);
         const int pix = inside ? im.pix_base + i * im.W + j : 0;
@@ -308,7 +306,8 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
         acc[k] = 0.f;
         my_last = max(my_last, lst[k]);
     }
-    const int wave_last = wave_max(my_last);
+    // wave- and workgroup-uniform: the splat loop runs on scalar registers
+    const int wave_last = __builtin_amdgcn_readfirstlane(wave_max(my_last));
     int block_last = wave_last;
     if constexpr (NW > 1) {
         if (tid == 0) s_last = -1;
@@ -317,18 +316,23 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
         __syncthreads();
         if (__lane_id() == 0) atomicMax(&s_last, wave_last);
         __syncthreads();
-        block_last = s_last;
+        block_last = __builtin_amdgcn_readfirstlane(s_last);
     }
     float *const dst[9] = {grad.pos_x,  grad.pos_y,   grad.cov_xx,  grad.cov_xy, grad.cov_yy,
                            grad.color_r, grad.color_g, grad.color_b, grad.opacity};
     // The field this lane adds after the transposed reduction, its array,
-    // and the factor 2 of d dist2 / d(x, y) and of the symmetric cxy term.
+    // and the factor that turns the wave sum into the gradient: the sums of
+    // the position and conic fields are taken with m = -2 dL/d dist2 (see
+    // below), so position x, y get -1 (d dist2/d(x, y) = 2 (u, w)), conic
+    // a and c get 1/2 and the symmetric conic b 1.
     const int lane = __lane_id();
     const int my_field = field_of_lane(lane);
     float *my_dst = nullptr;
 #pragma unroll
     for (int f = 0; f < 9; ++f) my_dst = my_field == f ? dst[f] : my_dst;
-    const float my_scale = (my_field == 0 || my_field == 1 || my_field == 3) ? 2.f : 1.f;
+    const float my_scale = (my_field == 0 || my_field == 1) ? -1.f
+                           : (my_field == 2 || my_field == 4) ? 0.5f
+                                                              : 1.f;
     const int end = min(range.y, block_last + 1);
     for (int bend = end - 1; bend >= range.x; bend -= NT) {
         const int bs = min(NT, bend + 1 - range.x);
@@ -378,13 +382,15 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
                     v_alpha = Tb * cg - ra * acc[k];
                 const bool gate = ok[k] & (e[k].og <= kMaxAlpha);  // alpha below its clamp
                 v[8] = fmaf(gate ? e[k].g : 0.f, v_alpha, v[8]);
-                const float s = gate ? -0.5f * e[k].og * v_alpha : 0.f;  // dL/d dist2
-                const float p = s * e[k].u, q = s * e[k].w;
-                v[0] += p;                      // x2 after the reduction
-                v[1] += q;                      // x2 after the reduction
-                v[2] = fmaf(-p, e[k].u, v[2]);
-                v[3] = fmaf(-p, e[k].w, v[3]);  // x2 after the reduction
-                v[4] = fmaf(-q, e[k].w, v[4]);
+                // m = -2 dL/d dist2 (dL/d dist2 = -o g v_alpha / 2); the
+                // per-field factors are applied once, after the reduction
+                const float m = gate ? e[k].og * v_alpha : 0.f;
+                const float pu = m * e[k].u, pw = m * e[k].w;
+                v[0] += pu;
+                v[1] += pw;
+                v[2] = fmaf(pu, e[k].u, v[2]);
+                v[3] = fmaf(pu, e[k].w, v[3]);
+                v[4] = fmaf(pw, e[k].w, v[4]);
                 acc[k] = fmaf(fac, cg, acc[k]);
                 T[k] = Tb;
             }

@@ -29,6 +29,7 @@ struct MultiInst {
 
 // mi / tinst are NULL for one instance (W, H, ntx describe it); otherwise
 // tile t belongs to instance tinst[t] with the layout mi[tinst[t]].
+// Workgroup b composites tile order[b] (NULL: tile b).
 struct CompositeArgs {
     const float4 *rec;
     const uint32_t *flat;
@@ -36,6 +37,7 @@ struct CompositeArgs {
     int W, H, ntx, ntiles;
     const MultiInst *mi;
     const uint16_t *tinst;
+    const uint32_t *order;
 };
 
 // pix_per_thread: 1, 2 or 4 pixels per thread (256, 128 or 64 threads per tile).

@@ -21,9 +21,9 @@
 //    stably sorted on the tile bits only. The result equals a stable sort of
 //    the reference's 64-bit keys.
 //  * Compositing: one 256-thread workgroup (4 wave64) per 16x16 tile,
-//    256-record batches in LDS, workgroup-to-tile mapping that keeps
-//    neighbouring tiles on one XCD's L2. The MSE loss and its gradient can
-//    be fused into the forward epilogue.
+//    256-record batches in LDS, tiles dispatched heaviest first (pair count
+//    bucketed by powers of two) so that the grid ends evenly. The MSE loss
+//    and its gradient can be fused into the forward epilogue.
 //  * Backward: wave64 reductions, then LDS float atomics across the four
 //    waves, then one global atomic per (splat, field, tile).
 #include <float.h>
@@ -575,6 +575,57 @@ __global__ __launch_bounds__(256) void k_tile_scatter(const uint32_t *__restrict
     }
 }
 
+// Dispatch order of the compositing workgroups, heaviest tiles first: tiles
+// bucketed by floor(log2(pairs)) + 1 (empty tiles: bucket 0), buckets in
+// descending order (the order inside a bucket is arbitrary; results do not
+// depend on it). One workgroup; each wave ranks its 64 tiles per bucket
+// with ballots and reserves their slots with one LDS atomic per bucket.
+constexpr int kOrderThreads = 1024;
+__device__ __forceinline__ uint32_t tile_bucket(const int2 *ranges, int t) {
+    const int2 r = ranges[t];
+    const uint32_t c = (uint32_t)max(r.y - r.x, 0);
+    return c ? (uint32_t)min(31, 32 - __clz(c)) : 0u;
+}
+
+__global__ __launch_bounds__(kOrderThreads) void k_tile_order(const int2 *__restrict__ ranges,
+                                                              int ntiles,
+                                                              uint32_t *__restrict__ order) {
+    __shared__ uint32_t h[32];
+    const int tid = threadIdx.x;
+    if (tid < 32) h[tid] = 0u;
+    __syncthreads();
+    for (int t = tid; t < ntiles; t += kOrderThreads) lds_count(h, tile_bucket(ranges, t));
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (int b = 31; b >= 0; --b) {
+            const uint32_t v = h[b];
+            h[b] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+    for (int t0 = 0; t0 < ntiles; t0 += kOrderThreads) {
+        const int t = t0 + tid;
+        const bool valid = t < ntiles;
+        const uint32_t b = valid ? tile_bucket(ranges, t) : 0u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 5; ++bit) {
+            const bool on = (b >> bit) & 1u;
+            const uint64_t bb = __ballot(on);
+            peers &= on ? bb : ~bb;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        uint32_t base = 0;
+        if (valid && rank == 0) base = atomicAdd(&h[b], (uint32_t)__popcll(peers));
+        const int leader = valid ? __ffsll((unsigned long long)peers) - 1 : 0;
+        base = (uint32_t)__shfl((int)base, leader, kWave);
+        if (valid) order[base + rank] = (uint32_t)t;
+    }
+}
+
 int grid_cap(uint64_t n, int per, int cap) {
     uint64_t b = (n + per - 1) / per;
     if (b < 1) b = 1;
@@ -594,6 +645,7 @@ struct ds_raster {
     int grad_mode = DS_GRAD_EXACT;
     int fwd_ppt = 1;  // compositing forward: pixels per thread (1, 2 or 4)
     int bwd_ppt = 1;  // compositing backward: pixels per thread
+    int tile_order = DS_TILE_ORDER_HEAVY_FIRST;  // compositing dispatch order
     int tile_bits = 1;
     int tile_passes = 1;  // 8-bit radix passes over the tile ids
     // Several fit instances in one rasterizer (ninst > 0): splats, tiles and
@@ -615,6 +667,7 @@ struct ds_raster {
     ds::DevBuf<uint32_t> tcnt, tseg, ttot, tfirst;
     // per tile / pixel
     ds::DevBuf<int2> ranges;
+    ds::DevBuf<uint32_t> order;  // compositing dispatch order (heavy-first)
     ds::DevBuf<float> T;
     ds::DevBuf<int32_t> last;
     // scratch: device scalars, depth-digit histograms, look-back status words
@@ -623,6 +676,7 @@ struct ds_raster {
     ds::DevBuf<uint32_t> scalars, ghist, hist, part;
     ds::DevBuf<uint64_t> lb_depth, lb_scan;
     // results of the last forward
+    bool order_valid = false;  // `order` was computed by the last forward
     const uint32_t *flat = nullptr;
     int last_count = -1;
     bool has_fwd = false;
@@ -668,6 +722,7 @@ static ds_status raster_alloc_tiles(ds_raster *r) {
         r->tfirst.release();
     }
     DS_TRY(r->ranges.alloc((size_t)r->ntiles));
+    DS_TRY(r->order.alloc((size_t)r->ntiles));
     DS_TRY(r->T.alloc((size_t)r->npix));
     DS_TRY(r->last.alloc((size_t)r->npix));
     r->has_fwd = false;
@@ -679,7 +734,9 @@ static CompositeArgs composite_args(const ds_raster *r) {
     return CompositeArgs{(const float4 *)r->rec.ptr, r->flat, (const int2 *)r->ranges.ptr,
                          r->W, r->H, r->ntx, r->ntiles,
                          multi ? (const MultiInst *)r->mtab.ptr : nullptr,
-                         multi ? (const uint16_t *)r->tinst.ptr : nullptr};
+                         multi ? (const uint16_t *)r->tinst.ptr : nullptr,
+                         r->tile_order == DS_TILE_ORDER_HEAVY_FIRST ? (const uint32_t *)r->order.ptr
+                                                                    : nullptr};
 }
 
 static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_rgb,
@@ -814,6 +871,12 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
     } else {
         r->flat = r->tval_a.ptr;
     }
+    if (r->tile_order == DS_TILE_ORDER_HEAVY_FIRST) {
+        hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, s,
+                           (const int2 *)r->ranges.ptr, r->ntiles, r->order.ptr);
+        DS_CHECK_LAUNCH("tile_order");
+    }
+    r->order_valid = r->tile_order == DS_TILE_ORDER_HEAVY_FIRST;
     const CompositeArgs ca = composite_args(r);
     DS_TRY(launch_composite_fwd(ca, r->fwd_ppt, out_rgb, r->T.ptr, r->last.ptr, target,
                                 loss, grad, s));
@@ -827,7 +890,8 @@ static ds_status raster_backward(ds_raster *r, const ds_splat2d &in, const float
     DS_REQUIRE(r->has_fwd, "ds_raster_bwd called before ds_raster_fwd");
     DS_REQUIRE(in.count == r->last_count, "splat count differs from the last forward");
     DS_REQUIRE(grad_rgb != nullptr, "grad_rgb is NULL");
-    const CompositeArgs ca = composite_args(r);
+    CompositeArgs ca = composite_args(r);
+    if (!r->order_valid) ca.order = nullptr;  // order switched on after the forward
     DS_TRY(launch_composite_bwd(ca, r->bwd_ppt, r->grad_mode, r->T.ptr, r->last.ptr,
                                 grad_rgb, gin, s));
     return DS_OK;
@@ -1039,6 +1103,22 @@ extern "C" ds_status ds_raster_set_tuning(ds_raster *r, int fwd_ppt, int bwd_ppt
     DS_REQUIRE(ok(fwd_ppt) && ok(bwd_ppt), "pixels per thread must be 1, 2 or 4");
     r->fwd_ppt = fwd_ppt;
     r->bwd_ppt = bwd_ppt;
+    return DS_OK;
+}
+
+extern "C" ds_status ds_raster_set_tile_order(ds_raster *r, int order) {
+    clear_error();
+    DS_REQUIRE(r, "NULL raster");
+    DS_REQUIRE(order == DS_TILE_ORDER_ROWS || order == DS_TILE_ORDER_HEAVY_FIRST,
+               "order must be DS_TILE_ORDER_ROWS or DS_TILE_ORDER_HEAVY_FIRST");
+    r->tile_order = order;
+    return DS_OK;
+}
+
+extern "C" ds_status ds_raster_get_tile_order(ds_raster *r, int *order) {
+    clear_error();
+    DS_REQUIRE(r && order, "NULL argument");
+    *order = r->tile_order;
     return DS_OK;
 }
 

@@ -286,6 +286,18 @@ ds_status ds_raster_get_view(ds_raster *r, ds_raster_view *out);
 ds_status ds_raster_set_tuning(ds_raster *r, int fwd_pix_per_thread, int bwd_pix_per_thread);
 ds_status ds_raster_get_tuning(ds_raster *r, int *fwd_pix_per_thread, int *bwd_pix_per_thread);
 
+/* Dispatch order of the compositing workgroups (results do not depend on
+ * it; no counterpart in the reference, which launches tiles row-major):
+ * DS_TILE_ORDER_HEAVY_FIRST (default) starts the tiles with the most pairs
+ * first (power-of-two buckets of the pair count, computed on the device
+ * after the sort), so the longest tiles do not end the grid late;
+ * DS_TILE_ORDER_ROWS is row-major. The backward uses the order of its
+ * forward. */
+#define DS_TILE_ORDER_ROWS 0
+#define DS_TILE_ORDER_HEAVY_FIRST 1
+ds_status ds_raster_set_tile_order(ds_raster *r, int order);
+ds_status ds_raster_get_tile_order(ds_raster *r, int *order);
+
 /* ------------------------------------------------------------------ */
 /* Stand-alone sort (the radix sort inside ds_raster_fwd)              */
 /* ------------------------------------------------------------------ */

@@ -209,6 +209,32 @@ def test_composite_backward(scene, grad_mode, ppt):
     check_close(f"composite_bwd_{scene}_mode{grad_mode}_ppt{ppt}", gs2, rg, mask)
 
 
+@pytest.mark.parametrize("scene", ["c0", "ragged"])
+def test_tile_dispatch_order_leaves_results_unchanged(scene):
+    """Heavy-first and row-major compositing dispatch give the same image,
+    transmittance, last ids and MSE gradient bit for bit, and the same
+    splat gradients up to the order of float atomics; switching the order
+    between a forward and its backward is safe."""
+    seed, n, W, H, ls = SCENES[scene]
+    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    target = ds.target_image(seed, W, H, 32)
+    outs = []
+    for fwd_order, bwd_order in ((1, 1), (0, 0), (0, 1), (1, 0)):
+        st = Stages(params, n, 0, W, H, view, proj, cam, pix_per_thread=1)
+        ds.check(ds.lib().ds_raster_set_tile_order(st.ras.h, fwd_order), "tile order")
+        st.activ_project_fwd()
+        out = st.raster_fwd(target)
+        ds.check(ds.lib().ds_raster_set_tile_order(st.ras.h, bwd_order), "tile order")
+        out["gs2"] = st.raster_bwd(out["grad_rgb"])
+        outs.append(out)
+    ref = outs[0]
+    for out in outs[1:]:
+        for k in ("rgb", "T", "last_ids", "grad_rgb", "flatten_ids", "tile_ranges"):
+            assert np.array_equal(out[k], ref[k]), k
+        ok, mabs, _ = close(out["gs2"], ref["gs2"], atol=1e-7, rtol=1e-5)
+        assert ok, mabs
+
+
 @pytest.mark.parametrize("grad_mode", [0, 1])
 @pytest.mark.parametrize("deg", [0, 3])
 def test_projection_activation_backward(deg, grad_mode):
