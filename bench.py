@@ -67,6 +67,10 @@ def parse():
     ap.add_argument("--eval-images", type=int, default=64,
                     help="held-out images of the time-to-30dB comparison")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=500,
+                    help="steps of every instance in the batched-fitting leg")
+    ap.add_argument("--no-batch-compare", action="store_true",
+                    help="batched leg: only ds_multifit (no per-instance stream/sequential runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (counter calibration)")
     ap.add_argument("--ppt", type=str, default=os.environ.get("DS_PPT", "auto"),
@@ -221,7 +225,7 @@ def learned_vs_plain(ds, n_train=10000, train_steps=10000, n_eval=64, W=256, H=2
                     "cost reported separately"}
 
 
-def batched_fit(ds, count=64, steps=500, seq_steps=10):
+def batched_fit(ds, count=64, steps=500, seq_steps=10, compare=True, graph=True):
     """Batched fitting: `count` independent fits of uneven sizes (areas
     log-uniform 128^2..1024x576, splat counts log-uniform 2k..100k, the
     synthetic target and random frustum init of each), `steps` steps each.
@@ -243,7 +247,7 @@ def batched_fit(ds, count=64, steps=500, seq_steps=10):
     mf.forward()
     pairs0 = mf.raster_counts()[0]
     mtune = mf.autotune(reps=3)
-    mf.set_graph(True)
+    mf.set_graph(graph)
     mf.step(1)
     mf.synchronize()
     t0 = time.perf_counter()
@@ -254,6 +258,22 @@ def batched_fit(ds, count=64, steps=500, seq_steps=10):
     m_psnr = [ds.psnr_from_loss(float(x)) for x in mf.losses()]
     pairs1 = mf.raster_counts()[0]
     mf.close()
+    m_ms = 1e3 * m_el / steps
+    out = {"workload": f"{count} uneven fits (128^2..1024x576 pixels, 2k..100k splats), "
+                       f"{steps} steps each",
+           "instances": count, "steps": steps,
+           "pixels": int(sum(w * h for w, h, _, _ in sizes)),
+           "splats": int(sum(n for _, _, n, _ in sizes)),
+           "instance_steps_per_s": round(count * steps / m_el, 1),
+           "ms_per_batch_step": round(m_ms, 4),
+           "method": "ds_multifit: one launch per phase over all instances" +
+                     (", one HIP graph per step" if graph else ", eager launches"),
+           "compositing_tuning": mtune, "pairs_first_step": int(pairs0),
+           "pairs_last_step": int(pairs1),
+           "psnr_db_mean": round(float(np.mean(m_psnr)), 3),
+           "psnr_db_min": round(float(np.min(m_psnr)), 3)}
+    if not compare:
+        return out
 
     fits = []
     for w, h, n, seed in sizes:
@@ -287,27 +307,14 @@ def batched_fit(ds, count=64, steps=500, seq_steps=10):
     batch.close()
     for f in fits:
         f.close()
-    m_ms = 1e3 * m_el / steps
-    return {"workload": f"{count} uneven fits (128^2..1024x576 pixels, 2k..100k splats), "
-                        f"{steps} steps each",
-            "instances": count, "steps": steps,
-            "pixels": int(sum(w * h for w, h, _, _ in sizes)),
-            "splats": int(sum(n for _, _, n, _ in sizes)),
-            "instance_steps_per_s": round(count * steps / m_el, 1),
-            "ms_per_batch_step": round(m_ms, 4),
-            "method": "ds_multifit: one launch per phase over all instances, one HIP graph "
-                      "per step",
-            "compositing_tuning": mtune, "pairs_first_step": int(pairs0),
-            "pairs_last_step": int(pairs1),
-            "psnr_db_mean": round(float(np.mean(m_psnr)), 3),
-            "psnr_db_min": round(float(np.min(m_psnr)), 3),
-            "streams_batch": {"streams": streams, "seconds": round(el, 4),
-                              "instance_steps_per_s": round(count * steps / el, 1),
-                              "ms_per_batch_step": round(1e3 * el / steps, 4),
-                              "psnr_db_mean": round(float(np.mean(psnr)), 3)},
-            "sequential_ms_per_batch_step": round(seq_ms, 4),
-            "speedup_vs_sequential": round(seq_ms / m_ms, 2),
-            "speedup_vs_streams": round((1e3 * el / steps) / m_ms, 2)}
+    out.update({"streams_batch": {"streams": streams, "seconds": round(el, 4),
+                                  "instance_steps_per_s": round(count * steps / el, 1),
+                                  "ms_per_batch_step": round(1e3 * el / steps, 4),
+                                  "psnr_db_mean": round(float(np.mean(psnr)), 3)},
+                "sequential_ms_per_batch_step": round(seq_ms, 4),
+                "speedup_vs_sequential": round(seq_ms / m_ms, 2),
+                "speedup_vs_streams": round((1e3 * el / steps) / m_ms, 2)})
+    return out
 
 
 def viewer_fps(ds, n=1_000_000, W=1920, H=1080, n_cams=120):
@@ -416,7 +423,8 @@ def main():
         print(f"[bench] time to psnr {ttp}", file=sys.stderr, flush=True)
     batched = None
     if not args.no_batch:
-        batched = batched_fit(ds)
+        batched = batched_fit(ds, steps=args.batch_steps, compare=not args.no_batch_compare,
+                              graph=bool(args.graph))
         print(f"[bench] batched {batched}", file=sys.stderr, flush=True)
     if not args.no_roofline or not args.no_cpu:
         import oracle

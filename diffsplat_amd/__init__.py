@@ -326,6 +326,8 @@ class Rasterizer:
             "last_ids": download_ptr(v.last_ids, np.int32, npix),
             "num_tiles_x": v.num_tiles_x, "num_tiles_y": v.num_tiles_y,
         }
+        if v.tile_order:
+            st["tile_order"] = download_ptr(v.tile_order, np.uint32, ntiles)
         if v.isect_keys_sorted:
             st["keys"] = download_ptr(v.isect_keys_sorted, np.uint64, n_isects)
         return st
@@ -385,6 +387,8 @@ class MultiRasterizer(Rasterizer):
             "T": download_ptr(v.transmittance, np.float32, self.num_pixels),
             "last_ids": download_ptr(v.last_ids, np.int32, self.num_pixels),
         }
+        if v.tile_order:
+            st["tile_order"] = download_ptr(v.tile_order, np.uint32, self.num_tiles)
         if v.isect_keys_sorted:
             st["keys"] = download_ptr(v.isect_keys_sorted, np.uint64, n_isects)
         return st

@@ -238,12 +238,16 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
 // tiles costs no more than thousands of small ones: slot p belongs to the
 // last depth rank d with off[d] <= p (off is the exclusive scan of the tile
 // counts in depth order, so that splat's range [off[d], off[d+1]) holds p and
-// is not empty), found by binary search. The first thread also totals the
-// visible-count replicas into the device scalar. With MULTI the tile id is
-// global: the splat's instance's first tile plus the tile in its image.
+// is not empty), found by binary search between the owners of the first
+// slots of p's chunk of kEmitChunk slots and of the next chunk (recorded by
+// the offset scan). The first thread also totals the visible-count
+// replicas into the device scalar. With MULTI the tile id is global: the
+// splat's instance's first tile plus the tile in its image.
+constexpr int kEmitChunkShift = 12;
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_emit(const uint32_t *__restrict__ order,
                                               const uint32_t *__restrict__ off,
+                                              const uint32_t *__restrict__ chunk_first,
                                               const uint2 *__restrict__ rect, uint32_t *sc,
                                               const uint32_t *__restrict__ ghist, int ntx,
                                               uint32_t *__restrict__ tkey,
@@ -258,8 +262,13 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t *__restrict__ order
         for (int r = 0; r < kVisReps; ++r) v += ghist[kVisOffset + r * kVisStride];
         sc[SC_VISIBLE] = v;
     }
+    const uint32_t nch = (m + (1u << kEmitChunkShift) - 1) >> kEmitChunkShift;
     for (int64_t p = gtid(); p < m; p += gstride()) {
-        uint32_t lo = 0, hi = n;  // off[lo] <= p (off[0] == 0); the answer lies in [lo, hi)
+        // off[lo] <= p; the answer lies in [lo, hi): the owner of the chunk's
+        // first slot up to the owner of the next chunk's first slot
+        const uint32_t c = (uint32_t)p >> kEmitChunkShift;
+        uint32_t lo = chunk_first[c];
+        uint32_t hi = (c + 1 < nch) ? chunk_first[c + 1] + 1 : n;
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
             if (off[mid] <= (uint32_t)p)
@@ -576,27 +585,73 @@ __global__ __launch_bounds__(256) void k_tile_scatter(const uint32_t *__restrict
 }
 
 // Dispatch order of the compositing workgroups, heaviest tiles first: tiles
-// bucketed by floor(log2(pairs)) + 1 (empty tiles: bucket 0), buckets in
-// descending order (the order inside a bucket is arbitrary; results do not
-// depend on it). One workgroup; each wave ranks its 64 tiles per bucket
-// with ballots and reserves their slots with one LDS atomic per bucket.
+// bucketed by floor(log2(pairs)) + 1 (empty tiles: bucket 0, 2^30 and more
+// pairs share bucket 31), buckets in descending order (the order inside a
+// bucket is arbitrary; results do not depend on it). One workgroup. Pass 1
+// reads the ranges (kOrderUnroll loads in flight per thread), keeps each
+// tile's bucket in LDS when the tiles fit (kOrderLdsTiles) and counts the
+// buckets per wave: the lanes of one bucket are found with five ballots and
+// their first lane adds their number to the wave's row (distinct buckets,
+// distinct words: no atomics). Pass 2 reserves each wave's slots with one
+// LDS atomic per bucket.
 constexpr int kOrderThreads = 1024;
-__device__ __forceinline__ uint32_t tile_bucket(const int2 *ranges, int t) {
-    const int2 r = ranges[t];
+constexpr int kOrderWaves = kOrderThreads / kWave;
+constexpr int kOrderUnroll = 4;
+constexpr int kOrderLdsTiles = 49152;
+__device__ __forceinline__ uint32_t tile_bucket(int2 r) {
     const uint32_t c = (uint32_t)max(r.y - r.x, 0);
     return c ? (uint32_t)min(31, 32 - __clz(c)) : 0u;
+}
+
+// Lanes of this wave (among `valid` ones) whose bucket equals this lane's.
+__device__ __forceinline__ uint64_t same_bucket(uint32_t b, bool valid) {
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 5; ++bit) {
+        const bool on = (b >> bit) & 1u;
+        const uint64_t bb = __ballot(on);
+        peers &= on ? bb : ~bb;
+    }
+    return peers;
 }
 
 __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const int2 *__restrict__ ranges,
                                                               int ntiles,
                                                               uint32_t *__restrict__ order) {
     __shared__ uint32_t h[32];
-    const int tid = threadIdx.x;
-    if (tid < 32) h[tid] = 0u;
+    __shared__ uint32_t wh[kOrderWaves][32];
+    __shared__ uint8_t s_b[kOrderLdsTiles];
+    const int tid = threadIdx.x, w = tid / kWave;
+    const bool cache = ntiles <= kOrderLdsTiles;
+    const uint64_t lt = lanemask_lt();
+    for (int i = tid; i < kOrderWaves * 32; i += kOrderThreads) (&wh[0][0])[i] = 0u;
     __syncthreads();
-    for (int t = tid; t < ntiles; t += kOrderThreads) lds_count(h, tile_bucket(ranges, t));
+    constexpr int kStep = kOrderThreads * kOrderUnroll;
+    for (int t0 = 0; t0 < ntiles; t0 += kStep) {
+        int2 r[kOrderUnroll];
+#pragma unroll
+        for (int u = 0; u < kOrderUnroll; ++u) {
+            const int t = t0 + u * kOrderThreads + tid;
+            r[u] = t < ntiles ? ranges[t] : make_int2(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < kOrderUnroll; ++u) {
+            const int t = t0 + u * kOrderThreads + tid;
+            const bool valid = t < ntiles;
+            const uint32_t b = tile_bucket(r[u]);
+            if (valid && cache) s_b[t] = (uint8_t)b;
+            const uint64_t peers = same_bucket(b, valid);
+            if (valid && (peers & lt) == 0) wh[w][b] += (uint32_t)__popcll(peers);
+        }
+    }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < 32) {  // bucket totals
+        uint32_t v = 0;
+        for (int q = 0; q < kOrderWaves; ++q) v += wh[q][tid];
+        h[tid] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {  // descending exclusive scan: the first slot of each bucket
         uint32_t run = 0;
         for (int b = 31; b >= 0; --b) {
             const uint32_t v = h[b];
@@ -605,18 +660,11 @@ __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const int2 *__rest
         }
     }
     __syncthreads();
-    const uint64_t lt = lanemask_lt();
     for (int t0 = 0; t0 < ntiles; t0 += kOrderThreads) {
         const int t = t0 + tid;
         const bool valid = t < ntiles;
-        const uint32_t b = valid ? tile_bucket(ranges, t) : 0u;
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int bit = 0; bit < 5; ++bit) {
-            const bool on = (b >> bit) & 1u;
-            const uint64_t bb = __ballot(on);
-            peers &= on ? bb : ~bb;
-        }
+        const uint32_t b = !valid ? 0u : cache ? (uint32_t)s_b[t] : tile_bucket(ranges[t]);
+        const uint64_t peers = same_bucket(b, valid);
         const uint32_t rank = (uint32_t)__popcll(peers & lt);
         uint32_t base = 0;
         if (valid && rank == 0) base = atomicAdd(&h[b], (uint32_t)__popcll(peers));
@@ -709,17 +757,16 @@ static ds_status raster_alloc_tiles(ds_raster *r) {
     while ((1 << bits) < r->ntiles) ++bits;
     r->tile_bits = bits;
     r->tile_passes = (bits + kRadixBits - 1) / kRadixBits;
+    const size_t max_chunks = ((size_t)r->isect_cap + kTileChunk - 1) / kTileChunk;
+    DS_TRY(r->tfirst.alloc(max_chunks));
     if (one_pass_tile_sort(r)) {
-        const size_t max_chunks = ((size_t)r->isect_cap + kTileChunk - 1) / kTileChunk;
         DS_TRY(r->tcnt.alloc(max_chunks * (size_t)r->ntiles));
         DS_TRY(r->tseg.alloc((size_t)kTileSegs * r->ntiles));
         DS_TRY(r->ttot.alloc((size_t)r->ntiles));
-        DS_TRY(r->tfirst.alloc(max_chunks));
     } else {
         r->tcnt.release();
         r->tseg.release();
         r->ttot.release();
-        r->tfirst.release();
     }
     DS_TRY(r->ranges.alloc((size_t)r->ntiles));
     DS_TRY(r->order.alloc((size_t)r->ntiles));
@@ -802,9 +849,8 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
         const uint32_t *order = va;
         const uint64_t max_chunks = ((uint64_t)r->isect_cap + kTileChunk - 1) / kTileChunk;
         static_assert(kTileChunk == 1 << 12, "chunk heads use shift 12");
-        const ScanHeads heads = one_pass_tiles
-                                    ? ScanHeads{r->tfirst.ptr, 12u, (uint32_t)max_chunks}
-                                    : ScanHeads{nullptr, 0u, 0u};
+        static_assert(kEmitChunkShift == 12, "one chunk size for both emissions");
+        const ScanHeads heads{r->tfirst.ptr, 12u, (uint32_t)max_chunks};
         if (one_kernel)
             DS_TRY(scan_chained(r->ntiles_of.ptr, order, r->pair_off.ptr, sc + SC_COUNT,
                                 (uint32_t)n, sc + SC_WANTED, sc + SC_NISECT, r->isect_cap,
@@ -841,12 +887,14 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
             const dim3 egrid(grid_cap((uint64_t)r->isect_cap, 256, 4096));
             if (multi)
                 hipLaunchKernelGGL(k_emit<true>, egrid, dim3(256), 0, s, order,
-                                   (const uint32_t *)r->pair_off.ptr, (const uint2 *)r->rect.ptr,
+                                   (const uint32_t *)r->pair_off.ptr,
+                                   (const uint32_t *)r->tfirst.ptr, (const uint2 *)r->rect.ptr,
                                    sc, (const uint32_t *)gh, r->ntx, r->tkey_a.ptr,
                                    r->tval_a.ptr, mi, si);
             else
                 hipLaunchKernelGGL(k_emit<false>, egrid, dim3(256), 0, s, order,
-                                   (const uint32_t *)r->pair_off.ptr, (const uint2 *)r->rect.ptr,
+                                   (const uint32_t *)r->pair_off.ptr,
+                                   (const uint32_t *)r->tfirst.ptr, (const uint2 *)r->rect.ptr,
                                    sc, (const uint32_t *)gh, r->ntx, r->tkey_a.ptr,
                                    r->tval_a.ptr, mi, si);
             DS_CHECK_LAUNCH("emit");
@@ -1093,6 +1141,7 @@ extern "C" ds_status ds_raster_get_view(ds_raster *r, ds_raster_view *v) {
     v->num_tiles_y = r->nty;
     v->width = r->W;
     v->height = r->H;
+    v->tile_order = r->order_valid ? r->order.ptr : nullptr;
     return DS_OK;
 }
 

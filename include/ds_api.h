@@ -220,6 +220,8 @@ typedef struct ds_raster_view {
     const uint32_t *visible_count;     /* device scalar                                      */
     int32_t num_tiles_x, num_tiles_y;
     int32_t width, height;
+    const uint32_t *tile_order;        /* [num_tiles] compositing dispatch order of the last
+                                          forward, or NULL when it was row-major          */
 } ds_raster_view;
 
 /* TileRasterizeLayer::allocate (src/layers/tile_rasterize_layer.cu:591-616). */

@@ -106,6 +106,12 @@ def test_multi_raster_matches_one_instance_rasterizers():
     losses = d_loss.download(np.float32, len(SIZES))
     gs2_all = d_gs2.download(np.float32, 10 * total).reshape(10, total)
     assert st["n_isects"] == sum(s.get("n_isects", 0) for s in singles)
+    # heavy-first dispatch over the tiles of all instances
+    counts = st["tile_ranges"].reshape(-1, 2) @ np.array([-1, 1])
+    order = st["tile_order"]
+    assert np.array_equal(np.sort(order), np.arange(mr.num_tiles))
+    bucket = np.where(counts > 0, np.floor(np.log2(np.maximum(counts, 1))) + 1, 0)
+    assert np.all(np.diff(bucket[order]) <= 0)
     ids, ranges, keys = st["flatten_ids"], st["tile_ranges"].reshape(-1, 2), st["keys"]
     pair_base = 0
     for k, ((w, h, n, _), s) in enumerate(zip(SIZES, singles)):

@@ -235,6 +235,26 @@ def test_tile_dispatch_order_leaves_results_unchanged(scene):
         assert ok, mabs
 
 
+def heavy_first_ok(order, ranges):
+    """order is a permutation of the tiles with non-increasing buckets
+    floor(log2(pairs)) + 1 (0 for empty tiles)."""
+    counts = (ranges[:, 1] - ranges[:, 0]).astype(np.int64)
+    assert np.array_equal(np.sort(order), np.arange(len(counts)))
+    b = np.where(counts > 0, np.floor(np.log2(np.maximum(counts, 1))).astype(np.int64) + 1, 0)
+    assert np.all(np.diff(b[order]) <= 0)
+
+
+@pytest.mark.parametrize("W,H,n", [(256, 256, 4096), (6144, 3456, 6000)])
+def test_heavy_first_order_is_a_sorted_permutation(W, H, n):
+    """256 tiles keep their buckets in LDS, 82944 tiles re-read them."""
+    params, view, proj, cam = frustum_scene(7, n, W, H, -2.0)
+    st = Stages(params, n, 0, W, H, view, proj, cam, pix_per_thread=1)
+    st.activ_project_fwd()
+    out = st.raster_fwd()
+    assert "tile_order" in out
+    heavy_first_ok(out["tile_order"], out["tile_ranges"].reshape(-1, 2))
+
+
 @pytest.mark.parametrize("grad_mode", [0, 1])
 @pytest.mark.parametrize("deg", [0, 3])
 def test_projection_activation_backward(deg, grad_mode):
