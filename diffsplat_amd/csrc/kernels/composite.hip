@@ -131,39 +131,6 @@ __device__ __forceinline__ int field_of_lane(int lane) {
     }
 }
 
-// One thread's slot of the next LDS batch of splat records, loaded ahead:
-// the records of the slot `i` (next batch) and the splat id of slot
-// `i + step` (the batch after). Slots at or past `end` load nothing.
-// Forward batches step by +NT, backward batches by -NT (then `end` is the
-// first slot of the range and slots below it load nothing).
-struct RecPrefetch {
-    float4 A, B, C;
-    uint32_t id_after;
-    template <int DIR = 1>
-    __device__ __forceinline__ bool in(int i, int end) const {
-        return DIR > 0 ? i < end : i >= end;
-    }
-    template <int DIR = 1>
-    __device__ __forceinline__ void start(const CompositeArgs &a, int i, int end, int step) {
-        if (in<DIR>(i, end)) load_rec(a, a.flat[i]);
-        const int j = i + DIR * step;
-        id_after = in<DIR>(j, end) ? a.flat[j] : 0u;
-    }
-    // Moves to slot i (the batch after the one just stored to LDS).
-    template <int DIR = 1>
-    __device__ __forceinline__ void advance(const CompositeArgs &a, int i, int end, int step) {
-        if (in<DIR>(i, end)) load_rec(a, id_after);
-        const int j = i + DIR * step;
-        if (in<DIR>(j, end)) id_after = a.flat[j];
-    }
-    __device__ __forceinline__ void load_rec(const CompositeArgs &a, uint32_t id) {
-        const float4 *r = a.rec + 3 * (size_t)id;
-        A = r[0];
-        B = r[1];
-        C = r[2];
-    }
-};
-
 // The image a tile belongs to: its size, tiles per row, the tile's index in
 // it, where its pixels start in the (concatenated) pixel arrays, and which
 // instance it is (0 for one instance).
@@ -213,24 +180,19 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
         cur[k] = 0;
     }
     const int2 range = a.ranges[tile];
-    // Software pipeline: while batch k is composited, this thread's record
-    // of batch k + 1 and its splat id of batch k + 2 are in flight, so the
-    // dependent id -> record loads of a tile's next batch do not stall the
-    // workgroup at the barrier.
-    RecPrefetch pf;
-    pf.start(a, range.x + tid, range.y, NT);
     for (int b = range.x; b < range.y; b += NT) {
         bool live = false;
 #pragma unroll
         for (int k = 0; k < P; ++k) live |= !done[k];
         if (__syncthreads_count(live) == 0) break;
-        if (b + tid < range.y) {
-            sA[tid] = pf.A;
-            sB[tid] = pf.B;
-            sC[tid] = pf.C;
+        const int idx = b + tid;
+        if (idx < range.y) {
+            const float4 *r = a.rec + 3 * (size_t)a.flat[idx];
+            sA[tid] = r[0];
+            sB[tid] = r[1];
+            sC[tid] = r[2];
         }
         __syncthreads();
-        pf.advance(a, b + NT + tid, range.y, NT);
         // Every exit below is wave-uniform (a ballot), so the loop runs on
         // scalar branches; lanes whose pixels are done ride along adding
         // exact zeros. A wave whose pixels are all done skips the batch, and
@@ -372,19 +334,17 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
                            : (my_field == 2 || my_field == 4) ? 0.5f
                                                               : 1.f;
     const int end = min(range.y, block_last + 1);
-    // Software pipeline as in the forward, back to front.
-    RecPrefetch pf;
-    pf.start<-1>(a, end - 1 - tid, range.x, NT);
     for (int bend = end - 1; bend >= range.x; bend -= NT) {
         const int bs = min(NT, bend + 1 - range.x);
         __syncthreads();
-        if (bend - tid >= range.x) {
-            sA[tid] = pf.A;
-            sB[tid] = pf.B;
-            sC[tid] = pf.C;
+        const int li = bend - tid;
+        if (li >= range.x) {
+            const float4 *r = a.rec + 3 * (size_t)a.flat[li];
+            sA[tid] = r[0];
+            sB[tid] = r[1];
+            sC[tid] = r[2];
         }
         __syncthreads();
-        pf.advance<-1>(a, bend - NT - tid, range.x, NT);
         for (int t = max(0, bend - wave_last); t < bs; ++t) {
             const int idx = bend - t;
             const float4 A = sA[t], B = sB[t];
