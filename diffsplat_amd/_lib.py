@@ -148,6 +148,8 @@ _SIGS = {
     "ds_raster_resize": (C.c_int, [_P, C.c_int, C.c_int]),
     "ds_raster_fwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, _P, _P, _P, _P]),
     "ds_raster_bwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, C.POINTER(Splat2D), _P]),
+    "ds_raster_fwd_bwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, _P, _P, _P, C.POINTER(Splat2D),
+                                    _P]),
     "ds_splat2d_zero": (C.c_int, [C.POINTER(Splat2D), _P]),
     "ds_raster_counts": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                    C.POINTER(C.c_uint32)]),

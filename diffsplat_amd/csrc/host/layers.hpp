@@ -184,13 +184,30 @@ public:
         ds_splat2d gi = grad_input.view();
         check(ds_splat2d_zero(&gi, stream), "TileRasterizeLayer::zeroGrad");
     }
+    // With the MSE fused into the forward (setLossTarget), the backward can
+    // run inside the forward too (ds_raster_fwd_bwd): forward() then also
+    // accumulates dL/dSplat2D and the next backward() does nothing. zeroGrad
+    // must still come before forward().
+    void setFuseBackward(bool on) { fuse_backward = on; }
     void forward() override {
         require_wired("TileRasterizeLayer::forward");
         ds_splat2d in = input->view();
-        check(ds_raster_fwd(ras, &in, output.ptr, loss_target, loss_out, loss_grad, stream),
-              "TileRasterizeLayer::forward");
+        fused_last = fuse_backward && loss_target != nullptr;
+        if (fused_last) {
+            ds_splat2d gi = grad_input.view();
+            check(ds_raster_fwd_bwd(ras, &in, output.ptr, loss_target, loss_out, loss_grad, &gi,
+                                    stream),
+                  "TileRasterizeLayer::forward");
+        } else {
+            check(ds_raster_fwd(ras, &in, output.ptr, loss_target, loss_out, loss_grad, stream),
+                  "TileRasterizeLayer::forward");
+        }
     }
     void backward() override {
+        if (fused_last) {  // done by the forward
+            fused_last = false;
+            return;
+        }
         ds_splat2d in = input->view();
         ds_splat2d gi = grad_input.view();
         check(ds_raster_bwd(ras, &in, grad_output->ptr, &gi, stream),
@@ -206,6 +223,8 @@ public:
 private:
     ds_raster *ras = nullptr;
     int w = 0, h = 0;
+    bool fuse_backward = false;
+    bool fused_last = false;  // the last forward also ran the backward
     const float *loss_target = nullptr;
     float *loss_out = nullptr;
     float *loss_grad = nullptr;

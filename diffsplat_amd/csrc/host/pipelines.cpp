@@ -136,6 +136,7 @@ void ImageFitter::initLayers() {
         mse_layer.setFusedIntoRasterizer(true);
         ras_layer.setLossTarget(d_target_pixels.ptr, mse_layer.lossDevice(),
                                 mse_layer.getGradInput().ptr);
+        ras_layer.setFuseBackward(true);
         pipeline.add(&gp_layer);
         pipeline.add(&ras_layer);
         pipeline.add(&mse_layer);
@@ -155,6 +156,7 @@ void ImageFitter::initLayers() {
         atv_layer.setGradOutput(&psp_layer.getGradInput());
         mse_layer.setFusedIntoRasterizer(false);
         ras_layer.setLossTarget(nullptr, nullptr, nullptr);
+        ras_layer.setFuseBackward(false);
         pipeline.add(&atv_layer);
         pipeline.add(&psp_layer);
         pipeline.add(&ras_layer);
@@ -203,7 +205,12 @@ void ImageFitter::step() {
     hip_check(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch");
 }
 
-void ImageFitter::forwardOnly() { pipeline.forward(); }
+void ImageFitter::forwardOnly() {
+    // a forward alone must not run the fused backward
+    ras_layer.setFuseBackward(false);
+    pipeline.forward();
+    ras_layer.setFuseBackward(fused);
+}
 
 void ImageFitter::forwardBackward() {
     pipeline.zeroGrad();
@@ -249,6 +256,7 @@ void ImageFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd
     check(ds_raster_get_tuning(r, &fwd0, &bwd0), "autotune");
     const int opts[3] = {1, 2, 4};
     float tfw[3], tbw[3];
+    ras_layer.setFuseBackward(false);  // forward and backward timed apart
     for (int i = 0; i < 3; ++i) {
         check(ds_raster_set_tuning(r, opts[i], bwd0), "autotune");
         tfw[i] = timed([&] { pipeline.forward(); });
@@ -270,6 +278,7 @@ void ImageFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd
             ras_layer.backward();
         });
     });
+    ras_layer.setFuseBackward(fused);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     dropGraph();  // the captured step (if any) used the old launch shapes
@@ -522,11 +531,13 @@ void MultiFitter::forwardOnly() {
 void MultiFitter::forwardBackward() {
     ds_splat2d gv = gs2.view();
     check(ds_splat2d_zero(&gv, stream), "MultiFitter::zeroGrad");
-    forwardOnly();
     ds_gauss3d pv = params.view();
     ds_gauss3d gp = grads.view();
     ds_splat2d sv = s2.view();
-    check(ds_raster_bwd(ras, &sv, grad_rgb.ptr, &gv, stream), "MultiFitter::backward");
+    check(ds_activ_project_fwd_multi(&pv, &sv, sh_degree, cams.ptr, splat_inst, stream),
+          "MultiFitter::forward");
+    check(ds_raster_fwd_bwd(ras, &sv, image.ptr, target.ptr, loss.ptr, grad_rgb.ptr, &gv, stream),
+          "MultiFitter::forward_backward");
     check(ds_activ_project_bwd_multi(&pv, &gv, &gp, sh_degree, cams.ptr, splat_inst, grad_mode,
                                      stream),
           "MultiFitter::backward");

@@ -151,35 +151,60 @@ __device__ __forceinline__ TileImage tile_image(const CompositeArgs &a, int tile
     }
 }
 
-template <bool MSE, int P, bool MULTI>
-__global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
-    CompositeArgs a, float *__restrict__ out_rgb, float *__restrict__ out_T,
-    int32_t *__restrict__ out_last, const float *__restrict__ target, float *loss,
-    float *__restrict__ grad, float inv_npix_one) {
-    constexpr int NT = kTilePix / P;
-    constexpr int NW = NT / kWave;
-    __shared__ float4 sA[NT], sB[NT], sC[NT];
-    __shared__ float s_red[NW];
-    const int tile = tile_of_block(a);
-    const TileImage im = tile_image<MULTI>(a, tile, inv_npix_one);
-    const float inv_npix = im.inv_npix;
-    const int tid = threadIdx.x;
+// Pixels of one thread: column j (NDC px), P rows (NDC py), inside the image.
+template <int P>
+struct TilePixels {
+    int j;
+    float px;
+    int row[P];
+    float py[P];
+    bool inside[P];
+};
+
+template <int P>
+__device__ __forceinline__ TilePixels<P> tile_pixels(const TileImage &im, int tid) {
+    TilePixels<P> q;
     const int trow = im.local / im.ntx, tcol = im.local % im.ntx;
-    const int j = tcol * kTile + (tid & 15);
-    const float px = pixel_ndc_x(j, im.W);
-    float py[P], T[P], cr[P], cg[P], cb[P];
-    int32_t cur[P];
+    q.j = tcol * kTile + (tid & 15);
+    q.px = pixel_ndc_x(q.j, im.W);
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        q.row[k] = trow * kTile + (tid >> 4) * P + k;
+        q.py[k] = pixel_ndc_y(q.row[k], im.H);
+        q.inside[k] = q.row[k] < im.H && q.j < im.W;
+    }
+    return q;
+}
+
+// Per wave, one bit per splat of the tile's list for its first kSkipSplats
+// splats: did the forward find a pixel of this wave that the splat reaches
+// (alpha >= 1/255, pixel not yet finished)? The backward's test for the
+// same (wave, splat) is the same test (a pixel's last contributor is where
+// it finished), so it skips the cleared bits without evaluating them.
+constexpr int kSkipWords = 64;
+constexpr int kSkipSplats = 64 * kSkipWords;
+
+// Front to back over the tile's list (compositeFwdKernel). With BITS the
+// per-wave reach bits are recorded in s_bits. Every exit is wave-uniform (a
+// ballot), so the loop runs on scalar branches; lanes whose pixels are done
+// ride along adding exact zeros. A wave whose pixels are all done skips the
+// batch, and leaves it once the last of them finishes: only a splat that
+// reaches a pixel can finish it.
+template <int P, bool BITS>
+__device__ __forceinline__ void composite_front_to_back(
+    const CompositeArgs &a, int2 range, const TilePixels<P> &q, float4 *sA, float4 *sB,
+    float4 *sC, uint64_t (*s_bits)[kSkipWords], float (&T)[P], float (&cr)[P], float (&cg)[P],
+    float (&cb)[P], int32_t (&cur)[P]) {
+    constexpr int NT = kTilePix / P;
+    const int tid = threadIdx.x, w = tid / kWave;
     bool done[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        const int i = trow * kTile + (tid >> 4) * P + k;
-        py[k] = pixel_ndc_y(i, im.H);
-        done[k] = !(i < im.H && j < im.W);
+        done[k] = !q.inside[k];
         T[k] = 1.f;
         cr[k] = cg[k] = cb[k] = 0.f;
         cur[k] = 0;
     }
-    const int2 range = a.ranges[tile];
     for (int b = range.x; b < range.y; b += NT) {
         bool live = false;
 #pragma unroll
@@ -193,116 +218,75 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
             sC[tid] = r[2];
         }
         __syncthreads();
-        // Every exit below is wave-uniform (a ballot), so the loop runs on
-        // scalar branches; lanes whose pixels are done ride along adding
-        // exact zeros. A wave whose pixels are all done skips the batch, and
-        // leaves it (t = bs) once the last of them finishes: only a splat
-        // that reaches a pixel can finish it.
         bool live_w = false;
 #pragma unroll
         for (int k = 0; k < P; ++k) live_w |= !done[k];
         const int bs = __ballot(live_w) ? min(NT, range.y - b) : 0;
-        for (int t = 0; t < bs; ++t) {
-            const float4 A = sA[t], B = sB[t];
-            // Pass 1: which of this lane's pixels the splat reaches; a wave
-            // whose pixels it reaches none of skips the rest.
-            AlphaEval e[P];
-            bool ok[P];
-            bool anyv = false;
+        bool finished = false;
+        // 64 splats at a time: one word of reach bits (NT is a multiple of 64)
+        for (int t0 = 0; t0 < bs && !finished; t0 += 64) {
+            const int t1 = min(bs, t0 + 64);
+            uint64_t reach = 0;
+            for (int t = t0; t < t1; ++t) {
+                const float4 A = sA[t], B = sB[t];
+                // Pass 1: which of this lane's pixels the splat reaches; a
+                // wave whose pixels it reaches none of skips the rest.
+                AlphaEval e[P];
+                bool ok[P];
+                bool anyv = false;
 #pragma unroll
-            for (int k = 0; k < P; ++k) {
-                ok[k] = eval_alpha(A, B, px, py[k], e[k]) & !done[k];
-                anyv |= ok[k];
+                for (int k = 0; k < P; ++k) {
+                    ok[k] = eval_alpha(A, B, q.px, q.py[k], e[k]) & !done[k];
+                    anyv |= ok[k];
+                }
+                if (__ballot(anyv) == 0) continue;
+                reach |= 1ull << (t - t0);
+                const float4 C = sC[t];
+                bool any = false;
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    // A skipped splat gets alpha 0, which leaves T and the
+                    // colour exactly unchanged (T * (1 - 0) == T).
+                    const float alpha = ok[k] ? e[k].alpha : 0.f;
+                    const float vis = alpha * T[k];
+                    cr[k] += C.x * vis;
+                    cg[k] += C.y * vis;
+                    cb[k] += C.z * vis;
+                    cur[k] = ok[k] ? b + t : cur[k];
+                    T[k] = T[k] * (1.f - alpha);
+                    done[k] |= T[k] <= kTThres;
+                    any |= !done[k];
+                }
+                if (__ballot(any) == 0) {
+                    finished = true;
+                    t = t1;
+                }
             }
-            if (__ballot(anyv) == 0) continue;
-            const float4 C = sC[t];
-            bool any = false;
-#pragma unroll
-            for (int k = 0; k < P; ++k) {
-                // A skipped splat gets alpha 0, which leaves T and the colour
-                // exactly unchanged (T * (1 - 0) == T).
-                const float alpha = ok[k] ? e[k].alpha : 0.f;
-                const float vis = alpha * T[k];
-                cr[k] += C.x * vis;
-                cg[k] += C.y * vis;
-                cb[k] += C.z * vis;
-                cur[k] = ok[k] ? b + t : cur[k];
-                T[k] = T[k] * (1.f - alpha);
-                done[k] |= T[k] <= kTThres;
-                any |= !done[k];
+            if constexpr (BITS) {
+                const int word = (b - range.x + t0) >> 6;
+                if (word < kSkipWords && __lane_id() == 0) s_bits[w][word] = reach;
             }
-            if (__ballot(any) == 0) t = bs;
-        }
-    }
-    float l = 0.f;
-#pragma unroll
-    for (int k = 0; k < P; ++k) {
-        const int i = trow * kTile + (tid >> 4) * P + k;
-        if (!(i < im.H && j < im.W)) continue;
-        const int pix = im.pix_base + i * im.W + j;
-        out_rgb[3 * pix + 0] = cr[k];
-        out_rgb[3 * pix + 1] = cg[k];
-        out_rgb[3 * pix + 2] = cb[k];
-        out_T[pix] = T[k];
-        out_last[pix] = cur[k];
-        if constexpr (MSE) {
-            const float dr = cr[k] - target[3 * pix + 0];
-            const float dg = cg[k] - target[3 * pix + 1];
-            const float db = cb[k] - target[3 * pix + 2];
-            const float s2 = 2.f * inv_npix;
-            grad[3 * pix + 0] = s2 * dr;
-            grad[3 * pix + 1] = s2 * dg;
-            grad[3 * pix + 2] = s2 * db;
-            l += (dr * dr + dg * dg + db * db) * inv_npix;
-        }
-    }
-    if constexpr (MSE) {
-        l = wave_sum(l);
-        if (__lane_id() == 0) s_red[tid / kWave] = l;
-        __syncthreads();
-        if (tid == 0) {
-            float s = 0.f;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) s += s_red[w];
-            atomicAdd(loss + im.inst, s);
         }
     }
 }
 
-template <int GRAD_MODE, int P, bool MULTI>
-__global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
-    CompositeArgs a, const float *__restrict__ Tfin, const int32_t *__restrict__ last,
-    const float *__restrict__ gout, ds_splat2d grad) {
+// Back to front over the tile's list from each pixel's last contributor
+// (compositeBwdKernel), from the final transmittance T, last index lst and
+// dL/dC (gr, gg, gb) of each pixel. With BITS, (wave, splat) pairs whose
+// reach bit the forward cleared are skipped without evaluation.
+template <int GRAD_MODE, int P, bool BITS>
+__device__ __forceinline__ void composite_back_to_front(
+    const CompositeArgs &a, int2 range, const TilePixels<P> &q, float4 *sA, float4 *sB,
+    float4 *sC, float (*sG)[kTilePix / P > kWave ? kTilePix / P : 1], int *s_last,
+    const uint64_t (*s_bits)[kSkipWords], float (&T)[P], const int32_t (&lst)[P],
+    const float (&gr)[P], const float (&gg)[P], const float (&gb)[P], const ds_splat2d &grad) {
     constexpr int NT = kTilePix / P;
     constexpr int NW = NT / kWave;
-    constexpr int GW = NW > 1 ? NT : 1;
-    __shared__ float4 sA[NT], sB[NT], sC[NT];
-    __shared__ float sG[9][GW];
-    __shared__ int s_last;
-    const int tile = tile_of_block(a);
-    const int2 range = a.ranges[tile];
-    if (range.y <= range.x) return;
-    const TileImage im = tile_image<MULTI>(a, tile, 0.f);
-    const int tid = threadIdx.x;
-    const int trow = im.local / im.ntx, tcol = im.local % im.ntx;
-    const int j = tcol * kTile + (tid & 15);
-    const float px = pixel_ndc_x(j, im.W);
-    // Per pixel: transmittance (walked back from T_final), last contributor,
-    // dL/dC, and acc = (colour of the splats behind) . dL/dC.
-    float py[P], T[P], gr[P], gg[P], gb[P], acc[P];
-    int32_t lst[P];
+    const int tid = threadIdx.x, w = tid / kWave;
+    float acc[P];
     int my_last = -1;
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        const int i = trow * kTile + (tid >> 4) * P + k;
-        py[k] = pixel_ndc_y(i, im.H);
-        const bool inside = (i < im.H && j < im.W);
-        const int pix = inside ? im.pix_base + i * im.W + j : 0;
-        T[k] = inside ? Tfin[pix] : 1.f;
-        lst[k] = inside ? last[pix] : -1;
-        gr[k] = inside ? gout[3 * pix + 0] : 0.f;
-        gg[k] = inside ? gout[3 * pix + 1] : 0.f;
-        gb[k] = inside ? gout[3 * pix + 2] : 0.f;
         acc[k] = 0.f;
         my_last = max(my_last, lst[k]);
     }
@@ -310,13 +294,14 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
     const int wave_last = __builtin_amdgcn_readfirstlane(wave_max(my_last));
     int block_last = wave_last;
     if constexpr (NW > 1) {
-        if (tid == 0) s_last = -1;
+        __syncthreads();  // LDS of a previous phase is free
+        if (tid == 0) *s_last = -1;
 #pragma unroll
         for (int f = 0; f < 9; ++f) sG[f][tid] = 0.f;
         __syncthreads();
-        if (__lane_id() == 0) atomicMax(&s_last, wave_last);
+        if (__lane_id() == 0) atomicMax(s_last, wave_last);
         __syncthreads();
-        block_last = __builtin_amdgcn_readfirstlane(s_last);
+        block_last = __builtin_amdgcn_readfirstlane(*s_last);
     }
     float *const dst[9] = {grad.pos_x,  grad.pos_y,   grad.cov_xx,  grad.cov_xy, grad.cov_yy,
                            grad.color_r, grad.color_g, grad.color_b, grad.opacity};
@@ -345,8 +330,23 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
             sC[tid] = r[2];
         }
         __syncthreads();
+        uint64_t reach = 0;
+        int word = -1;
         for (int t = max(0, bend - wave_last); t < bs; ++t) {
             const int idx = bend - t;
+            if constexpr (BITS) {
+                const int qi = idx - range.x;
+                if (qi < kSkipSplats) {
+                    if ((qi >> 6) != word) {
+                        word = qi >> 6;
+                        const uint64_t v = s_bits[w][word];
+                        reach = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32))
+                                 << 32) |
+                                __builtin_amdgcn_readfirstlane((uint32_t)v);
+                    }
+                    if (!((reach >> (qi & 63)) & 1ull)) continue;
+                }
+            }
             const float4 A = sA[t], B = sB[t];
             // Pass 1: which of this lane's pixels the splat reaches. A wave
             // whose pixels it reaches none of skips the rest.
@@ -355,7 +355,7 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
             bool anyv = false;
 #pragma unroll
             for (int k = 0; k < P; ++k) {
-                ok[k] = eval_alpha(A, B, px, py[k], e[k]) & (idx <= lst[k]);
+                ok[k] = eval_alpha(A, B, q.px, q.py[k], e[k]) & (idx <= lst[k]);
                 anyv |= ok[k];
             }
             if (__ballot(anyv) == 0) continue;
@@ -374,12 +374,12 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
                 v[5] = fmaf(fac, gr[k], v[5]);
                 v[6] = fmaf(fac, gg[k], v[6]);
                 v[7] = fmaf(fac, gb[k], v[7]);
-                const float cg = fmaf(C.x, gr[k], fmaf(C.y, gg[k], C.z * gb[k]));
+                const float cgk = fmaf(C.x, gr[k], fmaf(C.y, gg[k], C.z * gb[k]));
                 float v_alpha;
                 if constexpr (GRAD_MODE == DS_GRAD_REFERENCE)
-                    v_alpha = Tb * (cg - acc[k]);
+                    v_alpha = Tb * (cgk - acc[k]);
                 else
-                    v_alpha = Tb * cg - ra * acc[k];
+                    v_alpha = Tb * cgk - ra * acc[k];
                 const bool gate = ok[k] & (e[k].og <= kMaxAlpha);  // alpha below its clamp
                 v[8] = fmaf(gate ? e[k].g : 0.f, v_alpha, v[8]);
                 // m = -2 dL/d dist2 (dL/d dist2 = -o g v_alpha / 2); the
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
                 v[2] = fmaf(pu, e[k].u, v[2]);
                 v[3] = fmaf(pu, e[k].w, v[3]);
                 v[4] = fmaf(pw, e[k].w, v[4]);
-                acc[k] = fmaf(fac, cg, acc[k]);
+                acc[k] = fmaf(fac, cgk, acc[k]);
                 T[k] = Tb;
             }
             const float sum = my_scale * wave_sum9_transposed(v, lane);
@@ -418,6 +418,141 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
             }
         }
     }
+}
+
+// Writes a tile's image, transmittance and last index; with MSE also
+// dL/dC = 2 (C - target) / (W H) (returned in gr, gg, gb) and adds the
+// tile's share of the loss (mseLossKernel / mseGradKernel).
+template <bool MSE, int P>
+__device__ __forceinline__ void composite_epilogue(
+    const TileImage &im, const TilePixels<P> &q, const float (&T)[P], const float (&cr)[P],
+    const float (&cg)[P], const float (&cb)[P], const int32_t (&cur)[P],
+    float *__restrict__ out_rgb, float *__restrict__ out_T, int32_t *__restrict__ out_last,
+    const float *__restrict__ target, float *loss, float *__restrict__ grad, float *s_red,
+    float (&gr)[P], float (&gg)[P], float (&gb)[P]) {
+    constexpr int NW = kTilePix / P / kWave;
+    const int tid = threadIdx.x;
+    float l = 0.f;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        gr[k] = gg[k] = gb[k] = 0.f;
+        if (!q.inside[k]) continue;
+        const int pix = im.pix_base + q.row[k] * im.W + q.j;
+        out_rgb[3 * pix + 0] = cr[k];
+        out_rgb[3 * pix + 1] = cg[k];
+        out_rgb[3 * pix + 2] = cb[k];
+        out_T[pix] = T[k];
+        out_last[pix] = cur[k];
+        if constexpr (MSE) {
+            const float dr = cr[k] - target[3 * pix + 0];
+            const float dg = cg[k] - target[3 * pix + 1];
+            const float db = cb[k] - target[3 * pix + 2];
+            const float s2 = 2.f * im.inv_npix;
+            gr[k] = s2 * dr;
+            gg[k] = s2 * dg;
+            gb[k] = s2 * db;
+            grad[3 * pix + 0] = gr[k];
+            grad[3 * pix + 1] = gg[k];
+            grad[3 * pix + 2] = gb[k];
+            l += (dr * dr + dg * dg + db * db) * im.inv_npix;
+        }
+    }
+    if constexpr (MSE) {
+        l = wave_sum(l);
+        if (__lane_id() == 0) s_red[tid / kWave] = l;
+        __syncthreads();
+        if (tid == 0) {
+            float s = 0.f;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) s += s_red[w];
+            atomicAdd(loss + im.inst, s);
+        }
+    }
+}
+
+template <bool MSE, int P, bool MULTI>
+__global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
+    CompositeArgs a, float *__restrict__ out_rgb, float *__restrict__ out_T,
+    int32_t *__restrict__ out_last, const float *__restrict__ target, float *loss,
+    float *__restrict__ grad, float inv_npix_one) {
+    constexpr int NT = kTilePix / P;
+    constexpr int NW = NT / kWave;
+    __shared__ float4 sA[NT], sB[NT], sC[NT];
+    __shared__ float s_red[NW];
+    const int tile = tile_of_block(a);
+    const TileImage im = tile_image<MULTI>(a, tile, inv_npix_one);
+    const TilePixels<P> q = tile_pixels<P>(im, threadIdx.x);
+    const int2 range = a.ranges[tile];
+    float T[P], cr[P], cg[P], cb[P], gr[P], gg[P], gb[P];
+    int32_t cur[P];
+    composite_front_to_back<P, false>(a, range, q, sA, sB, sC, nullptr, T, cr, cg, cb, cur);
+    composite_epilogue<MSE, P>(im, q, T, cr, cg, cb, cur, out_rgb, out_T, out_last, target, loss,
+                               grad, s_red, gr, gg, gb);
+}
+
+template <int GRAD_MODE, int P, bool MULTI>
+__global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
+    CompositeArgs a, const float *__restrict__ Tfin, const int32_t *__restrict__ last,
+    const float *__restrict__ gout, ds_splat2d grad) {
+    constexpr int NT = kTilePix / P;
+    constexpr int NW = NT / kWave;
+    constexpr int GW = NW > 1 ? NT : 1;
+    __shared__ float4 sA[NT], sB[NT], sC[NT];
+    __shared__ float sG[9][GW];
+    __shared__ int s_last;
+    const int tile = tile_of_block(a);
+    const int2 range = a.ranges[tile];
+    if (range.y <= range.x) return;
+    const TileImage im = tile_image<MULTI>(a, tile, 0.f);
+    const TilePixels<P> q = tile_pixels<P>(im, threadIdx.x);
+    // Per pixel: transmittance (walked back from T_final), last contributor,
+    // dL/dC.
+    float T[P], gr[P], gg[P], gb[P];
+    int32_t lst[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const int pix = q.inside[k] ? im.pix_base + q.row[k] * im.W + q.j : 0;
+        T[k] = q.inside[k] ? Tfin[pix] : 1.f;
+        lst[k] = q.inside[k] ? last[pix] : -1;
+        gr[k] = q.inside[k] ? gout[3 * pix + 0] : 0.f;
+        gg[k] = q.inside[k] ? gout[3 * pix + 1] : 0.f;
+        gb[k] = q.inside[k] ? gout[3 * pix + 2] : 0.f;
+    }
+    composite_back_to_front<GRAD_MODE, P, false>(a, range, q, sA, sB, sC, sG, &s_last, nullptr,
+                                                 T, lst, gr, gg, gb, grad);
+}
+
+// Forward with the fused MSE, then the backward of the same tile in the same
+// workgroup: the per-pixel state (T, last index, dL/dC) stays in registers
+// and the backward skips the (wave, splat) pairs the forward found unreached.
+template <int GRAD_MODE, int P, bool MULTI>
+__global__ __launch_bounds__(kTilePix / P) void k_composite_fused(
+    CompositeArgs a, float *__restrict__ out_rgb, float *__restrict__ out_T,
+    int32_t *__restrict__ out_last, const float *__restrict__ target, float *loss,
+    float *__restrict__ grad_rgb, float inv_npix_one, ds_splat2d grad) {
+    constexpr int NT = kTilePix / P;
+    constexpr int NW = NT / kWave;
+    constexpr int GW = NW > 1 ? NT : 1;
+    __shared__ float4 sA[NT], sB[NT], sC[NT];
+    __shared__ float sG[9][GW];
+    __shared__ float s_red[NW];
+    __shared__ int s_last;
+    __shared__ uint64_t s_bits[NW][kSkipWords];
+    const int tile = tile_of_block(a);
+    const TileImage im = tile_image<MULTI>(a, tile, inv_npix_one);
+    const TilePixels<P> q = tile_pixels<P>(im, threadIdx.x);
+    const int2 range = a.ranges[tile];
+    float T[P], cr[P], cg[P], cb[P], gr[P], gg[P], gb[P];
+    int32_t cur[P];
+    composite_front_to_back<P, true>(a, range, q, sA, sB, sC, s_bits, T, cr, cg, cb, cur);
+    composite_epilogue<true, P>(im, q, T, cr, cg, cb, cur, out_rgb, out_T, out_last, target,
+                                loss, grad_rgb, s_red, gr, gg, gb);
+    if (range.y <= range.x) return;
+    int32_t lst[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) lst[k] = q.inside[k] ? cur[k] : -1;
+    composite_back_to_front<GRAD_MODE, P, true>(a, range, q, sA, sB, sC, sG, &s_last, s_bits, T,
+                                                lst, gr, gg, gb, grad);
 }
 
 }  // namespace
@@ -453,6 +588,41 @@ static void composite_bwd(const CompositeArgs &a, int P, const float *T, const i
     DS_PIX_DISPATCH(P, hipLaunchKernelGGL((k_composite_bwd<MODE, PP, MULTI>), dim3(a.ntiles),
                                           dim3(kTilePix / PP), 0, s, a, T, last, grad_rgb,
                                           grad));
+}
+
+template <int MODE, bool MULTI>
+static void composite_fused(const CompositeArgs &a, int P, float *out_rgb, float *out_T,
+                            int32_t *out_last, const float *target, float *loss, float *grad_rgb,
+                            const ds_splat2d &grad, hipStream_t s) {
+    const float inv_npix = 1.f / ((float)a.W * (float)a.H);
+    DS_PIX_DISPATCH(P, hipLaunchKernelGGL((k_composite_fused<MODE, PP, MULTI>), dim3(a.ntiles),
+                                          dim3(kTilePix / PP), 0, s, a, out_rgb, out_T, out_last,
+                                          target, loss, grad_rgb, inv_npix, grad));
+}
+
+ds_status launch_composite_fused(const CompositeArgs &a, int pix_per_thread, int grad_mode,
+                                 float *out_rgb, float *out_T, int32_t *out_last,
+                                 const float *target, float *loss, float *grad_rgb,
+                                 const ds_splat2d &grad, hipStream_t s) {
+    if (a.ntiles <= 0) return DS_OK;
+    const int P = pix_per_thread;
+    if (grad_mode == DS_GRAD_REFERENCE) {
+        if (a.mi)
+            composite_fused<DS_GRAD_REFERENCE, true>(a, P, out_rgb, out_T, out_last, target, loss,
+                                                     grad_rgb, grad, s);
+        else
+            composite_fused<DS_GRAD_REFERENCE, false>(a, P, out_rgb, out_T, out_last, target,
+                                                      loss, grad_rgb, grad, s);
+    } else {
+        if (a.mi)
+            composite_fused<DS_GRAD_EXACT, true>(a, P, out_rgb, out_T, out_last, target, loss,
+                                                 grad_rgb, grad, s);
+        else
+            composite_fused<DS_GRAD_EXACT, false>(a, P, out_rgb, out_T, out_last, target, loss,
+                                                  grad_rgb, grad, s);
+    }
+    DS_CHECK_LAUNCH("composite_fused");
+    return DS_OK;
 }
 
 ds_status launch_composite_fwd(const CompositeArgs &a, int pix_per_thread, float *out_rgb,

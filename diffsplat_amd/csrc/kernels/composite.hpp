@@ -44,6 +44,12 @@ struct CompositeArgs {
 ds_status launch_composite_fwd(const CompositeArgs &a, int pix_per_thread, float *out_rgb,
                                float *out_T, int32_t *out_last, const float *target,
                                float *loss, float *grad, hipStream_t s);
+// Forward with the fused MSE (target required) and the backward of each tile
+// in one launch; the backward accumulates into grad like launch_composite_bwd.
+ds_status launch_composite_fused(const CompositeArgs &a, int pix_per_thread, int grad_mode,
+                                 float *out_rgb, float *out_T, int32_t *out_last,
+                                 const float *target, float *loss, float *grad_rgb,
+                                 const ds_splat2d &grad, hipStream_t s);
 ds_status launch_composite_bwd(const CompositeArgs &a, int pix_per_thread, int grad_mode,
                                const float *T, const int32_t *last, const float *grad_rgb,
                                const ds_splat2d &grad, hipStream_t s);

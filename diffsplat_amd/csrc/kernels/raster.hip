@@ -786,8 +786,13 @@ static CompositeArgs composite_args(const ds_raster *r) {
                                                                     : nullptr};
 }
 
+// Tile assignment, sorts, ranges and dispatch order, then compositing: the
+// forward alone, or (fused_grad non-NULL, target given, equal launch shapes)
+// the forward and backward of each tile in one kernel accumulating into
+// *fused_grad.
 static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_rgb,
-                                const float *target, float *loss, float *grad, hipStream_t s) {
+                                const float *target, float *loss, float *grad, hipStream_t s,
+                                const ds_splat2d *fused_grad = nullptr) {
     const int n = in.count;
     DS_REQUIRE(n >= 0 && n <= r->count_cap, "splat count exceeds the rasterizer's count_cap");
     DS_REQUIRE(r->ninst == 0 || n == r->count_cap,
@@ -926,10 +931,17 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
     }
     r->order_valid = r->tile_order == DS_TILE_ORDER_HEAVY_FIRST;
     const CompositeArgs ca = composite_args(r);
-    DS_TRY(launch_composite_fwd(ca, r->fwd_ppt, out_rgb, r->T.ptr, r->last.ptr, target,
-                                loss, grad, s));
+    if (fused_grad && target && r->fwd_ppt == r->bwd_ppt)
+        DS_TRY(launch_composite_fused(ca, r->fwd_ppt, r->grad_mode, out_rgb, r->T.ptr,
+                                      r->last.ptr, target, loss, grad, *fused_grad, s));
+    else
+        DS_TRY(launch_composite_fwd(ca, r->fwd_ppt, out_rgb, r->T.ptr, r->last.ptr, target,
+                                    loss, grad, s));
     r->last_count = n;
     r->has_fwd = true;
+    if (fused_grad && !(target && r->fwd_ppt == r->bwd_ppt))
+        DS_TRY(launch_composite_bwd(ca, r->bwd_ppt, r->grad_mode, r->T.ptr, r->last.ptr,
+                                    grad, *fused_grad, s));
     return DS_OK;
 }
 
@@ -1101,6 +1113,18 @@ extern "C" ds_status ds_raster_fwd(ds_raster *r, const ds_splat2d *in, float *ou
     clear_error();
     DS_REQUIRE(r && in, "NULL argument");
     return raster_forward(r, *in, out_rgb, target_rgb, loss_dev, grad_rgb, as_stream(s));
+}
+
+extern "C" ds_status ds_raster_fwd_bwd(ds_raster *r, const ds_splat2d *in, float *out_rgb,
+                                       const float *target_rgb, float *loss_dev,
+                                       float *grad_rgb, const ds_splat2d *grad_in,
+                                       ds_stream_t s) {
+    clear_error();
+    DS_REQUIRE(r && in && grad_in, "NULL argument");
+    DS_REQUIRE(target_rgb && loss_dev && grad_rgb, "ds_raster_fwd_bwd needs the MSE target");
+    DS_REQUIRE(grad_in->count == in->count, "grad_in count differs from the input's");
+    return raster_forward(r, *in, out_rgb, target_rgb, loss_dev, grad_rgb, as_stream(s),
+                          grad_in);
 }
 
 extern "C" ds_status ds_raster_bwd(ds_raster *r, const ds_splat2d *in, const float *grad_rgb,

@@ -263,6 +263,19 @@ ds_status ds_raster_fwd(ds_raster *r, const ds_splat2d *in, float *out_rgb,
                         const float *target_rgb, float *loss_dev, float *grad_rgb,
                         ds_stream_t s);
 
+/* TileRasterizeLayer::forward followed by MSELossLayer and TileRasterizeLayer::backward
+ * (src/layers/tile_rasterize_layer.cu:637-728, src/layers/mse_loss_layer.cu:22-67)
+ * in one call: the forward with the fused MSE as ds_raster_fwd, then dL/dSplat2D
+ * accumulated into grad_in as ds_raster_bwd (zero it first). When the forward and
+ * backward launch shapes are equal, each tile's forward and backward run in one
+ * compositing kernel, which keeps the per-pixel state in registers and skips in the
+ * backward the (wave, splat) pairs the forward found unreached. The image,
+ * transmittance, last ids, loss and dL/dimage are written as by ds_raster_fwd, and
+ * ds_raster_bwd may still be called afterwards. No counterpart in the reference. */
+ds_status ds_raster_fwd_bwd(ds_raster *r, const ds_splat2d *in, float *out_rgb,
+                            const float *target_rgb, float *loss_dev, float *grad_rgb,
+                            const ds_splat2d *grad_in, ds_stream_t s);
+
 /* TileRasterizeLayer::backward (src/layers/tile_rasterize_layer.cu:710-728).
  * Gradients are ACCUMULATED into grad_in (atomics), as in the reference: the
  * caller zeroes them first (ds_splat2d_zero). grad_in->pos_z is not written. */

@@ -235,6 +235,47 @@ def test_tile_dispatch_order_leaves_results_unchanged(scene):
         assert ok, mabs
 
 
+@pytest.mark.parametrize("fwd_ppt,bwd_ppt,grad_mode", [(1, 1, 0), (2, 2, 1), (4, 4, 0),
+                                                        (1, 2, 0)])
+@pytest.mark.parametrize("scene", ["c0", "ragged"])
+def test_fused_forward_backward_matches_separate_calls(scene, fwd_ppt, bwd_ppt, grad_mode):
+    """ds_raster_fwd_bwd (one compositing kernel per tile for forward and
+    backward when the launch shapes are equal, with the backward skipping the
+    pairs the forward found unreached) against ds_raster_fwd + ds_raster_bwd:
+    identical image, transmittance, last ids and dL/dC, loss and gradients
+    up to the order of float atomics; and against the FP64 oracle."""
+    seed, n, W, H, ls = SCENES[scene]
+    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    target = ds.target_image(seed, W, H, 32)
+    st = Stages(params, n, 0, W, H, view, proj, cam, grad_mode=grad_mode)
+    ds.check(ds.lib().ds_raster_set_tuning(st.ras.h, fwd_ppt, bwd_ppt), "tuning")
+    s2 = st.activ_project_fwd()
+    sep = st.raster_fwd(target)
+    sep["gs2"] = st.raster_bwd(sep["grad_rgb"])
+    d_t = ds.DeviceArray.from_numpy(target)
+    d_loss = ds.DeviceArray(4)
+    ds.check(ds.lib().ds_splat2d_zero(st.gs2, None), "zero")
+    ds.check(ds.lib().ds_raster_fwd_bwd(st.ras.h, st.s2, st.d_rgb.ptr, d_t.ptr, d_loss.ptr,
+                                        st.d_grad_rgb.ptr, st.gs2, None), "ds_raster_fwd_bwd")
+    fused = st.ras.state()
+    fused["rgb"] = st.d_rgb.download(np.float32, 3 * W * H)
+    fused["grad_rgb"] = st.d_grad_rgb.download(np.float32, 3 * W * H)
+    fused["loss"] = float(d_loss.download(np.float32, 1)[0])
+    fused["gs2"] = st.d_gs2.download(np.float32, 10 * n)
+    for k in ("rgb", "T", "last_ids", "grad_rgb", "flatten_ids", "tile_ranges"):
+        assert np.array_equal(fused[k], sep[k]), k
+    assert abs(fused["loss"] - sep["loss"]) <= 1e-6 * sep["loss"]
+    ok, mabs, _ = close(fused["gs2"], sep["gs2"], atol=1e-7, rtol=1e-5)
+    assert ok, mabs
+    ref = oracle.comp_fwd(s2, n, sep["flatten_ids"], sep["tile_ranges"], W, H)
+    rg, _ = oracle.comp_bwd(s2, n, sep["flatten_ids"], sep["tile_ranges"], W, H, ref["T"],
+                            ref["last"], sep["grad_rgb"], grad_mode=grad_mode)
+    mask = np.ones(10 * n, bool)
+    mask[2 * n:3 * n] = False
+    check_close(f"fused_fwd_bwd_{scene}_ppt{fwd_ppt}{bwd_ppt}_mode{grad_mode}", fused["gs2"], rg,
+                mask)
+
+
 def heavy_first_ok(order, ranges):
     """order is a permutation of the tiles with non-increasing buckets
     floor(log2(pairs)) + 1 (0 for empty tiles)."""
