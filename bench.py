@@ -6,10 +6,10 @@ zero grads, activation+projection, tile assign + sort + ranges, compositing
 with fused MSE, compositing backward, projection/activation backward, Adam.
 Inputs are synthetic and resident in HBM before the timed region starts.
 
-Prints ONE JSON line. `roofline` is for the compositing backward (the
-dominant kernel), timed with HIP events on the fitter's stream at the state
-reached after the timed steps; its algorithmic work (splat evaluations) is
-counted by the oracle on the same state. `cpu_baseline` is the oracle's fp32
+Prints ONE JSON line. `roofline` is for the fused compositing forward +
+backward (the dominant kernel), timed with HIP events on the fitter's stream
+at the state reached after the timed steps; its algorithmic work (splat
+evaluations) is counted by the oracle on the same state. `cpu_baseline` is the oracle's fp32
 OpenMP transcription of the same step on this host's cores. Beside the line:
 `time_to_30db` (the metric's second half: wall time to 30 dB PSNR at
 256x256 / 4096 splats from the learned initializer vs from random splats),
@@ -33,17 +33,19 @@ sys.path.insert(0, ROOT)
 METRIC = "fit steps/s @720p 60k splats; time-to-30dB PSNR learned-init vs plain"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-# Algorithmic FP32 work per splat evaluation of the compositing backward
-# (FMA = 2 FLOP): falloff, alpha, the colour / alpha / conic gradients.
+# Algorithmic FP32 work per splat evaluation (FMA = 2 FLOP) of the
+# compositing forward (falloff, alpha, colour and transmittance update) and
+# backward (falloff, alpha, the colour / alpha / conic gradients).
+FWD_FLOP_PER_EVAL = 24.0
 BWD_FLOP_PER_EVAL = 60.0
-# HBM bytes per launch of the compositing backward (one pixel per thread),
-# from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over 30 steps of this
-# workload, scaled by a 256 MiB copy (fetch x2.000, write x1.000): see
-# profiles/fit_720p_60k_counters.md. Counters cannot be read live here; the
-# write bytes are the kernel's device-scope float atomics.
-BWD_TRAFFIC_READ = 27.73e6
-BWD_TRAFFIC_WRITE = 133.16e6
-BWD_TRAFFIC_STATE = "fit step 32 (counter passes over 30 steps after 2 warm-up steps)"
+# HBM bytes per launch of the fused compositing kernel (one pixel per
+# thread), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over 30 steps of
+# this workload, scaled by a 256 MiB copy (fetch x2.000, write x1.000): see
+# profiles/fit_720p_60k_counters.md. Counters cannot be read live here; most
+# write bytes are the backward's device-scope float atomics.
+FUSED_TRAFFIC_READ = None
+FUSED_TRAFFIC_WRITE = None
+FUSED_TRAFFIC_STATE = "not measured yet"
 
 CONFIGS = {
     # name: (W, H, N, n_ellipses, seed)
@@ -78,7 +80,13 @@ def parse():
     return ap.parse_args()
 
 
-def roofline_bwd(ds, oracle, fit, W, H, N, reps=20):
+def roofline_composite(ds, oracle, fit, target, W, H, N, reps=20):
+    """The dominant kernel of the step, k_composite_fused (each tile's
+    forward with the fused MSE, then its backward), timed with HIP events on
+    the fitter's stream over `reps` launches on the state reached after the
+    timed steps (ds_raster_composite_fwd_bwd re-runs it on the last forward's
+    sorted pairs). Algorithmic work counted by the oracle on the same state:
+    24 FLOP per forward splat evaluation, 60 per backward evaluation."""
     L = ds.lib()
     g = ds._lib.Gauss3D()
     s2 = ds._lib.Splat2D()
@@ -91,14 +99,17 @@ def roofline_bwd(ds, oracle, fit, W, H, N, reps=20):
     stream = fit.stream()
     fit.forward()
     fit.synchronize()
+    d_target = ds.DeviceArray.from_numpy(np.ascontiguousarray(target, np.float32))
+    d_rgb = ds.DeviceArray(4 * 3 * W * H)
+    d_loss = ds.DeviceArray(4)
+    args = (ras, C.byref(s2), d_rgb.ptr, d_target.ptr, d_loss.ptr, grad_rgb, C.byref(gs2), stream)
     e0, e1 = C.c_void_p(), C.c_void_p()
     ds.check(L.ds_event_create(C.byref(e0)), "event")
     ds.check(L.ds_event_create(C.byref(e1)), "event")
-    ds.check(L.ds_raster_bwd(ras, C.byref(s2), grad_rgb, C.byref(gs2), stream), "warm")
+    ds.check(L.ds_raster_composite_fwd_bwd(*args), "warm")
     ds.check(L.ds_event_record(e0, stream), "record")
     for _ in range(reps):
-        ds.check(L.ds_raster_bwd(ras, C.byref(s2), grad_rgb, C.byref(gs2), stream),
-                 "ds_raster_bwd")
+        ds.check(L.ds_raster_composite_fwd_bwd(*args), "ds_raster_composite_fwd_bwd")
     ds.check(L.ds_event_record(e1, stream), "record")
     ds.check(L.ds_event_synchronize(e1), "sync")
     ms = C.c_float()
@@ -110,20 +121,25 @@ def roofline_bwd(ds, oracle, fit, W, H, N, reps=20):
     st = fit.raster_state()
     s2_host = ds.download_ptr(s2.pos_x, np.float32, 10 * N)
     g_host = ds.download_ptr(grad_rgb.value, np.float32, 3 * W * H)
+    fw = oracle.comp_fwd(s2_host, N, st["flatten_ids"], st["tile_ranges"], W, H,
+                         dtype=np.float32)
     _, e_bwd = oracle.comp_bwd(s2_host, N, st["flatten_ids"], st["tile_ranges"], W, H, st["T"],
                                st["last_ids"], g_host, dtype=np.float32)
-    flops = e_bwd * BWD_FLOP_PER_EVAL
+    e_fwd = int(fw["E"])
+    flops = e_fwd * FWD_FLOP_PER_EVAL + e_bwd * BWD_FLOP_PER_EVAL
     achieved = flops / avg_s / 1e12
-    return {"kernel": "k_composite_bwd", "bound": "mfma", "unit": "TFLOP/s",
+    return {"kernel": "k_composite_fused", "bound": "mfma", "unit": "TFLOP/s",
             "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": BWD_TRAFFIC_READ + BWD_TRAFFIC_WRITE,
-            "traffic_read_bytes": BWD_TRAFFIC_READ, "traffic_write_bytes": BWD_TRAFFIC_WRITE,
-            "traffic_state": BWD_TRAFFIC_STATE,
-            "avg_launch_ms": round(avg_s * 1e3, 4), "splat_evals_per_launch": int(e_bwd),
-            "flop_per_eval": BWD_FLOP_PER_EVAL, "n_isects": int(st["n_isects"]),
+            "traffic": (FUSED_TRAFFIC_READ + FUSED_TRAFFIC_WRITE
+                        if FUSED_TRAFFIC_READ is not None else None),
+            "traffic_read_bytes": FUSED_TRAFFIC_READ, "traffic_write_bytes": FUSED_TRAFFIC_WRITE,
+            "traffic_state": FUSED_TRAFFIC_STATE,
+            "avg_launch_ms": round(avg_s * 1e3, 4), "splat_evals_fwd": e_fwd,
+            "splat_evals_bwd": int(e_bwd), "flop_per_eval_fwd": FWD_FLOP_PER_EVAL,
+            "flop_per_eval_bwd": BWD_FLOP_PER_EVAL, "n_isects": int(st["n_isects"]),
             "note": "fp32 vector (VALU) work priced against the 157.3 TFLOP/s fp32 peak "
-                    "(which counts packed FMA); the kernel is VALU-issue-bound"}
+                    "(which counts packed FMA, not used here); the kernel is VALU-issue-bound"}
 
 
 TTP_POS_DECAY = 0.998  # position lr schedule of the time-to-PSNR fits
@@ -429,7 +445,7 @@ def main():
     if not args.no_roofline or not args.no_cpu:
         import oracle
     if not args.no_roofline:
-        roof = roofline_bwd(ds, oracle, fit, W, H, N)
+        roof = roofline_composite(ds, oracle, fit, target, W, H, N)
         print(f"[bench] roofline {roof}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         pv = ds.mat_mul(proj, view)

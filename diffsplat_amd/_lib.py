@@ -150,6 +150,8 @@ _SIGS = {
     "ds_raster_bwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, C.POINTER(Splat2D), _P]),
     "ds_raster_fwd_bwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, _P, _P, _P, C.POINTER(Splat2D),
                                     _P]),
+    "ds_raster_composite_fwd_bwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, _P, _P, _P,
+                                              C.POINTER(Splat2D), _P]),
     "ds_splat2d_zero": (C.c_int, [C.POINTER(Splat2D), _P]),
     "ds_raster_counts": (C.c_int, [_P, _P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                    C.POINTER(C.c_uint32)]),

@@ -38,6 +38,28 @@ static void tune_tile_order(ds_raster *r, F &&timed) {
     check(ds_raster_set_tile_order(r, t[other] < 0.95f * t[cur] ? other : cur), "autotune");
 }
 
+// With the forward and backward launch shapes chosen apart (bf, bb), times
+// the whole compositing (timed_fwd_bwd, ms) for (bf, bb) and for the equal
+// shapes (bf, bf) and (bb, bb), with which the forward and backward run in
+// one fused kernel, and keeps the fastest (the separate pair unless another
+// is at least 5% faster). Returns the kept shapes in bf, bb.
+template <class F>
+static void tune_fused_shapes(ds_raster *r, int &bf, int &bb, F &&timed_fwd_bwd) {
+    if (bf == bb) return;
+    const int cand[3][2] = {{bf, bb}, {bf, bf}, {bb, bb}};
+    float t[3];
+    for (int i = 0; i < 3; ++i) {
+        check(ds_raster_set_tuning(r, cand[i][0], cand[i][1]), "autotune");
+        t[i] = timed_fwd_bwd();
+    }
+    int best = 0;
+    for (int i = 1; i < 3; ++i)
+        if (t[i] < 0.95f * t[best]) best = i;
+    bf = cand[best][0];
+    bb = cand[best][1];
+    check(ds_raster_set_tuning(r, bf, bb), "autotune");
+}
+
 // ============================================================ ImageFitter
 
 ImageFitter::ImageFitter() = default;
@@ -261,7 +283,7 @@ void ImageFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd
         check(ds_raster_set_tuning(r, opts[i], bwd0), "autotune");
         tfw[i] = timed([&] { pipeline.forward(); });
     }
-    const int bf = opts[pick_with_margin(tfw, fwd0)];
+    int bf = opts[pick_with_margin(tfw, fwd0)];
     check(ds_raster_set_tuning(r, bf, bwd0), "autotune");
     pipeline.forward();
     for (int i = 0; i < 3; ++i) {
@@ -269,16 +291,19 @@ void ImageFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd
         tbw[i] = timed([&] { ras_layer.backward(); });
     }
     const int ib = pick_with_margin(tbw, bwd0);
-    const int bb = opts[ib];
+    int bb = opts[ib];
     const float tf = tfw[bf == 1 ? 0 : bf == 2 ? 1 : 2], tb = tbw[ib];
     check(ds_raster_set_tuning(r, bf, bb), "autotune");
-    tune_tile_order(r, [&] {
+    // the step's compositing: fused when the shapes are equal (fused mode)
+    ras_layer.setFuseBackward(fused);
+    auto fwd_bwd = [&] {
         return timed([&] {
             pipeline.forward();
             ras_layer.backward();
         });
-    });
-    ras_layer.setFuseBackward(fused);
+    };
+    if (fused) tune_fused_shapes(r, bf, bb, fwd_bwd);
+    tune_tile_order(r, fwd_bwd);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     dropGraph();  // the captured step (if any) used the old launch shapes
@@ -602,7 +627,7 @@ void MultiFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd
         check(ds_raster_set_tuning(ras, opts[i], bwd0), "autotune");
         tfw[i] = timed([&] { forwardOnly(); });
     }
-    const int bf = opts[pick_with_margin(tfw, fwd0)];
+    int bf = opts[pick_with_margin(tfw, fwd0)];
     check(ds_raster_set_tuning(ras, bf, bwd0), "autotune");
     forwardOnly();
     ds_splat2d sv = s2.view(), gv = gs2.view();
@@ -611,15 +636,22 @@ void MultiFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd
         tbw[i] = timed([&] { check(ds_raster_bwd(ras, &sv, grad_rgb.ptr, &gv, stream), "bwd"); });
     }
     const int ib = pick_with_margin(tbw, bwd0);
-    const int bb = opts[ib];
+    int bb = opts[ib];
     const float tf = tfw[bf == 1 ? 0 : bf == 2 ? 1 : 2], tb = tbw[ib];
     check(ds_raster_set_tuning(ras, bf, bb), "autotune");
-    tune_tile_order(ras, [&] {
+    // the step's compositing: one fused kernel when the shapes are equal
+    ds_gauss3d pv = params.view();
+    auto fwd_bwd = [&] {
         return timed([&] {
-            forwardOnly();
-            check(ds_raster_bwd(ras, &sv, grad_rgb.ptr, &gv, stream), "bwd");
+            check(ds_activ_project_fwd_multi(&pv, &sv, sh_degree, cams.ptr, splat_inst, stream),
+                  "forward");
+            check(ds_raster_fwd_bwd(ras, &sv, image.ptr, target.ptr, loss.ptr, grad_rgb.ptr, &gv,
+                                    stream),
+                  "forward_backward");
         });
-    });
+    };
+    tune_fused_shapes(ras, bf, bb, fwd_bwd);
+    tune_tile_order(ras, fwd_bwd);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     dropGraph();

@@ -1127,6 +1127,25 @@ extern "C" ds_status ds_raster_fwd_bwd(ds_raster *r, const ds_splat2d *in, float
                           grad_in);
 }
 
+extern "C" ds_status ds_raster_composite_fwd_bwd(ds_raster *r, const ds_splat2d *in,
+                                                 float *out_rgb, const float *target_rgb,
+                                                 float *loss_dev, float *grad_rgb,
+                                                 const ds_splat2d *grad_in, ds_stream_t s) {
+    clear_error();
+    DS_REQUIRE(r && in && grad_in, "NULL argument");
+    DS_REQUIRE(out_rgb && target_rgb && loss_dev && grad_rgb, "NULL image argument");
+    DS_REQUIRE(r->has_fwd, "ds_raster_composite_fwd_bwd called before a forward");
+    DS_REQUIRE(in->count == r->last_count, "splat count differs from the last forward");
+    DS_REQUIRE(r->fwd_ppt == r->bwd_ppt,
+               "the fused compositing needs equal forward and backward launch shapes");
+    CompositeArgs ca = composite_args(r);
+    if (!r->order_valid) ca.order = nullptr;
+    hipStream_t st = as_stream(s);
+    DS_TRY(fill_u32(loss_dev, 0u, r->ninst > 0 ? (size_t)r->ninst : 1, st));
+    return launch_composite_fused(ca, r->fwd_ppt, r->grad_mode, out_rgb, r->T.ptr, r->last.ptr,
+                                  target_rgb, loss_dev, grad_rgb, *grad_in, st);
+}
+
 extern "C" ds_status ds_raster_bwd(ds_raster *r, const ds_splat2d *in, const float *grad_rgb,
                                    const ds_splat2d *grad_in, ds_stream_t s) {
     clear_error();

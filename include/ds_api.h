@@ -276,6 +276,17 @@ ds_status ds_raster_fwd_bwd(ds_raster *r, const ds_splat2d *in, float *out_rgb,
                             const float *target_rgb, float *loss_dev, float *grad_rgb,
                             const ds_splat2d *grad_in, ds_stream_t s);
 
+/* The compositing of ds_raster_fwd_bwd alone (one fused kernel), on the sorted
+ * pairs and tile ranges of the last forward: the splats must be those of that
+ * forward. Needs equal forward and backward launch shapes. Zeroes the loss
+ * (one float per instance), writes the image, transmittance, last ids and
+ * dL/dimage, accumulates into grad_in. For timing and for re-differentiating
+ * against another target without sorting again. No counterpart in the
+ * reference. */
+ds_status ds_raster_composite_fwd_bwd(ds_raster *r, const ds_splat2d *in, float *out_rgb,
+                                      const float *target_rgb, float *loss_dev, float *grad_rgb,
+                                      const ds_splat2d *grad_in, ds_stream_t s);
+
 /* TileRasterizeLayer::backward (src/layers/tile_rasterize_layer.cu:710-728).
  * Gradients are ACCUMULATED into grad_in (atomics), as in the reference: the
  * caller zeroes them first (ds_splat2d_zero). grad_in->pos_z is not written. */
