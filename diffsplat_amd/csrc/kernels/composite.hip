@@ -190,8 +190,9 @@ constexpr int kSkipSplats = 64 * kSkipWords;
 // ride along adding exact zeros. A wave whose pixels are all done skips the
 // batch, and leaves it once the last of them finishes: only a splat that
 // reaches a pixel can finish it.
+// Returns the first list index of the batch left in sA/sB/sC (-1 if none).
 template <int P, bool BITS>
-__device__ __forceinline__ void composite_front_to_back(
+__device__ __forceinline__ int composite_front_to_back(
     const CompositeArgs &a, int2 range, const TilePixels<P> &q, float4 *sA, float4 *sB,
     float4 *sC, uint64_t (*s_bits)[kSkipWords], float (&T)[P], float (&cr)[P], float (&cg)[P],
     float (&cb)[P], int32_t (&cur)[P]) {
@@ -205,6 +206,7 @@ __device__ __forceinline__ void composite_front_to_back(
         cr[k] = cg[k] = cb[k] = 0.f;
         cur[k] = 0;
     }
+    int loaded = -1;
     for (int b = range.x; b < range.y; b += NT) {
         bool live = false;
 #pragma unroll
@@ -218,6 +220,7 @@ __device__ __forceinline__ void composite_front_to_back(
             sC[tid] = r[2];
         }
         __syncthreads();
+        loaded = b;
         bool live_w = false;
 #pragma unroll
         for (int k = 0; k < P; ++k) live_w |= !done[k];
@@ -268,17 +271,20 @@ __device__ __forceinline__ void composite_front_to_back(
             }
         }
     }
+    return loaded;
 }
 
 // Back to front over the tile's list from each pixel's last contributor
 // (compositeBwdKernel), from the final transmittance T, last index lst and
-// dL/dC (gr, gg, gb) of each pixel. With BITS, (wave, splat) pairs whose
-// reach bit the forward cleared are skipped without evaluation.
+// dL/dC (gr, gg, gb) of each pixel, in the forward's batches of NT splats.
+// The batch starting at list index `loaded` is already in sA/sB/sC (-1:
+// none). With BITS, (wave, splat) pairs whose reach bit the forward
+// cleared are skipped without evaluation.
 template <int GRAD_MODE, int P, bool BITS>
 __device__ __forceinline__ void composite_back_to_front(
     const CompositeArgs &a, int2 range, const TilePixels<P> &q, float4 *sA, float4 *sB,
     float4 *sC, float (*sG)[kTilePix / P > kWave ? kTilePix / P : 1], int *s_last,
-    const uint64_t (*s_bits)[kSkipWords], float (&T)[P], const int32_t (&lst)[P],
+    const uint64_t (*s_bits)[kSkipWords], int loaded, float (&T)[P], const int32_t (&lst)[P],
     const float (&gr)[P], const float (&gg)[P], const float (&gb)[P], const ds_splat2d &grad) {
     constexpr int NT = kTilePix / P;
     constexpr int NW = NT / kWave;
@@ -319,21 +325,24 @@ __device__ __forceinline__ void composite_back_to_front(
                            : (my_field == 2 || my_field == 4) ? 0.5f
                                                               : 1.f;
     const int end = min(range.y, block_last + 1);
-    for (int bend = end - 1; bend >= range.x; bend -= NT) {
-        const int bs = min(NT, bend + 1 - range.x);
-        __syncthreads();
-        const int li = bend - tid;
-        if (li >= range.x) {
-            const float4 *r = a.rec + 3 * (size_t)a.flat[li];
-            sA[tid] = r[0];
-            sB[tid] = r[1];
-            sC[tid] = r[2];
+    if (end <= range.x) return;  // workgroup-uniform
+    for (int bb = range.x + (end - 1 - range.x) / NT * NT; bb >= range.x; bb -= NT) {
+        const int bs = min(NT, end - bb);  // slots [0, bs) hold list indices bb + slot
+        if (bb != loaded) {
+            __syncthreads();
+            if (tid < bs) {
+                const float4 *r = a.rec + 3 * (size_t)a.flat[bb + tid];
+                sA[tid] = r[0];
+                sB[tid] = r[1];
+                sC[tid] = r[2];
+            }
+            __syncthreads();
         }
-        __syncthreads();
+        loaded = -1;  // only the forward's last batch is reused
         uint64_t reach = 0;
         int word = -1;
-        for (int t = max(0, bend - wave_last); t < bs; ++t) {
-            const int idx = bend - t;
+        for (int t = min(bs - 1, wave_last - bb); t >= 0; --t) {
+            const int idx = bb + t;
             if constexpr (BITS) {
                 const int qi = idx - range.x;
                 if (qi < kSkipSplats) {
@@ -485,7 +494,8 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
     const int2 range = a.ranges[tile];
     float T[P], cr[P], cg[P], cb[P], gr[P], gg[P], gb[P];
     int32_t cur[P];
-    composite_front_to_back<P, false>(a, range, q, sA, sB, sC, nullptr, T, cr, cg, cb, cur);
+    (void)composite_front_to_back<P, false>(a, range, q, sA, sB, sC, nullptr, T, cr, cg, cb,
+                                            cur);
     composite_epilogue<MSE, P>(im, q, T, cr, cg, cb, cur, out_rgb, out_T, out_last, target, loss,
                                grad, s_red, gr, gg, gb);
 }
@@ -519,7 +529,7 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
         gb[k] = q.inside[k] ? gout[3 * pix + 2] : 0.f;
     }
     composite_back_to_front<GRAD_MODE, P, false>(a, range, q, sA, sB, sC, sG, &s_last, nullptr,
-                                                 T, lst, gr, gg, gb, grad);
+                                                 -1, T, lst, gr, gg, gb, grad);
 }
 
 // Forward with the fused MSE, then the backward of the same tile in the same
@@ -544,15 +554,16 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fused(
     const int2 range = a.ranges[tile];
     float T[P], cr[P], cg[P], cb[P], gr[P], gg[P], gb[P];
     int32_t cur[P];
-    composite_front_to_back<P, true>(a, range, q, sA, sB, sC, s_bits, T, cr, cg, cb, cur);
+    const int loaded =
+        composite_front_to_back<P, true>(a, range, q, sA, sB, sC, s_bits, T, cr, cg, cb, cur);
     composite_epilogue<true, P>(im, q, T, cr, cg, cb, cur, out_rgb, out_T, out_last, target,
                                 loss, grad_rgb, s_red, gr, gg, gb);
     if (range.y <= range.x) return;
     int32_t lst[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) lst[k] = q.inside[k] ? cur[k] : -1;
-    composite_back_to_front<GRAD_MODE, P, true>(a, range, q, sA, sB, sC, sG, &s_last, s_bits, T,
-                                                lst, gr, gg, gb, grad);
+    composite_back_to_front<GRAD_MODE, P, true>(a, range, q, sA, sB, sC, sG, &s_last, s_bits,
+                                                loaded, T, lst, gr, gg, gb, grad);
 }
 
 }  // namespace
