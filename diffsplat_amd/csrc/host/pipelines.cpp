@@ -201,7 +201,12 @@ void ImageFitter::stepOnce() {
     optimizer.step(stream);
 }
 
-void ImageFitter::setUseGraph(bool on) { use_graph = on; }
+// A captured step keeps the launch shapes and dispatch order it was captured
+// with, so (re)setting the mode drops it; the next step captures again.
+void ImageFitter::setUseGraph(bool on) {
+    use_graph = on;
+    dropGraph();
+}
 
 void ImageFitter::step() {
     ++iter_count;

@@ -160,7 +160,10 @@ public:
     void forwardBackward();
     void resetOptimizer();
     void losses(float *host);  // one per instance, of the last forward (synchronises)
-    void setUseGraph(bool on) { use_graph = on; }
+    void setUseGraph(bool on) {  // drops a captured step (see ImageFitter::setUseGraph)
+        use_graph = on;
+        dropGraph();
+    }
     void autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd, float *ms_bwd);
     int getIterCount() const { return iter_count; }
     hipStream_t getStream() const { return stream; }

@@ -131,8 +131,13 @@ struct ChunkLds {
 
 // Loads chunk c and ranks its keys stably within their digit: rank[j] is the
 // number of earlier keys of the chunk with the same digit. Leaves the digit
-// counts in L.run and their exclusive prefix in L.start. Ranking matches
-// the 8 digit bits across the wave with ballots, then orders the waves.
+// counts in L.run and their exclusive prefix in L.start. Wave w owns the
+// contiguous keys [w * kSortChunk / NW, (w + 1) * kSortChunk / NW) of the
+// chunk and walks them 64 at a time in order, matching the 8 digit bits
+// across the wave with ballots and keeping a running count per digit in its
+// own LDS row (lanes of one digit share one first lane, so the row's words
+// written in one step are distinct); the waves' counts are then scanned per
+// digit. Three barriers per chunk.
 template <typename K>
 __device__ __forceinline__ void chunk_rank(const K *__restrict__ kin,
                                            const uint32_t *__restrict__ vin, uint32_t base,
@@ -140,12 +145,13 @@ __device__ __forceinline__ void chunk_rank(const K *__restrict__ kin,
                                            uint32_t (&val)[kSortItems],
                                            uint32_t (&rank)[kSortItems], ChunkLds<K> &L) {
     constexpr int NW = ChunkLds<K>::NW;
+    constexpr int kWaveKeys = kSortChunk / NW;
     const int tid = threadIdx.x;
-    const int w = tid / kWave;
+    const int w = tid / kWave, lane = __lane_id();
     const uint64_t lt = lanemask_lt();
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
-        const uint32_t e = j * kSortThreads + tid;
+        const uint32_t e = (uint32_t)(w * kWaveKeys + j * kWave + lane);
         if (e < cnt) {
             key[j] = kin[base + e];
             val[j] = vin[base + e];
@@ -154,13 +160,12 @@ __device__ __forceinline__ void chunk_rank(const K *__restrict__ kin,
             val[j] = 0;
         }
     }
-    L.run[tid] = 0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) L.w[q][tid] = 0;
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
-#pragma unroll
-        for (int q = 0; q < NW; ++q) L.w[q][tid] = 0;
-        __syncthreads();
-        const bool valid = (uint32_t)(j * kSortThreads + tid) < cnt;
+        const bool valid = (uint32_t)(w * kWaveKeys + j * kWave + lane) < cnt;
         const uint32_t d = digit_of(key[j], shift);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -170,22 +175,25 @@ __device__ __forceinline__ void chunk_rank(const K *__restrict__ kin,
             peers &= on ? b : ~b;
         }
         const uint32_t r = (uint32_t)__popcll(peers & lt);
-        if (valid && r == 0) L.w[w][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {
-            uint32_t run = L.run[tid];
-#pragma unroll
-            for (int q = 0; q < NW; ++q) {
-                const uint32_t cq = L.w[q][tid];
-                L.w[q][tid] = run;
-                run += cq;
-            }
-            L.run[tid] = run;
-        }
-        __syncthreads();
-        rank[j] = valid ? L.w[w][d] + r : 0u;
-        __syncthreads();
+        uint32_t before = 0;
+        if (valid) before = L.w[w][d];
+        if (valid && r == 0) L.w[w][d] = before + (uint32_t)__popcll(peers);
+        rank[j] = before + r;  // among this wave's keys of digit d
     }
+    __syncthreads();
+    {  // per digit: the waves' first ranks and the chunk's count
+        uint32_t run = 0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+            const uint32_t cq = L.w[q][tid];
+            L.w[q][tid] = run;
+            run += cq;
+        }
+        L.run[tid] = run;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) rank[j] += L.w[w][digit_of(key[j], shift)];
     L.start[tid] = block_excl_scan<kSortThreads>(L.run[tid], L.tmp, nullptr);
 }
 
@@ -198,9 +206,11 @@ __device__ __forceinline__ void chunk_scatter(const K (&key)[kSortItems],
                                               int shift, ChunkLds<K> &L, K *__restrict__ kout,
                                               uint32_t *__restrict__ vout) {
     const int tid = threadIdx.x;
+    constexpr int kWaveKeys = kSortChunk / ChunkLds<K>::NW;
+    const int w = tid / kWave, lane = __lane_id();
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
-        if ((uint32_t)(j * kSortThreads + tid) < cnt) {
+        if ((uint32_t)(w * kWaveKeys + j * kWave + lane) < cnt) {
             const uint32_t p = L.start[digit_of(key[j], shift)] + rank[j];
             L.k[p] = key[j];
             L.v[p] = val[j];

@@ -430,7 +430,9 @@ ds_status ds_fitter_synchronize(ds_fitter *f);
 ds_status ds_fitter_download_image(ds_fitter *f, float *host_rgb);
 ds_status ds_fitter_get_raster(ds_fitter *f, ds_raster **r);
 ds_status ds_fitter_get_stream(ds_fitter *f, ds_stream_t *s);
-/* 1: capture one step into a HIP graph on first use and replay it. */
+/* 1: capture one step into a HIP graph on first use and replay it. Every
+ * call drops a captured step, so call it again after changing the
+ * rasterizer's launch shapes or dispatch order directly (ds_raster_set_*). */
 ds_status ds_fitter_set_graph(ds_fitter *f, int enable);
 /* Autotuner: times every compositing launch shape on the current parameters
  * (forward and backward only; parameters and Adam state are unchanged; `reps`
@@ -570,7 +572,8 @@ ds_status ds_multifit_forward_backward(ds_multifit *m);
 /* Loss of every instance from the last forward, host [n_inst]. Synchronises. */
 ds_status ds_multifit_losses(ds_multifit *m, float *losses);
 ds_status ds_multifit_synchronize(ds_multifit *m);
-/* 1: capture one step into a HIP graph on first use and replay it. */
+/* 1: capture one step into one HIP graph on first use and replay it. Every
+ * call drops a captured step (as ds_fitter_set_graph). */
 ds_status ds_multifit_set_graph(ds_multifit *m, int enable);
 /* As ds_fitter_autotune, over the compositing launches of all instances. */
 ds_status ds_multifit_autotune(ds_multifit *m, int reps, int *fwd_pix_per_thread,
