@@ -37,14 +37,17 @@ __device__ __forceinline__ int tile_of_block(const CompositeArgs &a) {
 // the reference (-1/4 ddx^2, -1/4 ddy^2, -2 b dist2 - 2 dx dy / det) are
 // -u^2, -w^2 and -2 u w (the last by u w = b dist2 + dx dy / det).
 struct AlphaEval {
-    float u, w, dist2, g, og, alpha;
+    float u, w, dist2, g, og;
+    // alpha = min(0.99, o g); alpha >= 1/255 exactly when o g >= 1/255
+    __device__ __forceinline__ float alpha() const { return fminf(kMaxAlpha, og); }
 };
 
-// Gaussian falloff and alpha of one splat at one pixel, without branches.
-// Forward and backward run this same code so their discrete decisions agree
-// bit for bit. Returns false when the splat is skipped (sigma < 0 or
-// alpha < 1/255); g, og and alpha are then meaningless (g may be inf) and
-// callers select them away rather than multiplying by zero.
+// Gaussian falloff of one splat at one pixel, without branches. Forward and
+// backward run this same code so their discrete decisions agree bit for bit.
+// Returns false when the splat is skipped (sigma < 0 or alpha < 1/255,
+// tested as o g < 1/255 since 0.99 > 1/255); g and og are then meaningless
+// (g may be inf) and callers select them away rather than multiplying by
+// zero.
 __device__ __forceinline__ bool eval_alpha(const float4 &A, const float4 &B, float px, float py,
                                            AlphaEval &e) {
     constexpr float kNegHalfLog2e = -0.72134752044448170f;  // -0.5 / ln 2
@@ -55,8 +58,7 @@ __device__ __forceinline__ bool eval_alpha(const float4 &A, const float4 &B, flo
     e.dist2 = fmaf(dx, e.u, dy * e.w);
     e.g = __builtin_amdgcn_exp2f(e.dist2 * kNegHalfLog2e);
     e.og = B.y * e.g;
-    e.alpha = fminf(kMaxAlpha, e.og);
-    return (e.dist2 >= 0.f) & (e.alpha >= kAlphaThres);  // sigma = dist2 / 2 >= 0
+    return (e.dist2 >= 0.f) & (e.og >= kAlphaThres);  // sigma = dist2 / 2 >= 0
 }
 
 __device__ __forceinline__ float pixel_ndc_x(int j, int W) {
@@ -250,7 +252,7 @@ __device__ __forceinline__ int composite_front_to_back(
                 for (int k = 0; k < P; ++k) {
                     // A skipped splat gets alpha 0, which leaves T and the
                     // colour exactly unchanged (T * (1 - 0) == T).
-                    const float alpha = ok[k] ? e[k].alpha : 0.f;
+                    const float alpha = ok[k] ? e[k].alpha() : 0.f;
                     const float vis = alpha * T[k];
                     cr[k] += C.x * vis;
                     cg[k] += C.y * vis;
@@ -376,7 +378,7 @@ __device__ __forceinline__ void composite_back_to_front(
             for (int f = 0; f < 9; ++f) v[f] = 0.f;
 #pragma unroll
             for (int k = 0; k < P; ++k) {
-                const float alpha = ok[k] ? e[k].alpha : 0.f;
+                const float alpha = ok[k] ? e[k].alpha() : 0.f;
                 const float ra = __builtin_amdgcn_rcpf(fmaxf(1.f - alpha, 1e-6f));
                 const float Tb = ok[k] ? T[k] * ra : T[k];  // transmittance in front of the splat
                 const float fac = alpha * Tb;

@@ -80,6 +80,25 @@ def test_invalid_arguments_are_rejected_before_any_device_call():
     assert L.ds_batch_create(nulls, 1, C.byref(h)) == 1
     assert L.ds_batch_step(None, 1) == 1
     assert L.ds_renderer_autotune(None, 3, None, None) == 1
+    # many fits as one, fused compositing, dispatch order
+    inst = (_lib.RasterInstance * 2)(_lib.RasterInstance(64, 48, 10), _lib.RasterInstance(0, 8, 1))
+    assert L.ds_raster_create_multi(inst, 0, 0, 0, 0, C.byref(h)) == 1
+    assert L.ds_raster_create_multi(inst, 2, 0, 0, 0, C.byref(h)) == 1  # width 0
+    assert b"size" in L.ds_last_error()
+    assert L.ds_raster_create_multi(inst, 1, 0, 0, 5, C.byref(h)) == 1  # grad_mode
+    assert L.ds_raster_splat_instances(None, None) == 1
+    mfi = (_lib.MultiFitInstance * 1)()
+    assert L.ds_multifit_create(mfi, 1, None, 0, C.byref(h)) == 1
+    fc2 = _lib.FitterConfig()
+    L.ds_fitter_default_config(32, 32, 4, C.byref(fc2))
+    assert L.ds_multifit_create(mfi, 0, C.byref(fc2), 0, C.byref(h)) == 1
+    mfi[0].width, mfi[0].height, mfi[0].count = 16, 16, -1
+    assert L.ds_multifit_create(mfi, 1, C.byref(fc2), 0, C.byref(h)) == 1
+    assert L.ds_multifit_step(None, 1) == 1 and L.ds_multifit_losses(None, None) == 1
+    assert L.ds_raster_fwd_bwd(None, None, None, None, None, None, None, None) == 1
+    assert L.ds_raster_composite_fwd_bwd(None, None, None, None, None, None, None, None) == 1
+    assert L.ds_raster_set_tile_order(None, 1) == 1
+    assert L.ds_activ_project_fwd_multi(None, None, 0, None, None, None) == 1
     with pytest.raises(ds.DiffsplatError):
         ds.check(L.ds_raster_create(None, C.byref(h)), "ds_raster_create")
 
