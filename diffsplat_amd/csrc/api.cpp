@@ -464,7 +464,7 @@ ds_status ds_fitter_step(ds_fitter *f, int n_steps) {
     clear_error();
     DS_REQUIRE(f && n_steps >= 0, "bad argument");
     DS_GUARD_BEGIN
-    for (int k = 0; k < n_steps; ++k) f->impl.step();
+    f->impl.steps(n_steps);
     return DS_OK;
     DS_GUARD_END
 }

@@ -125,7 +125,18 @@ ds_status launch_mse_fwd(const float *p, const float *t, int64_t npix, float *lo
                          hipStream_t s);
 ds_status launch_mse_bwd(const float *p, const float *t, int64_t npix, float *grad,
                          hipStream_t s);
+// With step_dev, *step_dev is incremented first (tick) and used; tick =
+// false uses it as it is (a kernel earlier in the step incremented it).
 ds_status launch_adam(const ds_adam_group *groups, int n_groups, const ds_adam_hparams &hp,
-                      int32_t step, int32_t *step_dev, hipStream_t s);
+                      int32_t step, int32_t *step_dev, hipStream_t s, bool tick = true);
+
+// One fitter step's rasterizer work with its prelude folded in (no
+// counterpart in the reference): the first kernel of the forward also zeroes
+// the Splat2D gradients grad_in (ImageFitter's zeroGrad) and increments the
+// Adam step counter *adam_tick (the optimizer then runs with tick = false);
+// then the forward with the fused MSE and the backward, as ds_raster_fwd_bwd.
+ds_status raster_step_fwd_bwd(ds_raster *r, const ds_splat2d &in, float *out_rgb,
+                              const float *target, float *loss, float *grad_rgb,
+                              const ds_splat2d &grad_in, int32_t *adam_tick, hipStream_t s);
 
 }  // namespace ds

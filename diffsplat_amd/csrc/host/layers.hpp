@@ -351,6 +351,16 @@ public:
     int getStepCount() const { return step_count; }
     const AdamParams &getParams() const { return hp; }
     int32_t *stepDevice() const { return step_dev.ptr; }
+    // The groups as the C-ABI takes them, for callers that launch the step
+    // themselves (then countStep() keeps the host count).
+    std::vector<ds_adam_group> table() {
+        if (!built) throw std::runtime_error("[diffsplat] Adam::table before reset");
+        std::vector<ds_adam_group> tab;
+        tab.reserve(groups.size());
+        for (auto &g : groups) tab.push_back({g.param, g.grad, g.m, g.v, g.n, g.lr, g.decay});
+        return tab;
+    }
+    void countStep() { ++step_count; }
 
 private:
     AdamParams hp;

@@ -1,6 +1,8 @@
 // ImageFitter / BatchFitter / SplatRenderer implementation (see pipelines.hpp).
 #include "pipelines.hpp"
 
+#include "../ds_internal.hpp"
+
 #include <algorithm>
 #include <chrono>
 
@@ -73,8 +75,10 @@ ImageFitter::~ImageFitter() {
 void ImageFitter::dropGraph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
-    graph_exec = nullptr;
-    graph = nullptr;
+    if (graph_exec_k) (void)hipGraphExecDestroy(graph_exec_k);
+    if (graph_k) (void)hipGraphDestroy(graph_k);
+    graph_exec = graph_exec_k = nullptr;
+    graph = graph_k = nullptr;
 }
 
 void ImageFitter::bindStream(hipStream_t s) {
@@ -195,6 +199,29 @@ void ImageFitter::initLayers() {
 }
 
 void ImageFitter::stepOnce() {
+    if (fused) {
+        // The reference's step (zeroGrad, forward, backward, Adam) with the
+        // zeroing of the Splat2D gradients and the Adam step counter folded
+        // into the rasterizer's first kernel, and the compositing forward,
+        // MSE and compositing backward in one kernel: 15 launches.
+        gp_layer.forward();
+        ds_splat2d in = gp_layer.getOutput().view();
+        ds_splat2d gi = ras_layer.getGradInput().view();
+        check(ds::raster_step_fwd_bwd(ras_layer.raster(), in, ras_layer.getOutput().ptr,
+                                      d_target_pixels.ptr, mse_layer.lossDevice(),
+                                      mse_layer.getGradInput().ptr, gi, optimizer.stepDevice(),
+                                      stream),
+              "ImageFitter::step");
+        gp_layer.backward();
+        std::vector<ds_adam_group> tab = optimizer.table();
+        const AdamParams &a = optimizer.getParams();
+        ds_adam_hparams h{a.beta1, a.beta2, a.epsilon};
+        check(ds::launch_adam(tab.data(), (int)tab.size(), h, 0, optimizer.stepDevice(), stream,
+                              false),
+              "ImageFitter::step");
+        optimizer.countStep();
+        return;
+    }
     pipeline.zeroGrad();
     pipeline.forward();
     pipeline.backward();
@@ -208,28 +235,38 @@ void ImageFitter::setUseGraph(bool on) {
     dropGraph();
 }
 
-void ImageFitter::step() {
-    ++iter_count;
-    if (!use_graph) {
-        stepOnce();
-        return;
-    }
-    if (!graph_exec) {
+// Replays nsteps captured steps (captured on first use).
+void ImageFitter::launchCaptured(int nsteps, hipGraph_t &g, hipGraphExec_t &ge) {
+    if (!ge) {
         hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal),
                   "hipStreamBeginCapture");
         try {
-            stepOnce();
+            for (int k = 0; k < nsteps; ++k) stepOnce();
         } catch (...) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(stream, &g);
-            if (g) (void)hipGraphDestroy(g);
+            hipGraph_t bad = nullptr;
+            (void)hipStreamEndCapture(stream, &bad);
+            if (bad) (void)hipGraphDestroy(bad);
             throw;
         }
-        hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
-        hip_check(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0),
-                  "hipGraphInstantiate");
+        hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+        hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
     }
-    hip_check(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch");
+    hip_check(hipGraphLaunch(ge, stream), "hipGraphLaunch");
+}
+
+void ImageFitter::step() { steps(1); }
+
+void ImageFitter::steps(int n) {
+    if (n <= 0) return;
+    iter_count += n;
+    if (!use_graph) {
+        for (int k = 0; k < n; ++k) stepOnce();
+        return;
+    }
+    // Each graph launch costs the GPU a few microseconds between graphs, so
+    // long runs replay a graph of kGraphSteps steps.
+    for (; n >= kGraphSteps; n -= kGraphSteps) launchCaptured(kGraphSteps, graph_k, graph_exec_k);
+    for (; n > 0; --n) launchCaptured(1, graph, graph_exec);
 }
 
 void ImageFitter::forwardOnly() {

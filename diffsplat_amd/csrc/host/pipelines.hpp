@@ -42,6 +42,7 @@ public:
     void initLayers();
 
     void step();          // zeroGrad, forward, backward, Adam
+    void steps(int n);    // n steps (graph mode: replays of a multi-step graph)
     void forwardOnly();   // forward (render + loss)
     void forwardBackward();  // zeroGrad, forward, backward (no Adam update)
     void setTargetDevice(const float *dev_rgb);  // device-to-device, on the fitter's stream
@@ -86,8 +87,11 @@ private:
     DeviceBuffer<float> d_target_pixels;
     int iter_count = 0;  // steps since the optimizer was last reset (graph replays too)
     bool use_graph = false;
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
+    // captured steps: one step, and kGraphSteps steps (fewer graph launches)
+    static constexpr int kGraphSteps = 16;
+    hipGraphExec_t graph_exec = nullptr, graph_exec_k = nullptr;
+    hipGraph_t graph = nullptr, graph_k = nullptr;
+    void launchCaptured(int nsteps, hipGraph_t &g, hipGraphExec_t &ge);
 };
 
 // Many independent fits stepped together (no counterpart in the reference):

@@ -126,7 +126,7 @@ ds_status launch_mse_bwd(const float *p, const float *t, int64_t npix, float *gr
 }
 
 ds_status launch_adam(const ds_adam_group *groups, int n_groups, const ds_adam_hparams &hp,
-                      int32_t step, int32_t *step_dev, hipStream_t s) {
+                      int32_t step, int32_t *step_dev, hipStream_t s, bool tick) {
     AdamTable tab{};
     int64_t total = 0, largest = 0;
     for (int k = 0; k < n_groups; ++k) {
@@ -141,7 +141,7 @@ ds_status launch_adam(const ds_adam_group *groups, int n_groups, const ds_adam_h
         if (groups[k].n > largest) largest = groups[k].n;
     }
     tab.count = n_groups;
-    if (step_dev) {
+    if (step_dev && tick) {
         hipLaunchKernelGGL(k_adam_tick, dim3(1), dim3(1), 0, s, step_dev);
         DS_CHECK_LAUNCH("adam_tick");
     }

@@ -86,15 +86,35 @@ __device__ __forceinline__ int tile_floor(float v, int hi) {
     return (int)f;
 }
 
+// Zeroes up to ten arrays of n floats in one launch.
+struct ZeroList {
+    float *p[10];
+};
+
+// Work a fitter step folds into the first kernel of its forward: zero the
+// ten Splat2D gradient arrays (n floats each) and increment the Adam step
+// counter (tick may be NULL).
+struct StepPrelude {
+    ZeroList z;
+    int n;
+    int32_t *tick;
+};
+
 // Start of a forward: device scalars, a new look-back epoch, zero tickets,
 // digit histograms and visible-count replicas, empty tile ranges, zero
 // losses (one per instance), zero segment sums of the tile sort (zero_words
-// words at zero).
+// words at zero); and a fitter step's prelude (pre.n = 0: none).
 __global__ __launch_bounds__(256) void k_begin(uint32_t *sc, uint32_t count, int2 *ranges,
                                                int ntiles, float *loss, int nloss,
-                                               uint32_t *ghist, uint32_t *zero,
-                                               int zero_words) {
+                                               uint32_t *ghist, uint32_t *zero, int zero_words,
+                                               StepPrelude pre) {
     const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i == 0 && pre.tick) *pre.tick += 1;
+    for (int t = i; t < pre.n; t += gridDim.x * 256) {
+#pragma unroll
+        for (int f = 0; f < 10; ++f)
+            if (pre.z.p[f]) pre.z.p[f][t] = 0.f;
+    }
     if (i < SC_EPOCH) sc[i] = (i == SC_COUNT) ? count : 0u;
     if (i == SC_EPOCH) {
         const uint32_t e = sc[SC_EPOCH] + 1u;
@@ -133,10 +153,6 @@ __device__ __forceinline__ void flush_histograms(const uint32_t (*s_h)[kRadix], 
     }
 }
 
-// Zeroes up to ten arrays of n floats in one launch.
-struct ZeroList {
-    float *p[10];
-};
 __global__ __launch_bounds__(256) void k_zero10(ZeroList z, int n) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
 #pragma unroll
@@ -792,7 +808,8 @@ static CompositeArgs composite_args(const ds_raster *r) {
 // *fused_grad.
 static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_rgb,
                                 const float *target, float *loss, float *grad, hipStream_t s,
-                                const ds_splat2d *fused_grad = nullptr) {
+                                const ds_splat2d *fused_grad = nullptr,
+                                StepPrelude pre = StepPrelude{}) {
     const int n = in.count;
     DS_REQUIRE(n >= 0 && n <= r->count_cap, "splat count exceeds the rasterizer's count_cap");
     DS_REQUIRE(r->ninst == 0 || n == r->count_cap,
@@ -811,12 +828,12 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
     const MultiInst *mi = multi ? r->mtab.ptr : nullptr;
     const uint16_t *si = multi ? r->sinst.ptr : nullptr;
     const int zero_words = one_pass_tiles ? kTileSegs * r->ntiles : 0;
-    const uint64_t begin_items =
-        std::max<uint64_t>({(uint64_t)r->ntiles, (uint64_t)zero_words / 4, kGhistWords / 4});
+    const uint64_t begin_items = std::max<uint64_t>(
+        {(uint64_t)r->ntiles, (uint64_t)zero_words / 4, kGhistWords / 4, (uint64_t)pre.n / 4});
     const int nloss = multi ? r->ninst : 1;
     hipLaunchKernelGGL(k_begin, dim3(grid_cap(begin_items, 256, 1024)), dim3(256), 0, s, sc,
                        (uint32_t)n, r->ranges.ptr, r->ntiles, loss, nloss, gh, r->tseg.ptr,
-                       zero_words);
+                       zero_words, pre);
     DS_CHECK_LAUNCH("begin");
     if (n > 0) {
         const bool one_kernel = (uint32_t)n <= kOneKernelMaxKeys;
@@ -943,6 +960,19 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
         DS_TRY(launch_composite_bwd(ca, r->bwd_ppt, r->grad_mode, r->T.ptr, r->last.ptr,
                                     grad, *fused_grad, s));
     return DS_OK;
+}
+
+ds_status raster_step_fwd_bwd(ds_raster *r, const ds_splat2d &in, float *out_rgb,
+                              const float *target, float *loss, float *grad_rgb,
+                              const ds_splat2d &grad_in, int32_t *adam_tick, hipStream_t s) {
+    DS_REQUIRE(r, "NULL raster");
+    DS_REQUIRE(target && loss && grad_rgb, "the fitter step needs the MSE target");
+    DS_REQUIRE(grad_in.count == in.count, "grad_in count differs from the input's");
+    const StepPrelude pre{ZeroList{{grad_in.pos_x, grad_in.pos_y, grad_in.pos_z, grad_in.cov_xx,
+                                    grad_in.cov_xy, grad_in.cov_yy, grad_in.color_r,
+                                    grad_in.color_g, grad_in.color_b, grad_in.opacity}},
+                          grad_in.count, adam_tick};
+    return raster_forward(r, in, out_rgb, target, loss, grad_rgb, s, &grad_in, pre);
 }
 
 static ds_status raster_backward(ds_raster *r, const ds_splat2d &in, const float *grad_rgb,
