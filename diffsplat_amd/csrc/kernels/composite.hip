@@ -32,12 +32,14 @@ __device__ __forceinline__ int tile_of_block(const CompositeArgs &a) {
     return a.order ? (int)a.order[blockIdx.x] : (int)blockIdx.x;
 }
 
-// u = (Sigma^-1 d)_x, w = (Sigma^-1 d)_y for d = splat centre - pixel, so that
-// dist2 = dx u + dy w, d dist2/d(x,y) = 2 (u, w), and the conic gradients of
-// the reference (-1/4 ddx^2, -1/4 ddy^2, -2 b dist2 - 2 dx dy / det) are
-// -u^2, -w^2 and -2 u w (the last by u w = b dist2 + dx dy / det).
+// u = k (Sigma^-1 d)_x, w = k (Sigma^-1 d)_y for d = splat centre - pixel
+// and the record's scale k, so that p = dx u + dy w = k dist2,
+// d dist2/d(x,y) = 2 (u, w) / k, and the covariance gradients of the
+// reference (-1/4 ddx^2, -1/4 ddy^2, -2 b dist2 - 2 dx dy / det) are
+// -u^2, -w^2 and -2 u w, each over k^2 (the last by u w = k^2 (b dist2 +
+// dx dy / det)).
 struct AlphaEval {
-    float u, w, dist2, g, og;
+    float u, w, p, g, og;
     // alpha = min(0.99, o g); alpha >= 1/255 exactly when o g >= 1/255
     __device__ __forceinline__ float alpha() const { return fminf(kMaxAlpha, og); }
 };
@@ -50,15 +52,14 @@ struct AlphaEval {
 // zero.
 __device__ __forceinline__ bool eval_alpha(const float4 &A, const float4 &B, float px, float py,
                                            AlphaEval &e) {
-    constexpr float kNegHalfLog2e = -0.72134752044448170f;  // -0.5 / ln 2
     const float dx = A.x - px;
     const float dy = A.y - py;
     e.u = fmaf(A.z, dx, A.w * dy);
     e.w = fmaf(A.w, dx, B.x * dy);
-    e.dist2 = fmaf(dx, e.u, dy * e.w);
-    e.g = __builtin_amdgcn_exp2f(e.dist2 * kNegHalfLog2e);
+    e.p = fmaf(dx, e.u, dy * e.w);  // k dist2 = -sigma / ln 2
+    e.g = __builtin_amdgcn_exp2f(e.p);
     e.og = B.y * e.g;
-    return (e.dist2 >= 0.f) & (e.og >= kAlphaThres);  // sigma = dist2 / 2 >= 0
+    return (e.p <= 0.f) & (e.og >= kAlphaThres);  // sigma >= 0
 }
 
 __device__ __forceinline__ float pixel_ndc_x(int j, int W) {
@@ -315,16 +316,19 @@ __device__ __forceinline__ void composite_back_to_front(
                            grad.color_r, grad.color_g, grad.color_b, grad.opacity};
     // The field this lane adds after the transposed reduction, its array,
     // and the factor that turns the wave sum into the gradient: the sums of
-    // the position and conic fields are taken with m = -2 dL/d dist2 (see
-    // below), so position x, y get -1 (d dist2/d(x, y) = 2 (u, w)), conic
-    // a and c get 1/2 and the symmetric conic b 1.
+    // the position and covariance fields are taken with m = -2 dL/d dist2
+    // (see below) and the record's scaled u, w, so position x, y get -1/k
+    // (d dist2/d(x, y) = 2 (u, w) / k), xx and yy get 1/(2 k^2) and the
+    // symmetric xy 1/k^2.
     const int lane = __lane_id();
     const int my_field = field_of_lane(lane);
     float *my_dst = nullptr;
 #pragma unroll
     for (int f = 0; f < 9; ++f) my_dst = my_field == f ? dst[f] : my_dst;
-    const float my_scale = (my_field == 0 || my_field == 1) ? -1.f
-                           : (my_field == 2 || my_field == 4) ? 0.5f
+    constexpr float kInvK = 1.f / kFalloffScale;
+    const float my_scale = (my_field == 0 || my_field == 1) ? -kInvK
+                           : (my_field == 2 || my_field == 4) ? 0.5f * kInvK * kInvK
+                           : my_field == 3                    ? kInvK * kInvK
                                                               : 1.f;
     const int end = min(range.y, block_last + 1);
     if (end <= range.x) return;  // workgroup-uniform

@@ -14,9 +14,12 @@ constexpr float kAlphaThres = 1.0f / 255.0f;
 constexpr float kTThres = 0.0001f;
 
 // Packed per-splat render record written by tile assignment (3 x float4):
-//   [0] = {x, y, conic_a, conic_b}   [1] = {conic_c, opacity, 1/det, splat id bits}
+//   [0] = {x, y, k a, k b}   [1] = {k c, opacity, 1/det, splat id bits}
 //   [2] = {r, g, b, 0}
-// conic = inverse 2D covariance: a = cyy/det, b = -cxy/det, c = cxx/det.
+// conic = inverse 2D covariance: a = cyy/det, b = -cxy/det, c = cxx/det,
+// stored scaled by k = -1/(2 ln 2), so that the falloff exp(-d^2/2) is
+// exp2 of the scaled quadratic form directly.
+constexpr float kFalloffScale = -0.72134752044448170f;  // -0.5 / ln 2
 
 // One fit instance of a multi-instance rasterizer: its image, its run of the
 // global tile ids and of the concatenated pixel arrays, 1 / (W H) for its loss.

@@ -228,8 +228,10 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
             ntiles[i] = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
             key = depth_bits(z);
             const float idet = 1.f / det;
-            rec[3 * i + 0] = make_float4(x, y, cyy * idet, -cxy * idet);
-            rec[3 * i + 1] = make_float4(cxx * idet, in.opacity[i], idet, __uint_as_float(i));
+            const float k = kFalloffScale;
+            rec[3 * i + 0] = make_float4(x, y, cyy * idet * k, -cxy * idet * k);
+            rec[3 * i + 1] =
+                make_float4(cxx * idet * k, in.opacity[i], idet, __uint_as_float(i));
             rec[3 * i + 2] = make_float4(in.color_r[i], in.color_g[i], in.color_b[i], 0.f);
         } else {
             ntiles[i] = 0;
