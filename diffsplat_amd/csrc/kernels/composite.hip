@@ -209,6 +209,13 @@ __device__ __forceinline__ int composite_front_to_back(
         cr[k] = cg[k] = cb[k] = 0.f;
         cur[k] = 0;
     }
+    // The wave's band of pixel centres (NDC): lane 0 holds its first column
+    // and top row, lane 63 its last column and bottom row.
+    const float wx0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.px), 0));
+    const float wx1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.px), kWave - 1));
+    const float wy1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.py[0]), 0));
+    const float wy0 =
+        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.py[P - 1]), kWave - 1));
     int loaded = -1;
     for (int b = range.x; b < range.y; b += NT) {
         bool live = false;
@@ -232,8 +239,26 @@ __device__ __forceinline__ int composite_front_to_back(
         // 64 splats at a time: one word of reach bits (NT is a multiple of 64)
         for (int t0 = 0; t0 < bs && !finished; t0 += 64) {
             const int t1 = min(bs, t0 + 64);
+            // Candidates: the splats whose reach box (record B.z, C.w: no
+            // pixel outside it can pass the alpha test) meets the band; the
+            // others are skipped without evaluation.
+            uint64_t cand;
+            {
+                const int t = t0 + __lane_id();
+                bool hit = false;
+                if (t < t1) {
+                    const float2 c = *(const float2 *)&sA[t];
+                    const float rx = sB[t].z, ry = sC[t].w;
+                    hit = (c.x - rx <= wx1) & (c.x + rx >= wx0) & (c.y - ry <= wy1) &
+                          (c.y + ry >= wy0);
+                }
+                cand = __ballot(hit);
+            }
             uint64_t reach = 0;
-            for (int t = t0; t < t1; ++t) {
+            while (cand) {
+                const int s = (int)__builtin_ctzll(cand);
+                cand &= cand - 1;
+                const int t = t0 + s;
                 const float4 A = sA[t], B = sB[t];
                 // Pass 1: which of this lane's pixels the splat reaches; a
                 // wave whose pixels it reaches none of skips the rest.
@@ -246,7 +271,7 @@ __device__ __forceinline__ int composite_front_to_back(
                     anyv |= ok[k];
                 }
                 if (__ballot(anyv) == 0) continue;
-                reach |= 1ull << (t - t0);
+                reach |= 1ull << s;
                 const float4 C = sC[t];
                 bool any = false;
 #pragma unroll
@@ -265,7 +290,7 @@ __device__ __forceinline__ int composite_front_to_back(
                 }
                 if (__ballot(any) == 0) {
                     finished = true;
-                    t = t1;
+                    break;
                 }
             }
             if constexpr (BITS) {

@@ -161,6 +161,28 @@ __global__ __launch_bounds__(256) void k_zero10(ZeroList z, int n) {
     }
 }
 
+// Half-extents (NDC, x and y) of a box holding every point where the
+// compositing's float evaluation of the record's falloff can pass the alpha
+// test. With Q' = -(qa, qb; qb, qc) (positive definite), a point d passes
+// when d^T Q' d <= t2 = log2(o / (1/255)) up to rounding: the float value of
+// p = -d^T Q' d (two fma chains) is within 12 eps (tr^2 / det) |p| of the
+// exact one (eps = 2^-24), exp2 and the opacity product within 2^-20
+// relative (1e-5 in t2). The box of the widened ellipse d^T Q' d <= tm has
+// half-extents sqrt(tm Q'yy / det), sqrt(tm Q'xx / det). Computed in double
+// and rounded outwards; a conic too ill-conditioned for the bound (or not
+// positive definite in float) gets an unbounded box, an opacity below the
+// threshold an empty one (-inf: no comparison passes).
+__device__ float2 reach_extent(float qa, float qb, float qc, float o) {
+    if (!(o >= kAlphaThres)) return make_float2(-INFINITY, -INFINITY);
+    const double a = -(double)qa, b = -(double)qb, c = -(double)qc;
+    const double det = a * c - b * b, tr = a + c;
+    const double slack = 16.0 * 0x1p-24 * (tr * tr / det);
+    if (!(a > 0.0 && det > 0.0 && slack < 0.5)) return make_float2(INFINITY, INFINITY);
+    const double tm = (log2((double)o / (double)kAlphaThres) + 1e-5) / (1.0 - slack);
+    const double rx = sqrt(tm * c / det), ry = sqrt(tm * a / det);
+    return make_float2((float)(rx * (1.0 + 1e-6) + 1e-6), (float)(ry * (1.0 + 1e-6) + 1e-6));
+}
+
 // Per splat: cull test and tile rectangle exactly as tileAssignKernel
 // (src/layers/tile_rasterize_layer.cu:76-125), with FP contraction off so
 // the decisions are reproducible on the host. With `hist` the four depth-byte
@@ -229,10 +251,12 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
             key = depth_bits(z);
             const float idet = 1.f / det;
             const float k = kFalloffScale;
-            rec[3 * i + 0] = make_float4(x, y, cyy * idet * k, -cxy * idet * k);
-            rec[3 * i + 1] =
-                make_float4(cxx * idet * k, in.opacity[i], idet, __uint_as_float(i));
-            rec[3 * i + 2] = make_float4(in.color_r[i], in.color_g[i], in.color_b[i], 0.f);
+            const float qa = cyy * idet * k, qb = -cxy * idet * k, qc = cxx * idet * k;
+            const float o = in.opacity[i];
+            const float2 ext = reach_extent(qa, qb, qc, o);
+            rec[3 * i + 0] = make_float4(x, y, qa, qb);
+            rec[3 * i + 1] = make_float4(qc, o, ext.x, __uint_as_float(i));
+            rec[3 * i + 2] = make_float4(in.color_r[i], in.color_g[i], in.color_b[i], ext.y);
         } else {
             ntiles[i] = 0;
         }
