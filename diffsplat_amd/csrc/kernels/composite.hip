@@ -370,76 +370,81 @@ __device__ __forceinline__ void composite_back_to_front(
             __syncthreads();
         }
         loaded = -1;  // only the forward's last batch is reused
-        uint64_t reach = 0;
-        int word = -1;
-        for (int t = min(bs - 1, wave_last - bb); t >= 0; --t) {
-            const int idx = bb + t;
+        // 64 slots at a time, last first (bb - range.x is a multiple of 64,
+        // so a chunk is one word of reach bits); the set bits of the chunk's
+        // mask are visited from the highest down.
+        const int tmax = min(bs - 1, wave_last - bb);
+        for (int c0 = tmax < 0 ? -64 : tmax & ~63; c0 >= 0; c0 -= 64) {
+            const int lim = tmax - c0;  // highest slot of the chunk in range
+            uint64_t mask = lim >= 63 ? ~0ull : (2ull << lim) - 1ull;
             if constexpr (BITS) {
-                const int qi = idx - range.x;
-                if (qi < kSkipSplats) {
-                    if ((qi >> 6) != word) {
-                        word = qi >> 6;
-                        const uint64_t v = s_bits[w][word];
-                        reach = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32))
-                                 << 32) |
-                                __builtin_amdgcn_readfirstlane((uint32_t)v);
-                    }
-                    if (!((reach >> (qi & 63)) & 1ull)) continue;
+                const int word = (bb - range.x + c0) >> 6;
+                if (word < kSkipWords) {
+                    const uint64_t v = s_bits[w][word];
+                    mask &= ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+                            __builtin_amdgcn_readfirstlane((uint32_t)v);
                 }
             }
-            const float4 A = sA[t], B = sB[t];
-            // Pass 1: which of this lane's pixels the splat reaches. A wave
-            // whose pixels it reaches none of skips the rest.
-            AlphaEval e[P];
-            bool ok[P];
-            bool anyv = false;
+            while (mask) {
+                const int sl = 63 - (int)__builtin_clzll(mask);
+                mask &= ~(1ull << sl);
+                const int t = c0 + sl;
+                const int idx = bb + t;
+                const float4 A = sA[t], B = sB[t];
+                // Pass 1: which of this lane's pixels the splat reaches. A wave
+                // whose pixels it reaches none of skips the rest.
+                AlphaEval e[P];
+                bool ok[P];
+                bool anyv = false;
 #pragma unroll
-            for (int k = 0; k < P; ++k) {
-                ok[k] = eval_alpha(A, B, q.px, q.py[k], e[k]) & (idx <= lst[k]);
-                anyv |= ok[k];
-            }
-            if (__ballot(anyv) == 0) continue;
-            // Pass 2, without branches: a skipped pixel gets alpha 0 and gate
-            // false, which adds exact zeros and keeps T and acc unchanged.
-            const float4 C = sC[t];
-            float v[9];
+                for (int k = 0; k < P; ++k) {
+                    ok[k] = eval_alpha(A, B, q.px, q.py[k], e[k]) & (idx <= lst[k]);
+                    anyv |= ok[k];
+                }
+                if (__ballot(anyv) == 0) continue;
+                // Pass 2, without branches: a skipped pixel gets alpha 0 and gate
+                // false, which adds exact zeros and keeps T and acc unchanged.
+                const float4 C = sC[t];
+                float v[9];
 #pragma unroll
-            for (int f = 0; f < 9; ++f) v[f] = 0.f;
+                for (int f = 0; f < 9; ++f) v[f] = 0.f;
 #pragma unroll
-            for (int k = 0; k < P; ++k) {
-                const float alpha = ok[k] ? e[k].alpha() : 0.f;
-                const float ra = __builtin_amdgcn_rcpf(fmaxf(1.f - alpha, 1e-6f));
-                const float Tb = ok[k] ? T[k] * ra : T[k];  // transmittance in front of the splat
-                const float fac = alpha * Tb;
-                v[5] = fmaf(fac, gr[k], v[5]);
-                v[6] = fmaf(fac, gg[k], v[6]);
-                v[7] = fmaf(fac, gb[k], v[7]);
-                const float cgk = fmaf(C.x, gr[k], fmaf(C.y, gg[k], C.z * gb[k]));
-                float v_alpha;
-                if constexpr (GRAD_MODE == DS_GRAD_REFERENCE)
-                    v_alpha = Tb * (cgk - acc[k]);
-                else
-                    v_alpha = Tb * cgk - ra * acc[k];
-                const bool gate = ok[k] & (e[k].og <= kMaxAlpha);  // alpha below its clamp
-                v[8] = fmaf(gate ? e[k].g : 0.f, v_alpha, v[8]);
-                // m = -2 dL/d dist2 (dL/d dist2 = -o g v_alpha / 2); the
-                // per-field factors are applied once, after the reduction
-                const float m = gate ? e[k].og * v_alpha : 0.f;
-                const float pu = m * e[k].u, pw = m * e[k].w;
-                v[0] += pu;
-                v[1] += pw;
-                v[2] = fmaf(pu, e[k].u, v[2]);
-                v[3] = fmaf(pu, e[k].w, v[3]);
-                v[4] = fmaf(pw, e[k].w, v[4]);
-                acc[k] = fmaf(fac, cgk, acc[k]);
-                T[k] = Tb;
-            }
-            const float sum = my_scale * wave_sum9_transposed(v, lane);
-            if (my_field >= 0) {  // nine lanes of row 0, one field each
-                if constexpr (NW == 1) {
-                    if (sum != 0.f) atomicAdd(my_dst + __float_as_uint(B.w), sum);
-                } else {
-                    atomicAdd(&sG[my_field][t], sum);
+                for (int k = 0; k < P; ++k) {
+                    const float alpha = ok[k] ? e[k].alpha() : 0.f;
+                    const float ra = __builtin_amdgcn_rcpf(fmaxf(1.f - alpha, 1e-6f));
+                    // transmittance in front of the splat
+                    const float Tb = ok[k] ? T[k] * ra : T[k];
+                    const float fac = alpha * Tb;
+                    v[5] = fmaf(fac, gr[k], v[5]);
+                    v[6] = fmaf(fac, gg[k], v[6]);
+                    v[7] = fmaf(fac, gb[k], v[7]);
+                    const float cgk = fmaf(C.x, gr[k], fmaf(C.y, gg[k], C.z * gb[k]));
+                    float v_alpha;
+                    if constexpr (GRAD_MODE == DS_GRAD_REFERENCE)
+                        v_alpha = Tb * (cgk - acc[k]);
+                    else
+                        v_alpha = Tb * cgk - ra * acc[k];
+                    const bool gate = ok[k] & (e[k].og <= kMaxAlpha);  // alpha below its clamp
+                    v[8] = fmaf(gate ? e[k].g : 0.f, v_alpha, v[8]);
+                    // m = -2 dL/d dist2 (dL/d dist2 = -o g v_alpha / 2); the
+                    // per-field factors are applied once, after the reduction
+                    const float m = gate ? e[k].og * v_alpha : 0.f;
+                    const float pu = m * e[k].u, pw = m * e[k].w;
+                    v[0] += pu;
+                    v[1] += pw;
+                    v[2] = fmaf(pu, e[k].u, v[2]);
+                    v[3] = fmaf(pu, e[k].w, v[3]);
+                    v[4] = fmaf(pw, e[k].w, v[4]);
+                    acc[k] = fmaf(fac, cgk, acc[k]);
+                    T[k] = Tb;
+                }
+                const float sum = my_scale * wave_sum9_transposed(v, lane);
+                if (my_field >= 0) {  // nine lanes of row 0, one field each
+                    if constexpr (NW == 1) {
+                        if (sum != 0.f) atomicAdd(my_dst + __float_as_uint(B.w), sum);
+                    } else {
+                        atomicAdd(&sG[my_field][t], sum);
+                    }
                 }
             }
         }

@@ -56,13 +56,35 @@ SCENES = {
     # name: (seed, n, W, H, ls_mean)
     "c0": (0, 4096, 256, 256, -3.0),
     "ragged": (5, 1500, 200, 120, -2.6),
+    "needles": (9, 2000, 200, 120, -2.8),
 }
+
+
+def scene_params(scene):
+    """Splats and camera of a named scene. "needles": half the splats
+    stretched 7.4:1 in a random orientation and 40% with opacities between 1
+    and 3 times the 1/255 alpha threshold, so that many pixels sit near the
+    edge of a splat's alpha-threshold ellipse (the compositing forward skips
+    the splats whose box around that ellipse misses a wave's pixels)."""
+    seed, n, W, H, ls = SCENES[scene]
+    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    if scene == "needles":
+        rng = np.random.default_rng(seed)
+        slab = params.reshape(14, n)
+        thin = rng.random(n) < 0.5
+        slab[3, thin] += 1.0
+        slab[4:6, thin] -= 1.0
+        faint = rng.random(n) < 0.4
+        o = rng.uniform(1.0, 3.0, n) / 255.0
+        slab[13, faint] = np.log(o / (1.0 - o))[faint]
+        params = slab.reshape(-1).astype(np.float32)
+    return params, view, proj, cam
 
 
 @pytest.mark.parametrize("deg", [0, 1, 2, 3])
 def test_activation_projection_forward(deg):
     seed, n, W, H, ls = SCENES["c0"]
-    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    params, view, proj, cam = scene_params("c0")
     cam = np.array([0.3, -0.2, 0.5], np.float32)
     params = with_sh(params, n, deg, seed) if deg else params
     st = Stages(params, n, deg, W, H, view, proj, cam)
@@ -120,7 +142,7 @@ def test_tile_sort_with_screen_sized_splats(W, H, cap):
 
 def test_isect_cap_drops_farthest_pairs():
     seed, n, W, H, ls = SCENES["c0"]
-    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    params, view, proj, cam = scene_params("c0")
     cap = 5000
     st = Stages(params, n, 0, W, H, view, proj, cam, isect_cap=cap)
     s2 = st.activ_project_fwd()
@@ -165,10 +187,10 @@ def test_sort_pairs_u64(n, bits):
 
 
 @pytest.mark.parametrize("ppt", [1, 2, 4])
-@pytest.mark.parametrize("scene", ["c0", "ragged"])
+@pytest.mark.parametrize("scene", ["c0", "ragged", "needles"])
 def test_composite_forward(scene, ppt):
     seed, n, W, H, ls = SCENES[scene]
-    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    params, view, proj, cam = scene_params(scene)
     st = Stages(params, n, 0, W, H, view, proj, cam, pix_per_thread=ppt)
     s2 = st.activ_project_fwd()
     out = st.raster_fwd()
@@ -184,10 +206,10 @@ def test_composite_forward(scene, ppt):
 
 @pytest.mark.parametrize("ppt", [1, 2, 4])
 @pytest.mark.parametrize("grad_mode", [0, 1])
-@pytest.mark.parametrize("scene", ["c0", "ragged"])
+@pytest.mark.parametrize("scene", ["c0", "ragged", "needles"])
 def test_composite_backward(scene, grad_mode, ppt):
     seed, n, W, H, ls = SCENES[scene]
-    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    params, view, proj, cam = scene_params(scene)
     target = ds.target_image(seed, W, H, 32)
     st = Stages(params, n, 0, W, H, view, proj, cam, grad_mode=grad_mode, pix_per_thread=ppt)
     s2 = st.activ_project_fwd()
@@ -216,7 +238,7 @@ def test_tile_dispatch_order_leaves_results_unchanged(scene):
     splat gradients up to the order of float atomics; switching the order
     between a forward and its backward is safe."""
     seed, n, W, H, ls = SCENES[scene]
-    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    params, view, proj, cam = scene_params(scene)
     target = ds.target_image(seed, W, H, 32)
     outs = []
     for fwd_order, bwd_order in ((1, 1), (0, 0), (0, 1), (1, 0)):
@@ -237,7 +259,7 @@ def test_tile_dispatch_order_leaves_results_unchanged(scene):
 
 @pytest.mark.parametrize("fwd_ppt,bwd_ppt,grad_mode", [(1, 1, 0), (2, 2, 1), (4, 4, 0),
                                                         (1, 2, 0)])
-@pytest.mark.parametrize("scene", ["c0", "ragged"])
+@pytest.mark.parametrize("scene", ["c0", "ragged", "needles"])
 def test_fused_forward_backward_matches_separate_calls(scene, fwd_ppt, bwd_ppt, grad_mode):
     """ds_raster_fwd_bwd (one compositing kernel per tile for forward and
     backward when the launch shapes are equal, with the backward skipping the
@@ -245,7 +267,7 @@ def test_fused_forward_backward_matches_separate_calls(scene, fwd_ppt, bwd_ppt, 
     identical image, transmittance, last ids and dL/dC, loss and gradients
     up to the order of float atomics; and against the FP64 oracle."""
     seed, n, W, H, ls = SCENES[scene]
-    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    params, view, proj, cam = scene_params(scene)
     target = ds.target_image(seed, W, H, 32)
     st = Stages(params, n, 0, W, H, view, proj, cam, grad_mode=grad_mode)
     ds.check(ds.lib().ds_raster_set_tuning(st.ras.h, fwd_ppt, bwd_ppt), "tuning")
@@ -300,7 +322,7 @@ def test_heavy_first_order_is_a_sorted_permutation(W, H, n):
 @pytest.mark.parametrize("deg", [0, 3])
 def test_projection_activation_backward(deg, grad_mode):
     seed, n, W, H, ls = SCENES["c0"]
-    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    params, view, proj, cam = scene_params("c0")
     cam = np.array([0.3, -0.2, 0.5], np.float32)
     params = with_sh(params, n, deg, seed) if deg else params
     st = Stages(params, n, deg, W, H, view, proj, cam, grad_mode=grad_mode)
@@ -370,7 +392,7 @@ def test_mse_and_adam():
 def test_fitter_step_matches_oracle_chain(fused, grad_mode):
     """One full ImageFitter step: gradients and post-Adam parameters."""
     seed, n, W, H, ls = SCENES["c0"]
-    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    params, view, proj, cam = scene_params("c0")
     target = ds.target_image(seed, W, H, 32)
     lr = {"lr_pos": 1e-2, "lr_sh_dc": 1e-2, "lr_scale": 5e-3}
     fit = ds.ImageFitter(W, H, n, view=view, proj=proj, cam_pos=cam, lr=lr, fused=fused,
