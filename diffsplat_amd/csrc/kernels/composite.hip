@@ -179,13 +179,12 @@ __device__ __forceinline__ TilePixels<P> tile_pixels(const TileImage &im, int ti
     return q;
 }
 
-// Per wave, one bit per splat of the tile's list for its first kSkipSplats
-// splats: did the forward find a pixel of this wave that the splat reaches
+// Per wave, one bit per splat of the tile's list for its first 64 *
+// kSkipWords splats: did the forward find a pixel of this wave that the splat reaches
 // (alpha >= 1/255, pixel not yet finished)? The backward's test for the
 // same (wave, splat) is the same test (a pixel's last contributor is where
 // it finished), so it skips the cleared bits without evaluating them.
 constexpr int kSkipWords = 64;
-constexpr int kSkipSplats = 64 * kSkipWords;
 
 // Front to back over the tile's list (compositeFwdKernel). With BITS the
 // per-wave reach bits are recorded in s_bits. Every exit is wave-uniform (a

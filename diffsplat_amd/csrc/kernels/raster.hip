@@ -53,9 +53,8 @@ enum {
     SC_NUM = 32
 };
 
-// Look-back slots (and digit-histogram rows): the four depth passes, the
-// pair-offset scan.
-enum { LB_DEPTH = 0, LB_SCAN = 4, LB_SLOTS = 8 };
+// Look-back slots (and digit-histogram rows): the four depth passes.
+enum { LB_DEPTH = 0, LB_SLOTS = 8 };
 constexpr int kHistRows = 4;  // digit histograms of the depth passes
 // Largest splat count sorted with one-kernel passes (64 chunks).
 constexpr uint32_t kOneKernelMaxKeys = 64u * kSortChunk;
@@ -70,7 +69,12 @@ constexpr int kHistRepWords = kHistRows * kRadix;
 constexpr int kVisReps = 16;
 constexpr int kVisStride = 32;
 constexpr int kVisOffset = kHistReps * kHistRepWords;
-constexpr int kGhistWords = kVisOffset + kVisReps * kVisStride;
+// kHistReps copies of 256 64-bit sums: per top depth byte, the tile count of
+// the splats with that byte (the last depth pass turns them into the pair
+// offsets). 64-bit aligned: kWhistOffset is even.
+constexpr int kWhistOffset = kVisOffset + kVisReps * kVisStride;
+constexpr int kGhistWords = kWhistOffset + 2 * kHistReps * kRadix;
+static_assert(kWhistOffset % 2 == 0, "64-bit sums");
 
 __device__ __forceinline__ uint32_t depth_bits(float z) {
     uint32_t u = __float_as_uint(z);
@@ -143,6 +147,26 @@ __device__ __forceinline__ void lds_count(uint32_t *h, uint32_t d) {
     }
 }
 
+// Adds w to bin d of an LDS histogram for every lane of the wave (all lanes
+// active). When the lanes share the bin (the top depth byte of a run of
+// splats) one lane adds the wave's sum.
+__device__ __forceinline__ void lds_add_weight(uint32_t *h, uint32_t d, uint32_t w) {
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    if (__ballot(d == d0) == __ballot(true)) {
+        uint32_t sum = w;
+#pragma unroll
+        for (int m = 1; m < kWave; m <<= 1) sum += __shfl_xor(sum, m, kWave);
+        if (__lane_id() == 0 && sum) atomicAdd(&h[d0], sum);
+    } else if (w) {
+        atomicAdd(&h[d], w);
+    }
+}
+
+// Copy r of the per-top-byte tile-count sums in ghist.
+__device__ __forceinline__ unsigned long long *whist_row(uint32_t *ghist, int r) {
+    return reinterpret_cast<unsigned long long *>(ghist + kWhistOffset) + (size_t)r * kRadix;
+}
+
 // Adds a 256-thread workgroup's LDS digit histograms (rows x 256 bins; row r
 // counts bits [8r, 8r + 8) of the keys) to the global rows.
 __device__ __forceinline__ void flush_histograms(const uint32_t (*s_h)[kRadix], int rows,
@@ -202,16 +226,19 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
                                               const uint16_t *__restrict__ sinst) {
 #pragma clang fp contract(off)
     __shared__ uint32_t s_h[4][kRadix];  // the four depth-byte histograms
+    __shared__ uint32_t s_w[kRadix];     // tile counts per top depth byte
     __shared__ uint32_t s_vis;
     if (hist) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) s_h[r][threadIdx.x] = 0u;
+        s_w[threadIdx.x] = 0u;
     }
     if (threadIdx.x == 0) s_vis = 0u;
     __syncthreads();
     const int i = blockIdx.x * 256 + threadIdx.x;
     bool visible = false;
     uint32_t key = 0xFFFFFFFFu;  // culled splats sort last
+    uint32_t nt = 0;             // tiles of the splat
     if (i < n) {
         if constexpr (MULTI) {
             const MultiInst I = mi[sinst[i]];
@@ -247,7 +274,8 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
         if (visible) {
             rect[i] = make_uint2((uint32_t)tx0 | ((uint32_t)tx1 << 16),
                                  (uint32_t)ty0 | ((uint32_t)ty1 << 16));
-            ntiles[i] = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
+            nt = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
+            ntiles[i] = nt;
             key = depth_bits(z);
             const float idet = 1.f / det;
             const float k = kFalloffScale;
@@ -266,12 +294,19 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
             for (int r = 0; r < 4; ++r) lds_count(s_h[r], (key >> (8 * r)) & 0xFFu);
         }
     }
+    if (hist) lds_add_weight(s_w, key >> 24, nt);
     const uint64_t vis = __ballot(visible);
     if (__lane_id() == 0 && vis) atomicAdd(&s_vis, (uint32_t)__popcll(vis));
     __syncthreads();
     if (threadIdx.x == 0 && s_vis)
         atomicAdd(&ghist[kVisOffset + (blockIdx.x % kVisReps) * kVisStride], s_vis);
-    if (hist) flush_histograms(s_h, 4, ghist + (blockIdx.x % kHistReps) * kHistRepWords);
+    if (hist) {
+        flush_histograms(s_h, 4, ghist + (blockIdx.x % kHistReps) * kHistRepWords);
+        const uint32_t wsum = s_w[threadIdx.x];
+        if (wsum)
+            atomicAdd(whist_row(ghist, blockIdx.x % kHistReps) + threadIdx.x,
+                      (unsigned long long)wsum);
+    }
 }
 
 // Writes the pairs of the splats, taken in depth order, each splat's tiles in
@@ -764,7 +799,7 @@ struct ds_raster {
     // of the depth passes and of the pair-offset scan, and the per-chunk
     // histograms and scan partials of the tile passes
     ds::DevBuf<uint32_t> scalars, ghist, hist, part;
-    ds::DevBuf<uint64_t> lb_depth, lb_scan;
+    ds::DevBuf<uint64_t> lb_depth;
     // results of the last forward
     bool order_valid = false;  // `order` was computed by the last forward
     const uint32_t *flat = nullptr;
@@ -875,35 +910,47 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
                                mi, si);
         DS_CHECK_LAUNCH("prep");
         // Depth sort of splats: four 8-bit LSD passes over the 32 depth bits,
-        // then the pair offsets in depth order (total and capped count). Up
-        // to kOneKernelMaxKeys splats each pass and the scan are one kernel
-        // with look-back (launches dominate there); above, reduce-then-scan
-        // passes win, as the look-back traffic grows with the chunk count.
+        // and the pair offsets in depth order (total and capped count). Up
+        // to kOneKernelMaxKeys splats each pass is one kernel with look-back
+        // (launches dominate there) and the last pass also writes the
+        // offsets; above, reduce-then-scan passes and a separate scan win, as
+        // the look-back traffic grows with the chunk count.
         uint32_t *ka = r->dkey_a.ptr, *kb = r->dkey_b.ptr, *va = r->didx_a.ptr,
                  *vb = r->didx_b.ptr;
         const size_t dstride = lookback_words((uint64_t)r->count_cap, kRadix);
-        for (int pass = 0; pass < 4; ++pass) {
-            if (one_kernel)
-                DS_TRY(onesweep_pass(ka, va, kb, vb, sc + SC_COUNT, (uint32_t)n, 8 * pass,
-                                     gh + (LB_DEPTH + pass) * kRadix, kHistReps,
-                                     kHistRepWords,
-                                     slot(r->lb_depth.ptr + pass * dstride, LB_DEPTH + pass), s));
-            else
-                DS_TRY(radix_pass<uint32_t>(ka, va, kb, vb, sc + SC_COUNT, (uint32_t)n, 8 * pass,
-                                            r->hist.ptr, r->part.ptr, s));
-            std::swap(ka, kb);
-            std::swap(va, vb);
-        }
-        const uint32_t *order = va;
         const uint64_t max_chunks = ((uint64_t)r->isect_cap + kTileChunk - 1) / kTileChunk;
         static_assert(kTileChunk == 1 << 12, "chunk heads use shift 12");
         static_assert(kEmitChunkShift == 12, "one chunk size for both emissions");
         const ScanHeads heads{r->tfirst.ptr, 12u, (uint32_t)max_chunks};
-        if (one_kernel)
-            DS_TRY(scan_chained(r->ntiles_of.ptr, order, r->pair_off.ptr, sc + SC_COUNT,
-                                (uint32_t)n, sc + SC_WANTED, sc + SC_NISECT, r->isect_cap,
-                                slot(r->lb_scan.ptr, LB_SCAN), s, heads));
-        else
+        for (int pass = 0; pass < 4; ++pass) {
+            if (one_kernel) {
+                // the last pass also writes the pair offsets in depth order
+                const PairOffsets po =
+                    pass == 3 ? PairOffsets{r->ntiles_of.ptr,
+                                            (const unsigned long long *)(gh + kWhistOffset),
+                                            kHistReps,
+                                            r->pair_off.ptr,
+                                            r->lb_depth.ptr + 4 * dstride,
+                                            sc + SC_WANTED,
+                                            sc + SC_NISECT,
+                                            r->isect_cap,
+                                            heads}
+                              : PairOffsets{nullptr, nullptr, 0, nullptr, nullptr,
+                                            nullptr, nullptr, 0, ScanHeads{nullptr, 0, 0}};
+                DS_TRY(onesweep_pass(ka, va, kb, vb, sc + SC_COUNT, (uint32_t)n, 8 * pass,
+                                     gh + (LB_DEPTH + pass) * kRadix, kHistReps,
+                                     kHistRepWords,
+                                     slot(r->lb_depth.ptr + pass * dstride, LB_DEPTH + pass), s,
+                                     po));
+            } else {
+                DS_TRY(radix_pass<uint32_t>(ka, va, kb, vb, sc + SC_COUNT, (uint32_t)n, 8 * pass,
+                                            r->hist.ptr, r->part.ptr, s));
+            }
+            std::swap(ka, kb);
+            std::swap(va, vb);
+        }
+        const uint32_t *order = va;
+        if (!one_kernel)
             DS_TRY(scan_exclusive_u32(r->ntiles_of.ptr, order, r->pair_off.ptr,
                                       ScanLen{sc + SC_COUNT, 1, 1}, (uint32_t)n, r->part.ptr,
                                       sc + SC_WANTED, sc + SC_NISECT, r->isect_cap, s, heads));
@@ -1038,11 +1085,10 @@ static ds_status raster_finish_create(ds_raster *r, ds_status st, ds_raster **ou
     if (st == DS_OK) st = r->tval_b.alloc(cap);
     if (st == DS_OK && r->keep_keys) st = r->keys64.alloc(cap);
     // look-back status words start at zero: epoch 0 is never used
-    const size_t lbd = 4 * lookback_words(nc, kRadix), lbs = lookback_words(nc, 1);
+    // four depth passes and the last pass's pair-offset sums
+    const size_t lbd = 5 * lookback_words(nc, kRadix);
     if (st == DS_OK) st = r->lb_depth.alloc(lbd);
     if (st == DS_OK) st = fill_u32((uint32_t *)r->lb_depth.ptr, 0u, 2 * lbd, nullptr);
-    if (st == DS_OK) st = r->lb_scan.alloc(lbs);
-    if (st == DS_OK) st = fill_u32((uint32_t *)r->lb_scan.ptr, 0u, 2 * lbs, nullptr);
     if (st == DS_OK) st = r->ghist.alloc((size_t)kGhistWords);
     if (st == DS_OK)
         st = r->hist.alloc(sort_hist_words(std::max<uint64_t>((uint64_t)cap, (uint64_t)nc)));

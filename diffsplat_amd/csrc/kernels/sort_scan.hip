@@ -314,13 +314,62 @@ __device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
     return (uint32_t)min((uint64_t)a + b, (uint64_t)kLookbackMaxCount);
 }
 
-// One stable radix pass in one kernel (see LookbackSlot).
+// LDS of the pair-offset side output: the chunk's weights in digit order,
+// then their exclusive scan (entry kSortChunk: the chunk's total), and per
+// digit the output offset of the chunk's first key of that digit.
+struct OffsetLds {
+    uint32_t w[kSortChunk];
+    uint32_t s[kSortChunk + 1];
+    uint32_t wb[kRadix];
+    unsigned long long tmp[kSortThreads / kWave];
+};
+template <bool ON>
+struct OffsetLdsIf {
+    OffsetLds v;
+};
+template <>
+struct OffsetLdsIf<false> {};
+
+// Exclusive scan of one 64-bit value per thread over a 256-thread block.
+__device__ __forceinline__ unsigned long long block_excl_scan64(unsigned long long v,
+                                                                unsigned long long *s_tmp,
+                                                                unsigned long long *total) {
+    constexpr int NW = kSortThreads / kWave;
+    const int w = threadIdx.x / kWave, lane = __lane_id();
+    unsigned long long inc = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const unsigned long long y = __shfl_up(inc, d, kWave);
+        if (lane >= d) inc += y;
+    }
+    if (lane == kWave - 1) s_tmp[w] = inc;
+    __syncthreads();
+    unsigned long long base = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        base += (k < w) ? s_tmp[k] : 0ull;
+        tot += s_tmp[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return base + inc - v;
+}
+
+// One stable radix pass in one kernel (see LookbackSlot). With OFF it also
+// writes the exclusive scan of weight[value] in the output order (see
+// PairOffsets): a key's offset is the weight of all keys of smaller digits
+// (from whist) plus that of the keys of its digit in earlier chunks (a
+// second look-back over per-digit weight sums) plus that of the keys of its
+// digit before it in its chunk (a scan of the chunk's weights in digit
+// order, in LDS).
+template <bool OFF>
 __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, const uint32_t *n_dev, int shift,
     const uint32_t *__restrict__ ghist, int hist_reps, uint32_t hist_rep_stride,
-    LookbackSlot lb) {
+    LookbackSlot lb, PairOffsets po) {
     __shared__ ChunkLds<uint32_t> L;
+    __shared__ OffsetLdsIf<OFF> O;
     const uint32_t n = *n_dev;
     const uint32_t nch = (n + kSortChunk - 1) / kSortChunk;
     const uint32_t epoch = *lb.epoch;
@@ -329,10 +378,23 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     // one memory wait.
     uint32_t count = 0;
     for (int r = 0; r < hist_reps; ++r) count += ghist[(size_t)r * hist_rep_stride + tid];
+    unsigned long long wdig = 0;  // weight of all keys of digit tid
+    if constexpr (OFF)
+        for (int r = 0; r < po.whist_reps; ++r) wdig += po.whist[(size_t)r * kRadix + tid];
     if (tid == 0) L.ticket = atomicAdd(lb.ticket, 1u);
     __syncthreads();
     // first position of each digit in the output
     const uint32_t dbase = block_excl_scan<kSortThreads>(count, L.tmp, nullptr);
+    unsigned long long wbase = 0;  // weight of all keys of smaller digits
+    if constexpr (OFF) {
+        unsigned long long wtot;
+        wbase = block_excl_scan64(wdig, O.v.tmp, &wtot);
+        if (blockIdx.x == 0 && tid == 0) {
+            const uint32_t total = (uint32_t)min(wtot, (unsigned long long)kLookbackMaxCount);
+            if (po.total_out) *po.total_out = total;
+            if (po.clamped_out) *po.clamped_out = min(total, po.cap);
+        }
+    }
     for (bool first = true;; first = false) {
         if (!first) {
             if (tid == 0) L.ticket = atomicAdd(lb.ticket, 1u);
@@ -347,82 +409,90 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
         uint64_t *col = lb.status + (size_t)c * kRadix + tid;
         const uint32_t mine = L.run[tid];
         uint32_t excl = 0;
-        if (c == 0) {
-            lb_publish(col, epoch, kLbPrefix, mine);
-        } else {
-            lb_publish(col, epoch, kLbAggregate, mine);
-            excl = lb_exclusive(lb.status + tid, kRadix, c, epoch, lb.err);
-            lb_publish(col, epoch, kLbPrefix, excl + mine);
-        }
-        L.gb[tid] = dbase + excl;
-        __syncthreads();
-        chunk_scatter(key, val, rank, cnt, shift, L, kout, vout);
-    }
-}
-
-// Exclusive scan in one kernel: chunks of kScanChunk elements, one status
-// word per chunk. Prefixes and the total saturate at kLookbackMaxCount
-// (2^30 - 1); the chunk sums themselves must stay below 2^32.
-__global__ __launch_bounds__(kScanThreads) void k_scan_chained(
-    const uint32_t *__restrict__ in, const uint32_t *__restrict__ gidx, const uint32_t *n_dev,
-    uint32_t *__restrict__ out, uint32_t *total_out, uint32_t *clamped_out, uint32_t cap,
-    LookbackSlot lb, ScanHeads H) {
-    __shared__ uint32_t s_tmp[kScanThreads / kWave];
-    __shared__ uint32_t s_ticket, s_excl;
-    const uint32_t n = *n_dev;
-    const uint32_t nch = (n + kScanChunk - 1) / kScanChunk;
-    const uint32_t epoch = *lb.epoch;
-    const int tid = threadIdx.x;
-    if (nch == 0) {
-        if (blockIdx.x == 0 && tid == 0) {
-            if (total_out) *total_out = 0u;
-            if (clamped_out) *clamped_out = 0u;
-        }
-        return;
-    }
-    for (;;) {
-        if (tid == 0) s_ticket = atomicAdd(lb.ticket, 1u);
-        __syncthreads();
-        const uint32_t c = s_ticket;
-        if (c >= nch) break;  // uniform
-        const uint32_t base = c * kScanChunk + tid * kScanItems;
-        uint32_t v[kScanItems];
-        uint32_t sum = 0;
-#pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
-            const uint32_t e = base + j;
-            v[j] = (e < n) ? (gidx ? in[gidx[e]] : in[e]) : 0u;
-            sum += v[j];
-        }
-        uint32_t tot;
-        const uint32_t run = block_excl_scan<kScanThreads>(sum, s_tmp, &tot);
-        if (tid == 0) {
-            const uint32_t agg = min(tot, kLookbackMaxCount);
-            uint32_t excl = 0;
+        if constexpr (!OFF) {
             if (c == 0) {
-                lb_publish(lb.status, epoch, kLbPrefix, agg);
+                lb_publish(col, epoch, kLbPrefix, mine);
             } else {
-                lb_publish(lb.status + c, epoch, kLbAggregate, agg);
-                excl = lb_exclusive(lb.status, 1, c, epoch, lb.err);
-                lb_publish(lb.status + c, epoch, kLbPrefix, sat_add(excl, agg));
+                lb_publish(col, epoch, kLbAggregate, mine);
+                excl = lb_exclusive(lb.status + tid, kRadix, c, epoch, lb.err);
+                lb_publish(col, epoch, kLbPrefix, excl + mine);
             }
-            s_excl = excl;
-            if (c == nch - 1) {
-                const uint32_t total = sat_add(excl, agg);
-                if (total_out) *total_out = total;
-                if (clamped_out) *clamped_out = min(total, cap);
-            }
-        }
-        __syncthreads();
-        uint32_t acc = sat_add(s_excl, run);
+            L.gb[tid] = dbase + excl;
+            __syncthreads();
+            chunk_scatter(key, val, rank, cnt, shift, L, kout, vout);
+        } else {
+            OffsetLds &Q = O.v;
+            if (c == 0)
+                lb_publish(col, epoch, kLbPrefix, mine);
+            else
+                lb_publish(col, epoch, kLbAggregate, mine);
+            // the chunk's keys, values and weights in digit order
+            constexpr int kWaveKeys = kSortChunk / ChunkLds<uint32_t>::NW;
+            const int w = tid / kWave, lane = __lane_id();
+            uint32_t wt[kSortItems];
 #pragma unroll
-        for (int j = 0; j < kScanItems; ++j) {
-            const uint32_t e = base + j;
-            if (e < n) out[e] = acc;
-            mark_heads(H, e, acc, v[j]);
-            acc = sat_add(acc, v[j]);
+            for (int j = 0; j < kSortItems; ++j) {
+                const bool valid = (uint32_t)(w * kWaveKeys + j * kWave + lane) < cnt;
+                wt[j] = valid ? po.weight[val[j]] : 0u;
+            }
+            __syncthreads();  // every digit's L.start is written
+#pragma unroll
+            for (int j = 0; j < kSortItems; ++j) {
+                if ((uint32_t)(w * kWaveKeys + j * kWave + lane) < cnt) {
+                    const uint32_t p = L.start[digit_of(key[j], shift)] + rank[j];
+                    L.k[p] = key[j];
+                    L.v[p] = val[j];
+                    Q.w[p] = wt[j];
+                }
+            }
+            __syncthreads();
+            {  // exclusive scan of the weights over the digit order
+                uint32_t x[kSortItems], sum = 0;
+#pragma unroll
+                for (int j = 0; j < kSortItems; ++j) {
+                    const uint32_t p = (uint32_t)(tid * kSortItems + j);
+                    x[j] = p < cnt ? Q.w[p] : 0u;
+                    sum += x[j];
+                }
+                uint32_t tot;
+                uint32_t run = block_excl_scan<kSortThreads>(sum, L.tmp, &tot);
+#pragma unroll
+                for (int j = 0; j < kSortItems; ++j) {
+                    Q.s[tid * kSortItems + j] = run;
+                    run += x[j];
+                }
+                if (tid == 0) Q.s[kSortChunk] = tot;
+            }
+            __syncthreads();
+            // this chunk's weight of digit tid: its keys occupy [start, start + run)
+            const uint32_t d0 = L.start[tid];
+            const uint32_t wmine = Q.s[min(d0 + mine, cnt)] - Q.s[min(d0, cnt)];
+            uint64_t *wcol = po.status + (size_t)c * kRadix + tid;
+            uint32_t wexcl = 0;
+            if (c == 0) {
+                lb_publish(wcol, epoch, kLbPrefix, min(wmine, kLookbackMaxCount));
+            } else {
+                lb_publish(wcol, epoch, kLbAggregate, min(wmine, kLookbackMaxCount));
+                excl = lb_exclusive(lb.status + tid, kRadix, c, epoch, lb.err);
+                lb_publish(col, epoch, kLbPrefix, excl + mine);
+                wexcl = lb_exclusive(po.status + tid, kRadix, c, epoch, lb.err);
+                lb_publish(wcol, epoch, kLbPrefix, sat_add(wexcl, wmine));
+            }
+            L.gb[tid] = dbase + excl;
+            Q.wb[tid] = (uint32_t)min(wbase + wexcl, (unsigned long long)kLookbackMaxCount);
+            __syncthreads();
+            for (uint32_t p = tid; p < cnt; p += kSortThreads) {
+                const uint32_t k = L.k[p];
+                const uint32_t d = digit_of(k, shift);
+                const uint32_t pos = L.gb[d] + (p - L.start[d]);
+                kout[pos] = k;
+                vout[pos] = L.v[p];
+                const uint32_t off = sat_add(Q.wb[d], Q.s[p] - Q.s[L.start[d]]);
+                po.out[pos] = off;
+                mark_heads(po.heads, pos, off, Q.s[p + 1] - Q.s[p]);
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -487,26 +557,19 @@ template ds_status radix_pass<uint64_t>(const uint64_t *, const uint32_t *, uint
 ds_status onesweep_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                         uint32_t *vout, const uint32_t *n_dev, uint32_t max_n, int shift,
                         const uint32_t *ghist, int hist_reps, uint32_t hist_rep_stride,
-                        LookbackSlot lb, hipStream_t s) {
+                        LookbackSlot lb, hipStream_t s, PairOffsets po) {
     if (max_n > kLookbackMaxCount)
         return set_error(DS_ERR_CAPACITY, "radix pass longer than 2^30 - 1 keys");
     const uint64_t max_chunks = ((uint64_t)max_n + kSortChunk - 1) / kSortChunk;
     if (max_chunks == 0) return DS_OK;
-    hipLaunchKernelGGL(k_onesweep, dim3(clamp_grid(max_chunks, 2048)), dim3(kSortThreads), 0, s,
-                       kin, vin, kout, vout, n_dev, shift, ghist, hist_reps, hist_rep_stride,
-                       lb);
+    const dim3 grid(clamp_grid(max_chunks, 2048));
+    if (po.weight)
+        hipLaunchKernelGGL(k_onesweep<true>, grid, dim3(kSortThreads), 0, s, kin, vin, kout, vout,
+                           n_dev, shift, ghist, hist_reps, hist_rep_stride, lb, po);
+    else
+        hipLaunchKernelGGL(k_onesweep<false>, grid, dim3(kSortThreads), 0, s, kin, vin, kout,
+                           vout, n_dev, shift, ghist, hist_reps, hist_rep_stride, lb, po);
     DS_CHECK_LAUNCH("onesweep");
-    return DS_OK;
-}
-
-ds_status scan_chained(const uint32_t *in, const uint32_t *gidx, uint32_t *out,
-                       const uint32_t *n_dev, uint32_t max_n, uint32_t *total_out,
-                       uint32_t *clamped_out, uint32_t cap, LookbackSlot lb, hipStream_t s,
-                       ScanHeads heads) {
-    const uint64_t max_chunks = ((uint64_t)max_n + kScanChunk - 1) / kScanChunk;
-    hipLaunchKernelGGL(k_scan_chained, dim3(clamp_grid(max_chunks, 1024)), dim3(kScanThreads), 0,
-                       s, in, gidx, n_dev, out, total_out, clamped_out, cap, lb, heads);
-    DS_CHECK_LAUNCH("scan_chained");
     return DS_OK;
 }
 

@@ -105,7 +105,7 @@ inline uint32_t sort_hist_words(uint64_t max_n) {
 // ------------------------------------------------------------------
 // One-kernel passes with decoupled look-back.
 //
-// A radix pass or a scan is one kernel: each workgroup draws the next chunk
+// A radix pass is one kernel: each workgroup draws the next chunk
 // index from a ticket counter, publishes its chunk's per-digit counts, and
 // looks back over earlier chunks for its exclusive prefix. A workgroup only
 // waits on chunks with lower tickets, whose workgroups are running and
@@ -116,7 +116,7 @@ inline uint32_t sort_hist_words(uint64_t max_n) {
 // 1). The spin is bounded: if it ever gives up it sets *err and the result
 // is wrong but the kernel ends.
 struct LookbackSlot {
-    uint64_t *status;       // kRadix (or 1 for a scan) words per chunk
+    uint64_t *status;       // kRadix words per chunk
     uint32_t *ticket;       // zero before the pass
     const uint32_t *epoch;  // current forward's epoch, never 0
     uint32_t *err;
@@ -129,19 +129,34 @@ inline size_t lookback_words(uint64_t max_n, int digits) {
     return (size_t)digits * (size_t)((max_n + kSortChunk - 1) / kSortChunk);
 }
 
+// Optional side output of the last radix pass: the exclusive scan of a
+// per-value weight (weight[v] for the value v of a key) in the pass's output
+// order, so that no separate scan pass over the sorted values is needed.
+// whist holds, per digit of the pass, the sum of the weights of all keys
+// with that digit (reps partial copies of kRadix words, copy r at whist +
+// r * kRadix). The scan, total and heads follow scan_chained: prefixes and the
+// total saturate at kLookbackMaxCount, clamped = min(total, cap), and
+// heads marks the owner of every multiple of 2^heads.shift.
+struct PairOffsets {
+    const uint32_t *weight;  // nullptr: no side output
+    const unsigned long long *whist;
+    int whist_reps;
+    uint32_t *out;
+    uint64_t *status;  // look-back words of the weight sums, kRadix per chunk
+    uint32_t *total_out, *clamped_out;
+    uint32_t cap;
+    ScanHeads heads;
+};
+
 // Stable pass on bits [shift, shift + 8) of 32-bit keys. The pass's 256-bin
 // digit histogram over all n keys is the sum of hist_reps partial copies,
 // copy r at ghist + r * hist_rep_stride.
 ds_status onesweep_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                         uint32_t *vout, const uint32_t *n_dev, uint32_t max_n, int shift,
                         const uint32_t *ghist, int hist_reps, uint32_t hist_rep_stride,
-                        LookbackSlot lb, hipStream_t s);
-
-// Exclusive scan of in[gidx[e]] (or in[e]) for e < n_dev[0], in one kernel;
-// writes the total and min(total, cap).
-ds_status scan_chained(const uint32_t *in, const uint32_t *gidx, uint32_t *out,
-                       const uint32_t *n_dev, uint32_t max_n, uint32_t *total_out,
-                       uint32_t *clamped_out, uint32_t cap, LookbackSlot lb, hipStream_t s,
-                       ScanHeads heads = ScanHeads{nullptr, 0, 0});
+                        LookbackSlot lb, hipStream_t s,
+                        PairOffsets po = PairOffsets{nullptr, nullptr, 0, nullptr, nullptr,
+                                                     nullptr, nullptr, 0,
+                                                     ScanHeads{nullptr, 0, 0}});
 
 }  // namespace ds

@@ -101,10 +101,13 @@ def test_activation_projection_forward(deg):
 
 # c1: the fit workload (one-kernel depth passes); big: above 131072 splats,
 # where the depth passes are reduce-then-scan kernels.
-BIG_SCENES = {"c1": (1000, 60000, 1280, 720, -3.0), "big": (1001, 200000, 640, 360, -3.5)}
+# "c1": 30 sort chunks; "chunks64": the most splats sorted by one-kernel passes
+# whose last pass writes the pair offsets (64 chunks of 2048); "big": above it
+BIG_SCENES = {"c1": (1000, 60000, 1280, 720, -3.0), "chunks64": (1002, 131072, 640, 360, -3.5),
+              "big": (1001, 200000, 640, 360, -3.5)}
 
 
-@pytest.mark.parametrize("scene", ["c0", "ragged", "c1", "big"])
+@pytest.mark.parametrize("scene", ["c0", "ragged", "c1", "chunks64", "big"])
 def test_tile_assign_sort_ranges_bitexact(scene):
     seed, n, W, H, ls = BIG_SCENES[scene] if scene in BIG_SCENES else SCENES[scene]
     params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
