@@ -179,6 +179,37 @@ __device__ __forceinline__ TilePixels<P> tile_pixels(const TileImage &im, int ti
     return q;
 }
 
+// Whether a splat's reach ellipse (record: centre A.xy, conic A.z, A.w, B.x
+// scaled so that p = -q(d), threshold B.z) meets a band of pixel centres
+// [wx0, wx1] x [wy0, wy1]: the least q(d) = a dx^2 + 2 b dx dy + c dy^2 over
+// the band (d = centre - pixel) is at most B.z. q is convex, so the least
+// value is 0 when the centre is in the band and otherwise the least of the
+// four edges' minima (each edge's 1-D minimiser clamped to the edge). The
+// threshold covers the rounding of this float evaluation. A conic that is
+// not positive definite has threshold +inf; any NaN on the way counts as a
+// hit.
+__device__ __forceinline__ float edge_min(float a, float b, float c, float rc, float dx, float lo,
+                                          float hi) {
+    const float dy = fminf(fmaxf(-b * dx * rc, lo), hi);
+    return fmaf(dx, fmaf(2.f * b, dy, a * dx), c * dy * dy);
+}
+
+__device__ __forceinline__ bool reaches_band(const float4 &A, const float4 &B, float wx0,
+                                             float wx1, float wy0, float wy1) {
+    const float a = -A.z, b = -A.w, c = -B.x;
+    const float dxl = A.x - wx1, dxh = A.x - wx0, dyl = A.y - wy1, dyh = A.y - wy0;
+    const float ra = __builtin_amdgcn_rcpf(a), rc = __builtin_amdgcn_rcpf(c);
+    // edges dx = dxl, dxh (minimising over dy), then dy = dyl, dyh (over dx)
+    const float mx =
+        fminf(edge_min(a, b, c, rc, dxl, dyl, dyh), edge_min(a, b, c, rc, dxh, dyl, dyh));
+    const float my =
+        fminf(edge_min(c, b, a, ra, dyl, dxl, dxh), edge_min(c, b, a, ra, dyh, dxl, dxh));
+    float m = fminf(mx, my);
+    const bool inside = (dxl <= 0.f) & (dxh >= 0.f) & (dyl <= 0.f) & (dyh >= 0.f);
+    m = inside ? 0.f : m;
+    return !(m > B.z);
+}
+
 // Per wave, one bit per splat of the tile's list for its first 64 *
 // kSkipWords splats: did the forward find a pixel of this wave that the splat reaches
 // (alpha >= 1/255, pixel not yet finished)? The backward's test for the
@@ -238,19 +269,14 @@ __device__ __forceinline__ int composite_front_to_back(
         // 64 splats at a time: one word of reach bits (NT is a multiple of 64)
         for (int t0 = 0; t0 < bs && !finished; t0 += 64) {
             const int t1 = min(bs, t0 + 64);
-            // Candidates: the splats whose reach box (record B.z, C.w: no
-            // pixel outside it can pass the alpha test) meets the band; the
-            // others are skipped without evaluation.
+            // Candidates: the splats whose reach ellipse meets the band (no
+            // pixel outside it can pass the alpha test); the others are
+            // skipped without evaluation. Lane l tests splat t0 + l.
             uint64_t cand;
             {
                 const int t = t0 + __lane_id();
                 bool hit = false;
-                if (t < t1) {
-                    const float2 c = *(const float2 *)&sA[t];
-                    const float rx = sB[t].z, ry = sC[t].w;
-                    hit = (c.x - rx <= wx1) & (c.x + rx >= wx0) & (c.y - ry <= wy1) &
-                          (c.y + ry >= wy0);
-                }
+                if (t < t1) hit = reaches_band(sA[t], sB[t], wx0, wx1, wy0, wy1);
                 cand = __ballot(hit);
             }
             uint64_t reach = 0;

@@ -185,26 +185,27 @@ __global__ __launch_bounds__(256) void k_zero10(ZeroList z, int n) {
     }
 }
 
-// Half-extents (NDC, x and y) of a box holding every point where the
-// compositing's float evaluation of the record's falloff can pass the alpha
-// test. With Q' = -(qa, qb; qb, qc) (positive definite), a point d passes
-// when d^T Q' d <= t2 = log2(o / (1/255)) up to rounding: the float value of
-// p = -d^T Q' d (two fma chains) is within 12 eps (tr^2 / det) |p| of the
-// exact one (eps = 2^-24), exp2 and the opacity product within 2^-20
-// relative (1e-5 in t2). The box of the widened ellipse d^T Q' d <= tm has
-// half-extents sqrt(tm Q'yy / det), sqrt(tm Q'xx / det). Computed in double
-// and rounded outwards; a conic too ill-conditioned for the bound (or not
-// positive definite in float) gets an unbounded box, an opacity below the
-// threshold an empty one (-inf: no comparison passes).
-__device__ float2 reach_extent(float qa, float qb, float qc, float o) {
-    if (!(o >= kAlphaThres)) return make_float2(-INFINITY, -INFINITY);
+// Reach threshold of a splat: with Q' = -(qa, qb; qb, qc) (the record's
+// conic, positive definite), every point d where the compositing's float
+// evaluation of the falloff can pass the alpha test has d^T Q' d <= tm, and
+// the returned value bounds tm plus the rounding of the compositing's own
+// test of d^T Q' d against it. A point passes when d^T Q' d <= t2 = log2(o /
+// (1/255)) up to rounding: the float value of p = -d^T Q' d (two fma chains)
+// is within 12 eps (tr^2 / det) |p| of the exact one (eps = 2^-24), exp2 and
+// the opacity product within 2^-20 relative (1e-5 in t2); so tm = (t2 +
+// 1e-5) / (1 - slack) with slack = 16 eps tr^2 / det, and the float form in
+// the test is within slack of exact again. Computed in double and rounded
+// up; a conic too ill-conditioned for the bound (or not positive definite in
+// float) gets +inf (always evaluated), an opacity below the threshold -inf
+// (never).
+__device__ float reach_threshold(float qa, float qb, float qc, float o) {
+    if (!(o >= kAlphaThres)) return -INFINITY;
     const double a = -(double)qa, b = -(double)qb, c = -(double)qc;
     const double det = a * c - b * b, tr = a + c;
     const double slack = 16.0 * 0x1p-24 * (tr * tr / det);
-    if (!(a > 0.0 && det > 0.0 && slack < 0.5)) return make_float2(INFINITY, INFINITY);
+    if (!(a > 0.0 && c > 0.0 && det > 0.0 && slack < 0.25)) return INFINITY;
     const double tm = (log2((double)o / (double)kAlphaThres) + 1e-5) / (1.0 - slack);
-    const double rx = sqrt(tm * c / det), ry = sqrt(tm * a / det);
-    return make_float2((float)(rx * (1.0 + 1e-6) + 1e-6), (float)(ry * (1.0 + 1e-6) + 1e-6));
+    return (float)(tm * (1.0 + slack) * (1.0 + 1e-6) + 1e-6);
 }
 
 // Per splat: cull test and tile rectangle exactly as tileAssignKernel
@@ -281,10 +282,9 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
             const float k = kFalloffScale;
             const float qa = cyy * idet * k, qb = -cxy * idet * k, qc = cxx * idet * k;
             const float o = in.opacity[i];
-            const float2 ext = reach_extent(qa, qb, qc, o);
             rec[3 * i + 0] = make_float4(x, y, qa, qb);
-            rec[3 * i + 1] = make_float4(qc, o, ext.x, __uint_as_float(i));
-            rec[3 * i + 2] = make_float4(in.color_r[i], in.color_g[i], in.color_b[i], ext.y);
+            rec[3 * i + 1] = make_float4(qc, o, reach_threshold(qa, qb, qc, o), __uint_as_float(i));
+            rec[3 * i + 2] = make_float4(in.color_r[i], in.color_g[i], in.color_b[i], 0.f);
         } else {
             ntiles[i] = 0;
         }
