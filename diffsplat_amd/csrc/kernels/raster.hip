@@ -403,8 +403,10 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t *__restrict__ tke
 //   k_tile_scatter  per pair: output slot = (chunk, tile) slot + rank
 // Within a tile the pairs keep their emission order, i.e. depth order.
 constexpr int kTileChunk = 4096;
-constexpr int kTileThreads = 256;
-constexpr int kTileWaves = kTileThreads / kWave;         // 4
+// 512 threads: a chunk's workgroup is alone on its CU (LDS), so its latency
+// is the kernel's; more waves shorten its per-thread loops.
+constexpr int kTileThreads = 512;
+constexpr int kTileWaves = kTileThreads / kWave;         // 8
 constexpr int kTileWaveSlots = kTileChunk / kTileWaves;  // consecutive slots per wave
 constexpr int kTileGroups = kTileWaveSlots / kWave;      // 64-slot groups per wave
 constexpr int kTileItems = kTileChunk / kTileThreads;    // slots per thread in the head scan
@@ -417,7 +419,7 @@ __device__ __forceinline__ uint32_t tile_seg_len(uint32_t nch) {
     return (nch + kTileSegs - 1) / kTileSegs;
 }
 
-// LDS: per-wave tile counters (32 KB at kTileBinsMax), the owner of each slot
+// LDS: per-wave tile counters (64 KB at kTileBinsMax), the owner of each slot
 // and each owner's first slot (16-bit: both stay below kTileChunk), and per
 // splat of the chunk its id and tile box. Three memory round trips per chunk:
 // the chunk's first and last splats, their offsets and ids, their boxes.
