@@ -104,6 +104,23 @@ __device__ __forceinline__ void sh_basis_rest(float3 v, float *Y) {
 }
 
 // ---------------------------------------------------------------- wave64
+// One Adam element (adamKernel, src/optimizers/adam.cu:12-37) given the
+// step's learning rate and bias corrections bc1 = 1 / (1 - beta1^t), bc2 =
+// 1 / (1 - beta2^t); shared by the optimiser kernel and the fused projection
+// backward so that both compute the same expression.
+__device__ __forceinline__ void adam_elem(float &p, float &m, float &v, float g, float lr,
+                                          float bc1, float bc2, float beta1, float beta2,
+                                          float eps) {
+    m = beta1 * m + (1.f - beta1) * g;
+    v = beta2 * v + (1.f - beta2) * g * g;
+    p -= lr * (m * bc1) / (sqrtf(v * bc2) + eps);
+}
+
+// Learning rate of a group at step t: lr * decay^(t - 1) (decay 1: constant).
+__device__ __forceinline__ float adam_lr(float lr, float decay, int32_t t) {
+    return decay == 1.f ? lr : lr * powf(decay, (float)(t - 1));
+}
+
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 __device__ __forceinline__ float wave_sum(float v) {

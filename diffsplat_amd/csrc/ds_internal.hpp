@@ -120,6 +120,27 @@ ds_status launch_activ_project_bwd_multi(const ds_gauss3d &in, const ds_splat2d 
                                          const ds_gauss3d &gin, int deg, const float4 *cam_tab,
                                          const uint16_t *splat_inst, int mode, hipStream_t s);
 
+// The fitter's Adam step fused into the activation/projection backward (no
+// counterpart in the reference; the reference's ImageFitter runs Adam as its
+// own pass): after a splat's gradients are written, each of its parameters
+// takes its Adam step. Field f of the table is, in order: position x, y, z,
+// log-scale x, y, z, rotation w, x, y, z, SH DC r, g, b, opacity logit (14),
+// then the SH rest r, g, b slabs (band-major, n * bands each) when the degree
+// is above 0. step: the device step counter, already incremented.
+constexpr int kAdamFields = 17;
+struct AdamFields {
+    float *p[kAdamFields];
+    float *m[kAdamFields];
+    float *v[kAdamFields];
+    float lr[kAdamFields];
+    float decay[kAdamFields];
+    float beta1, beta2, eps;
+    const int32_t *step;
+};
+ds_status launch_activ_project_bwd_adam(const ds_gauss3d &in, const ds_splat2d &gout,
+                                        const ds_gauss3d &gin, int deg, Cam3 cam, const PV &pv,
+                                        int mode, const AdamFields &af, hipStream_t s);
+
 // loss / optimiser (kernels/optim.hip)
 ds_status launch_mse_fwd(const float *p, const float *t, int64_t npix, float *loss,
                          hipStream_t s);

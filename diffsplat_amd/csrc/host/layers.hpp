@@ -142,6 +142,9 @@ public:
         check(ds_activ_project_bwd(&in, &go, &gi, sh_degree, cam.data(), pv, grad_mode, stream),
               "GaussProjectLayer::backward");
     }
+    int shDegree() const { return sh_degree; }
+    const std::array<float, 3> &cameraPosition() const { return cam; }
+    const float *viewProj() const { return pv; }
 
 private:
     int sh_degree = 0;

@@ -4,6 +4,7 @@
 #include "../ds_internal.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <chrono>
 
 namespace dsh {
@@ -198,12 +199,49 @@ void ImageFitter::initLayers() {
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
+// The optimiser's groups as the fused projection backward's fields (see
+// ds::AdamFields): false unless every field of the slab (and nothing else) is
+// a group with that field's parameters and gradients.
+static bool adam_fields_of(const std::vector<ds_adam_group> &tab, const ds_gauss3d &p,
+                           const ds_gauss3d &g, int deg, const AdamParams &hp,
+                           int32_t *step, ds::AdamFields &af) {
+    const float *pf[ds::kAdamFields] = {
+        p.pos_x,   p.pos_y,   p.pos_z,   p.scale_x,   p.scale_y,   p.scale_z,
+        p.rot_w,   p.rot_x,   p.rot_y,   p.rot_z,     p.sh_dc_r,   p.sh_dc_g,
+        p.sh_dc_b, p.logit_opacity, p.sh_rest_r, p.sh_rest_g, p.sh_rest_b};
+    const float *gf[ds::kAdamFields] = {
+        g.pos_x,   g.pos_y,   g.pos_z,   g.scale_x,   g.scale_y,   g.scale_z,
+        g.rot_w,   g.rot_x,   g.rot_y,   g.rot_z,     g.sh_dc_r,   g.sh_dc_g,
+        g.sh_dc_b, g.logit_opacity, g.sh_rest_r, g.sh_rest_g, g.sh_rest_b};
+    const int nfields = deg > 0 ? ds::kAdamFields : 14;
+    if ((int)tab.size() != nfields) return false;
+    af = ds::AdamFields{};
+    for (int f = 0; f < nfields; ++f) {
+        const int64_t n = (int64_t)p.count * (f < 14 ? 1 : ds_sh_bands(deg));
+        const ds_adam_group *hit = nullptr;
+        for (const auto &k : tab)
+            if (k.param == pf[f] && k.grad == gf[f] && k.n == n) hit = &k;
+        if (!hit) return false;
+        af.p[f] = hit->param;
+        af.m[f] = hit->m;
+        af.v[f] = hit->v;
+        af.lr[f] = hit->lr;
+        af.decay[f] = hit->decay > 0.f ? hit->decay : 1.f;
+    }
+    af.beta1 = hp.beta1;
+    af.beta2 = hp.beta2;
+    af.eps = hp.epsilon;
+    af.step = step;
+    return true;
+}
+
 void ImageFitter::stepOnce() {
     if (fused) {
         // The reference's step (zeroGrad, forward, backward, Adam) with the
         // zeroing of the Splat2D gradients and the Adam step counter folded
-        // into the rasterizer's first kernel, and the compositing forward,
-        // MSE and compositing backward in one kernel: 15 launches.
+        // into the rasterizer's first kernel, the compositing forward, MSE
+        // and compositing backward in one kernel, and Adam in the
+        // projection backward: 13 launches.
         gp_layer.forward();
         ds_splat2d in = gp_layer.getOutput().view();
         ds_splat2d gi = ras_layer.getGradInput().view();
@@ -212,13 +250,27 @@ void ImageFitter::stepOnce() {
                                       mse_layer.getGradInput().ptr, gi, optimizer.stepDevice(),
                                       stream),
               "ImageFitter::step");
-        gp_layer.backward();
         std::vector<ds_adam_group> tab = optimizer.table();
         const AdamParams &a = optimizer.getParams();
-        ds_adam_hparams h{a.beta1, a.beta2, a.epsilon};
-        check(ds::launch_adam(tab.data(), (int)tab.size(), h, 0, optimizer.stepDevice(), stream,
-                              false),
-              "ImageFitter::step");
+        const ds_gauss3d params = gaussian_params.view();
+        const ds_gauss3d pgrads = gp_layer.getGradInput().view();
+        ds::AdamFields af;
+        if (adam_fields_of(tab, params, pgrads, gp_layer.shDegree(), a, optimizer.stepDevice(),
+                           af)) {
+            const auto &c = gp_layer.cameraPosition();
+            ds::PV pv;
+            std::memcpy(pv.m, gp_layer.viewProj(), sizeof(pv.m));
+            check(ds::launch_activ_project_bwd_adam(params, gi, pgrads, gp_layer.shDegree(),
+                                                    ds::Cam3{c[0], c[1], c[2]}, pv,
+                                                    opt.grad_mode, af, stream),
+                  "ImageFitter::step");
+        } else {
+            gp_layer.backward();
+            ds_adam_hparams h{a.beta1, a.beta2, a.epsilon};
+            check(ds::launch_adam(tab.data(), (int)tab.size(), h, 0, optimizer.stepDevice(),
+                                  stream, false),
+                  "ImageFitter::step");
+        }
         optimizer.countStep();
         return;
     }

@@ -493,11 +493,33 @@ __global__ __launch_bounds__(kBlock) void k_activ_project_fwd(ds_gauss3d in, ds_
     out.opacity[i] = sigmoid(in.logit_opacity[i]);
 }
 
-template <int DEG, int MODE, class CS>
+template <int DEG, int MODE, class CS, bool ADAM = false>
 __global__ __launch_bounds__(kBlock) void k_activ_project_bwd(ds_gauss3d in, ds_splat2d go,
-                                                              ds_gauss3d gi, CS cs) {
+                                                              ds_gauss3d gi, CS cs,
+                                                              AdamFields af) {
+    // With ADAM: the step's learning rate per field and bias corrections
+    __shared__ float s_lr[ADAM ? kAdamFields : 1];
+    __shared__ float s_bc[2];
+    if constexpr (ADAM) {
+        const int32_t t = *af.step;
+        if (threadIdx.x < kAdamFields)
+            s_lr[threadIdx.x] = adam_lr(af.lr[threadIdx.x], af.decay[threadIdx.x], t);
+        if (threadIdx.x == 0) {
+            s_bc[0] = 1.f / (1.f - powf(af.beta1, (float)t));
+            s_bc[1] = 1.f / (1.f - powf(af.beta2, (float)t));
+        }
+        __syncthreads();
+    }
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= in.count) return;
+    // Adam step of field f at element e of its slab, gradient g
+    auto adam = [&](int f, int64_t e, float g) {
+        float pp = af.p[f][e], mm = af.m[f][e], vv = af.v[f][e];
+        adam_elem(pp, mm, vv, g, s_lr[f], s_bc[0], s_bc[1], af.beta1, af.beta2, af.eps);
+        af.p[f][e] = pp;
+        af.m[f][e] = mm;
+        af.v[f][e] = vv;
+    };
     Cam3 cam;
     PV pv;
     cs.load(i, cam, pv);
@@ -509,6 +531,13 @@ __global__ __launch_bounds__(kBlock) void k_activ_project_bwd(ds_gauss3d in, ds_
         gi.rot_w[i] = 0.f; gi.rot_x[i] = 0.f; gi.rot_y[i] = 0.f; gi.rot_z[i] = 0.f;
         gi.logit_opacity[i] = 0.f;
         sh_backward<DEG>(gi, i, px, py, pz, cam, 0.f, 0.f, 0.f);
+        if constexpr (ADAM) {  // a zero gradient still moves the moments
+#pragma unroll
+            for (int f = 0; f < 14; ++f) adam(f, i, 0.f);
+            constexpr int nb = sh_bands(DEG);
+            for (int k = 0; k < nb; ++k)
+                for (int c = 0; c < 3; ++c) adam(14 + c, (int64_t)k * in.count + i, 0.f);
+        }
         return;
     }
     // projection backward
@@ -540,7 +569,29 @@ __global__ __launch_bounds__(kBlock) void k_activ_project_bwd(ds_gauss3d in, ds_
     gi.pos_y[i] = gp[1];
     gi.pos_z[i] = gp[2];
     const float s = sigmoid(in.logit_opacity[i]);
-    gi.logit_opacity[i] = go.opacity[i] * s * (1.f - s);
+    const float g_op = go.opacity[i] * s * (1.f - s);
+    gi.logit_opacity[i] = g_op;
+    if constexpr (ADAM) {
+        // every read of this splat's parameters is done: step them in place
+        adam(0, i, gp[0]);
+        adam(1, i, gp[1]);
+        adam(2, i, gp[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) adam(3 + c, i, g.ds[c]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) adam(6 + c, i, g.dq[c]);
+        adam(10, i, dR * SH_C0);
+        adam(11, i, dG * SH_C0);
+        adam(12, i, dB * SH_C0);
+        adam(13, i, g_op);
+        constexpr int nb = sh_bands(DEG);
+        const float *rest[3] = {gi.sh_rest_r, gi.sh_rest_g, gi.sh_rest_b};
+        for (int k = 0; k < nb; ++k)
+            for (int c = 0; c < 3; ++c) {
+                const int64_t e = (int64_t)k * in.count + i;
+                adam(14 + c, e, rest[c][e]);
+            }
+    }
 }
 
 inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
@@ -625,27 +676,28 @@ static ds_status activ_project_fwd(const ds_gauss3d &in, const ds_splat2d &out, 
     return DS_OK;
 }
 
-template <int MODE, class CS>
+template <int MODE, class CS, bool ADAM>
 static void activ_project_bwd_mode(const ds_gauss3d &in, const ds_splat2d &go,
-                                   const ds_gauss3d &gi, int deg, const CS &cs, hipStream_t s) {
+                                   const ds_gauss3d &gi, int deg, const CS &cs,
+                                   const AdamFields &af, hipStream_t s) {
     const dim3 grid(grid_for(in.count)), block(kBlock);
     switch (deg) {
-        case 0: hipLaunchKernelGGL((k_activ_project_bwd<0, MODE, CS>), grid, block, 0, s, in, go, gi, cs); break;
-        case 1: hipLaunchKernelGGL((k_activ_project_bwd<1, MODE, CS>), grid, block, 0, s, in, go, gi, cs); break;
-        case 2: hipLaunchKernelGGL((k_activ_project_bwd<2, MODE, CS>), grid, block, 0, s, in, go, gi, cs); break;
-        default: hipLaunchKernelGGL((k_activ_project_bwd<3, MODE, CS>), grid, block, 0, s, in, go, gi, cs); break;
+        case 0: hipLaunchKernelGGL((k_activ_project_bwd<0, MODE, CS, ADAM>), grid, block, 0, s, in, go, gi, cs, af); break;
+        case 1: hipLaunchKernelGGL((k_activ_project_bwd<1, MODE, CS, ADAM>), grid, block, 0, s, in, go, gi, cs, af); break;
+        case 2: hipLaunchKernelGGL((k_activ_project_bwd<2, MODE, CS, ADAM>), grid, block, 0, s, in, go, gi, cs, af); break;
+        default: hipLaunchKernelGGL((k_activ_project_bwd<3, MODE, CS, ADAM>), grid, block, 0, s, in, go, gi, cs, af); break;
     }
 }
 
-template <class CS>
+template <class CS, bool ADAM = false>
 static ds_status activ_project_bwd(const ds_gauss3d &in, const ds_splat2d &go,
                                    const ds_gauss3d &gi, int deg, const CS &cs, int mode,
-                                   hipStream_t s) {
+                                   hipStream_t s, const AdamFields &af = AdamFields{}) {
     if (in.count <= 0) return DS_OK;
     if (mode == DS_GRAD_REFERENCE)
-        activ_project_bwd_mode<DS_GRAD_REFERENCE>(in, go, gi, deg, cs, s);
+        activ_project_bwd_mode<DS_GRAD_REFERENCE, CS, ADAM>(in, go, gi, deg, cs, af, s);
     else
-        activ_project_bwd_mode<DS_GRAD_EXACT>(in, go, gi, deg, cs, s);
+        activ_project_bwd_mode<DS_GRAD_EXACT, CS, ADAM>(in, go, gi, deg, cs, af, s);
     DS_CHECK_LAUNCH("activ_project_bwd");
     return DS_OK;
 }
@@ -659,6 +711,14 @@ ds_status launch_activ_project_bwd(const ds_gauss3d &in, const ds_splat2d &go,
                                    const ds_gauss3d &gi, int deg, Cam3 cam, const PV &pv,
                                    int mode, hipStream_t s) {
     return activ_project_bwd(in, go, gi, deg, CamUniform{cam, pv}, mode, s);
+}
+
+ds_status launch_activ_project_bwd_adam(const ds_gauss3d &in, const ds_splat2d &go,
+                                        const ds_gauss3d &gi, int deg, Cam3 cam, const PV &pv,
+                                        int mode, const AdamFields &af, hipStream_t s) {
+    if (!af.step) return set_error(DS_ERR_INVALID_ARG, "fused Adam step without a step counter");
+    return activ_project_bwd<CamUniform, true>(in, go, gi, deg, CamUniform{cam, pv}, mode, s,
+                                               af);
 }
 
 ds_status launch_activ_project_fwd_multi(const ds_gauss3d &in, const ds_splat2d &out, int deg,

@@ -72,22 +72,19 @@ __global__ __launch_bounds__(kBlock) void k_adam(AdamTable tab, float beta1, flo
     const int32_t t = step_dev ? *step_dev : step_host;
     const float bc1 = 1.f / (1.f - powf(beta1, (float)t));
     const float bc2 = 1.f / (1.f - powf(beta2, (float)t));
-    const float ob1 = 1.f - beta1, ob2 = 1.f - beta2;
     float *__restrict__ P = tab.param[k];
     const float *__restrict__ G = tab.grad[k];
     float *__restrict__ M = tab.m[k];
     float *__restrict__ V = tab.v[k];
-    const float d = tab.decay[k];
-    const float lr = d == 1.f ? tab.lr[k] : tab.lr[k] * powf(d, (float)(t - 1));
+    const float lr = adam_lr(tab.lr[k], tab.decay[k], t);
     const int64_t n = tab.n[k];
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const float g = G[i];
-        const float m = beta1 * M[i] + ob1 * g;
-        const float v = beta2 * V[i] + ob2 * g * g;
+        float p = P[i], m = M[i], v = V[i];
+        adam_elem(p, m, v, G[i], lr, bc1, bc2, beta1, beta2, eps);
         M[i] = m;
         V[i] = v;
-        P[i] -= lr * (m * bc1) / (sqrtf(v * bc2) + eps);
+        P[i] = p;
     }
 }
 

@@ -96,3 +96,48 @@ def test_fit_200_steps_matches_cpu_fit():
     print(f"[parity] fit200: psnr_init={psnr0:.3f} gpu={psnr_gpu:.3f} cpu={psnr_cpu:.3f}")
     assert psnr_gpu > psnr0 + 8.0
     assert abs(psnr_gpu - psnr_cpu) <= 1.0
+
+
+@pytest.mark.parametrize("deg,pos_decay", [(0, 0.99), (3, 1.0)])
+def test_fused_adam_steps_the_reported_gradients(deg, pos_decay):
+    """The fitter's Adam runs inside the projection backward: two steps'
+    parameters against Adam (FP64) applied to the gradients each step
+    reports, with per-group learning rates, the position schedule and the
+    SH rest coefficients (degree 3)."""
+    n = 1500
+    params = ds.frustum_splats(SEED + 7, n, aspect=W / H)
+    if deg:
+        nb = ds.sh_bands(deg)
+        rng = np.random.default_rng(7)
+        params = np.concatenate([params[:13 * n], rng.normal(0, 0.2, 3 * nb * n),
+                                 params[13 * n:]]).astype(np.float32)
+    view, proj, cam = ds.fitter_camera(W, H)
+    lr = dict(LR, lr_sh_rest=2e-3, lr_pos_decay=pos_decay)
+    fit = ds.ImageFitter(W, H, n, view=view, proj=proj, cam_pos=cam, lr=lr, sh_degree=deg)
+    fit.set_target(ds.target_image(SEED + 7, W, H, 32))
+    fit.upload_params(params)
+    b1, b2, eps = fit.cfg.beta1, fit.cfg.beta2, fit.cfg.epsilon
+    nb = ds.sh_bands(deg)
+    # slab rows: position 3, scale 3, rotation 4, SH DC 3, SH rest 3 * nb, opacity
+    rows = ([lr["lr_pos"]] * 3 + [lr["lr_scale"]] * 3 + [lr["lr_rot"]] * 4
+            + [lr["lr_sh_dc"]] * 3 + [lr["lr_sh_rest"]] * (3 * nb) + [lr["lr_opacity"]])
+    lr_row = np.repeat(np.array(rows, np.float64), n)
+    is_pos = np.zeros(len(rows) * n, bool)
+    is_pos[:3 * n] = True
+    p = fit.params().astype(np.float64)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for t in (1, 2):
+        fit.step(1)
+        fit.synchronize()
+        g = fit.grads().astype(np.float64)
+        m = b1 * m + (1 - b1) * g
+        v = b2 * v + (1 - b2) * g * g
+        lr_t = np.where(is_pos, lr_row * pos_decay ** (t - 1), lr_row)
+        p = p - lr_t * (m / (1 - b1 ** t)) / (np.sqrt(v / (1 - b2 ** t)) + eps)
+        got = fit.params().astype(np.float64)
+        ok, mabs, mrel = close(got, p, atol=1e-6, rtol=1e-5)
+        print(f"[parity] fused_adam_deg{deg}_step{t}: max_abs={mabs:.3e}")
+        assert ok, (t, mabs, mrel)
+        p = got  # the next step starts from the GPU's parameters
+    fit.close()
