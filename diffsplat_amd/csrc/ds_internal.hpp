@@ -151,13 +151,42 @@ ds_status launch_mse_bwd(const float *p, const float *t, int64_t npix, float *gr
 ds_status launch_adam(const ds_adam_group *groups, int n_groups, const ds_adam_hparams &hp,
                       int32_t step, int32_t *step_dev, hipStream_t s, bool tick = true);
 
+// The work of a rasterizer forward's first kernel, in neutral terms so that
+// another kernel of a fitter step can do it instead (no counterpart in the
+// reference): word ranges set to zero (the word at set_word to set_value
+// instead), a counter stepped by one and never to 0 (epoch), and a fitter
+// step's prelude: the Splat2D gradient arrays zeroed (ngrad floats each) and
+// the Adam step counter incremented (tick may be NULL).
+constexpr int kStartRanges = 6;
+struct ForwardStart {
+    uint32_t *zero[kStartRanges];
+    uint32_t nzero[kStartRanges];
+    uint32_t *set_word;
+    uint32_t set_value;
+    uint32_t *epoch;
+    int32_t *tick;
+    float *grad[10];
+    int ngrad;
+};
+
 // One fitter step's rasterizer work with its prelude folded in (no
 // counterpart in the reference): the first kernel of the forward also zeroes
 // the Splat2D gradients grad_in (ImageFitter's zeroGrad) and increments the
 // Adam step counter *adam_tick (the optimizer then runs with tick = false);
 // then the forward with the fused MSE and the backward, as ds_raster_fwd_bwd.
+// With started, that first kernel's work (raster_step_start) was already
+// done by an earlier kernel of the step and is not launched.
 ds_status raster_step_fwd_bwd(ds_raster *r, const ds_splat2d &in, float *out_rgb,
                               const float *target, float *loss, float *grad_rgb,
-                              const ds_splat2d &grad_in, int32_t *adam_tick, hipStream_t s);
+                              const ds_splat2d &grad_in, int32_t *adam_tick, hipStream_t s,
+                              bool started = false);
+// The first kernel's work of raster_step_fwd_bwd for n splats.
+ds_status raster_step_start(ds_raster *r, int n, float *loss, const ds_splat2d &grad_in,
+                            int32_t *adam_tick, ForwardStart *out);
+// Activation + projection that also does a forward's first-kernel work (n >
+// 0: every thread of its grid takes a share).
+ds_status launch_activ_project_fwd_start(const ds_gauss3d &in, const ds_splat2d &out, int deg,
+                                         Cam3 cam, const PV &pv, const ForwardStart &fs,
+                                         hipStream_t s);
 
 }  // namespace ds

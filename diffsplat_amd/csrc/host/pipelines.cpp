@@ -241,14 +241,35 @@ void ImageFitter::stepOnce() {
         // zeroing of the Splat2D gradients and the Adam step counter folded
         // into the rasterizer's first kernel, the compositing forward, MSE
         // and compositing backward in one kernel, and Adam in the
-        // projection backward: 13 launches.
-        gp_layer.forward();
-        ds_splat2d in = gp_layer.getOutput().view();
+        // projection backward; the rasterizer's first-kernel work (scalars,
+        // counters, that prelude) runs in the activation/projection kernel:
+        // 12 launches.
         ds_splat2d gi = ras_layer.getGradInput().view();
+        const int n = gaussian_params.count;
+        bool started = false;
+        if (n > 0) {
+            ds::ForwardStart fs;
+            check(ds::raster_step_start(ras_layer.raster(), n, mse_layer.lossDevice(), gi,
+                                        optimizer.stepDevice(), &fs),
+                  "ImageFitter::step");
+            gp_layer.getOutput().count = n;
+            const auto &c = gp_layer.cameraPosition();
+            ds::PV pv;
+            std::memcpy(pv.m, gp_layer.viewProj(), sizeof(pv.m));
+            check(ds::launch_activ_project_fwd_start(gaussian_params.view(),
+                                                     gp_layer.getOutput().view(),
+                                                     gp_layer.shDegree(),
+                                                     ds::Cam3{c[0], c[1], c[2]}, pv, fs, stream),
+                  "ImageFitter::step");
+            started = true;
+        } else {
+            gp_layer.forward();
+        }
+        ds_splat2d in = gp_layer.getOutput().view();
         check(ds::raster_step_fwd_bwd(ras_layer.raster(), in, ras_layer.getOutput().ptr,
                                       d_target_pixels.ptr, mse_layer.lossDevice(),
                                       mse_layer.getGradInput().ptr, gi, optimizer.stepDevice(),
-                                      stream),
+                                      stream, started),
               "ImageFitter::step");
         std::vector<ds_adam_group> tab = optimizer.table();
         const AdamParams &a = optimizer.getParams();

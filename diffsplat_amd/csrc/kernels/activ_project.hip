@@ -14,6 +14,7 @@
 
 #include "../ds_device.hpp"
 #include "../ds_internal.hpp"
+#include "forward_start.hpp"
 
 namespace ds {
 namespace {
@@ -461,9 +462,11 @@ struct CamTable {
     }
 };
 
-template <int DEG, class CS>
+template <int DEG, class CS, bool START = false>
 __global__ __launch_bounds__(kBlock) void k_activ_project_fwd(ds_gauss3d in, ds_splat2d out,
-                                                              CS cs) {
+                                                              CS cs, ForwardStart fs) {
+    // a rasterizer forward's first-kernel work, shared by this grid's threads
+    if constexpr (START) forward_start(fs, gtid(), gstride());
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= in.count) return;
     Cam3 cam;
@@ -661,16 +664,17 @@ ds_status launch_project_bwd(const ds_splat3d &in, const ds_splat2d &go, const d
     return DS_OK;
 }
 
-template <class CS>
+template <class CS, bool START = false>
 static ds_status activ_project_fwd(const ds_gauss3d &in, const ds_splat2d &out, int deg,
-                                   const CS &cs, hipStream_t s) {
+                                   const CS &cs, hipStream_t s,
+                                   const ForwardStart &fs = ForwardStart{}) {
     if (in.count <= 0) return DS_OK;
     const dim3 grid(grid_for(in.count)), block(kBlock);
     switch (deg) {
-        case 0: hipLaunchKernelGGL((k_activ_project_fwd<0, CS>), grid, block, 0, s, in, out, cs); break;
-        case 1: hipLaunchKernelGGL((k_activ_project_fwd<1, CS>), grid, block, 0, s, in, out, cs); break;
-        case 2: hipLaunchKernelGGL((k_activ_project_fwd<2, CS>), grid, block, 0, s, in, out, cs); break;
-        default: hipLaunchKernelGGL((k_activ_project_fwd<3, CS>), grid, block, 0, s, in, out, cs); break;
+        case 0: hipLaunchKernelGGL((k_activ_project_fwd<0, CS, START>), grid, block, 0, s, in, out, cs, fs); break;
+        case 1: hipLaunchKernelGGL((k_activ_project_fwd<1, CS, START>), grid, block, 0, s, in, out, cs, fs); break;
+        case 2: hipLaunchKernelGGL((k_activ_project_fwd<2, CS, START>), grid, block, 0, s, in, out, cs, fs); break;
+        default: hipLaunchKernelGGL((k_activ_project_fwd<3, CS, START>), grid, block, 0, s, in, out, cs, fs); break;
     }
     DS_CHECK_LAUNCH("activ_project_fwd");
     return DS_OK;
@@ -711,6 +715,13 @@ ds_status launch_activ_project_bwd(const ds_gauss3d &in, const ds_splat2d &go,
                                    const ds_gauss3d &gi, int deg, Cam3 cam, const PV &pv,
                                    int mode, hipStream_t s) {
     return activ_project_bwd(in, go, gi, deg, CamUniform{cam, pv}, mode, s);
+}
+
+ds_status launch_activ_project_fwd_start(const ds_gauss3d &in, const ds_splat2d &out, int deg,
+                                         Cam3 cam, const PV &pv, const ForwardStart &fs,
+                                         hipStream_t s) {
+    if (in.count <= 0) return set_error(DS_ERR_INVALID_ARG, "forward start needs splats");
+    return activ_project_fwd<CamUniform, true>(in, out, deg, CamUniform{cam, pv}, s, fs);
 }
 
 ds_status launch_activ_project_bwd_adam(const ds_gauss3d &in, const ds_splat2d &go,

@@ -34,6 +34,7 @@
 #include "../ds_device.hpp"
 #include "../ds_internal.hpp"
 #include "composite.hpp"
+#include "forward_start.hpp"
 #include "sort_scan.hpp"
 
 namespace ds {
@@ -90,46 +91,12 @@ __device__ __forceinline__ int tile_floor(float v, int hi) {
     return (int)f;
 }
 
-// Zeroes up to ten arrays of n floats in one launch.
-struct ZeroList {
-    float *p[10];
-};
-
-// Work a fitter step folds into the first kernel of its forward: zero the
-// ten Splat2D gradient arrays (n floats each) and increment the Adam step
-// counter (tick may be NULL).
-struct StepPrelude {
-    ZeroList z;
-    int n;
-    int32_t *tick;
-};
-
-// Start of a forward: device scalars, a new look-back epoch, zero tickets,
-// digit histograms and visible-count replicas, empty tile ranges, zero
-// losses (one per instance), zero segment sums of the tile sort (zero_words
-// words at zero); and a fitter step's prelude (pre.n = 0: none).
-__global__ __launch_bounds__(256) void k_begin(uint32_t *sc, uint32_t count, int2 *ranges,
-                                               int ntiles, float *loss, int nloss,
-                                               uint32_t *ghist, uint32_t *zero, int zero_words,
-                                               StepPrelude pre) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i == 0 && pre.tick) *pre.tick += 1;
-    for (int t = i; t < pre.n; t += gridDim.x * 256) {
-#pragma unroll
-        for (int f = 0; f < 10; ++f)
-            if (pre.z.p[f]) pre.z.p[f][t] = 0.f;
-    }
-    if (i < SC_EPOCH) sc[i] = (i == SC_COUNT) ? count : 0u;
-    if (i == SC_EPOCH) {
-        const uint32_t e = sc[SC_EPOCH] + 1u;
-        sc[SC_EPOCH] = e ? e : 1u;
-    }
-    if (i == SC_ERR || (i >= SC_TICKET && i < SC_TICKET + LB_SLOTS)) sc[i] = 0u;
-    if (loss)
-        for (int t = i; t < nloss; t += gridDim.x * 256) loss[t] = 0.f;
-    for (int t = i; t < kGhistWords; t += gridDim.x * 256) ghist[t] = 0u;
-    for (int t = i; t < ntiles; t += gridDim.x * 256) ranges[t] = make_int2(0, 0);
-    for (int t = i; t < zero_words; t += gridDim.x * 256) zero[t] = 0u;
+// Start of a forward (see ForwardStart): device scalars, a new look-back
+// epoch, zero tickets, digit histograms and visible-count replicas, empty
+// tile ranges, zero losses (one per instance), zero segment sums of the tile
+// sort; and a fitter step's prelude.
+__global__ __launch_bounds__(256) void k_begin(ForwardStart f) {
+    forward_start(f, gtid(), gstride());
 }
 
 // Adds 1 to bin d of an LDS histogram for every active lane. When all active
@@ -176,6 +143,11 @@ __device__ __forceinline__ void flush_histograms(const uint32_t (*s_h)[kRadix], 
         if (c) atomicAdd(&grows[r * kRadix + threadIdx.x], c);
     }
 }
+
+// Zeroes up to ten arrays of n floats in one launch.
+struct ZeroList {
+    float *p[10];
+};
 
 __global__ __launch_bounds__(256) void k_zero10(ZeroList z, int n) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -869,10 +841,44 @@ static CompositeArgs composite_args(const ds_raster *r) {
 // forward alone, or (fused_grad non-NULL, target given, equal launch shapes)
 // the forward and backward of each tile in one kernel accumulating into
 // *fused_grad.
+// The first kernel's work for n splats: the forward's scalars and counters,
+// and (grad_zero, tick non-NULL) a fitter step's prelude.
+static ForwardStart forward_start_of(ds_raster *r, int n, float *loss,
+                                     const ds_splat2d *grad_zero, int32_t *tick) {
+    uint32_t *sc = r->scalars.ptr;
+    const bool multi = r->ninst > 0;
+    ForwardStart f{};
+    f.zero[0] = sc;  // SC_NISECT .. SC_EPOCH - 1, with SC_COUNT = n
+    f.nzero[0] = SC_EPOCH;
+    f.set_word = sc + SC_COUNT;
+    f.set_value = (uint32_t)n;
+    f.zero[1] = sc + SC_ERR;  // the error flag and the tickets
+    f.nzero[1] = SC_TICKET + LB_SLOTS - SC_ERR;
+    f.zero[2] = loss ? (uint32_t *)loss : nullptr;
+    f.nzero[2] = loss ? (uint32_t)(multi ? r->ninst : 1) : 0u;
+    f.zero[3] = r->ghist.ptr;
+    f.nzero[3] = kGhistWords;
+    f.zero[4] = (uint32_t *)r->ranges.ptr;
+    f.nzero[4] = 2u * (uint32_t)r->ntiles;
+    f.zero[5] = r->tseg.ptr;
+    f.nzero[5] = one_pass_tile_sort(r) ? (uint32_t)(kTileSegs * r->ntiles) : 0u;
+    f.epoch = sc + SC_EPOCH;
+    f.tick = tick;
+    if (grad_zero) {
+        const ds_splat2d &g = *grad_zero;
+        float *arrs[10] = {g.pos_x, g.pos_y,   g.pos_z,   g.cov_xx,  g.cov_xy,
+                           g.cov_yy, g.color_r, g.color_g, g.color_b, g.opacity};
+        for (int k = 0; k < 10; ++k) f.grad[k] = arrs[k];
+        f.ngrad = g.count;
+    }
+    return f;
+}
+
 static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_rgb,
                                 const float *target, float *loss, float *grad, hipStream_t s,
                                 const ds_splat2d *fused_grad = nullptr,
-                                StepPrelude pre = StepPrelude{}) {
+                                const ds_splat2d *grad_zero = nullptr, int32_t *tick = nullptr,
+                                bool started = false) {
     const int n = in.count;
     DS_REQUIRE(n >= 0 && n <= r->count_cap, "splat count exceeds the rasterizer's count_cap");
     DS_REQUIRE(r->ninst == 0 || n == r->count_cap,
@@ -890,14 +896,13 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
     const bool multi = r->ninst > 0;
     const MultiInst *mi = multi ? r->mtab.ptr : nullptr;
     const uint16_t *si = multi ? r->sinst.ptr : nullptr;
-    const int zero_words = one_pass_tiles ? kTileSegs * r->ntiles : 0;
-    const uint64_t begin_items = std::max<uint64_t>(
-        {(uint64_t)r->ntiles, (uint64_t)zero_words / 4, kGhistWords / 4, (uint64_t)pre.n / 4});
-    const int nloss = multi ? r->ninst : 1;
-    hipLaunchKernelGGL(k_begin, dim3(grid_cap(begin_items, 256, 1024)), dim3(256), 0, s, sc,
-                       (uint32_t)n, r->ranges.ptr, r->ntiles, loss, nloss, gh, r->tseg.ptr,
-                       zero_words, pre);
-    DS_CHECK_LAUNCH("begin");
+    if (!started) {
+        const ForwardStart f = forward_start_of(r, n, loss, grad_zero, tick);
+        uint64_t words = (uint64_t)f.ngrad;
+        for (int k = 0; k < kStartRanges; ++k) words = std::max<uint64_t>(words, f.nzero[k]);
+        hipLaunchKernelGGL(k_begin, dim3(grid_cap(words / 4, 256, 1024)), dim3(256), 0, s, f);
+        DS_CHECK_LAUNCH("begin");
+    }
     if (n > 0) {
         const bool one_kernel = (uint32_t)n <= kOneKernelMaxKeys;
         if (multi)
@@ -1039,15 +1044,21 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
 
 ds_status raster_step_fwd_bwd(ds_raster *r, const ds_splat2d &in, float *out_rgb,
                               const float *target, float *loss, float *grad_rgb,
-                              const ds_splat2d &grad_in, int32_t *adam_tick, hipStream_t s) {
+                              const ds_splat2d &grad_in, int32_t *adam_tick, hipStream_t s,
+                              bool started) {
     DS_REQUIRE(r, "NULL raster");
     DS_REQUIRE(target && loss && grad_rgb, "the fitter step needs the MSE target");
     DS_REQUIRE(grad_in.count == in.count, "grad_in count differs from the input's");
-    const StepPrelude pre{ZeroList{{grad_in.pos_x, grad_in.pos_y, grad_in.pos_z, grad_in.cov_xx,
-                                    grad_in.cov_xy, grad_in.cov_yy, grad_in.color_r,
-                                    grad_in.color_g, grad_in.color_b, grad_in.opacity}},
-                          grad_in.count, adam_tick};
-    return raster_forward(r, in, out_rgb, target, loss, grad_rgb, s, &grad_in, pre);
+    return raster_forward(r, in, out_rgb, target, loss, grad_rgb, s, &grad_in, &grad_in,
+                          adam_tick, started);
+}
+
+ds_status raster_step_start(ds_raster *r, int n, float *loss, const ds_splat2d &grad_in,
+                            int32_t *adam_tick, ForwardStart *out) {
+    DS_REQUIRE(r && out, "NULL argument");
+    DS_REQUIRE(n >= 0 && n <= r->count_cap, "splat count exceeds the rasterizer's count_cap");
+    *out = forward_start_of(r, n, loss, &grad_in, adam_tick);
+    return DS_OK;
 }
 
 static ds_status raster_backward(ds_raster *r, const ds_splat2d &in, const float *grad_rgb,
