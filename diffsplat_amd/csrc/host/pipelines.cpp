@@ -243,7 +243,7 @@ void ImageFitter::stepOnce() {
         // and compositing backward in one kernel, and Adam in the
         // projection backward; the rasterizer's first-kernel work (scalars,
         // counters, that prelude) runs in the activation/projection kernel:
-        // 12 launches.
+        // 13 launches.
         ds_splat2d gi = ras_layer.getGradInput().view();
         const int n = gaussian_params.count;
         bool started = false;
