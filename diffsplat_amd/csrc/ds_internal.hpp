@@ -188,5 +188,15 @@ ds_status raster_step_start(ds_raster *r, int n, float *loss, const ds_splat2d &
 ds_status launch_activ_project_fwd_start(const ds_gauss3d &in, const ds_splat2d &out, int deg,
                                          Cam3 cam, const PV &pv, const ForwardStart &fs,
                                          hipStream_t s);
+// The same for many fits at once (cameras per instance, as
+// launch_activ_project_fwd_multi), and the fused Adam backward likewise.
+ds_status launch_activ_project_fwd_start_multi(const ds_gauss3d &in, const ds_splat2d &out,
+                                               int deg, const float4 *cam_tab,
+                                               const uint16_t *splat_inst,
+                                               const ForwardStart &fs, hipStream_t s);
+ds_status launch_activ_project_bwd_adam_multi(const ds_gauss3d &in, const ds_splat2d &gout,
+                                              const ds_gauss3d &gin, int deg,
+                                              const float4 *cam_tab, const uint16_t *splat_inst,
+                                              int mode, const AdamFields &af, hipStream_t s);
 
 }  // namespace ds

@@ -683,9 +683,37 @@ void MultiFitter::forwardBackward() {
           "MultiFitter::backward");
 }
 
+// As ImageFitter's step: the rasterizer's first-kernel work (with the
+// zeroing of the Splat2D gradients and the Adam step counter) in the
+// activation/projection kernel and Adam in the projection backward, when the
+// optimiser's groups are the slab's fields.
 void MultiFitter::stepOnce() {
-    forwardBackward();
-    optimizer.step(stream);
+    std::vector<ds_adam_group> tab = optimizer.table();
+    ds::AdamFields af;
+    const ds_gauss3d pv = params.view();
+    const ds_gauss3d gp = grads.view();
+    if (pv.count <= 0 ||
+        !adam_fields_of(tab, pv, gp, sh_degree, optimizer.getParams(), optimizer.stepDevice(),
+                        af)) {
+        forwardBackward();
+        optimizer.step(stream);
+        return;
+    }
+    ds_splat2d gv = gs2.view();
+    ds_splat2d sv = s2.view();
+    ds::ForwardStart fs;
+    check(ds::raster_step_start(ras, pv.count, loss.ptr, gv, optimizer.stepDevice(), &fs),
+          "MultiFitter::step");
+    const float4 *ct = reinterpret_cast<const float4 *>(cams.ptr);
+    check(ds::launch_activ_project_fwd_start_multi(pv, sv, sh_degree, ct, splat_inst, fs, stream),
+          "MultiFitter::step");
+    check(ds::raster_step_fwd_bwd(ras, sv, image.ptr, target.ptr, loss.ptr, grad_rgb.ptr, gv,
+                                  optimizer.stepDevice(), stream, true),
+          "MultiFitter::step");
+    check(ds::launch_activ_project_bwd_adam_multi(pv, gv, gp, sh_degree, ct, splat_inst,
+                                                  grad_mode, af, stream),
+          "MultiFitter::step");
+    optimizer.countStep();
 }
 
 void MultiFitter::step() {

@@ -724,6 +724,23 @@ ds_status launch_activ_project_fwd_start(const ds_gauss3d &in, const ds_splat2d 
     return activ_project_fwd<CamUniform, true>(in, out, deg, CamUniform{cam, pv}, s, fs);
 }
 
+ds_status launch_activ_project_fwd_start_multi(const ds_gauss3d &in, const ds_splat2d &out,
+                                               int deg, const float4 *cam_tab,
+                                               const uint16_t *splat_inst,
+                                               const ForwardStart &fs, hipStream_t s) {
+    if (in.count <= 0) return set_error(DS_ERR_INVALID_ARG, "forward start needs splats");
+    return activ_project_fwd<CamTable, true>(in, out, deg, CamTable{cam_tab, splat_inst}, s, fs);
+}
+
+ds_status launch_activ_project_bwd_adam_multi(const ds_gauss3d &in, const ds_splat2d &go,
+                                              const ds_gauss3d &gi, int deg,
+                                              const float4 *cam_tab, const uint16_t *splat_inst,
+                                              int mode, const AdamFields &af, hipStream_t s) {
+    if (!af.step) return set_error(DS_ERR_INVALID_ARG, "fused Adam step without a step counter");
+    return activ_project_bwd<CamTable, true>(in, go, gi, deg, CamTable{cam_tab, splat_inst},
+                                             mode, s, af);
+}
+
 ds_status launch_activ_project_bwd_adam(const ds_gauss3d &in, const ds_splat2d &go,
                                         const ds_gauss3d &gi, int deg, Cam3 cam, const PV &pv,
                                         int mode, const AdamFields &af, hipStream_t s) {
