@@ -515,6 +515,18 @@ __global__ __launch_bounds__(kBlock) void k_activ_project_bwd(ds_gauss3d in, ds_
     }
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= in.count) return;
+    // The 14 per-splat fields' parameters and moments are loaded up front so
+    // their latency overlaps the gradient computation (nothing before the
+    // Adam step writes them).
+    float ap[14], am[14], av[14];
+    if constexpr (ADAM) {
+#pragma unroll
+        for (int f = 0; f < 14; ++f) {
+            ap[f] = af.p[f][i];
+            am[f] = af.m[f][i];
+            av[f] = af.v[f][i];
+        }
+    }
     // Adam step of field f at element e of its slab, gradient g
     auto adam = [&](int f, int64_t e, float g) {
         float pp = af.p[f][e], mm = af.m[f][e], vv = af.v[f][e];
@@ -522,6 +534,13 @@ __global__ __launch_bounds__(kBlock) void k_activ_project_bwd(ds_gauss3d in, ds_
         af.p[f][e] = pp;
         af.m[f][e] = mm;
         af.v[f][e] = vv;
+    };
+    // the same for one of the 14 prefetched fields (element i)
+    auto adam14 = [&](int f, float g) {
+        adam_elem(ap[f], am[f], av[f], g, s_lr[f], s_bc[0], s_bc[1], af.beta1, af.beta2, af.eps);
+        af.p[f][i] = ap[f];
+        af.m[f][i] = am[f];
+        af.v[f][i] = av[f];
     };
     Cam3 cam;
     PV pv;
@@ -536,7 +555,7 @@ __global__ __launch_bounds__(kBlock) void k_activ_project_bwd(ds_gauss3d in, ds_
         sh_backward<DEG>(gi, i, px, py, pz, cam, 0.f, 0.f, 0.f);
         if constexpr (ADAM) {  // a zero gradient still moves the moments
 #pragma unroll
-            for (int f = 0; f < 14; ++f) adam(f, i, 0.f);
+            for (int f = 0; f < 14; ++f) adam14(f, 0.f);
             constexpr int nb = sh_bands(DEG);
             for (int k = 0; k < nb; ++k)
                 for (int c = 0; c < 3; ++c) adam(14 + c, (int64_t)k * in.count + i, 0.f);
@@ -576,17 +595,17 @@ __global__ __launch_bounds__(kBlock) void k_activ_project_bwd(ds_gauss3d in, ds_
     gi.logit_opacity[i] = g_op;
     if constexpr (ADAM) {
         // every read of this splat's parameters is done: step them in place
-        adam(0, i, gp[0]);
-        adam(1, i, gp[1]);
-        adam(2, i, gp[2]);
+        adam14(0, gp[0]);
+        adam14(1, gp[1]);
+        adam14(2, gp[2]);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) adam(3 + c, i, g.ds[c]);
+        for (int c = 0; c < 3; ++c) adam14(3 + c, g.ds[c]);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) adam(6 + c, i, g.dq[c]);
-        adam(10, i, dR * SH_C0);
-        adam(11, i, dG * SH_C0);
-        adam(12, i, dB * SH_C0);
-        adam(13, i, g_op);
+        for (int c = 0; c < 4; ++c) adam14(6 + c, g.dq[c]);
+        adam14(10, dR * SH_C0);
+        adam14(11, dG * SH_C0);
+        adam14(12, dB * SH_C0);
+        adam14(13, g_op);
         constexpr int nb = sh_bands(DEG);
         const float *rest[3] = {gi.sh_rest_r, gi.sh_rest_g, gi.sh_rest_b};
         for (int k = 0; k < nb; ++k)
