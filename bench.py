@@ -43,7 +43,7 @@ BWD_FLOP_PER_EVAL = 60.0
 # this workload, scaled by a 256 MiB copy (fetch x2.000, write x1.000): see
 # profiles/fit_720p_60k_counters.md. Counters cannot be read live here; most
 # write bytes are the backward's device-scope float atomics.
-FUSED_TRAFFIC_READ = 58.45e6
+FUSED_TRAFFIC_READ = 57.66e6
 FUSED_TRAFFIC_WRITE = 178.29e6
 FUSED_TRAFFIC_STATE = ("fit step 32: the 20 roofline launches of a 30-step counter run "
                        "(longer counter runs crash inside the profiler on the host)")
