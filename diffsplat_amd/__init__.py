@@ -13,9 +13,9 @@ import math
 import numpy as np
 
 from ._lib import (AdamGroup, AdamHParams, CameraInput, CameraSpeeds, DeviceArray,
-                   DiffsplatError, FitterConfig, MultiFitInstance,
+                   DiffsplatError, FitterConfig, Gauss3D, MultiFitInstance,
                    RasterConfig, RasterInstance, RasterView, check, download_ptr, farr, fptr,
-                   gauss_view, has_gpu, lib, splat2d_view, splat3d_view)
+                   Splat2D, gauss_view, has_gpu, lib, splat2d_view, splat3d_view)
 
 __all__ = [
     "lib", "has_gpu", "check", "DiffsplatError", "DeviceArray", "gauss_view", "splat3d_view",
@@ -440,6 +440,19 @@ class ImageFitter:
         check(lib().ds_fitter_download_grads(self.h, fptr(out)), "ds_fitter_download_grads")
         return out
 
+    def moments(self):
+        """Adam moments (m, v) in the parameter slab's layout."""
+        m = np.empty(slab_floats(self.count, self.sh_degree), np.float32)
+        v = np.empty_like(m)
+        check(lib().ds_fitter_download_moments(self.h, fptr(m), fptr(v)),
+              "ds_fitter_download_moments")
+        return m, v
+
+    def step_count(self) -> int:
+        c = C.c_int32()
+        check(lib().ds_fitter_step_count(self.h, C.byref(c)), "ds_fitter_step_count")
+        return c.value
+
     def step(self, n: int = 1) -> None:
         check(lib().ds_fitter_step(self.h, n), "ds_fitter_step")
 
@@ -722,6 +735,43 @@ class MultiFitter:
         s = C.c_void_p()
         check(lib().ds_multifit_get_stream(self.h, C.byref(s)), "ds_multifit_get_stream")
         return s.value
+
+    def bases(self):
+        """Per instance: its first global tile, first pixel and first splat."""
+        tb, pb, sb = [], [], []
+        t = p = q = 0
+        for w, h, n in self.inst:
+            tb.append(t)
+            pb.append(p)
+            sb.append(q)
+            t += ((w + 15) // 16) * ((h + 15) // 16)
+            p += w * h
+            q += n
+        return tb, pb, sb
+
+    def raster_state(self):
+        """Forward state of the last step over all instances (synchronises)."""
+        r = C.c_void_p()
+        check(lib().ds_multifit_get_raster(self.h, C.byref(r)), "ds_multifit_get_raster")
+        ras = MultiRasterizer.__new__(MultiRasterizer)
+        ras.sizes = list(self.inst)
+        ras.tile_base, ras.pix_base, ras.splat_base = self.bases()
+        ras.num_tiles = sum(((w + 15) // 16) * ((h + 15) // 16) for w, h, _ in self.inst)
+        ras.num_pixels = sum(w * h for w, h, _ in self.inst)
+        ras.num_splats = sum(n for _, _, n in self.inst)
+        ras.h, ras.width, ras.height = r.value, 0, 0
+        try:
+            return ras.state()
+        finally:
+            ras.h = None
+
+    def views(self):
+        """Device views (Splat2D, dL/dSplat2D, dL/d(image) pointer) over all
+        instances."""
+        g, s2, gs2, grad = Gauss3D(), Splat2D(), Splat2D(), C.c_void_p()
+        check(lib().ds_multifit_views(self.h, C.byref(g), C.byref(s2), C.byref(gs2),
+                                      C.byref(grad)), "ds_multifit_views")
+        return s2, gs2, grad.value
 
     def raster_counts(self):
         r = C.c_void_p()

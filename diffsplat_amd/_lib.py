@@ -176,6 +176,7 @@ _SIGS = {
     "ds_fitter_upload_params": (C.c_int, [_P, _F]),
     "ds_fitter_download_params": (C.c_int, [_P, _F]),
     "ds_fitter_download_grads": (C.c_int, [_P, _F]),
+    "ds_fitter_download_moments": (C.c_int, [_P, _F, _F]),
     "ds_fitter_step": (C.c_int, [_P, C.c_int]),
     "ds_fitter_forward": (C.c_int, [_P]),
     "ds_fitter_loss": (C.c_int, [_P, _F]),
@@ -244,6 +245,8 @@ _SIGS = {
     "ds_multifit_step_count": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "ds_multifit_get_raster": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
     "ds_multifit_get_stream": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
+    "ds_multifit_views": (C.c_int, [_P, C.POINTER(Gauss3D), C.POINTER(Splat2D),
+                                    C.POINTER(Splat2D), C.POINTER(C.c_void_p)]),
     "ds_renderer_render": (C.c_int, [_P, _F, _F, _F]),
     "ds_renderer_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), _F]),
     "ds_renderer_download_image": (C.c_int, [_P, _F]),

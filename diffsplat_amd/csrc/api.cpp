@@ -460,6 +460,15 @@ ds_status ds_fitter_download_grads(ds_fitter *f, float *host_slab) {
     DS_GUARD_END
 }
 
+ds_status ds_fitter_download_moments(ds_fitter *f, float *host_m, float *host_v) {
+    clear_error();
+    DS_REQUIRE(f && (host_m || host_v), "NULL argument");
+    DS_GUARD_BEGIN
+    f->impl.downloadMoments(host_m, host_v);
+    return DS_OK;
+    DS_GUARD_END
+}
+
 ds_status ds_fitter_step(ds_fitter *f, int n_steps) {
     clear_error();
     DS_REQUIRE(f && n_steps >= 0, "bad argument");
@@ -777,6 +786,16 @@ ds_status ds_multifit_get_raster(ds_multifit *m, ds_raster **r) {
     DS_REQUIRE(m && r, "NULL argument");
     *r = m->impl.raster();
     return DS_OK;
+}
+
+ds_status ds_multifit_views(ds_multifit *m, ds_gauss3d *params, ds_splat2d *splat2d,
+                            ds_splat2d *grad_splat2d, float **grad_rgb) {
+    clear_error();
+    DS_REQUIRE(m, "NULL multifit");
+    DS_GUARD_BEGIN
+    m->impl.views(params, splat2d, grad_splat2d, grad_rgb);
+    return DS_OK;
+    DS_GUARD_END
 }
 
 ds_status ds_multifit_get_stream(ds_multifit *m, ds_stream_t *s) {

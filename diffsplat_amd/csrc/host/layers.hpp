@@ -354,6 +354,10 @@ public:
     int getStepCount() const { return step_count; }
     const AdamParams &getParams() const { return hp; }
     int32_t *stepDevice() const { return step_dev.ptr; }
+    // First and second moments of every group, in the order the groups were
+    // added (one element per parameter).
+    const DeviceBuffer<float> &moment1() const { return m; }
+    const DeviceBuffer<float> &moment2() const { return v; }
     // The groups as the C-ABI takes them, for callers that launch the step
     // themselves (then countStep() keeps the host count).
     std::vector<ds_adam_group> table() {

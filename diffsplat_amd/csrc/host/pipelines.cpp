@@ -134,6 +134,23 @@ void ImageFitter::downloadGrads(float *host_slab) const {
     g.slab.download(host_slab, g.slab.size);
 }
 
+void ImageFitter::downloadMoments(float *host_m, float *host_v) const {
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    // registerGaussian3DGroups adds the 13 leading fields, then the opacity,
+    // then the SH rest bands; the slab keeps the opacity last.
+    const size_t n = (size_t)opt.count;
+    const size_t rest = 3 * n * (size_t)ds_sh_bands(opt.sh_degree);
+    std::vector<float> tmp(optimizer.moment1().size);
+    for (int which = 0; which < 2; ++which) {
+        float *dst = which ? host_v : host_m;
+        if (!dst) continue;
+        (which ? optimizer.moment2() : optimizer.moment1()).download(tmp.data(), tmp.size());
+        std::copy(tmp.begin(), tmp.begin() + 13 * n, dst);
+        std::copy(tmp.begin() + 14 * n, tmp.begin() + 14 * n + rest, dst + 13 * n);
+        std::copy(tmp.begin() + 13 * n, tmp.begin() + 14 * n, dst + 13 * n + rest);
+    }
+}
+
 void ImageFitter::initLayers() {
     const int n = opt.count;
     pipeline.clear();

@@ -39,6 +39,8 @@ public:
     void uploadParams(const float *host_slab);  // also resets Adam
     void downloadParams(float *host_slab) const;
     void downloadGrads(float *host_slab) const;
+    // Adam moments m and v in the parameter slab's layout (either may be NULL).
+    void downloadMoments(float *host_m, float *host_v) const;
     void initLayers();
 
     void step();          // zeroGrad, forward, backward, Adam
@@ -173,6 +175,14 @@ public:
     hipStream_t getStream() const { return stream; }
     ds_raster *raster() const { return ras; }
     int64_t totalSplats() const { return n_total; }
+    // Device views over all instances (instance k's splats are elements
+    // [splat base, + count) of every field, its pixels [pixel base, + W H)).
+    void views(ds_gauss3d *p, ds_splat2d *s2v, ds_splat2d *gs2v, float **grad) {
+        if (p) *p = params.view();
+        if (s2v) *s2v = s2.view();
+        if (gs2v) *gs2v = gs2.view();
+        if (grad) *grad = grad_rgb.ptr;
+    }
 
 private:
     void stepOnce();

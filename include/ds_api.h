@@ -418,6 +418,9 @@ ds_status ds_fitter_upload_params(ds_fitter *f, const float *host_slab);
 ds_status ds_fitter_download_params(ds_fitter *f, float *host_slab);
 /* Same layout, gradients of the last step. */
 ds_status ds_fitter_download_grads(ds_fitter *f, float *host_slab);
+/* Adam first and second moments (Adam::step state, src/optimizers/adam.cu:12-73)
+ * in the same slab layout; either pointer may be NULL (synchronises). */
+ds_status ds_fitter_download_moments(ds_fitter *f, float *host_m, float *host_v);
 /* ImageFitter::step (src/pipelines/image_fitter.cpp:103-112), n_steps times,
  * asynchronously on the fitter's stream. */
 ds_status ds_fitter_step(ds_fitter *f, int n_steps);
@@ -581,6 +584,12 @@ ds_status ds_multifit_autotune(ds_multifit *m, int reps, int *fwd_pix_per_thread
 ds_status ds_multifit_step_count(ds_multifit *m, int32_t *step);
 ds_status ds_multifit_get_raster(ds_multifit *m, ds_raster **r);
 ds_status ds_multifit_get_stream(ds_multifit *m, ds_stream_t *s);
+/* Device views of the multifit's own buffers over all instances, as
+ * ds_fitter_views: instance k's splats are elements [sum of the counts of
+ * instances < k, + its count) of every field; its pixels follow the pixels
+ * of instances < k in dL/d(image). Any pointer may be NULL. */
+ds_status ds_multifit_views(ds_multifit *m, ds_gauss3d *params, ds_splat2d *splat2d,
+                            ds_splat2d *grad_splat2d, float **grad_rgb);
 
 /* ------------------------------------------------------------------ */
 /* Forward-only renderer (SplatRenderer, src/pipelines/splat_renderer.{h,cpp}) */

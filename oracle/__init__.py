@@ -41,7 +41,8 @@ def lib():
             getattr(h, f"or_comp_fwd_{suf}").argtypes = [_P, C.c_int, _P, _P, C.c_int, C.c_int,
                                                          _P, _P, _P, _P, _P]
             getattr(h, f"or_comp_bwd_{suf}").argtypes = [_P, C.c_int, _P, _P, C.c_int, C.c_int,
-                                                         _P, _P, _P, _P, _P, C.c_int]
+                                                         _P, _P, _P, _P, _P, C.c_int, _P,
+                                                         C.c_double, _P]
             getattr(h, f"or_mse_{suf}").argtypes = [_P, _P, C.c_int64, _P]
             getattr(h, f"or_mse_{suf}").restype = T
             getattr(h, f"or_adam_{suf}").argtypes = [_P, _P, _P, _P, C.c_int64, T, T, T, T,
@@ -169,9 +170,15 @@ def comp_fwd(s2, n, flat, ranges, W, H, dtype=np.float64):
     return {"rgb": rgb, "T": T, "last": last, "margin": margin, "E": int(e.value)}
 
 
-def comp_bwd(s2, n, flat, ranges, W, H, T, last, dLdC, dtype=np.float64, grad_mode=0):
+def comp_bwd(s2, n, flat, ranges, W, H, T, last, dLdC, dtype=np.float64, grad_mode=0,
+             px_margin=None, amb_margin=5e-5):
     """grad_mode 0: exact dL/dalpha; 1: the reference's formula
-    (tile_rasterize_layer.cu:491-495, colour behind not renormalised)."""
+    (tile_rasterize_layer.cu:491-495, colour behind not renormalised).
+
+    With px_margin (comp_fwd's "margin"), also returns an allowance per
+    gradient element: the summed magnitude of the terms that pixels with a
+    decision within amb_margin of its threshold contribute (what a decision
+    flip between fp32 and this evaluation may change)."""
     s = _a(s2, dtype)
     f = _a(flat, np.uint32)
     r = _a(ranges, np.int32)
@@ -180,10 +187,15 @@ def comp_bwd(s2, n, flat, ranges, W, H, T, last, dLdC, dtype=np.float64, grad_mo
     g = _a(dLdC, dtype)
     gs2 = np.zeros(10 * n, dtype)
     e = C.c_int64()
-    getattr(lib(), f"or_comp_bwd_{_suffix(dtype)}")(s.ctypes.data, n, f.ctypes.data,
-                                                   r.ctypes.data, W, H, t.ctypes.data,
-                                                   l.ctypes.data, g.ctypes.data,
-                                                   gs2.ctypes.data, C.byref(e), int(grad_mode))
+    pm = _a(px_margin, np.float64) if px_margin is not None else None
+    amb = np.zeros(10 * n, dtype) if pm is not None else None
+    getattr(lib(), f"or_comp_bwd_{_suffix(dtype)}")(
+        s.ctypes.data, n, f.ctypes.data, r.ctypes.data, W, H, t.ctypes.data, l.ctypes.data,
+        g.ctypes.data, gs2.ctypes.data, C.byref(e), int(grad_mode),
+        pm.ctypes.data if pm is not None else None, C.c_double(amb_margin),
+        amb.ctypes.data if amb is not None else None)
+    if amb is not None:
+        return gs2, int(e.value), amb
     return gs2, int(e.value)
 
 

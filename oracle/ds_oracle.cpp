@@ -14,7 +14,9 @@
 //
 // Compositing records, per pixel, the smallest relative margin of any
 // discrete decision it took (alpha >= 1/255, o*g <= 0.99, T <= 1e-4), so a
-// test can tell decision flips between fp32 and FP64 from real errors.
+// test can tell decision flips between fp32 and FP64 from real errors. The
+// backward can bound, per gradient element, how much of it comes from such
+// pixels (the terms a decision flip there may change).
 //
 // Reference functions restated (file:line in the reference tree):
 //   act_fwd        src/layers/gauss_activ_layer.cu:37-170
@@ -524,12 +526,77 @@ inline void atomic_add(T *p, T v) {
 // grad_mode 1: the reference's T_before * (c - colour behind)
 //              (tile_rasterize_layer.cu:491-495), which drops the 1/T_{k+1}
 //              normalisation of the colour behind.
+//
+// With px_margin (comp_fwd's per-pixel margins) and gs2_amb: for every pixel
+// whose forward took a decision within amb_margin of its threshold, gs2_amb
+// receives a bound on how much a flip of that decision in fp32 changes each
+// gradient term the pixel contributes:
+//   - a splat whose own alpha-threshold or clamp decision is within the
+//     margin: its whole term (the threshold switches the term on or off, the
+//     clamp switches its falloff and conic parts);
+//   - every other splat at that pixel: 2 a_f / (1 - a_f) of its term's
+//     magnitude, where a_f is the largest alpha of a threshold flipper (a
+//     splat of alpha a_f appearing or vanishing scales T behind it by
+//     1 - a_f and moves the colour behind the splats in front by at most
+//     a_f T c); clamp flips move alpha by rounding only;
+//   - a termination (T <= 1e-4) within the margin: the whole term of every
+//     splat at the pixel, and of the next contributor after the stop, which
+//     fp32 may still include.
+template <typename T>
+struct BwdTerms {
+    T v[9];    // fields: 0 x, 1 y, 2 cxx, 3 cxy, 4 cyy, 5 r, 6 g, 7 b, 8 o
+    T mag[9];  // magnitude of each term (dL/dalpha's two products summed apart)
+};
+
+template <typename T>
+inline BwdTerms<T> bwd_terms(const Rec<T> &R, T dx, T dy, T d2, T g, T alpha, T Tb, T ra,
+                             T ar, T ag, T ab, T gr, T gg, T gb, int grad_mode) {
+    BwdTerms<T> t{};
+    const T fac = alpha * Tb;
+    t.v[5] = fac * gr;
+    t.v[6] = fac * gg;
+    t.v[7] = fac * gb;
+    for (int f = 5; f < 8; ++f) t.mag[f] = std::fabs(t.v[f]);
+    if (R.o * g <= (T)kMaxAlpha) {
+        const T va = grad_mode == 1
+                         ? Tb * ((R.r - ar) * gr + (R.g - ag) * gg + (R.bl - ab) * gb)
+                         : Tb * (R.r * gr + R.g * gg + R.bl * gb) -
+                               ra * (ar * gr + ag * gg + ab * gb);
+        const T va_mag =
+            grad_mode == 1
+                ? Tb * ((std::fabs(R.r) + std::fabs(ar)) * std::fabs(gr) +
+                        (std::fabs(R.g) + std::fabs(ag)) * std::fabs(gg) +
+                        (std::fabs(R.bl) + std::fabs(ab)) * std::fabs(gb))
+                : Tb * (std::fabs(R.r * gr) + std::fabs(R.g * gg) + std::fabs(R.bl * gb)) +
+                      ra * (std::fabs(ar * gr) + std::fabs(ag * gg) + std::fabs(ab * gb));
+        t.v[8] = g * va;
+        t.mag[8] = g * va_mag;
+        const T vd2 = (T)0.5 * (-R.o * g * va);
+        const T vd2_mag = (T)0.5 * R.o * g * va_mag;
+        const T ddx = 2 * (dx * R.a + dy * R.b);
+        const T ddy = 2 * (dx * R.b + dy * R.c);
+        t.v[0] = vd2 * ddx;
+        t.v[1] = vd2 * ddy;
+        t.v[2] = vd2 * ((T)-0.25 * ddx * ddx);
+        t.v[4] = vd2 * ((T)-0.25 * ddy * ddy);
+        t.v[3] = vd2 * ((T)-2 * R.b * d2 - 2 * dx * dy * R.idet);
+        t.mag[0] = vd2_mag * std::fabs(ddx);
+        t.mag[1] = vd2_mag * std::fabs(ddy);
+        t.mag[2] = vd2_mag * ((T)0.25 * ddx * ddx);
+        t.mag[4] = vd2_mag * ((T)0.25 * ddy * ddy);
+        t.mag[3] = vd2_mag * (std::fabs((T)2 * R.b * d2) + std::fabs(2 * dx * dy * R.idet));
+    }
+    return t;
+}
+
 template <typename T>
 void comp_bwd(const T *s2, int n, const uint32_t *flat, const int32_t *ranges, int W, int H,
               const T *Tfin, const int32_t *last, const T *dLdC, T *gs2, int64_t *e_bwd,
-              int grad_mode) {
+              int grad_mode, const double *px_margin = nullptr, double amb_margin = 0.0,
+              T *gs2_amb = nullptr) {
     const int ntx = (W + 15) / 16, nty = (H + 15) / 16;
     int64_t E = 0;
+    const bool want_amb = px_margin && gs2_amb;
     // gs2 fields: 0 x, 1 y, (2 z untouched), 3 cxx, 4 cxy, 5 cyy, 6 r, 7 g, 8 b, 9 o
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : E)
     for (int tile = 0; tile < ntx * nty; ++tile) {
@@ -540,6 +607,7 @@ void comp_bwd(const T *s2, int n, const uint32_t *flat, const int32_t *ranges, i
         recs.reserve(re - rs);
         for (int k = rs; k < re; ++k) recs.push_back(rec_of<T>(s2, n, flat[k]));
         std::vector<T> acc((size_t)(re - rs) * 9, (T)0);
+        std::vector<T> amb(want_amb ? (size_t)(re - rs) * 9 : 0, (T)0);
         for (int ly = 0; ly < 16; ++ly) {
             for (int lx = 0; lx < 16; ++lx) {
                 const int i = ty * 16 + ly, j = tx * 16 + lx;
@@ -549,8 +617,49 @@ void comp_bwd(const T *s2, int n, const uint32_t *flat, const int32_t *ranges, i
                 T Tr = Tfin[pix];
                 const int32_t lst = last[pix];
                 const T gr = dLdC[3 * pix], gg = dLdC[3 * pix + 1], gb = dLdC[3 * pix + 2];
+                const bool pamb = want_amb && px_margin[pix] <= amb_margin;
+                // at an ambiguous pixel: which kind of decision is near its
+                // threshold (a front-to-back pass as in comp_fwd)
+                bool term_amb = false;
+                T other_scale = 0;
+                if (pamb) {
+                    T Tf = 1, af = 0;
+                    for (int k = rs; k < re; ++k) {
+                        const Rec<T> &R = recs[(size_t)(k - rs)];
+                        const T dx = R.x - px, dy = R.y - py;
+                        const T d2 = dx * dx * R.a + 2 * dx * dy * R.b + dy * dy * R.c;
+                        if ((T)0.5 * d2 < 0) continue;
+                        const T og = R.o * std::exp((T)-0.5 * d2);
+                        const T alpha = std::min((T)kMaxAlpha, og);
+                        if (rel_margin(alpha, kAlphaThres) <= amb_margin)
+                            af = std::max(af, std::max(alpha, (T)kAlphaThres));
+                        if (alpha < (T)kAlphaThres) continue;
+                        Tf = Tf * (1 - alpha);
+                        if (rel_margin(Tf, kTThres) <= amb_margin) term_amb = true;
+                        if (Tf <= (T)kTThres) break;
+                    }
+                    other_scale = term_amb ? (T)1 : 2 * af / (1 - af);
+                }
                 T ar = 0, ag = 0, ab = 0;
                 const int kstart = std::min(re - 1, (int)lst);
+                if (pamb && term_amb && Tr <= (T)kTThres) {
+                    // the next contributor after a near-threshold stop
+                    for (int k = kstart + 1; k < re; ++k) {
+                        const Rec<T> &R = recs[(size_t)(k - rs)];
+                        const T dx = R.x - px, dy = R.y - py;
+                        const T d2 = dx * dx * R.a + 2 * dx * dy * R.b + dy * dy * R.c;
+                        if ((T)0.5 * d2 < 0) continue;
+                        const T g = std::exp((T)-0.5 * d2);
+                        const T alpha = std::min((T)kMaxAlpha, R.o * g);
+                        if (alpha < (T)kAlphaThres) continue;
+                        const T ra = (T)1 / std::max((T)1 - alpha, (T)1e-6);
+                        const BwdTerms<T> t = bwd_terms(R, dx, dy, d2, g, alpha, Tr, ra, (T)0,
+                                                        (T)0, (T)0, gr, gg, gb, grad_mode);
+                        T *B = &amb[(size_t)(k - rs) * 9];
+                        for (int f = 0; f < 9; ++f) B[f] += t.mag[f];
+                        break;
+                    }
+                }
                 for (int k = kstart; k >= rs; --k) {
                     ++E;
                     const Rec<T> &R = recs[(size_t)(k - rs)];
@@ -560,30 +669,40 @@ void comp_bwd(const T *s2, int n, const uint32_t *flat, const int32_t *ranges, i
                     if (sigma < 0) continue;
                     const T g = std::exp(-sigma);
                     const T alpha = std::min((T)kMaxAlpha, R.o * g);
-                    if (alpha < (T)kAlphaThres) continue;
+                    if (alpha < (T)kAlphaThres) {
+                        if (pamb && rel_margin(alpha, kAlphaThres) <= amb_margin) {
+                            // a splat fp32 may count: its terms at T before it
+                            const T ra = (T)1 / ((T)1 - (T)kAlphaThres);
+                            const BwdTerms<T> t =
+                                bwd_terms(R, dx, dy, d2, g, (T)kAlphaThres, Tr * ra, ra, ar, ag,
+                                          ab, gr, gg, gb, grad_mode);
+                            T *B = &amb[(size_t)(k - rs) * 9];
+                            for (int f = 0; f < 9; ++f) B[f] += t.mag[f];
+                        }
+                        continue;
+                    }
                     const T ra = (T)1 / std::max((T)1 - alpha, (T)1e-6);
                     const T Tb = Tr * ra;
                     const T fac = alpha * Tb;
+                    const BwdTerms<T> t =
+                        bwd_terms(R, dx, dy, d2, g, alpha, Tb, ra, ar, ag, ab, gr, gg, gb,
+                                  grad_mode);
                     T *A = &acc[(size_t)(k - rs) * 9];
-                    A[5] += fac * gr;
-                    A[6] += fac * gg;
-                    A[7] += fac * gb;
-                    const T va =
-                        grad_mode == 1
-                            ? Tb * ((R.r - ar) * gr + (R.g - ag) * gg + (R.bl - ab) * gb)
-                            : Tb * (R.r * gr + R.g * gg + R.bl * gb) -
-                                  ra * (ar * gr + ag * gg + ab * gb);
-                    if (R.o * g <= (T)kMaxAlpha) {
-                        A[8] += g * va;
-                        const T vs = -R.o * g * va;
-                        const T vd2 = (T)0.5 * vs;
-                        const T ddx = 2 * (dx * R.a + dy * R.b);
-                        const T ddy = 2 * (dx * R.b + dy * R.c);
-                        A[0] += vd2 * ddx;
-                        A[1] += vd2 * ddy;
-                        A[2] += vd2 * ((T)-0.25 * ddx * ddx);
-                        A[4] += vd2 * ((T)-0.25 * ddy * ddy);
-                        A[3] += vd2 * ((T)-2 * R.b * d2 - 2 * dx * dy * R.idet);
+                    for (int f = 0; f < 9; ++f) A[f] += t.v[f];
+                    if (pamb) {
+                        T *B = &amb[(size_t)(k - rs) * 9];
+                        const bool own = rel_margin(alpha, kAlphaThres) <= amb_margin ||
+                                         rel_margin(R.o * g, kMaxAlpha) <= amb_margin;
+                        const T sc = own ? (T)1 : other_scale;
+                        for (int f = 0; f < 9; ++f) B[f] += sc * t.mag[f];
+                        if (rel_margin(R.o * g, kMaxAlpha) <= amb_margin &&
+                            R.o * g > (T)kMaxAlpha) {
+                            // clamped, but fp32 may not clamp: the gated terms
+                            const BwdTerms<T> u =
+                                bwd_terms(R, dx, dy, d2, g * ((T)kMaxAlpha / (R.o * g)), alpha,
+                                          Tb, ra, ar, ag, ab, gr, gg, gb, grad_mode);
+                            for (int f = 0; f < 9; ++f) B[f] += u.mag[f];
+                        }
                     }
                     ar += R.r * fac;
                     ag += R.g * fac;
@@ -599,6 +718,11 @@ void comp_bwd(const T *s2, int n, const uint32_t *flat, const int32_t *ranges, i
             const T *A = &acc[(size_t)(k - rs) * 9];
             for (int f = 0; f < 9; ++f)
                 if (A[f] != 0) atomic_add(&gs2[(size_t)dst[f] * n + gi], A[f]);
+            if (want_amb) {
+                const T *B = &amb[(size_t)(k - rs) * 9];
+                for (int f = 0; f < 9; ++f)
+                    if (B[f] != 0) atomic_add(&gs2_amb[(size_t)dst[f] * n + gi], B[f]);
+            }
         }
     }
     if (e_bwd) *e_bwd = E;
@@ -752,8 +876,10 @@ extern "C" {
     }                                                                                         \
     void or_comp_bwd_##SUF(const T *s2, int n, const uint32_t *flat, const int32_t *ranges,   \
                            int W, int H, const T *Tfin, const int32_t *last, const T *dLdC,   \
-                           T *gs2, int64_t *e_bwd, int grad_mode) {                           \
-        comp_bwd<T>(s2, n, flat, ranges, W, H, Tfin, last, dLdC, gs2, e_bwd, grad_mode);      \
+                           T *gs2, int64_t *e_bwd, int grad_mode, const double *px_margin,    \
+                           double amb_margin, T *gs2_amb) {                                   \
+        comp_bwd<T>(s2, n, flat, ranges, W, H, Tfin, last, dLdC, gs2, e_bwd, grad_mode,       \
+                    px_margin, amb_margin, gs2_amb);                                          \
     }                                                                                         \
     T or_mse_##SUF(const T *rgb, const T *tgt, int64_t npix, T *grad) {                       \
         return mse<T>(rgb, tgt, npix, grad);                                                  \

@@ -95,13 +95,58 @@ def fp64_forward(params, n, deg, cam, pv, W, H, target, pairs=None):
     return dict(s3=s3, s2=s2, ta=ta, comp=comp, loss=loss, dLdC=dLdC)
 
 
-def fp64_backward(fw, params, n, deg, cam, pv, W, H, grad_mode=0):
-    gs2, _ = oracle.comp_bwd(fw["s2"], n, fw["ta"]["ids"], fw["ta"]["ranges"], W, H,
-                             fw["comp"]["T"], fw["comp"]["last"], fw["dLdC"],
-                             grad_mode=grad_mode)
+def fp64_forward_on(s2_f32, params, n, deg, cam, pv, W, H, target, pairs):
+    """FP64 forward whose compositing runs on a given fp32 Splat2D (the GPU's
+    own projection output): the discrete decisions of compositing then see
+    the same inputs as the GPU, and only the arithmetic differs. The FP64
+    activation and projection of `params` are returned too ("s2_fp64"), so
+    the fp32 Splat2D can itself be checked against them; the backward of
+    the result chains through the FP64 activation state ("s3")."""
+    s3 = oracle.act_fwd(params, n, deg, cam)
+    s2_64 = oracle.proj_fwd(s3, n, pv)
+    s2 = np.asarray(s2_f32, np.float32).astype(np.float64)
+    comp = oracle.comp_fwd(s2, n, pairs["ids"], pairs["ranges"], W, H)
+    loss, dLdC = oracle.mse(comp["rgb"], target, W * H)
+    return dict(s3=s3, s2=s2, s2_fp64=s2_64, ta=pairs, comp=comp, loss=loss, dLdC=dLdC)
+
+
+def fp64_backward(fw, params, n, deg, cam, pv, W, H, grad_mode=0, amb_margin=None):
+    """FP64 backward of fp64_forward. With amb_margin, "gs2_amb" is the
+    decision-flip allowance of each dL/dSplat2D element (see oracle.comp_bwd)
+    and "amb" flags the splats that have one."""
+    r = oracle.comp_bwd(fw["s2"], n, fw["ta"]["ids"], fw["ta"]["ranges"], W, H,
+                        fw["comp"]["T"], fw["comp"]["last"], fw["dLdC"], grad_mode=grad_mode,
+                        px_margin=fw["comp"]["margin"] if amb_margin is not None else None,
+                        amb_margin=amb_margin if amb_margin is not None else 0.0)
+    gs2 = r[0]
     gs3 = oracle.proj_bwd(fw["s3"], gs2, n, pv, grad_mode=grad_mode)
     gp = oracle.act_bwd(params, fw["s3"], gs3, n, deg, cam, grad_mode=grad_mode)
-    return dict(gs2=gs2, gs3=gs3, gparams=gp)
+    out = dict(gs2=gs2, gs3=gs3, gparams=gp)
+    if amb_margin is not None:
+        out["gs2_amb"] = r[2]
+        out["amb"] = (r[2].reshape(10, n) > 0).any(axis=0)
+    return out
+
+
+def param_allowance(fw, params, n, deg, cam, pv, gs2_amb, grad_mode=0):
+    """Propagates a dL/dSplat2D allowance to the parameter gradients. The
+    projection and activation backward map each splat's ten Splat2D
+    gradients linearly to its parameter gradients; the allowance of a
+    parameter gradient is sum_f |d g_param / d g_splat2d_f| * allowance_f,
+    with the Jacobian columns found by pushing unit Splat2D gradients
+    through the FP64 backward."""
+    out = np.zeros(oracle.bands(deg) * 3 * n + 14 * n, np.float64)
+    nf = len(out) // n
+    amb = np.asarray(gs2_amb, np.float64).reshape(10, n)
+    for f in (0, 1, 3, 4, 5, 6, 7, 8, 9):
+        if not np.any(amb[f]):
+            continue
+        e = np.zeros(10 * n, np.float64)
+        e[f * n:(f + 1) * n] = 1.0
+        gs3 = oracle.proj_bwd(fw["s3"], e, n, pv, grad_mode=grad_mode)
+        col = oracle.act_bwd(params, fw["s3"], gs3, n, deg, cam, grad_mode=grad_mode)
+        out += np.abs(col) * np.tile(amb[f], nf)
+    return out
 
 
 def fd_grad(scene, h=1e-6, pairs=None):

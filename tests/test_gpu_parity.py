@@ -322,7 +322,7 @@ def test_heavy_first_order_is_a_sorted_permutation(W, H, n):
 
 
 @pytest.mark.parametrize("grad_mode", [0, 1])
-@pytest.mark.parametrize("deg", [0, 3])
+@pytest.mark.parametrize("deg", [0, 1, 2, 3])
 def test_projection_activation_backward(deg, grad_mode):
     seed, n, W, H, ls = SCENES["c0"]
     params, view, proj, cam = scene_params("c0")
