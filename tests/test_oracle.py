@@ -501,3 +501,87 @@ def test_fp32_fit_step_composes_the_stages():
     np.testing.assert_allclose(p1, want, rtol=1e-5, atol=1e-6)
     losses = [fit.step() for _ in range(30)]
     assert losses[-1] < 0.8 * l0
+
+
+def _conic_g(s, j, px, py):
+    """FP64 falloff exp(-d^2/2) of splat j at an NDC pixel centre."""
+    Cm = np.array([[s[3, j], s[4, j]], [s[4, j], s[5, j]]])
+    d = np.array([s[0, j] - px, s[1, j] - py])
+    return math.exp(-0.5 * d @ np.linalg.solve(Cm, d))
+
+
+def _grads(s2, n, W, H, ta, dldc):
+    fw = oracle.comp_fwd(s2, n, ta["ids"], ta["ranges"], W, H)
+    g, _, amb = oracle.comp_bwd(s2, n, ta["ids"], ta["ranges"], W, H, fw["T"], fw["last"], dldc,
+                                px_margin=fw["margin"], amb_margin=5e-5)
+    return g, amb, fw
+
+
+@pytest.mark.parametrize("kind", ["threshold", "clamp", "termination"])
+def test_flip_allowance_covers_decision_flips(kind):
+    """The compositing backward's flip allowance (the bound the GPU parity
+    tests grant elements near a discrete decision) against real flips: one
+    splat's opacity is set so that a decision at one pixel lies 1e-7 (relative)
+    on either side of its threshold; the gradients on the two sides differ by
+    no more than the allowance computed on one side (plus 1e-4 relative for
+    the continuous change of the opacity itself), and the flip changes some
+    gradient by more than the strict 1e-4 relative tolerance."""
+    s2, n, W, H = small_scene(n=150 if kind == "termination" else 30, seed=11)
+    s = s2.reshape(10, n).astype(np.float64)
+    if kind == "termination":
+        s[9] = 0.95  # opaque: pixels stop at T <= 1e-4
+    s2 = s.reshape(-1)
+    ta = oracle.tile_assign_sort(s2.astype(np.float32), n, W, H)
+    rng = np.random.default_rng(5)
+    dldc = rng.normal(0, 1e-3, 3 * W * H)
+    fw = oracle.comp_fwd(s2, n, ta["ids"], ta["ranges"], W, H)
+    ntx = (W + 15) // 16
+    ranges = ta["ranges"].reshape(-1, 2)
+    # choose (pixel, splat): the pixel's last contributor for termination,
+    # else the splat of the pixel's list whose falloff there suits the kind
+    best = None
+    for i in range(H):
+        for jx in range(W):
+            pix = i * W + jx
+            t = (i // 16) * ntx + jx // 16
+            px = float(np.float32(2) * (np.float32(jx) + np.float32(0.5)) / np.float32(W) - 1)
+            py = float(1 - np.float32(2) * (np.float32(i) + np.float32(0.5)) / np.float32(H))
+            if kind == "termination":
+                if fw["T"][pix] > 1e-4:
+                    continue
+                k = int(fw["last"][pix])
+                j = int(ta["ids"][k])
+                g = _conic_g(s, j, px, py)
+                a_old = min(0.99, s[9, j] * g)
+                t_before = fw["T"][pix] / (1 - a_old)
+                a_go = 1 - 1e-4 * (1 + 1e-7) / t_before    # T stays above: continue
+                a_stop = 1 - 1e-4 * (1 - 1e-7) / t_before  # T drops below: stop
+                if 0.02 < a_go < 0.98 and a_stop / g <= 1.0:
+                    best = (pix, j, a_go / g, a_stop / g)
+                    break
+                continue
+            for k in range(ranges[t, 0], ranges[t, 1]):
+                if k > fw["last"][pix]:
+                    break
+                j = int(ta["ids"][k])
+                g = _conic_g(s, j, px, py)
+                if kind == "threshold" and 0.05 < g < 0.5:
+                    best = (pix, j, (1 / 255) * (1 + 1e-7) / g, (1 / 255) * (1 - 1e-7) / g)
+                elif kind == "clamp" and g > 0.992:
+                    best = (pix, j, 0.99 * (1 - 1e-7) / g, 0.99 * (1 + 1e-7) / g)
+                if best:
+                    break
+            if best:
+                break
+        if best:
+            break
+    assert best is not None
+    pix, j, o_a, o_b = best
+    sa, sb = s.copy(), s.copy()
+    sa[9, j], sb[9, j] = o_a, o_b
+    ga, amb, fwa = _grads(sa.reshape(-1), n, W, H, ta, dldc)
+    gb, _, fwb = _grads(sb.reshape(-1), n, W, H, ta, dldc)
+    assert fwa["margin"][pix] <= 5e-5  # the pixel is flagged ambiguous
+    d = np.abs(ga - gb)
+    assert np.all(d <= amb + 1e-4 * np.abs(ga) + 1e-15), float((d - amb).max())
+    assert np.any(d > 1e-4 * np.abs(ga) + 1e-15)  # the flip is visible
