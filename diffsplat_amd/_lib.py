@@ -59,7 +59,8 @@ class RasterView(C.Structure):
                 ("last_ids", C.c_void_p), ("n_isects", C.c_void_p),
                 ("n_isects_wanted", C.c_void_p), ("visible_count", C.c_void_p),
                 ("num_tiles_x", C.c_int32), ("num_tiles_y", C.c_int32),
-                ("width", C.c_int32), ("height", C.c_int32), ("tile_order", C.c_void_p)]
+                ("width", C.c_int32), ("height", C.c_int32), ("tile_order", C.c_void_p),
+                ("lookback_error", C.c_void_p)]
 
 
 class AdamGroup(C.Structure):

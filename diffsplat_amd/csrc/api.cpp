@@ -492,7 +492,7 @@ ds_status ds_fitter_loss(ds_fitter *f, float *loss) {
     DS_REQUIRE(f && loss, "NULL argument");
     DS_GUARD_BEGIN
     *loss = f->impl.getLoss();
-    return DS_OK;
+    return ds::raster_take_error(f->impl.raster(), f->impl.getStream());
     DS_GUARD_END
 }
 
@@ -500,7 +500,7 @@ ds_status ds_fitter_synchronize(ds_fitter *f) {
     clear_error();
     DS_REQUIRE(f, "NULL fitter");
     DS_TRY_HIP(hipStreamSynchronize(f->impl.getStream()));
-    return DS_OK;
+    return ds::raster_take_error(f->impl.raster(), f->impl.getStream());
 }
 
 ds_status ds_fitter_download_image(ds_fitter *f, float *host_rgb) {
@@ -509,7 +509,7 @@ ds_status ds_fitter_download_image(ds_fitter *f, float *host_rgb) {
     DS_TRY_HIP(hipStreamSynchronize(f->impl.getStream()));
     const size_t bytes = (size_t)f->impl.width() * f->impl.height() * 3 * sizeof(float);
     DS_TRY_HIP(hipMemcpy(host_rgb, f->impl.getOutput(), bytes, hipMemcpyDeviceToHost));
-    return DS_OK;
+    return ds::raster_take_error(f->impl.raster(), f->impl.getStream());
 }
 
 ds_status ds_fitter_get_raster(ds_fitter *f, ds_raster **r) {
@@ -746,7 +746,7 @@ ds_status ds_multifit_losses(ds_multifit *m, float *losses) {
     DS_REQUIRE(m && losses, "NULL argument");
     DS_GUARD_BEGIN
     m->impl.losses(losses);
-    return DS_OK;
+    return ds::raster_take_error(m->impl.raster(), m->impl.getStream());
     DS_GUARD_END
 }
 
@@ -754,7 +754,7 @@ ds_status ds_multifit_synchronize(ds_multifit *m) {
     clear_error();
     DS_REQUIRE(m, "NULL multifit");
     DS_TRY_HIP(hipStreamSynchronize(m->impl.getStream()));
-    return DS_OK;
+    return ds::raster_take_error(m->impl.raster(), m->impl.getStream());
 }
 
 ds_status ds_multifit_set_graph(ds_multifit *m, int enable) {
@@ -965,7 +965,7 @@ ds_status ds_renderer_download_image(ds_renderer *r, float *host_rgb) {
     DS_TRY_HIP(hipStreamSynchronize(r->impl.getStream()));
     const size_t bytes = (size_t)r->impl.width() * r->impl.height() * 3 * sizeof(float);
     DS_TRY_HIP(hipMemcpy(host_rgb, r->impl.getOutput(), bytes, hipMemcpyDeviceToHost));
-    return DS_OK;
+    return ds::raster_take_error(r->impl.raster(), r->impl.getStream());
 }
 
 ds_status ds_renderer_get_raster(ds_renderer *r, ds_raster **out) {
@@ -986,7 +986,7 @@ ds_status ds_renderer_synchronize(ds_renderer *r) {
     clear_error();
     DS_REQUIRE(r, "NULL renderer");
     DS_TRY_HIP(hipStreamSynchronize(r->impl.getStream()));
-    return DS_OK;
+    return ds::raster_take_error(r->impl.raster(), r->impl.getStream());
 }
 
 }  // extern "C"

@@ -180,6 +180,10 @@ ds_status raster_step_fwd_bwd(ds_raster *r, const ds_splat2d &in, float *out_rgb
                               const float *target, float *loss, float *grad_rgb,
                               const ds_splat2d &grad_in, int32_t *adam_tick, hipStream_t s,
                               bool started = false);
+// Reads the rasterizer's sticky look-back error flag (synchronises s); if a
+// look-back gave up since the last read, clears the flag and returns
+// DS_ERR_STATE with the detail set.
+ds_status raster_take_error(ds_raster *r, hipStream_t s);
 // The first kernel's work of raster_step_fwd_bwd for n splats.
 ds_status raster_step_start(ds_raster *r, int n, float *loss, const ds_splat2d &grad_in,
                             int32_t *adam_tick, ForwardStart *out);

@@ -43,16 +43,23 @@ namespace {
 
 // Scalars kept on the device: indices into ds_raster::scalars. Those below
 // SC_EPOCH and the tickets start each forward at zero (SC_COUNT at n).
+// SC_ERR is sticky: no forward clears it, so a give-up in any step of a
+// replayed multi-step graph survives until the host reads and clears it
+// (raster_take_error).
 enum {
     SC_NISECT = 0,
     SC_WANTED = 1,
     SC_VISIBLE = 2,
     SC_COUNT = 3,
     SC_EPOCH = 8,    // look-back epoch: +1 every forward, never 0
-    SC_ERR = 9,      // a look-back spin gave up in this forward
+    SC_ERR = 9,      // a look-back spin gave up since the host last read it
     SC_TICKET = 16,  // one ticket counter per look-back slot
     SC_NUM = 32
 };
+
+constexpr const char *kLookbackError =
+    "a sort or scan look-back gave up waiting in a forward since the error was last "
+    "reported; the results of that forward (and of the steps after it) are not valid";
 
 // Look-back slots (and digit-histogram rows): the four depth passes.
 enum { LB_DEPTH = 0, LB_SLOTS = 8 };
@@ -852,8 +859,8 @@ static ForwardStart forward_start_of(ds_raster *r, int n, float *loss,
     f.nzero[0] = SC_EPOCH;
     f.set_word = sc + SC_COUNT;
     f.set_value = (uint32_t)n;
-    f.zero[1] = sc + SC_ERR;  // the error flag and the tickets
-    f.nzero[1] = SC_TICKET + LB_SLOTS - SC_ERR;
+    f.zero[1] = sc + SC_TICKET;  // the tickets (not the sticky error flag)
+    f.nzero[1] = LB_SLOTS;
     f.zero[2] = loss ? (uint32_t *)loss : nullptr;
     f.nzero[2] = loss ? (uint32_t)(multi ? r->ninst : 1) : 0u;
     f.zero[3] = r->ghist.ptr;
@@ -1278,11 +1285,27 @@ extern "C" ds_status ds_raster_counts(ds_raster *r, ds_stream_t s, uint32_t *n_i
     if (n_isects) *n_isects = h[SC_NISECT];
     if (visible_count) *visible_count = h[SC_VISIBLE];
     if (n_wanted) *n_wanted = h[SC_WANTED];
-    if (h[SC_ERR])
-        return set_error(DS_ERR_STATE, "a sort or scan look-back gave up waiting in the last "
-                                       "forward; its results are not valid");
+    if (h[SC_ERR]) {
+        DS_TRY_HIP(hipMemsetAsync(r->scalars.ptr + SC_ERR, 0, sizeof(uint32_t), as_stream(s)));
+        DS_TRY_HIP(hipStreamSynchronize(as_stream(s)));
+        return set_error(DS_ERR_STATE, kLookbackError);
+    }
     return DS_OK;
 }
+
+namespace ds {
+ds_status raster_take_error(ds_raster *r, hipStream_t s) {
+    if (!r) return DS_OK;
+    uint32_t err = 0;
+    DS_TRY_HIP(hipMemcpyAsync(&err, r->scalars.ptr + SC_ERR, sizeof(err), hipMemcpyDeviceToHost,
+                              s));
+    DS_TRY_HIP(hipStreamSynchronize(s));
+    if (!err) return DS_OK;
+    DS_TRY_HIP(hipMemsetAsync(r->scalars.ptr + SC_ERR, 0, sizeof(uint32_t), s));
+    DS_TRY_HIP(hipStreamSynchronize(s));
+    return set_error(DS_ERR_STATE, kLookbackError);
+}
+}  // namespace ds
 
 extern "C" ds_status ds_raster_get_view(ds_raster *r, ds_raster_view *v) {
     clear_error();
@@ -1295,6 +1318,7 @@ extern "C" ds_status ds_raster_get_view(ds_raster *r, ds_raster_view *v) {
     v->n_isects = r->scalars.ptr + SC_NISECT;
     v->n_isects_wanted = r->scalars.ptr + SC_WANTED;
     v->visible_count = r->scalars.ptr + SC_VISIBLE;
+    v->lookback_error = r->scalars.ptr + SC_ERR;
     v->num_tiles_x = r->ntx;
     v->num_tiles_y = r->nty;
     v->width = r->W;
@@ -1366,20 +1390,9 @@ extern "C" ds_status ds_sort_pairs_u64(const uint64_t *keys_in, const uint32_t *
     DS_TRY(hist.alloc(sort_hist_words((uint64_t)n)));
     DS_TRY(part.alloc(kSpineMax));
     DS_TRY(nd.alloc(1));
-    DS_TRY(fill_u32(nd.ptr, (uint32_t)n, 1, st));
-    const int passes = (key_bits + kRadixBits - 1) / kRadixBits;
-    // ping-pong so the last pass lands in the output buffers
-    const uint64_t *ksrc = keys_in;
-    const uint32_t *vsrc = vals_in;
-    for (int p = 0; p < passes; ++p) {
-        const bool to_out = ((passes - 1 - p) % 2) == 0;
-        uint64_t *kd = to_out ? keys_out : kb.ptr;
-        uint32_t *vd = to_out ? vals_out : vb.ptr;
-        DS_TRY(radix_pass<uint64_t>(ksrc, vsrc, kd, vd, nd.ptr, (uint32_t)n, kRadixBits * p,
-                                    hist.ptr, part.ptr, st));
-        ksrc = kd;
-        vsrc = vd;
-    }
+    DS_TRY(sort_pairs_u64_scratch(keys_in, vals_in, keys_out, vals_out, (uint32_t)n, key_bits,
+                                  SortPairsScratch{kb.ptr, vb.ptr, hist.ptr, part.ptr, nd.ptr},
+                                  st));
     DS_TRY_HIP(hipStreamSynchronize(st));
     return DS_OK;
 }

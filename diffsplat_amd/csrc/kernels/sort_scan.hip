@@ -554,6 +554,27 @@ template ds_status radix_pass<uint64_t>(const uint64_t *, const uint32_t *, uint
                                         uint32_t *, const uint32_t *, uint32_t, int, uint32_t *,
                                         uint32_t *, hipStream_t);
 
+ds_status sort_pairs_u64_scratch(const uint64_t *keys_in, const uint32_t *vals_in,
+                                 uint64_t *keys_out, uint32_t *vals_out, uint32_t n,
+                                 int key_bits, const SortPairsScratch &scr, hipStream_t s) {
+    if (n == 0) return DS_OK;
+    DS_TRY(fill_u32(scr.count, n, 1, s));
+    const int passes = (key_bits + kRadixBits - 1) / kRadixBits;
+    // ping-pong so the last pass lands in the output buffers
+    const uint64_t *ksrc = keys_in;
+    const uint32_t *vsrc = vals_in;
+    for (int p = 0; p < passes; ++p) {
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint64_t *kd = to_out ? keys_out : scr.keys;
+        uint32_t *vd = to_out ? vals_out : scr.vals;
+        DS_TRY(radix_pass<uint64_t>(ksrc, vsrc, kd, vd, scr.count, n, kRadixBits * p, scr.hist,
+                                    scr.part, s));
+        ksrc = kd;
+        vsrc = vd;
+    }
+    return DS_OK;
+}
+
 ds_status onesweep_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                         uint32_t *vout, const uint32_t *n_dev, uint32_t max_n, int shift,
                         const uint32_t *ghist, int hist_reps, uint32_t hist_rep_stride,

@@ -102,6 +102,25 @@ inline uint32_t sort_hist_words(uint64_t max_n) {
     return (uint32_t)(kRadix * ((max_n + kSortChunk - 1) / kSortChunk));
 }
 
+// Scratch of a stand-alone (u64 key, u32 value) sort of up to n pairs: key
+// and value ping-pong buffers of n each, sort_hist_words(n) histogram words,
+// kSpineMax scan words and one count word. Nothing in it needs clearing: the
+// count word is set and every other word is written before it is read.
+struct SortPairsScratch {
+    uint64_t *keys;
+    uint32_t *vals;
+    uint32_t *hist;
+    uint32_t *part;
+    uint32_t *count;
+};
+
+// Stable LSD sort of n pairs on the low key_bits bits, 8 bits per pass,
+// ping-ponging through the scratch so the last pass lands in keys_out /
+// vals_out. Asynchronous on s; allocates nothing.
+ds_status sort_pairs_u64_scratch(const uint64_t *keys_in, const uint32_t *vals_in,
+                                 uint64_t *keys_out, uint32_t *vals_out, uint32_t n,
+                                 int key_bits, const SortPairsScratch &scr, hipStream_t s);
+
 // ------------------------------------------------------------------
 // One-kernel passes with decoupled look-back.
 //

@@ -222,6 +222,11 @@ typedef struct ds_raster_view {
     int32_t width, height;
     const uint32_t *tile_order;        /* [num_tiles] compositing dispatch order of the last
                                           forward, or NULL when it was row-major          */
+    const uint32_t *lookback_error;    /* device word, nonzero when a sort look-back gave up
+                                          in some forward since the last report; sticky
+                                          across forwards and graph replays, cleared by
+                                          the call that reports it (ds_raster_counts, the
+                                          fitters' loss / synchronize, ...)               */
 } ds_raster_view;
 
 /* TileRasterizeLayer::allocate (src/layers/tile_rasterize_layer.cu:591-616). */
@@ -426,7 +431,10 @@ ds_status ds_fitter_download_moments(ds_fitter *f, float *host_m, float *host_v)
 ds_status ds_fitter_step(ds_fitter *f, int n_steps);
 /* Forward only (render + loss) with the current parameters. */
 ds_status ds_fitter_forward(ds_fitter *f);
-/* Loss of the last forward (synchronises). */
+/* Loss of the last forward (synchronises). This call, ds_fitter_synchronize
+ * and ds_fitter_download_image return DS_ERR_STATE (and clear the flag) when
+ * a sort look-back of any step since the last report gave up waiting
+ * (ds_raster_view.lookback_error); *loss is still written. */
 ds_status ds_fitter_loss(ds_fitter *f, float *loss);
 ds_status ds_fitter_synchronize(ds_fitter *f);
 /* Rendered image of the last forward, host [H*W*3] (synchronises). */
@@ -573,6 +581,7 @@ ds_status ds_multifit_forward(ds_multifit *m);
 /* zero grads, forward, backward (no Adam update). */
 ds_status ds_multifit_forward_backward(ds_multifit *m);
 /* Loss of every instance from the last forward, host [n_inst]. Synchronises. */
+/* Both report a sticky look-back give-up as ds_fitter_loss does. */
 ds_status ds_multifit_losses(ds_multifit *m, float *losses);
 ds_status ds_multifit_synchronize(ds_multifit *m);
 /* 1: capture one step into one HIP graph on first use and replay it. Every
@@ -614,6 +623,8 @@ ds_status ds_renderer_autotune(ds_renderer *r, int reps, int *fwd_pix_per_thread
 ds_status ds_renderer_download_image(ds_renderer *r, float *host_rgb);
 ds_status ds_renderer_get_raster(ds_renderer *r, ds_raster **out);
 ds_status ds_renderer_get_stream(ds_renderer *r, ds_stream_t *s);
+/* This call and ds_renderer_download_image report a sticky look-back give-up
+ * as ds_fitter_loss does. */
 ds_status ds_renderer_synchronize(ds_renderer *r);
 
 /* ------------------------------------------------------------------ */
