@@ -164,6 +164,7 @@ _SIGS = {
     "ds_fitter_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _F,
                                      _F]),
     "ds_sort_pairs_u64": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_int, _P]),
+    "ds_sort_pairs_u64_noalloc": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_int, _P]),
     "ds_mse_fwd": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
     "ds_mse_bwd": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
     "ds_adam_step": (C.c_int, [C.POINTER(AdamGroup), C.c_int, C.POINTER(AdamHParams),

@@ -341,6 +341,18 @@ ds_status ds_sort_pairs_u64(const uint64_t *keys_in, const uint32_t *vals_in,
                             uint64_t *keys_out, uint32_t *vals_out, int64_t n, int key_bits,
                             ds_stream_t s);
 
+/* The same sort without any allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Its scratch
+ * is a fixed device array of about 12.6 MiB per GPU, held until the process
+ * ends; n is at most 1048576 (2^20) pairs, larger n returns DS_ERR_CAPACITY.
+ * Asynchronous on s (unlike ds_sort_pairs_u64). Every call on one GPU shares
+ * that GPU's scratch: calls that overlap in time on one GPU race. Output
+ * equals ds_sort_pairs_u64's bit for bit. */
+ds_status ds_sort_pairs_u64_noalloc(const uint64_t *keys_in, const uint32_t *vals_in,
+                                    uint64_t *keys_out, uint32_t *vals_out, int64_t n,
+                                    int key_bits, ds_stream_t s);
+
 /* ------------------------------------------------------------------ */
 /* MSE loss                                                            */
 /* ------------------------------------------------------------------ */
