@@ -42,21 +42,21 @@ static void tune_tile_order(ds_raster *r, F &&timed) {
 }
 
 // With the forward and backward launch shapes chosen apart (bf, bb), times
-// the whole compositing (timed_fwd_bwd, ms) for (bf, bb) and for the equal
-// shapes (bf, bf) and (bb, bb), with which the forward and backward run in
-// one fused kernel, and keeps the fastest (the separate pair unless another
-// is at least 5% faster). Returns the kept shapes in bf, bb.
+// the whole compositing (timed_fwd_bwd, ms) for (bf, bb) and for every equal
+// pair (1, 1), (2, 2), (4, 4), with which the forward and backward run in one
+// fused kernel (whose speed the separate timings do not predict), and keeps
+// the fastest (the separately chosen pair unless another is at least 5%
+// faster). Returns the kept shapes in bf, bb.
 template <class F>
 static void tune_fused_shapes(ds_raster *r, int &bf, int &bb, F &&timed_fwd_bwd) {
-    if (bf == bb) return;
-    const int cand[3][2] = {{bf, bb}, {bf, bf}, {bb, bb}};
-    float t[3];
-    for (int i = 0; i < 3; ++i) {
+    const int cand[4][2] = {{bf, bb}, {1, 1}, {2, 2}, {4, 4}};
+    float t[4];
+    for (int i = 0; i < 4; ++i) {
         check(ds_raster_set_tuning(r, cand[i][0], cand[i][1]), "autotune");
         t[i] = timed_fwd_bwd();
     }
     int best = 0;
-    for (int i = 1; i < 3; ++i)
+    for (int i = 1; i < 4; ++i)
         if (t[i] < 0.95f * t[best]) best = i;
     bf = cand[best][0];
     bb = cand[best][1];

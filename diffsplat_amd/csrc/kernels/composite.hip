@@ -40,8 +40,13 @@ __device__ __forceinline__ int tile_of_block(const CompositeArgs &a) {
 // dx dy / det)).
 struct AlphaEval {
     float u, w, p, g, og;
-    // alpha = min(0.99, o g); alpha >= 1/255 exactly when o g >= 1/255
-    __device__ __forceinline__ float alpha() const { return fminf(kMaxAlpha, og); }
+    // alpha = min(0.99, o g); alpha >= 1/255 exactly when o g >= 1/255. Only
+    // called where eval_alpha passed, so og is a number in [1/255, inf): the
+    // median of (og, 0, 0.99) is that minimum in one instruction (fminf
+    // quiets NaNs first, one more).
+    __device__ __forceinline__ float alpha() const {
+        return __builtin_amdgcn_fmed3f(og, 0.f, kMaxAlpha);
+    }
 };
 
 // Gaussian falloff of one splat at one pixel, without branches. Forward and
@@ -231,14 +236,24 @@ __device__ __forceinline__ int composite_front_to_back(
     float (&cb)[P], int32_t (&cur)[P]) {
     constexpr int NT = kTilePix / P;
     const int tid = threadIdx.x, w = tid / kWave;
-    bool done[P];
+    // Finished pixels as wave masks (scalar registers): bit l of dm[k] is
+    // set once lane l's k-th pixel is done (outside the image, or T <= 1e-4).
+    // The per-lane view of a mask is an inverse ballot, which costs no vector
+    // instruction, so the bookkeeping stays on the scalar unit.
+    uint64_t dm[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        done[k] = !q.inside[k];
+        dm[k] = __builtin_amdgcn_ballot_w64(!q.inside[k]);
         T[k] = 1.f;
         cr[k] = cg[k] = cb[k] = 0.f;
         cur[k] = 0;
     }
+    auto wave_live = [&]() {
+        bool live = false;
+#pragma unroll
+        for (int k = 0; k < P; ++k) live |= dm[k] != ~0ull;
+        return live;
+    };
     // The wave's band of pixel centres (NDC): lane 0 holds its first column
     // and top row, lane 63 its last column and bottom row.
     const float wx0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.px), 0));
@@ -248,10 +263,7 @@ __device__ __forceinline__ int composite_front_to_back(
         __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.py[P - 1]), kWave - 1));
     int loaded = -1;
     for (int b = range.x; b < range.y; b += NT) {
-        bool live = false;
-#pragma unroll
-        for (int k = 0; k < P; ++k) live |= !done[k];
-        if (__syncthreads_count(live) == 0) break;
+        if (__syncthreads_count(wave_live()) == 0) break;
         const int idx = b + tid;
         if (idx < range.y) {
             const float4 *r = a.rec + 3 * (size_t)a.flat[idx];
@@ -261,10 +273,7 @@ __device__ __forceinline__ int composite_front_to_back(
         }
         __syncthreads();
         loaded = b;
-        bool live_w = false;
-#pragma unroll
-        for (int k = 0; k < P; ++k) live_w |= !done[k];
-        const int bs = __ballot(live_w) ? min(NT, range.y - b) : 0;
+        const int bs = wave_live() ? min(NT, range.y - b) : 0;
         bool finished = false;
         // 64 splats at a time: one word of reach bits (NT is a multiple of 64)
         for (int t0 = 0; t0 < bs && !finished; t0 += 64) {
@@ -288,32 +297,32 @@ __device__ __forceinline__ int composite_front_to_back(
                 // Pass 1: which of this lane's pixels the splat reaches; a
                 // wave whose pixels it reaches none of skips the rest.
                 AlphaEval e[P];
-                bool ok[P];
+                uint64_t okm[P];
                 bool anyv = false;
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
-                    ok[k] = eval_alpha(A, B, q.px, q.py[k], e[k]) & !done[k];
-                    anyv |= ok[k];
+                    okm[k] = __builtin_amdgcn_ballot_w64(eval_alpha(A, B, q.px, q.py[k], e[k])) &
+                             ~dm[k];
+                    anyv |= okm[k] != 0;
                 }
-                if (__ballot(anyv) == 0) continue;
+                if (!anyv) continue;
                 reach |= 1ull << s;
                 const float4 C = sC[t];
-                bool any = false;
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
                     // A skipped splat gets alpha 0, which leaves T and the
                     // colour exactly unchanged (T * (1 - 0) == T).
-                    const float alpha = ok[k] ? e[k].alpha() : 0.f;
+                    const bool ok = __builtin_amdgcn_inverse_ballot_w64(okm[k]);
+                    const float alpha = ok ? e[k].alpha() : 0.f;
                     const float vis = alpha * T[k];
                     cr[k] += C.x * vis;
                     cg[k] += C.y * vis;
                     cb[k] += C.z * vis;
-                    cur[k] = ok[k] ? b + t : cur[k];
+                    cur[k] = ok ? b + t : cur[k];
                     T[k] = T[k] * (1.f - alpha);
-                    done[k] |= T[k] <= kTThres;
-                    any |= !done[k];
+                    dm[k] |= __builtin_amdgcn_ballot_w64(T[k] <= kTThres);
                 }
-                if (__ballot(any) == 0) {
+                if (!wave_live()) {
                     finished = true;
                     break;
                 }
@@ -436,7 +445,9 @@ __device__ __forceinline__ void composite_back_to_front(
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
                     const float alpha = ok[k] ? e[k].alpha() : 0.f;
-                    const float ra = __builtin_amdgcn_rcpf(fmaxf(1.f - alpha, 1e-6f));
+                    // 1 / max(1 - alpha, 1e-6) of the reference: alpha <= 0.99, so the
+                    // floor never applies
+                    const float ra = __builtin_amdgcn_rcpf(1.f - alpha);
                     // transmittance in front of the splat
                     const float Tb = ok[k] ? T[k] * ra : T[k];
                     const float fac = alpha * Tb;
