@@ -13,7 +13,8 @@ import math
 import numpy as np
 
 from ._lib import (AdamGroup, AdamHParams, CameraInput, CameraSpeeds, DeviceArray,
-                   DiffsplatError, FitterConfig, Gauss3D, MultiFitInstance,
+                   DensifyConfig, DensifyResult, DiffsplatError, FitterConfig, Gauss3D,
+                   MultiFitInstance,
                    RasterConfig, RasterInstance, RasterView, check, download_ptr, farr, fptr,
                    Splat2D, gauss_view, has_gpu, lib, splat2d_view, splat3d_view)
 
@@ -439,6 +440,36 @@ class ImageFitter:
         out = np.empty(slab_floats(self.count, self.sh_degree), np.float32)
         check(lib().ds_fitter_download_grads(self.h, fptr(out)), "ds_fitter_download_grads")
         return out
+
+    def set_densify_stats(self, on: bool = True) -> None:
+        """Accumulate per-splat gradient statistics in every later step."""
+        check(lib().ds_fitter_set_densify_stats(self.h, int(on)), "ds_fitter_set_densify_stats")
+
+    def densify_stats(self):
+        """(sum of |dL/d(NDC centre)|, visits) per splat."""
+        g = np.empty(self.count, np.float32)
+        v = np.empty(self.count, np.uint32)
+        check(lib().ds_fitter_densify_stats(self.h, fptr(g), v.ctypes.data),
+              "ds_fitter_densify_stats")
+        return g, v
+
+    def densify(self, **cfg):
+        """One clone / split / prune (ds_fitter_densify). Keyword arguments
+        override ds_densify_default_config. Returns (result dict, per-splat
+        action codes of the splats before)."""
+        c = DensifyConfig()
+        lib().ds_densify_default_config(C.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        r = DensifyResult()
+        acts = np.empty(max(self.count, 1), np.uint8)
+        check(lib().ds_fitter_densify(self.h, C.byref(c), C.byref(r), acts.ctypes.data),
+              "ds_fitter_densify")
+        n = C.c_int32()
+        check(lib().ds_fitter_count(self.h, C.byref(n)), "ds_fitter_count")
+        acts = acts[:self.count]
+        self.count = n.value
+        return {k: getattr(r, k) for k, _ in DensifyResult._fields_}, acts
 
     def moments(self):
         """Adam moments (m, v) in the parameter slab's layout."""

@@ -69,6 +69,18 @@ class AdamGroup(C.Structure):
                 ("decay", C.c_float)]
 
 
+class DensifyConfig(C.Structure):
+    _fields_ = [("grad_threshold", C.c_float), ("percent_dense", C.c_float),
+                ("scene_extent", C.c_float), ("min_opacity", C.c_float),
+                ("max_scale_frac", C.c_float), ("split_scale_div", C.c_float),
+                ("seed", C.c_uint64)]
+
+
+class DensifyResult(C.Structure):
+    _fields_ = [("count_before", C.c_int32), ("count_after", C.c_int32), ("pruned", C.c_int32),
+                ("cloned", C.c_int32), ("split", C.c_int32)]
+
+
 class AdamHParams(C.Structure):
     _fields_ = [("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float)]
 
@@ -179,6 +191,12 @@ _SIGS = {
     "ds_fitter_download_params": (C.c_int, [_P, _F]),
     "ds_fitter_download_grads": (C.c_int, [_P, _F]),
     "ds_fitter_download_moments": (C.c_int, [_P, _F, _F]),
+    "ds_densify_default_config": (None, [C.POINTER(DensifyConfig)]),
+    "ds_fitter_set_densify_stats": (C.c_int, [_P, C.c_int]),
+    "ds_fitter_densify_stats": (C.c_int, [_P, _F, _P]),
+    "ds_fitter_densify": (C.c_int, [_P, C.POINTER(DensifyConfig), C.POINTER(DensifyResult),
+                                    _P]),
+    "ds_fitter_count": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "ds_fitter_step": (C.c_int, [_P, C.c_int]),
     "ds_fitter_forward": (C.c_int, [_P]),
     "ds_fitter_loss": (C.c_int, [_P, _F]),

@@ -469,6 +469,52 @@ ds_status ds_fitter_download_moments(ds_fitter *f, float *host_m, float *host_v)
     DS_GUARD_END
 }
 
+void ds_densify_default_config(ds_densify_config *cfg) {
+    if (!cfg) return;
+    cfg->grad_threshold = 2e-4f;
+    cfg->percent_dense = 0.01f;
+    cfg->scene_extent = 0.f;
+    cfg->min_opacity = 0.005f;
+    cfg->max_scale_frac = 0.f;
+    cfg->split_scale_div = 1.6f;
+    cfg->seed = 0;
+}
+
+ds_status ds_fitter_set_densify_stats(ds_fitter *f, int enable) {
+    clear_error();
+    DS_REQUIRE(f, "NULL fitter");
+    DS_GUARD_BEGIN
+    f->impl.setDensifyStats(enable != 0);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_densify_stats(ds_fitter *f, float *host_grad_sum, uint32_t *host_visits) {
+    clear_error();
+    DS_REQUIRE(f && (host_grad_sum || host_visits), "NULL argument");
+    DS_GUARD_BEGIN
+    f->impl.downloadDensifyStats(host_grad_sum, host_visits);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_densify(ds_fitter *f, const ds_densify_config *cfg,
+                            ds_densify_result *result, uint8_t *host_actions) {
+    clear_error();
+    DS_REQUIRE(f && cfg, "NULL argument");
+    DS_GUARD_BEGIN
+    f->impl.densify(*cfg, result, host_actions);
+    return DS_OK;
+    DS_GUARD_END
+}
+
+ds_status ds_fitter_count(ds_fitter *f, int32_t *count) {
+    clear_error();
+    DS_REQUIRE(f && count, "NULL argument");
+    *count = f->impl.count();
+    return DS_OK;
+}
+
 ds_status ds_fitter_step(ds_fitter *f, int n_steps) {
     clear_error();
     DS_REQUIRE(f && n_steps >= 0, "bad argument");

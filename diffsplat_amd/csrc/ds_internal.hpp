@@ -180,6 +180,41 @@ ds_status raster_step_fwd_bwd(ds_raster *r, const ds_splat2d &in, float *out_rgb
                               const float *target, float *loss, float *grad_rgb,
                               const ds_splat2d &grad_in, int32_t *adam_tick, hipStream_t s,
                               bool started = false);
+// Adaptive density control (kernels/densify.hip). Thresholds are on the raw
+// parameters: log-scales and opacity logits.
+struct DensifyParams {
+    float grad_threshold;       // mean |dL/d(NDC centre)| over visited steps
+    float clone_max_log_scale;  // clone when the largest log-scale is <= this, else split
+    float min_logit;            // prune when the opacity logit is below this
+    float prune_max_log_scale;  // prune when the largest log-scale exceeds this (prune_big)
+    float split_log_div;        // log of the split's scale divisor (1.6)
+    int prune_big;
+    uint64_t seed;              // split samples
+};
+// Device scratch of one densification of n splats (all n words unless
+// noted): actions (bytes), survive flags, added counts, their exclusive
+// scans, the two totals (survivors, added), the element count word and
+// kSpineMax scan words.
+struct DensifyScratch {
+    uint8_t *action;
+    uint32_t *keep, *added, *keep_off, *add_off;
+    uint32_t *totals;
+    uint32_t *count;
+    uint32_t *part;
+};
+// Adds |dL/d(NDC centre)| and 1 visit for every splat with a tile (visible).
+ds_status launch_densify_accum(const float *grad_x, const float *grad_y, const uint32_t *ntiles,
+                               float *accum, uint32_t *visits, int n, hipStream_t s);
+// Classifies, scans and scatters: dst, dst_m, dst_v receive nfields fields of
+// stride n_out = totals[0] + totals[1] (at most 2 n) each; m and v are the
+// Adam moments of p's slab in any field order.
+ds_status launch_densify(const ds_gauss3d &p, const float *m, const float *v, int nfields,
+                         const float *accum, const uint32_t *visits, const DensifyParams &c,
+                         const DensifyScratch &scr, float *dst, float *dst_m, float *dst_v,
+                         hipStream_t s);
+// Per-splat tile counts of the rasterizer's last forward (0: culled).
+const uint32_t *raster_splat_tiles(ds_raster *r);
+
 // Reads the rasterizer's sticky look-back error flag (synchronises s); if a
 // look-back gave up since the last read, clears the flag and returns
 // DS_ERR_STATE with the detail set.

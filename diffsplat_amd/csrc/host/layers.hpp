@@ -368,6 +368,7 @@ public:
         return tab;
     }
     void countStep() { ++step_count; }
+    void setStepCount(int t) { step_count = t; }
 
 private:
     AdamParams hp;

@@ -4,6 +4,7 @@
 #include "../ds_internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <chrono>
 
@@ -310,12 +311,166 @@ void ImageFitter::stepOnce() {
                   "ImageFitter::step");
         }
         optimizer.countStep();
+        accumulateDensifyStats();
         return;
     }
     pipeline.zeroGrad();
     pipeline.forward();
     pipeline.backward();
     optimizer.step(stream);
+    accumulateDensifyStats();
+}
+
+void ImageFitter::accumulateDensifyStats() {
+    if (!densify_stats || gaussian_params.count <= 0) return;
+    const ds_splat2d gi = ras_layer.getGradInput().view();
+    check(ds::launch_densify_accum(gi.pos_x, gi.pos_y, ds::raster_splat_tiles(ras_layer.raster()),
+                                   dens_accum.ptr, dens_visits.ptr, gaussian_params.count, stream),
+          "ImageFitter::step");
+}
+
+void ImageFitter::setDensifyStats(bool on) {
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    densify_stats = on;
+    const size_t n = (size_t)gaussian_params.count;
+    dens_accum.allocate(n);
+    dens_visits.allocate(n);
+    dens_accum.zero(stream);
+    dens_visits.zero(stream);
+    dropGraph();  // a captured step has or lacks the accumulation
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+void ImageFitter::downloadDensifyStats(float *grad_sum, uint32_t *visits) {
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    const size_t n = (size_t)gaussian_params.count;
+    if (dens_accum.size != n)
+        throw std::runtime_error("[diffsplat] densify statistics are not enabled");
+    if (grad_sum) dens_accum.download(grad_sum, n);
+    if (visits) dens_visits.download(visits, n);
+}
+
+void ImageFitter::densify(const ds_densify_config &cfg, ds_densify_result *res,
+                          uint8_t *host_actions) {
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    const int n = gaussian_params.count;
+    const int nf = gaussian_params.nfields();
+    if (dens_accum.size != (size_t)n)
+        throw std::runtime_error("[diffsplat] densify needs ds_fitter_set_densify_stats(f, 1) "
+                                 "and some steps first");
+    if (cfg.percent_dense < 0.f || cfg.min_opacity < 0.f || cfg.min_opacity >= 1.f ||
+        cfg.split_scale_div <= 0.f || cfg.max_scale_frac < 0.f)
+        throw std::runtime_error("[diffsplat] densify: bad configuration");
+    if ((int64_t)2 * n >= ((int64_t)1 << 30))
+        throw std::runtime_error("[diffsplat] densify: too many splats");
+    // scene extent: the largest distance of a centre from their mean
+    float extent = cfg.scene_extent;
+    if (!(extent > 0.f) && n > 0) {
+        std::vector<float> pos((size_t)3 * n);
+        hip_check(hipMemcpy(pos.data(), gaussian_params.slab.ptr, sizeof(float) * 3 * n,
+                            hipMemcpyDeviceToHost),
+                  "hipMemcpy");
+        double c[3] = {0, 0, 0};
+        for (int d = 0; d < 3; ++d)
+            for (int i = 0; i < n; ++i) c[d] += pos[(size_t)d * n + i];
+        for (int d = 0; d < 3; ++d) c[d] /= n;
+        double r2 = 0;
+        for (int i = 0; i < n; ++i) {
+            double q = 0;
+            for (int d = 0; d < 3; ++d) {
+                const double x = pos[(size_t)d * n + i] - c[d];
+                q += x * x;
+            }
+            r2 = std::max(r2, q);
+        }
+        extent = (float)std::sqrt(r2);
+    }
+    ds::DensifyParams p{};
+    p.grad_threshold = cfg.grad_threshold;
+    p.clone_max_log_scale = cfg.percent_dense > 0.f && extent > 0.f
+                                ? (float)std::log((double)cfg.percent_dense * extent)
+                                : -INFINITY;
+    p.min_logit = cfg.min_opacity > 0.f
+                      ? (float)std::log((double)cfg.min_opacity / (1.0 - cfg.min_opacity))
+                      : -INFINITY;
+    p.prune_big = cfg.max_scale_frac > 0.f && extent > 0.f;
+    p.prune_max_log_scale = p.prune_big ? (float)std::log((double)cfg.max_scale_frac * extent)
+                                        : INFINITY;
+    p.split_log_div = (float)std::log((double)cfg.split_scale_div);
+    p.seed = cfg.seed;
+    // classify, scan and scatter on the device
+    DeviceBuffer<uint8_t> action;
+    DeviceBuffer<uint32_t> keep, added, keep_off, add_off, words, part;
+    DeviceBuffer<float> np, nm, nv;
+    action.allocate((size_t)std::max(n, 1));
+    for (auto *b : {&keep, &added, &keep_off, &add_off}) b->allocate((size_t)std::max(n, 1));
+    words.allocate(4);
+    part.allocate(8192);
+    const size_t cap = (size_t)2 * std::max(n, 1) * nf;
+    np.allocate(cap);
+    nm.allocate(cap);
+    nv.allocate(cap);
+    words.zero(stream);
+    const ds::DensifyScratch scr{action.ptr, keep.ptr,  added.ptr,     keep_off.ptr,
+                                 add_off.ptr, words.ptr, words.ptr + 2, part.ptr};
+    check(ds::launch_densify(gaussian_params.view(), optimizer.moment1().ptr,
+                             optimizer.moment2().ptr, nf, dens_accum.ptr, dens_visits.ptr, p, scr,
+                             np.ptr, nm.ptr, nv.ptr, stream),
+          "ImageFitter::densify");
+    uint32_t tot[2] = {0, 0};
+    std::vector<uint8_t> act((size_t)n);
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if (n > 0) {
+        words.download(tot, 2);
+        action.download(act.data(), (size_t)n);
+    }
+    const int n_out = (int)(tot[0] + tot[1]);
+    ds_densify_result r{n, n_out, 0, 0, 0};
+    for (uint8_t a : act) {
+        r.pruned += a == DS_DENSIFY_PRUNE;
+        r.cloned += a == DS_DENSIFY_CLONE;
+        r.split += a == DS_DENSIFY_SPLIT;
+    }
+    if (host_actions && n > 0) std::memcpy(host_actions, act.data(), (size_t)n);
+    // rebuild the fitter at the new count, keeping step, shapes and order
+    int fwd_ppt = 1, bwd_ppt = 1, order = DS_TILE_ORDER_HEAVY_FIRST;
+    check(ds_raster_get_tuning(ras_layer.raster(), &fwd_ppt, &bwd_ppt), "densify");
+    check(ds_raster_get_tile_order(ras_layer.raster(), &order), "densify");
+    int32_t step_now = 0;
+    hip_check(hipMemcpy(&step_now, optimizer.stepDevice(), sizeof(step_now),
+                        hipMemcpyDeviceToHost),
+              "hipMemcpy");
+    const int iters = iter_count;
+    dropGraph();
+    opt.count = n_out;
+    gaussian_params.allocate(n_out, opt.sh_degree);
+    const size_t nfl = (size_t)nf * n_out;
+    if (nfl)
+        hip_check(hipMemcpyAsync(gaussian_params.slab.ptr, np.ptr, sizeof(float) * nfl,
+                                 hipMemcpyDeviceToDevice, stream),
+                  "hipMemcpyAsync");
+    initLayers();  // new buffers, Adam groups at the new count (moments zero)
+    check(ds_raster_set_tuning(ras_layer.raster(), fwd_ppt, bwd_ppt), "densify");
+    check(ds_raster_set_tile_order(ras_layer.raster(), order), "densify");
+    if (nfl) {
+        hip_check(hipMemcpyAsync(optimizer.moment1().ptr, nm.ptr, sizeof(float) * nfl,
+                                 hipMemcpyDeviceToDevice, stream),
+                  "hipMemcpyAsync");
+        hip_check(hipMemcpyAsync(optimizer.moment2().ptr, nv.ptr, sizeof(float) * nfl,
+                                 hipMemcpyDeviceToDevice, stream),
+                  "hipMemcpyAsync");
+    }
+    hip_check(hipMemcpyAsync(optimizer.stepDevice(), &step_now, sizeof(step_now),
+                             hipMemcpyHostToDevice, stream),
+              "hipMemcpyAsync");
+    optimizer.setStepCount(step_now);
+    iter_count = iters;
+    dens_accum.allocate((size_t)n_out);
+    dens_visits.allocate((size_t)n_out);
+    dens_accum.zero(stream);
+    dens_visits.zero(stream);
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if (res) *res = r;
 }
 
 // A captured step keeps the launch shapes and dispatch order it was captured

@@ -70,9 +70,20 @@ public:
     int width() const { return opt.width; }
     int height() const { return opt.height; }
     void views(ds_gauss3d *params, ds_splat2d *s2, ds_splat2d *gs2, float **grad_rgb);
+    // Adaptive density control (kernels/densify.hip): per-step gradient
+    // statistics, then one clone/split/prune with device-side compaction of
+    // the parameters and Adam moments. The count changes; the step counter,
+    // launch shapes and everything else are kept.
+    void setDensifyStats(bool on);
+    void downloadDensifyStats(float *grad_sum, uint32_t *visits);
+    void densify(const ds_densify_config &cfg, ds_densify_result *res, uint8_t *host_actions);
 
 private:
     void stepOnce();
+    void accumulateDensifyStats();
+    bool densify_stats = false;
+    DeviceBuffer<float> dens_accum;
+    DeviceBuffer<uint32_t> dens_visits;
     void dropGraph();
     FitterOptions opt;
     bool fused = true;

@@ -1294,6 +1294,8 @@ extern "C" ds_status ds_raster_counts(ds_raster *r, ds_stream_t s, uint32_t *n_i
 }
 
 namespace ds {
+const uint32_t *raster_splat_tiles(ds_raster *r) { return r ? r->ntiles_of.ptr : nullptr; }
+
 ds_status raster_take_error(ds_raster *r, hipStream_t s) {
     if (!r) return DS_OK;
     uint32_t err = 0;

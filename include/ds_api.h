@@ -487,6 +487,56 @@ ds_status ds_fitter_set_target_device(ds_fitter *f, const float *dev_rgb);
 ds_status ds_fitter_reset_optimizer(ds_fitter *f);
 
 /* ------------------------------------------------------------------ */
+/* Adaptive density control (clone / split / prune). The reference       */
+/* lists it as a TODO (README.md:10); the rules are those of Kerbl et   */
+/* al., SIGGRAPH 2023, sec. 5.2.                                         */
+/* ------------------------------------------------------------------ */
+
+#define DS_DENSIFY_PRUNE 0
+#define DS_DENSIFY_KEEP 1
+#define DS_DENSIFY_CLONE 2
+#define DS_DENSIFY_SPLIT 3
+
+typedef struct ds_densify_config {
+    float grad_threshold;   /* mean |dL/d(NDC centre)| over the steps a splat was visible
+                               at or above which it is cloned or split (3DGS: 2e-4)      */
+    float percent_dense;    /* clone when the largest scale <= percent_dense * extent,
+                               else split (3DGS: 0.01)                                   */
+    float scene_extent;     /* world units; <= 0: the largest distance of a splat centre
+                               from their mean (as 3DGS's camera extent)                 */
+    float min_opacity;      /* prune below this opacity (3DGS: 0.005)                    */
+    float max_scale_frac;   /* prune when the largest scale > max_scale_frac * extent;
+                               0 = off (3DGS: 0.1, after its first opacity reset)       */
+    float split_scale_div;  /* split samples' scale divisor (3DGS: 1.6 = 0.8 * 2)        */
+    uint64_t seed;          /* split samples                                             */
+} ds_densify_config;
+
+typedef struct ds_densify_result {
+    int32_t count_before, count_after;
+    int32_t pruned, cloned, split;   /* splats of each action (a split adds 2)        */
+} ds_densify_result;
+
+/* 3DGS's defaults (above). */
+void ds_densify_default_config(ds_densify_config *cfg);
+/* 1: every later fit step also accumulates, per splat visible in that step,
+ * |dL/d(NDC centre)| and one visit (one more launch per step); 0: stops.
+ * Either way the statistics start from zero. */
+ds_status ds_fitter_set_densify_stats(ds_fitter *f, int enable);
+/* The accumulated statistics, host [count] each (synchronises). */
+ds_status ds_fitter_densify_stats(ds_fitter *f, float *host_grad_sum, uint32_t *host_visits);
+/* One densification from the accumulated statistics: classifies every splat
+ * (per-splat action codes into host_actions [count before] unless NULL),
+ * then compacts the parameter slab and the Adam moments on the device
+ * (survivors in order with their moments, then clones, then split samples
+ * with zero moments). The splat count changes; the step counter, learning
+ * rates, target, camera and launch shapes are kept, the statistics restart
+ * from zero and a captured step is captured again. Synchronises. */
+ds_status ds_fitter_densify(ds_fitter *f, const ds_densify_config *cfg,
+                            ds_densify_result *result, uint8_t *host_actions);
+/* Current splat count (changes with ds_fitter_densify). */
+ds_status ds_fitter_count(ds_fitter *f, int32_t *count);
+
+/* ------------------------------------------------------------------ */
 /* Learned initializer (no counterpart in the reference, which starts    */
 /* every fit from random splats, src/utils/splat_utils.h:10-51)           */
 /* ------------------------------------------------------------------ */

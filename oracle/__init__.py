@@ -15,6 +15,7 @@ Layouts (field-major, n = splat count):
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -47,6 +48,10 @@ def lib():
             getattr(h, f"or_mse_{suf}").restype = T
             getattr(h, f"or_adam_{suf}").argtypes = [_P, _P, _P, _P, C.c_int64, T, T, T, T,
                                                      C.c_int]
+        h.or_densify.argtypes = [_P, _P, _P, C.c_int, C.c_int, _P, _P, C.c_float, C.c_float,
+                                 C.c_float, C.c_float, C.c_int, C.c_float, C.c_uint64, _P, _P, _P,
+                                 _P]
+        h.or_densify.restype = C.c_int64
         h.or_tile_assign_sort.argtypes = [_P, C.c_int, C.c_int, C.c_int, C.c_int64, _P, _P,
                                           _P, _P, _P, _P]
         h.or_tile_assign_sort.restype = C.c_int64
@@ -197,6 +202,47 @@ def comp_bwd(s2, n, flat, ranges, W, H, T, last, dLdC, dtype=np.float64, grad_mo
     if amb is not None:
         return gs2, int(e.value), amb
     return gs2, int(e.value)
+
+
+def densify_params(grad_threshold=2e-4, percent_dense=0.01, scene_extent=1.0,
+                   min_opacity=0.005, max_scale_frac=0.0, split_scale_div=1.6, seed=0):
+    """The engine's decision thresholds on raw parameters (log-scales and
+    opacity logits), computed in double and rounded to float as the fitter
+    does."""
+    f32 = lambda x: float(np.float32(x))  # noqa: E731
+    prune_big = max_scale_frac > 0 and scene_extent > 0
+    return dict(
+        grad_threshold=f32(grad_threshold),
+        clone_max_log_scale=f32(math.log(percent_dense * scene_extent))
+        if percent_dense > 0 and scene_extent > 0 else -math.inf,
+        min_logit=f32(math.log(min_opacity / (1 - min_opacity))) if min_opacity > 0
+        else -math.inf,
+        prune_max_log_scale=f32(math.log(max_scale_frac * scene_extent)) if prune_big
+        else math.inf,
+        prune_big=int(prune_big), split_log_div=f32(math.log(split_scale_div)), seed=int(seed))
+
+
+def densify(slab, m, v, n, deg, accum, visits, p):
+    """Clone / split / prune (Kerbl et al. 2023, sec. 5.2) as the engine does
+    it, from float parameters, Adam moments and gradient statistics, with the
+    thresholds of densify_params. Returns the per-splat actions (0 prune, 1
+    keep, 2 clone, 3 split), the new count, and the new slab (float64: split
+    samples exact in double), moments."""
+    nf = 14 + 3 * bands(deg)
+    sl, mm, vv = (_a(x, np.float32) for x in (slab, m, v))
+    acc = _a(accum, np.float32)
+    vis = _a(visits, np.uint32)
+    act = np.zeros(max(n, 1), np.uint8)
+    args = [sl.ctypes.data, mm.ctypes.data, vv.ctypes.data, n, nf, acc.ctypes.data,
+            vis.ctypes.data, p["grad_threshold"], p["clone_max_log_scale"], p["min_logit"],
+            p["prune_max_log_scale"], p["prune_big"], p["split_log_div"], p["seed"],
+            act.ctypes.data]
+    n_out = lib().or_densify(*args, None, None, None)
+    out = np.zeros(nf * n_out, np.float64)
+    om = np.zeros(nf * n_out, np.float32)
+    ov = np.zeros(nf * n_out, np.float32)
+    lib().or_densify(*args, out.ctypes.data, om.ctypes.data, ov.ctypes.data)
+    return {"action": act[:n], "n_out": int(n_out), "params": out, "m": om, "v": ov}
 
 
 def mse(rgb, tgt, npix, dtype=np.float64):

@@ -35,6 +35,8 @@
 //   mse            src/layers/mse_loss_layer.cu:22-67
 //   adam           src/optimizers/adam.cu:12-37, 59-73
 //   step order     src/pipelines/image_fitter.cpp:103-112, pipeline.h:11-27
+//   densify        no reference code (README.md:10 lists it as a TODO): Kerbl
+//                  et al., SIGGRAPH 2023, sec. 5.2 (clone / split / prune)
 #include <omp.h>
 
 #include <algorithm>
@@ -849,10 +851,122 @@ struct Fit {
     float loss = 0;
 };
 
+// ---------------------------------------------------------------- densify
+// Adaptive density control (Kerbl et al., SIGGRAPH 2023, sec. 5.2; a TODO in
+// the reference, README.md:10): the engine's rules, restated. Decisions are
+// float comparisons on the raw parameters (as the engine takes them); the
+// split samples are computed in double from the same hashed uniforms.
+inline uint64_t dens_mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+inline double dens_uniform(uint64_t seed, uint32_t i, uint32_t s, uint32_t k) {
+    const uint64_t h = dens_mix64(seed ^ dens_mix64(((uint64_t)i << 8) | (s << 2) | k));
+    return (double)((uint32_t)(h >> 40) + 1u) / 16777216.0;
+}
+
+enum { DENS_PRUNE = 0, DENS_KEEP = 1, DENS_CLONE = 2, DENS_SPLIT = 3 };
+
+int64_t densify(const float *slab, const float *m, const float *v, int n, int nfields,
+                const float *accum, const uint32_t *visits, float grad_threshold,
+                float clone_max_log_scale, float min_logit, float prune_max_log_scale,
+                int prune_big, float split_log_div, uint64_t seed, uint8_t *action,
+                double *out, float *out_m, float *out_v) {
+    std::vector<uint32_t> keep((size_t)n), added((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const uint32_t nv = visits[i];
+        const float g = nv ? accum[i] / (float)nv : 0.f;
+        const float ls = std::max(std::max(slab[3 * (size_t)n + i], slab[4 * (size_t)n + i]),
+                                  slab[5 * (size_t)n + i]);
+        const bool faint = slab[13 * (size_t)n + (size_t)(nfields - 14) * n + i] < min_logit;
+        const bool hot = g >= grad_threshold;
+        const bool small = ls <= clone_max_log_scale;
+        const bool too_big = prune_big && ls > prune_max_log_scale;
+        const bool child_too_big = prune_big && ls - split_log_div > prune_max_log_scale;
+        int a;
+        uint32_t k = 0, mm = 0;
+        if (faint) {
+            a = DENS_PRUNE;
+        } else if (hot && small) {
+            a = DENS_CLONE;
+            k = mm = too_big ? 0u : 1u;
+        } else if (hot) {
+            a = DENS_SPLIT;
+            mm = child_too_big ? 0u : 2u;
+        } else {
+            a = too_big ? DENS_PRUNE : DENS_KEEP;
+            k = too_big ? 0u : 1u;
+        }
+        action[i] = (uint8_t)a;
+        keep[i] = k;
+        added[i] = mm;
+    }
+    uint64_t nkeep = 0, nadd = 0;
+    for (int i = 0; i < n; ++i) {
+        nkeep += keep[i];
+        nadd += added[i];
+    }
+    const size_t n_out = nkeep + nadd;
+    if (!out) return (int64_t)n_out;
+    size_t jk = 0, ja = nkeep;
+    for (int i = 0; i < n; ++i) {
+        if (keep[i]) {
+            for (int f = 0; f < nfields; ++f) {
+                out[f * n_out + jk] = slab[(size_t)f * n + i];
+                out_m[f * n_out + jk] = m[(size_t)f * n + i];
+                out_v[f * n_out + jk] = v[(size_t)f * n + i];
+            }
+            ++jk;
+        }
+        for (uint32_t s = 0; s < added[i]; ++s, ++ja) {
+            for (int f = 0; f < nfields; ++f) {
+                out[f * n_out + ja] = slab[(size_t)f * n + i];
+                out_m[f * n_out + ja] = 0.f;
+                out_v[f * n_out + ja] = 0.f;
+            }
+            if (action[i] != DENS_SPLIT) continue;
+            const double u0 = dens_uniform(seed, (uint32_t)i, s, 0);
+            const double u1 = dens_uniform(seed, (uint32_t)i, s, 1);
+            const double u2 = dens_uniform(seed, (uint32_t)i, s, 2);
+            const double u3 = dens_uniform(seed, (uint32_t)i, s, 3);
+            const double r0 = std::sqrt(-2.0 * std::log(u0)), r1 = std::sqrt(-2.0 * std::log(u2));
+            const double two_pi = 6.283185307179586;
+            const double z[3] = {r0 * std::cos(two_pi * u1), r0 * std::sin(two_pi * u1),
+                                 r1 * std::cos(two_pi * u3)};
+            const RSq<double> rs = make_rs<double>(
+                slab[3 * (size_t)n + i], slab[4 * (size_t)n + i], slab[5 * (size_t)n + i],
+                slab[6 * (size_t)n + i], slab[7 * (size_t)n + i], slab[8 * (size_t)n + i],
+                slab[9 * (size_t)n + i]);
+            for (int r = 0; r < 3; ++r) {
+                double d = 0;
+                for (int c = 0; c < 3; ++c) d += rs.M[r][c] * z[c];
+                out[r * n_out + ja] = (double)slab[(size_t)r * n + i] + d;
+            }
+            for (int f = 3; f < 6; ++f)
+                out[f * n_out + ja] = (double)(slab[(size_t)f * n + i] - split_log_div);
+        }
+    }
+    return (int64_t)n_out;
+}
+
 }  // namespace
 
 // ======================================================================= C ABI
 extern "C" {
+
+// Densification (see densify above); with out == NULL only the actions and
+// the new count.
+int64_t or_densify(const float *slab, const float *m, const float *v, int n, int nfields,
+                   const float *accum, const uint32_t *visits, float grad_threshold,
+                   float clone_max_log_scale, float min_logit, float prune_max_log_scale,
+                   int prune_big, float split_log_div, uint64_t seed, uint8_t *action,
+                   double *out, float *out_m, float *out_v) {
+    return densify(slab, m, v, n, nfields, accum, visits, grad_threshold, clone_max_log_scale,
+                   min_logit, prune_max_log_scale, prune_big, split_log_div, seed, action, out,
+                   out_m, out_v);
+}
 
 #define OR_DEFINE(SUF, T)                                                                     \
     void or_act_fwd_##SUF(const T *params, int n, int deg, const T *cam, T *s3) {             \
