@@ -4,9 +4,9 @@
 //   ds_image_save_png   ImageSaver::saveAsPNG (src/io/image_io.cpp:82-93)
 //
 // Decoding: PNG (bit depth 8 or 16, grey / RGB / palette / grey+alpha / RGBA,
-// not interlaced; alpha is dropped) through zlib, and binary PPM (P6, maxval
-// 255). The reference decodes with stb_image and also reads JPEG; JPEG is not
-// supported here. Placement follows the reference exactly: scale =
+// not interlaced; alpha is dropped) through zlib, JPEG (baseline and
+// progressive Huffman, jpeg.cpp) and binary PPM (P6, maxval 255). The
+// reference decodes with stb_image (src/io/image_io.cpp:26). Placement follows the reference exactly: scale =
 // min((W - 2 pad) / w, (H - 2 pad) / h), fit size = (int)(w * scale) x
 // (int)(h * scale), centred at ((W - fit_w) / 2, (H - fit_h) / 2), black
 // elsewhere, value / 255. Resampling differs: the reference uses
@@ -28,6 +28,7 @@
 
 #include "../../../include/ds_api.h"
 #include "../ds_internal.hpp"
+#include "jpeg.hpp"
 
 namespace {
 
@@ -180,35 +181,63 @@ ds_status decode_file(const char *path, Rgb8 &img) {
     std::vector<uint8_t> f((std::istreambuf_iterator<char>(s)), std::istreambuf_iterator<char>());
     if (f.size() >= 2 && f[0] == 'P' && f[1] == '6') return decode_ppm(f, img);
     if (f.size() >= 8 && f[0] == 137 && f[1] == 'P') return decode_png(f, img);
+    if (f.size() >= 2 && f[0] == 0xFF && f[1] == 0xD8) {
+        dsh::JpegImage j;
+        std::string err;
+        if (!dsh::decode_jpeg(f.data(), f.size(), j, err))
+            return ds::set_error(DS_ERR_INVALID_ARG, std::string(path) + ": " + err);
+        img.w = j.w;
+        img.h = j.h;
+        img.px = std::move(j.rgb);
+        return DS_OK;
+    }
     return ds::set_error(DS_ERR_INVALID_ARG,
-                         std::string("unsupported image format (PNG or binary PPM): ") + path);
+                         std::string("unsupported image format (PNG, JPEG or binary PPM): ") +
+                             path);
 }
 
-// Weights of the source samples for each output sample along one axis.
+// Weights of the source samples for each output sample along one axis: the
+// cubic filters of the reference's resize (stbir_resize_uint8_linear,
+// src/io/image_io.cpp:44, whose defaults are Catmull-Rom when an axis grows
+// and Mitchell-Netravali B = C = 1/3 when it shrinks, with clamped edges),
+// evaluated between pixel centres; a shrinking axis stretches the filter by
+// the reduction factor. Weights are normalised to sum to 1.
 struct Tap {
     int i;
     float w;
 };
+double cubic(double x, double B, double C) {
+    x = std::fabs(x);
+    if (x < 1.0)
+        return ((12 - 9 * B - 6 * C) * x * x * x + (-18 + 12 * B + 6 * C) * x * x + (6 - 2 * B)) /
+               6.0;
+    if (x < 2.0)
+        return ((-B - 6 * C) * x * x * x + (6 * B + 30 * C) * x * x + (-12 * B - 48 * C) * x +
+                (8 * B + 24 * C)) /
+               6.0;
+    return 0.0;
+}
 std::vector<std::vector<Tap>> axis_weights(int src, int dst) {
     std::vector<std::vector<Tap>> out((size_t)dst);
-    const double r = (double)src / dst;
+    const double s = (double)dst / src;
+    const bool up = s >= 1.0;
+    const double B = up ? 0.0 : 1.0 / 3.0, Cc = up ? 0.5 : 1.0 / 3.0;
+    const double support = up ? 2.0 : 2.0 / s;
     for (int o = 0; o < dst; ++o) {
-        auto &t = out[(size_t)o];
-        if (dst <= src) {  // area average over [o r, (o + 1) r)
-            const double a = o * r, b = (o + 1) * r;
-            for (int i = (int)std::floor(a); i < (int)std::ceil(b) && i < src; ++i) {
-                const double ov = std::min(b, (double)i + 1) - std::max(a, (double)i);
-                if (ov > 0) t.push_back({i, (float)(ov / r)});
-            }
-        } else {  // linear between pixel centres
-            double x = (o + 0.5) * r - 0.5;
-            x = std::min(std::max(x, 0.0), (double)(src - 1));
-            const int i0 = (int)std::floor(x);
-            const int i1 = std::min(i0 + 1, src - 1);
-            const float f = (float)(x - i0);
-            t.push_back({i0, 1.f - f});
-            if (i1 != i0) t.push_back({i1, f});
+        const double centre = (o + 0.5) / s;  // in source pixel units
+        std::vector<double> w((size_t)src, 0.0);
+        const int i0 = (int)std::floor(centre - support - 0.5), i1 = (int)std::ceil(centre + support);
+        double total = 0.0;
+        for (int i = i0; i <= i1; ++i) {
+            const double x = (i + 0.5) - centre;
+            const double k = up ? cubic(x, B, Cc) : cubic(x * s, B, Cc) * s;
+            if (k == 0.0) continue;
+            w[(size_t)std::min(std::max(i, 0), src - 1)] += k;
+            total += k;
         }
+        auto &t = out[(size_t)o];
+        for (int i = 0; i < src; ++i)
+            if (w[(size_t)i] != 0.0) t.push_back({i, (float)(w[(size_t)i] / total)});
     }
     return out;
 }
@@ -216,13 +245,13 @@ std::vector<std::vector<Tap>> axis_weights(int src, int dst) {
 Rgb8 resize(const Rgb8 &in, int w, int h) {
     if (w == in.w && h == in.h) return in;
     const auto wx = axis_weights(in.w, w), wy = axis_weights(in.h, h);
-    std::vector<float> tmp((size_t)in.h * w * 3, 0.f);  // horizontal pass
+    std::vector<float> tmp((size_t)in.h * w * 3, 0.f);  // horizontal pass, values / 255
     for (int y = 0; y < in.h; ++y)
         for (int x = 0; x < w; ++x)
             for (const Tap &t : wx[(size_t)x])
                 for (int c = 0; c < 3; ++c)
                     tmp[((size_t)y * w + x) * 3 + c] +=
-                        t.w * in.px[((size_t)y * in.w + t.i) * 3 + c];
+                        t.w * (in.px[((size_t)y * in.w + t.i) * 3 + c] * (1.f / 255.f));
     Rgb8 out;
     out.w = w;
     out.h = h;
@@ -233,7 +262,7 @@ Rgb8 resize(const Rgb8 &in, int w, int h) {
                 float v = 0.f;
                 for (const Tap &t : wy[(size_t)y]) v += t.w * tmp[((size_t)t.i * w + x) * 3 + c];
                 out.px[((size_t)y * w + x) * 3 + c] =
-                    (uint8_t)std::min(255.f, std::max(0.f, std::floor(v + 0.5f)));
+                    (uint8_t)std::min(255.f, std::max(0.f, std::floor(v * 255.f + 0.5f)));
             }
     return out;
 }

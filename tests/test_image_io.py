@@ -134,14 +134,54 @@ def test_fit_is_centred_with_black_borders(tmp_path):
     assert np.all(img[~inside] == 0)
 
 
-def test_shrinking_by_an_integer_factor_gives_block_means(tmp_path):
-    px = np.random.default_rng(7).integers(0, 256, (16, 24, 3), dtype=np.uint8)
+def cubic(x, B, C):
+    x = np.abs(x)
+    a = ((12 - 9 * B - 6 * C) * x ** 3 + (-18 + 12 * B + 6 * C) * x ** 2 + (6 - 2 * B)) / 6
+    b = ((-B - 6 * C) * x ** 3 + (6 * B + 30 * C) * x ** 2 + (-12 * B - 48 * C) * x +
+         (8 * B + 24 * C)) / 6
+    return np.where(x < 1, a, np.where(x < 2, b, 0.0))
+
+
+def resample_matrix(src, dst):
+    """dst x src weights of the reference's default resize filters
+    (stb_image_resize2: Catmull-Rom up, Mitchell B = C = 1/3 down, clamped
+    edges), as a dense numpy matrix built from the filter definitions."""
+    s = dst / src
+    up = s >= 1
+    B, C = (0.0, 0.5) if up else (1 / 3, 1 / 3)
+    M = np.zeros((dst, src))
+    for o in range(dst):
+        c = (o + 0.5) / s
+        i = np.arange(int(np.floor(c - 2 / min(s, 1) - 1)), int(np.ceil(c + 2 / min(s, 1))) + 1)
+        x = (i + 0.5) - c
+        k = cubic(x, B, C) if up else cubic(x * s, B, C) * s
+        np.add.at(M[o], np.clip(i, 0, src - 1), k)
+        M[o] /= k.sum()
+    return M
+
+
+@pytest.mark.parametrize("sw,sh,W,H", [(24, 16, 12, 8), (13, 9, 40, 30), (50, 31, 37, 23)])
+def test_resize_follows_the_reference_filters(tmp_path, sw, sh, W, H):
+    """The loader's resampling against a dense-matrix statement of the same
+    filters (to 1 of 255, the float rounding), and a constant image stays
+    constant."""
+    px = np.random.default_rng(7).integers(0, 256, (sh, sw, 3), dtype=np.uint8)
     p = tmp_path / "img.png"
     p.write_bytes(encode_png(px, 2))
-    img, _, _ = ds.load_image(p, 12, 8)
-    means = px.reshape(8, 2, 12, 2, 3).astype(np.float64).mean(axis=(1, 3))
-    want = np.floor(means + 0.5) / 255
-    np.testing.assert_allclose(img.reshape(8, 12, 3), want, atol=1e-7)
+    scale = min(np.float32(W) / np.float32(sw), np.float32(H) / np.float32(sh))
+    fw, fh = int(sw * scale), int(sh * scale)
+    img, _, _ = ds.load_image(p, W, H)
+    img = img.reshape(H, W, 3)
+    ox, oy = (W - fw) // 2, (H - fh) // 2
+    got = np.round(img[oy:oy + fh, ox:ox + fw] * 255)
+    Mx, My = resample_matrix(sw, fw), resample_matrix(sh, fh)
+    want = np.clip(np.floor(np.einsum("ys,sxc,xt->ytc", My, px / 255.0, Mx.T) * 255 + 0.5), 0,
+                   255)
+    assert np.abs(got - want).max() <= 1
+    p.write_bytes(encode_png(np.full((sh, sw, 3), 77, np.uint8), 2))
+    img, _, _ = ds.load_image(p, W, H)
+    img = img.reshape(H, W, 3)
+    assert np.all(np.round(img[oy:oy + fh, ox:ox + fw] * 255) == 77)
 
 
 def test_save_png_bytes_follow_the_reference_formula(tmp_path):
