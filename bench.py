@@ -12,7 +12,8 @@ at the state reached after the timed steps; its algorithmic work (splat
 evaluations) is counted by the oracle on the same state. `cpu_baseline` is the oracle's fp32
 OpenMP transcription of the same step on this host's cores. Beside the line:
 `time_to_30db` (the metric's second half: wall time to 30 dB PSNR at
-256x256 / 4096 splats from the learned initializer vs from random splats),
+256x256 / 4096 splats from the trained initializer shipped in
+diffsplat_amd/data vs from random splats, 512 held-out seeds),
 `viewer` (1M-splat 1080p frames/s) and `batched` (64 uneven fits at once).
 """
 from __future__ import annotations
@@ -67,7 +68,7 @@ def parse():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-viewer", action="store_true")
     ap.add_argument("--no-time-to-psnr", action="store_true")
-    ap.add_argument("--eval-images", type=int, default=64,
+    ap.add_argument("--eval-images", type=int, default=512,
                     help="held-out images of the time-to-30dB comparison")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=500,
@@ -143,103 +144,42 @@ def roofline_composite(ds, oracle, fit, target, W, H, N, reps=20):
                     "(which counts packed FMA, not used here); the kernel is VALU-issue-bound"}
 
 
-TTP_POS_DECAY = 0.998  # position lr schedule of the time-to-PSNR fits
-
-
-def fit_to_psnr(ds, net, image, d_image, seed, W, H, N, target_db=30.0, max_steps=3000,
-                check_every=10):
-    """Wall time for one fit of `image` to reach target_db PSNR: from the
-    random frustum init (net None) or from the learned initializer's output
-    (its inference is inside the timed region). HIP graph on (its capture on
-    the first step is inside the timed region too); the loss of the last
-    step's forward is read every check_every steps (one host sync). The
-    position learning rate decays by TTP_POS_DECAY per step: at a constant
-    rate the fit oscillates below 30 dB."""
-    view, proj, cam = ds.fitter_camera(W, H)
-    lr = {"lr_pos": 1e-2, "lr_sh_dc": 1e-2, "lr_scale": 5e-3, "lr_rot": 6e-4,
-          "lr_opacity": 6e-3, "lr_pos_decay": TTP_POS_DECAY}
-    fit = ds.ImageFitter(W, H, N, view=view, proj=proj, cam_pos=cam, lr=lr)
-    fit.set_target(image)
-    fit.upload_params(ds.frustum_splats(seed, N, aspect=W / H))
-    fit.set_graph(True)
-    fit.synchronize()
-    t0 = time.perf_counter()
-    if net is not None:
-        pslab, _ = fit.param_slabs()
-        net.infer(d_image, pslab, fit.stream())
-        fit.reset_optimizer()
-    steps, psnr = 0, 0.0
-    while steps < max_steps:
-        fit.step(check_every)
-        steps += check_every
-        psnr = ds.psnr_from_loss(fit.loss())
-        if psnr >= target_db:
-            break
-    el = time.perf_counter() - t0
-    fit.close()
-    return steps, el, psnr
-
-
-def learned_vs_plain(ds, n_train=10000, train_steps=10000, n_eval=64, W=256, H=256,
-                     target_db=30.0):
-    """Time to target_db on held-out synthetic images (seeds 90000+i), from
-    random init and from the learned initializer, both with the same fitter
-    settings. The initializer is first trained end to end through the
-    renderer on n_train synthetic images (seeds 4000+i), one image per step,
-    with a halving-every-3000-steps Adam rate from 2e-3."""
-    from concurrent.futures import ThreadPoolExecutor
-    N = (W // 4) * (H // 4)
-    img_floats = W * H * 3
-    t0 = time.perf_counter()
-    d_train = ds.DeviceArray(4 * img_floats * n_train)
-    chunk = 1024
-    with ThreadPoolExecutor(16) as ex:
-        for c0 in range(0, n_train, chunk):
-            seeds = range(4000 + c0, 4000 + min(n_train, c0 + chunk))
-            block = np.stack(list(ex.map(lambda s: ds.target_image(s, W, H, 32), seeds)))
-            ds.check(ds.lib().ds_memcpy_htod(C.c_void_p(d_train.ptr + 4 * img_floats * c0),
-                                             ds.fptr(block), block.nbytes), "upload")
-        evals = list(ex.map(lambda s: ds.target_image(s, W, H, 32),
-                            range(90000, 90000 + n_eval)))
-    gen_s = time.perf_counter() - t0
-    view, proj, cam = ds.fitter_camera(W, H)
-    net = ds.InitNet(W, H, seed=0)
-    trainer = ds.ImageFitter(W, H, N, view=view, proj=proj, cam_pos=cam)
-    t0 = time.perf_counter()
-    for k in range(train_steps):
-        img = C.c_void_p(d_train.ptr + 4 * img_floats * (k % n_train))
-        ds.check(ds.lib().ds_initnet_train_step(net.h, trainer.h, img,
-                                                2e-3 * 0.5 ** (k / 3000.0)), "train")
-    trainer.synchronize()
-    train_s = time.perf_counter() - t0
-    del d_train
-    d_evals = [ds.DeviceArray.from_numpy(e) for e in evals]
-    # PSNR of the initializer's output alone
-    pslab, _ = trainer.param_slabs()
-    init_psnr = []
-    for d in d_evals:
-        net.infer(d, pslab, trainer.stream())
-        trainer.set_target_device(d.ptr)
-        trainer.forward()
-        init_psnr.append(ds.psnr_from_loss(trainer.loss()))
-    trainer.close()
-    out = {}
-    for name, use in (("plain", None), ("learned", net)):
-        r = [fit_to_psnr(ds, use, evals[i], d_evals[i], 90000 + i, W, H, N, target_db)
-             for i in range(n_eval)]
-        out[name] = {"mean_seconds": round(float(np.mean([x[1] for x in r])), 5),
-                     "mean_steps": round(float(np.mean([x[0] for x in r])), 1),
-                     "reached": int(sum(x[2] >= target_db for x in r)), "fits": n_eval}
+def learned_vs_plain(ds, n_eval=512, target_db=30.0):
+    """Time to target_db on held-out synthetic images (256x256, 32 ellipses,
+    seeds 90000 + i, none of them a training image), from random splats and
+    from the trained initializer shipped with the package (diffsplat_amd/
+    data/initnet_256.npy: supervised on 10,000 (image, 1000-step fit) pairs,
+    then fine-tuned through the renderer), with the same fitter settings.
+    The learned time includes inference and the fallback (the initializer's
+    splats are kept only when they render with a lower loss than the random
+    ones). The training run's own measurements of the compared recipes
+    (supervised only, render loss only) come from its settings file."""
+    from diffsplat_amd import initnet_train as T
+    net = T.load(T.WEIGHTS)
+    with open(T.WEIGHTS.rsplit(".", 1)[0] + ".json") as f:
+        meta = json.load(f)
+    out = {"plain": T.evaluate(None, n_eval, target_db=target_db),
+           "learned": T.evaluate(net, n_eval, target_db=target_db)}
     net.close()
-    return {"workload": f"{W}x{H} synthetic targets, {N} splats, {n_eval} held-out seeds",
+    train = meta["time_to_30db"]
+    return {"workload": f"{T.W}x{T.H} synthetic targets, {(T.W // 4) * (T.H // 4)} splats, "
+                        f"{n_eval} held-out seeds from {T.EVAL_SEED0}",
             "target_db": target_db, "plain": out["plain"], "learned": out["learned"],
             "speedup": round(out["plain"]["mean_seconds"] / out["learned"]["mean_seconds"], 2),
-            "learned_init_psnr_db": round(float(np.mean(init_psnr)), 3),
-            "train_images": n_train, "train_steps": train_steps,
-            "train_seconds": round(train_s, 3), "generate_seconds": round(gen_s, 3),
-            "lr_pos_decay": TTP_POS_DECAY,
-            "note": "learned time includes the initializer's inference; training is a one-off "
-                    "cost reported separately"}
+            "steps_ratio": round(out["plain"]["mean_steps"] / out["learned"]["mean_steps"], 2),
+            "recipe": {"pairs": meta["data"]["pairs"], "fit_steps": meta["data"]["fit_steps"],
+                       "supervised_epochs": meta["epochs"], "render_steps": meta["render_steps"],
+                       "pairs_seconds": meta["data"]["seconds"],
+                       "supervised_seconds": meta["train_seconds"],
+                       "render_seconds": meta["render_seconds"]},
+            "compared_at_training": {
+                k: {"mean_seconds": v["mean_seconds"], "mean_steps": v["mean_steps"],
+                    "initial_psnr_db": v["initial_psnr_db"], "fits": v["fits"]}
+                for k, v in train.items()},
+            "lr_pos_decay": T.FIT_LR["lr_pos_decay"],
+            "note": "learned time includes the initializer's inference and the random-init "
+                    "fallback check; training is a one-off cost (python -m "
+                    "diffsplat_amd.initnet_train)"}
 
 
 def batched_fit(ds, count=64, steps=500, seq_steps=10, compare=True, graph=True):

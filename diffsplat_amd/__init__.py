@@ -570,8 +570,11 @@ class ImageFitter:
 
 
 class InitNet:
-    """ds_initnet: the learned initializer (a bf16 MFMA CNN producing one SH-0
-    splat per 4x4-pixel cell), trained end to end through the renderer."""
+    """ds_initnet: the learned initializer, a bf16 MFMA CNN of six 3x3
+    convolutions producing 16 SH-0 splats per 16x16-pixel cell (one per 4x4
+    pixels), trained on (image, fitted splats) pairs or through the renderer."""
+
+    HEAD_VALUES = 10  # per splat
 
     def __init__(self, width: int, height: int, fovy: float = math.radians(60.0), seed: int = 0):
         h = C.c_void_p()
@@ -599,9 +602,30 @@ class InitNet:
         check(lib().ds_initnet_infer(self.h, ip, sp, stream), "ds_initnet_infer")
 
     def head(self) -> np.ndarray:
-        out = np.empty(self.num_splats * 16, np.float32)
+        """Head values of the last forward: [cells][16 splats x 10]."""
+        out = np.empty(self.num_splats * self.HEAD_VALUES, np.float32)
         check(lib().ds_initnet_head(self.h, fptr(out)), "ds_initnet_head")
         return out
+
+    def head_targets(self, slab, targets, stream=None) -> None:
+        """Regression targets (device, head layout) of a fitted slab (device)."""
+        check(lib().ds_initnet_head_targets(self.h, getattr(slab, "ptr", slab),
+                                            getattr(targets, "ptr", targets), stream),
+              "ds_initnet_head_targets")
+
+    def set_loss_weights(self, weights=None) -> None:
+        """Per head value weights of the supervised loss (None: defaults)."""
+        if weights is None:
+            check(lib().ds_initnet_set_loss_weights(self.h, None), "ds_initnet_set_loss_weights")
+            return
+        w = np.ascontiguousarray(weights, np.float32).reshape(-1)
+        assert w.size == self.HEAD_VALUES
+        check(lib().ds_initnet_set_loss_weights(self.h, fptr(w)), "ds_initnet_set_loss_weights")
+
+    def train_step_supervised(self, image, targets, lr: float, loss=None, stream=None) -> None:
+        check(lib().ds_initnet_train_step_supervised(
+            self.h, getattr(image, "ptr", image), getattr(targets, "ptr", targets), lr,
+            getattr(loss, "ptr", loss), stream), "ds_initnet_train_step_supervised")
 
     def grads(self, image: "DeviceArray", grad_slab: "DeviceArray") -> np.ndarray:
         out = np.empty(self.num_params, np.float32)

@@ -235,6 +235,9 @@ _SIGS = {
     "ds_initnet_head": (C.c_int, [_P, _F]),
     "ds_initnet_grads": (C.c_int, [_P, _P, _P, _F]),
     "ds_initnet_train_step": (C.c_int, [_P, _P, _P, C.c_float]),
+    "ds_initnet_head_targets": (C.c_int, [_P, _P, _P, _P]),
+    "ds_initnet_train_step_supervised": (C.c_int, [_P, _P, _P, C.c_float, _P, _P]),
+    "ds_initnet_set_loss_weights": (C.c_int, [_P, _P]),
     "ds_fitter_forward_backward": (C.c_int, [_P]),
     "ds_fitter_param_slabs": (C.c_int, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "ds_fitter_set_target_device": (C.c_int, [_P, _P]),

@@ -1,9 +1,11 @@
-// Convolution kernels of the learned initializer (a small CNN that maps a
-// target image to one splat per 4x4-pixel cell), on bf16 MFMA with fp32
-// accumulation: forward, backward-data and backward-weight as implicit
-// GEMMs, plus the head that turns the CNN's per-cell outputs into splat
-// parameters and its backward. No counterpart in the reference, which starts
-// every fit from random splats (src/utils/splat_utils.h:10-51).
+// Convolution kernels of the learned initializer (a CNN of six 3x3
+// convolutions that maps a target image to a grid of 16x16-pixel cells with
+// 16 splats each), on bf16 MFMA with fp32 accumulation: forward,
+// backward-data and backward-weight as implicit GEMMs, plus the head that
+// turns the CNN's per-cell outputs into splat parameters, its backward, its
+// inverse (the regression targets of fitted splats) and the regression loss.
+// No counterpart in the reference, which starts every fit from random splats
+// (src/utils/splat_utils.h:10-51).
 //
 // Layout: activations NHWC bf16; weights [Cout][Kpad] bf16 with the GEMM
 // index k = (kh * KS + kw) * Cin + ci, zero-padded to Kpad (a multiple of
@@ -208,19 +210,23 @@ __global__ __launch_bounds__(64) void k_conv_bwd_weight(const __bf16 *__restrict
     }
 }
 
-// db[co] += sum over pixels of d_out[p][co]; COUT <= 64 threads per pixel row.
+// db[co] += sum over pixels of d_out[p][co]; RPB pixel rows of COUT <= 256
+// channels in flight per workgroup (threads past RPB * COUT idle).
 template <int COUT>
 __global__ __launch_bounds__(256) void k_bias_grad(const __bf16 *__restrict__ dout, int P,
                                                    float *__restrict__ db) {
-    constexpr int RPB = 256 / COUT;  // pixel rows in flight per workgroup
+    static_assert(COUT <= 256, "one thread per channel");
+    constexpr int RPB = 256 / COUT;
     __shared__ float s[COUT];
     const int c = threadIdx.x % COUT, r = threadIdx.x / COUT;
     if (threadIdx.x < COUT) s[threadIdx.x] = 0.f;
     __syncthreads();
-    float sum = 0.f;
-    for (int p = blockIdx.x * RPB + r; p < P; p += gridDim.x * RPB)
-        sum += (float)dout[(size_t)p * COUT + c];
-    atomicAdd(&s[c], sum);
+    if (r < RPB) {
+        float sum = 0.f;
+        for (int p = blockIdx.x * RPB + r; p < P; p += gridDim.x * RPB)
+            sum += (float)dout[(size_t)p * COUT + c];
+        atomicAdd(&s[c], sum);
+    }
     __syncthreads();
     if (threadIdx.x < COUT) atomicAdd(&db[threadIdx.x], s[threadIdx.x]);
 }
@@ -261,27 +267,45 @@ __global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ x, in
 
 __device__ __forceinline__ float sigmoidf(float x) { return 1.f / (1.f + expf(-x)); }
 
-// Per cell c (row-major over the gw x gh grid): 16 head values h -> the SH-0
-// splat of the cell in the slab (field-major, n = gw * gh splats).
+// Head values of splat (ax, ay): in its output cell, sub-position j.
+__device__ __forceinline__ size_t head_base(const HeadGeom &g, int ax, int ay) {
+    const int cell = (ay / kCellSub) * (g.gw / kCellSub) + ax / kCellSub;
+    const int j = (ay % kCellSub) * kCellSub + ax % kCellSub;
+    return (size_t)cell * kHeadChannels + (size_t)j * kHeadValues;
+}
+
+// Per splat c (row-major over the gw x gh anchors): its head values -> its
+// SH-0 parameters in the slab (field-major, n = gw * gh splats). The splat
+// sits on the ray through its image position (u, v) at depth d. Its
+// footprint is the log of its 2x2 apparent covariance (its size at depth
+// g.depth): mean log-scale m = log_scale0 + max_log_scale tanh(h2) and the
+// anisotropy vector (h3, h4) = r (cos 2 theta, sin 2 theta), half the
+// log-scale difference a = max_aniso tanh(sqrt(r^2 + eps^2) / max_aniso)
+// and the major axis at angle theta: log-scales m +- a, both shifted by
+// log(d / g.depth) so that d orders splats without changing footprints.
+constexpr float kAnisoEps2 = 1e-8f;
+
 __global__ __launch_bounds__(256) void k_head_fwd(const float *__restrict__ h, HeadGeom g,
                                                   float *__restrict__ slab) {
     const int n = g.gw * g.gh;
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= n) return;
-    const float *hc = h + (size_t)c * kHeadChannels;
     const int cx = c % g.gw, cy = c / g.gw;
+    const float *hc = h + head_base(g, cx, cy);
     const float u = g.cell * (cx + 0.5f) + g.max_off * tanhf(hc[0]);
     const float v = g.cell * (cy + 0.5f) + g.max_off * tanhf(hc[1]);
     const float xn = 2.f * u / g.W - 1.f, yn = 1.f - 2.f * v / g.H;
-    slab[0 * n + c] = xn * g.depth * g.tan_half_fovy * g.aspect;
-    slab[1 * n + c] = yn * g.depth * g.tan_half_fovy;
-    slab[2 * n + c] = -g.depth;
-    const float sx = g.log_scale0 + g.max_log_scale * tanhf(hc[2]);
-    const float sy = g.log_scale0 + g.max_log_scale * tanhf(hc[3]);
-    slab[3 * n + c] = sx;
-    slab[4 * n + c] = sy;
-    slab[5 * n + c] = 0.5f * (sx + sy) + g.log_scale_z;
-    const float th = 0.5f * hc[4];
+    const float d = g.depth + g.max_dz * tanhf(hc[9]);
+    slab[0 * n + c] = xn * d * g.tan_half_fovy * g.aspect;
+    slab[1 * n + c] = yn * d * g.tan_half_fovy;
+    slab[2 * n + c] = -d;
+    const float m = g.log_scale0 + g.max_log_scale * tanhf(hc[2]) + logf(d / g.depth);
+    const float r = sqrtf(hc[3] * hc[3] + hc[4] * hc[4] + kAnisoEps2);
+    const float a = g.max_aniso * tanhf(r / g.max_aniso);
+    slab[3 * n + c] = m + a;
+    slab[4 * n + c] = m - a;
+    slab[5 * n + c] = m + g.log_scale_z;
+    const float th = 0.25f * atan2f(hc[4], hc[3]);  // half of theta
     slab[6 * n + c] = cosf(th);
     slab[7 * n + c] = 0.f;
     slab[8 * n + c] = 0.f;
@@ -292,7 +316,7 @@ __global__ __launch_bounds__(256) void k_head_fwd(const float *__restrict__ h, H
 }
 
 // Chain rule of k_head_fwd: slab gradients -> head-value gradients (bf16,
-// the backward GEMMs' input; unused channels get 0).
+// the backward GEMMs' input).
 __global__ __launch_bounds__(256) void k_head_bwd(const float *__restrict__ h, HeadGeom g,
                                                   const float *__restrict__ gs,
                                                   float *__restrict__ dh32,
@@ -300,35 +324,149 @@ __global__ __launch_bounds__(256) void k_head_bwd(const float *__restrict__ h, H
     const int n = g.gw * g.gh;
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= n) return;
-    const float *hc = h + (size_t)c * kHeadChannels;
-    float d[kHeadChannels];
-#pragma unroll
-    for (int k = 0; k < kHeadChannels; ++k) d[k] = 0.f;
-    // position: u -> x_ndc -> pos_x, v -> y_ndc -> pos_y
-    const float dxn = gs[0 * n + c] * g.depth * g.tan_half_fovy * g.aspect;
-    const float dyn = gs[1 * n + c] * g.depth * g.tan_half_fovy;
-    const float t0 = tanhf(hc[0]), t1 = tanhf(hc[1]);
-    d[0] = dxn * (2.f / g.W) * g.max_off * (1.f - t0 * t0);
-    d[1] = -dyn * (2.f / g.H) * g.max_off * (1.f - t1 * t1);
-    // scales
-    const float gz = 0.5f * gs[5 * n + c];
-    const float t2 = tanhf(hc[2]), t3 = tanhf(hc[3]);
-    d[2] = (gs[3 * n + c] + gz) * g.max_log_scale * (1.f - t2 * t2);
-    d[3] = (gs[4 * n + c] + gz) * g.max_log_scale * (1.f - t3 * t3);
-    // rotation about the view axis
-    const float th = 0.5f * hc[4];
-    d[4] = 0.5f * (-sinf(th) * gs[6 * n + c] + cosf(th) * gs[9 * n + c]);
+    const int cx = c % g.gw, cy = c / g.gw;
+    const size_t base = head_base(g, cx, cy);
+    const float *hc = h + base;
+    float dv[kHeadValues];
+    const float t0 = tanhf(hc[0]), t1 = tanhf(hc[1]), t9 = tanhf(hc[9]);
+    const float u = g.cell * (cx + 0.5f) + g.max_off * t0;
+    const float v = g.cell * (cy + 0.5f) + g.max_off * t1;
+    const float xn = 2.f * u / g.W - 1.f, yn = 1.f - 2.f * v / g.H;
+    const float d = g.depth + g.max_dz * t9;
+    // position: u -> x_ndc -> pos_x, v -> y_ndc -> pos_y, d -> all three
+    const float gx = gs[0 * n + c] * g.tan_half_fovy * g.aspect;
+    const float gy = gs[1 * n + c] * g.tan_half_fovy;
+    dv[0] = gx * d * (2.f / g.W) * g.max_off * (1.f - t0 * t0);
+    dv[1] = -gy * d * (2.f / g.H) * g.max_off * (1.f - t1 * t1);
+    // footprint: m in all three log-scales, +-a in the in-plane two
+    const float t2 = tanhf(hc[2]);
+    const float gm = gs[3 * n + c] + gs[4 * n + c] + gs[5 * n + c];
+    dv[2] = gm * g.max_log_scale * (1.f - t2 * t2);
+    const float r2 = hc[3] * hc[3] + hc[4] * hc[4] + kAnisoEps2, r = sqrtf(r2);
+    const float ta = tanhf(r / g.max_aniso);
+    const float ga = (gs[3 * n + c] - gs[4 * n + c]) * (1.f - ta * ta) / r;  // dL/dr / r
+    // theta/2 = atan2(h4, h3) / 4; the quaternion (cos, 0, 0, sin) of it
+    const float th = 0.25f * atan2f(hc[4], hc[3]);
+    const float gth = -sinf(th) * gs[6 * n + c] + cosf(th) * gs[9 * n + c];
+    dv[3] = ga * hc[3] - 0.25f * gth * hc[4] / r2;
+    dv[4] = ga * hc[4] + 0.25f * gth * hc[3] / r2;
+    // depth: the position along the ray and log(d / depth) in every log-scale
+    dv[9] = (gx * xn + gy * yn - gs[2 * n + c] + gm / d) * g.max_dz * (1.f - t9 * t9);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const float s = sigmoidf(hc[5 + k]);
-        d[5 + k] = gs[(10 + k) * n + c] * s * (1.f - s) / kSHC0;
+        dv[5 + k] = gs[(10 + k) * n + c] * s * (1.f - s) / kSHC0;
     }
-    d[8] = gs[13 * n + c];
+    dv[8] = gs[13 * n + c];
 #pragma unroll
-    for (int k = 0; k < kHeadChannels; ++k) {
-        if (dh32) dh32[(size_t)c * kHeadChannels + k] = d[k];
-        dh[(size_t)c * kHeadChannels + k] = (__bf16)d[k];
+    for (int k = 0; k < kHeadValues; ++k) {
+        if (dh32) dh32[base + k] = dv[k];
+        dh[base + k] = (__bf16)dv[k];
     }
+}
+
+// Regression targets: the head values whose splat has the fitted splat's
+// footprint, each head function inverted (a tanh or sigmoid argument clamped
+// to +-kHeadClip of its range first):
+//   - image position (u, v) of the fitted centre, and its depth;
+//   - the footprint: the fitted 3D covariance projected along the view rays
+//     (K Sigma K^T with K = [[1, 0, x/d], [0, 1, y/d]], the pinhole camera's
+//     projection Jacobian divided by its focal terms), rescaled to the head
+//     depth g.depth; from its eigenvalues the mean log-scale and half their
+//     log difference a, from its major axis (angle theta) the anisotropy
+//     vector's direction (cos 2 theta, sin 2 theta). The head's splats are
+//     flat (z log-scale well below the in-plane ones), so this footprint is
+//     theirs up to that flat extent;
+//   - colour and opacity logit.
+constexpr float kHeadClip = 0.98f;
+
+__device__ __forceinline__ float atanh_clamped(float t) {
+    t = fminf(fmaxf(t, -kHeadClip), kHeadClip);
+    return 0.5f * logf((1.f + t) / (1.f - t));
+}
+
+__global__ __launch_bounds__(256) void k_head_targets(const float *__restrict__ slab, HeadGeom g,
+                                                      float *__restrict__ tgt) {
+    const int n = g.gw * g.gh;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const int cx = c % g.gw, cy = c / g.gw;
+    float *t = tgt + head_base(g, cx, cy);
+    const float x = slab[0 * n + c], y = slab[1 * n + c];
+    const float d = fmaxf(-slab[2 * n + c], 1e-3f);
+    const float xn = x / (d * g.tan_half_fovy * g.aspect);
+    const float yn = y / (d * g.tan_half_fovy);
+    const float u = 0.5f * (xn + 1.f) * g.W, v = 0.5f * (1.f - yn) * g.H;
+    t[0] = atanh_clamped((u - g.cell * (cx + 0.5f)) / g.max_off);
+    t[1] = atanh_clamped((v - g.cell * (cy + 0.5f)) / g.max_off);
+    t[9] = atanh_clamped((d - g.depth) / g.max_dz);
+    // Sigma = R diag(s^2) R^T of the normalised quaternion (w, x, y, z)
+    float qw = slab[6 * n + c], qx = slab[7 * n + c], qy = slab[8 * n + c], qz = slab[9 * n + c];
+    const float qn = rsqrtf(fmaxf(qw * qw + qx * qx + qy * qy + qz * qz, 1e-30f));
+    qw *= qn;
+    qx *= qn;
+    qy *= qn;
+    qz *= qn;
+    const float R[3][3] = {
+        {1.f - 2.f * (qy * qy + qz * qz), 2.f * (qx * qy - qw * qz), 2.f * (qx * qz + qw * qy)},
+        {2.f * (qx * qy + qw * qz), 1.f - 2.f * (qx * qx + qz * qz), 2.f * (qy * qz - qw * qx)},
+        {2.f * (qx * qz - qw * qy), 2.f * (qy * qz + qw * qx), 1.f - 2.f * (qx * qx + qy * qy)}};
+    const float s2[3] = {expf(2.f * slab[3 * n + c]), expf(2.f * slab[4 * n + c]),
+                         expf(2.f * slab[5 * n + c])};
+    // rows of K R: footprint A = sum_k s2_k (KR)_{.k} (KR)_{.k}^T
+    const float ax_ = x / d, ay_ = y / d;
+    float a = 0.f, b = 0.f, e = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float p = R[0][k] + ax_ * R[2][k], q = R[1][k] + ay_ * R[2][k];
+        a += s2[k] * p * p;
+        b += s2[k] * p * q;
+        e += s2[k] * q * q;
+    }
+    // at depth g.depth: scale by (g.depth / d)^2
+    const float k2 = (g.depth / d) * (g.depth / d);
+    const float tr = 0.5f * (a + e), half = sqrtf(0.25f * (a - e) * (a - e) + b * b);
+    const float l1 = k2 * (tr + half), l2 = fmaxf(k2 * (tr - half), 1e-12f * l1);
+    const float L1 = 0.5f * logf(l1), L2 = 0.5f * logf(l2);
+    t[2] = atanh_clamped((0.5f * (L1 + L2) - g.log_scale0) / g.max_log_scale);
+    const float r = g.max_aniso * atanh_clamped(0.5f * (L1 - L2) / g.max_aniso);
+    const float c2 = half > 0.f ? 0.5f * (a - e) / half : 1.f, s2_ = half > 0.f ? b / half : 0.f;
+    t[3] = r * c2;
+    t[4] = r * s2_;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float col = fminf(fmaxf(slab[(10 + k) * n + c] * kSHC0 + 0.5f, 1.f - kHeadClip),
+                                kHeadClip);
+        t[5 + k] = logf(col / (1.f - col));
+    }
+    t[8] = slab[13 * n + c] - g.logit_bias;
+}
+
+// Weighted squared error of the head values, mean over the splats, and its
+// gradient.
+__global__ __launch_bounds__(256) void k_head_regression(const float *__restrict__ h,
+                                                         const float *__restrict__ tgt,
+                                                         HeadGeom g, HeadWeights w,
+                                                         __bf16 *__restrict__ dh, float *loss) {
+    __shared__ float red[256 / kWave];
+    const int n = g.gw * g.gh;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    const float inv_n = 1.f / (float)n;
+    float l = 0.f;
+    if (c < n) {
+        const size_t base = head_base(g, c % g.gw, c / g.gw);
+#pragma unroll
+        for (int k = 0; k < kHeadValues; ++k) {
+            const float d = h[base + k] - tgt[base + k];
+            dh[base + k] = (__bf16)(2.f * w.w[k] * d * inv_n);
+            l += w.w[k] * d * d;
+        }
+    }
+    if (!loss) return;
+    l = wave_sum(l * inv_n);
+    if (__lane_id() == 0) red[threadIdx.x / kWave] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, red[0] + red[1] + red[2] + red[3]);
 }
 
 int blocks_for(long n, int cap = 4096) {
@@ -337,11 +475,16 @@ int blocks_for(long n, int cap = 4096) {
 }
 
 // Launchers of one layer shape. Output tiles are 16 x 16*TN with TN up to 4.
+// Column tiles per wave: the largest of 4, 3, 2, 1 dividing the 16-wide tiles.
+constexpr int tiles_per_wave(int tiles) {
+    return tiles % 4 == 0 ? 4 : tiles % 3 == 0 ? 3 : tiles % 2 == 0 ? 2 : 1;
+}
+
 template <int CI, int CO, int KS, int S>
 ds_status launch_fwd(const ConvShape &c, const __bf16 *in, const __bf16 *w, const float *bias,
                      void *out, bool relu, bool out_f32, hipStream_t s) {
     static_assert(CO % 16 == 0, "Cout must be a multiple of 16");
-    constexpr int TN = (CO / 16) > 4 ? 4 : (CO / 16);
+    constexpr int TN = tiles_per_wave(CO / 16);
     const dim3 grid((c.Ho * c.Wo + 63) / 64, CO / (16 * TN));
     if (relu && !out_f32)
         hipLaunchKernelGGL((k_conv_fwd<CI, CO, KS, S, TN, true, false>), grid, dim3(256), 0, s,
@@ -359,7 +502,7 @@ template <int CI, int CO, int KS, int S>
 ds_status launch_bwd_data(const ConvShape &c, const __bf16 *dout, const __bf16 *wt,
                           const __bf16 *y_in, __bf16 *din, hipStream_t s) {
     if constexpr (CI % 16 == 0) {
-        constexpr int TN = (CI / 16) > 4 ? 4 : (CI / 16);
+        constexpr int TN = tiles_per_wave(CI / 16);
         const dim3 grid((c.Hi * c.Wi + 63) / 64, CI / (16 * TN));
         hipLaunchKernelGGL((k_conv_bwd_data<CI, CO, KS, S, TN, true>), grid, dim3(256), 0, s,
                            dout, c.Ho, c.Wo, wt, c.kpad2, y_in, c.Hi, c.Wi, din);
@@ -386,18 +529,19 @@ ds_status launch_bwd_weight(const ConvShape &c, const __bf16 *dout, const __bf16
     return DS_OK;
 }
 
-// Calls F<CI, CO, KS, S>(args) for the layer shape of c (the five layers of
+// Calls F<CI, CO, KS, S>(args) for the layer shape of c (the six layers of
 // the initializer).
 template <template <int, int, int, int> class F, typename... A>
 ds_status dispatch_layer(const ConvShape &c, const char *what, A... args) {
 #define DS_LAYER(CI, CO, KS, S)                                                 \
     if (c.cin == CI && c.cout == CO && c.ks == KS && c.stride == S)             \
         return F<CI, CO, KS, S>::run(c, args...);
-    DS_LAYER(3, 32, 3, 1)
+    DS_LAYER(3, 32, 3, 2)
     DS_LAYER(32, 32, 3, 2)
     DS_LAYER(32, 64, 3, 2)
+    DS_LAYER(64, 64, 3, 2)
     DS_LAYER(64, 64, 3, 1)
-    DS_LAYER(64, kHeadChannels, 1, 1)
+    DS_LAYER(64, kHeadChannels, 3, 1)
 #undef DS_LAYER
     return set_error(DS_ERR_INVALID_ARG, std::string(what) + ": unsupported layer shape");
 }
@@ -476,13 +620,28 @@ ds_status head_bwd(const float *h, const HeadGeom &g, const float *grad_slab, fl
     return DS_OK;
 }
 
+ds_status head_targets(const float *slab, const HeadGeom &g, float *target, hipStream_t s) {
+    hipLaunchKernelGGL(k_head_targets, dim3(blocks_for((long)g.gw * g.gh, 1 << 20)), dim3(256), 0,
+                       s, slab, g, target);
+    DS_CHECK_LAUNCH("head_targets");
+    return DS_OK;
+}
+
+ds_status head_regression(const float *h, const float *target, const HeadGeom &g,
+                          const HeadWeights &w, __bf16 *dh, float *loss, hipStream_t s) {
+    hipLaunchKernelGGL(k_head_regression, dim3(blocks_for((long)g.gw * g.gh, 1 << 20)), dim3(256),
+                       0, s, h, target, g, w, dh, loss);
+    DS_CHECK_LAUNCH("head_regression");
+    return DS_OK;
+}
+
 }  // namespace ds
 
 // ====================================================================== net
 
 namespace {
 
-constexpr int kLayers = 5;
+constexpr int kLayers = 6;
 
 // Deterministic normal samples (splitmix64 + Box-Muller) for the weight init.
 struct InitRng {
@@ -507,6 +666,7 @@ int round32(int k) { return (k + 31) / 32 * 32; }
 struct ds_initnet {
     int W = 0, H = 0, gw = 0, gh = 0;
     ds::HeadGeom geom{};
+    ds::HeadWeights loss_w = ds::kDefaultHeadWeights;
     ds::ConvShape L[kLayers]{};
     size_t w_off[kLayers]{}, b_off[kLayers]{};  // into the parameter buffer
     size_t n_params = 0;                          // padded, fp32
@@ -549,10 +709,9 @@ ds_status net_forward(ds_initnet *n, const float *rgb, float *slab, hipStream_t 
     return DS_OK;
 }
 
-// Gradients of all weights and biases (into n->grads) from the slab gradient
-// of the last forward.
-ds_status net_backward(ds_initnet *n, const float *grad_slab, hipStream_t s) {
-    DS_TRY(head_bwd(n->h.ptr, n->geom, grad_slab, nullptr, bf(n->dh), s));
+// Gradients of all weights and biases (into n->grads) from the head-value
+// gradient n->dh of the last forward.
+ds_status net_backward_from_head(ds_initnet *n, hipStream_t s) {
     DS_TRY(fill_u32(n->grads.ptr, 0u, n->n_params, s));
     const __bf16 *dout = bf(n->dh);
     for (int l = kLayers - 1; l >= 0; --l) {
@@ -566,6 +725,12 @@ ds_status net_backward(ds_initnet *n, const float *grad_slab, hipStream_t s) {
         }
     }
     return DS_OK;
+}
+
+// The same from the slab gradient of the last forward.
+ds_status net_backward(ds_initnet *n, const float *grad_slab, hipStream_t s) {
+    DS_TRY(head_bwd(n->h.ptr, n->geom, grad_slab, nullptr, bf(n->dh), s));
+    return net_backward_from_head(n, s);
 }
 
 ds_status net_update(ds_initnet *n, float lr, hipStream_t s) {
@@ -610,19 +775,22 @@ extern "C" ds_status ds_initnet_create(int width, int height, float fovy, uint64
                                        ds_initnet **out) {
     clear_error();
     DS_REQUIRE(out, "NULL argument");
-    DS_REQUIRE(width >= 16 && height >= 16 && width % 4 == 0 && height % 4 == 0 &&
+    DS_REQUIRE(width >= 16 && height >= 16 && width % 16 == 0 && height % 16 == 0 &&
                    width <= 4096 && height <= 4096,
-               "initializer image size must be multiples of 4 in [16, 4096]");
+               "initializer image size must be multiples of 16 in [16, 4096]");
     DS_REQUIRE(fovy > 0.f && fovy < 3.1f, "fovy out of range");
     auto n = std::make_unique<ds_initnet>();
     n->W = width;
     n->H = height;
-    n->gw = width / 4;
+    n->gw = width / 4;  // splat anchors, 4 pixels apart
     n->gh = height / 4;
-    const int cin[kLayers] = {3, 32, 32, 64, 64};
-    const int cout[kLayers] = {32, 32, 64, 64, kHeadChannels};
-    const int ks[kLayers] = {3, 3, 3, 3, 1};
-    const int st[kLayers] = {1, 2, 2, 1, 1};
+    // six 3x3 convolutions: four of stride 2 down to one output cell per
+    // 16x16 pixels, then two at that resolution, the last with the head's
+    // values of the cell's 16 splats
+    const int cin[kLayers] = {3, 32, 32, 64, 64, 64};
+    const int cout[kLayers] = {32, 32, 64, 64, 64, kHeadChannels};
+    const int ks[kLayers] = {3, 3, 3, 3, 3, 3};
+    const int st[kLayers] = {2, 2, 2, 2, 1, 1};
     int hi = height, wi = width;
     size_t off = 0;
     for (int l = 0; l < kLayers; ++l) {
@@ -646,13 +814,14 @@ extern "C" ds_status ds_initnet_create(int width, int height, float fovy, uint64
         wi = c.Wo;
     }
     n->n_params = off;
-    DS_REQUIRE(hi == n->gh && wi == n->gw, "initializer layer sizes do not reach the cell grid");
+    DS_REQUIRE(hi * kCellSub == n->gh && wi * kCellSub == n->gw,
+               "initializer layer sizes do not reach the cell grid");
     const float th = std::tan(0.5f * fovy);
     const float depth = 4.f;
     const float px_world = 2.f * depth * th / (float)height;
-    n->geom = HeadGeom{width,      height, n->gw, n->gh, 4.f, 4.f, depth, th,
+    n->geom = HeadGeom{width, height, n->gw, n->gh, 4.f, 16.f, depth, 1.f, th,
                        (float)width / (float)height,
-                       std::log(2.f * px_world), 1.5f, -1.f, 1.f};
+                       std::log(2.f * px_world), 2.f, 2.f, -3.f, 1.f};
     // weights: He-normal for the ReLU layers, small for the head; biases 0
     std::vector<float> host(n->n_params, 0.f);
     InitRng rng{seed * 0x2545F4914F6CDD1Dull + 1};
@@ -681,8 +850,8 @@ extern "C" ds_status ds_initnet_create(int width, int height, float fovy, uint64
         }
     }
     tryall(n->x0.alloc((size_t)width * height * 3));
-    tryall(n->h.alloc((size_t)n->gw * n->gh * kHeadChannels));
-    tryall(n->dh.alloc((size_t)n->gw * n->gh * kHeadChannels));
+    tryall(n->h.alloc((size_t)n->gw * n->gh * kHeadValues));
+    tryall(n->dh.alloc((size_t)n->gw * n->gh * kHeadValues));
     if (stt != DS_OK) return stt;
     // the transposed copies' padding is never written: zero it once
     for (int l = 1; l < kLayers; ++l)
@@ -768,6 +937,39 @@ extern "C" ds_status ds_initnet_grads(ds_initnet *n, const float *dev_rgb,
                          hipMemcpyDeviceToHost));
     pack_export(n, dev, host_grads);
     return DS_OK;
+}
+
+extern "C" ds_status ds_initnet_head_targets(ds_initnet *n, const float *dev_slab,
+                                             float *dev_targets, ds_stream_t s) {
+    clear_error();
+    DS_REQUIRE(n && dev_slab && dev_targets, "NULL argument");
+    return head_targets(dev_slab, n->geom, dev_targets, as_stream(s));
+}
+
+extern "C" ds_status ds_initnet_set_loss_weights(ds_initnet *n, const float *weights) {
+    clear_error();
+    DS_REQUIRE(n, "NULL argument");
+    if (!weights) {
+        n->loss_w = kDefaultHeadWeights;
+        return DS_OK;
+    }
+    for (int k = 0; k < kHeadValues; ++k)
+        DS_REQUIRE(std::isfinite(weights[k]) && weights[k] >= 0.f,
+                   "loss weights must be finite and non-negative");
+    for (int k = 0; k < kHeadValues; ++k) n->loss_w.w[k] = weights[k];
+    return DS_OK;
+}
+
+extern "C" ds_status ds_initnet_train_step_supervised(ds_initnet *n, const float *dev_rgb,
+                                                      const float *dev_targets, float lr,
+                                                      float *dev_loss, ds_stream_t st) {
+    clear_error();
+    DS_REQUIRE(n && dev_rgb && dev_targets, "NULL argument");
+    hipStream_t s = as_stream(st);
+    DS_TRY(net_forward(n, dev_rgb, nullptr, s));
+    DS_TRY(head_regression(n->h.ptr, dev_targets, n->geom, n->loss_w, bf(n->dh), dev_loss, s));
+    DS_TRY(net_backward_from_head(n, s));
+    return net_update(n, lr, s);
 }
 
 extern "C" ds_status ds_initnet_train_step(ds_initnet *n, ds_fitter *f, const float *dev_rgb,

@@ -6,7 +6,10 @@
 
 namespace ds {
 
-constexpr int kHeadChannels = 16;  // per-cell head values (9 used, padded for the MFMA tile)
+// per splat: offset x, y, mean log-scale, anisotropy (2), rgb, opacity, depth
+constexpr int kHeadValues = 10;
+constexpr int kCellSub = 4;        // splats per cell side (16 per cell)
+constexpr int kHeadChannels = kHeadValues * kCellSub * kCellSub;  // per cell: 160
 constexpr float kSHC0 = 0.28209479177387814f;
 
 // One convolution layer: NHWC input Hi x Wi x cin -> Ho x Wo x cout, square
@@ -18,16 +21,27 @@ struct ConvShape {
     int kpad, kpad2;
 };
 
-// Geometry of the head: a gw x gh grid of cells of `cell` pixels over a W x H
-// image seen by the pinhole camera at the origin looking down -z. Each cell's
-// splat sits on the plane at `depth`, up to max_off pixels from the cell
-// centre; log-scales are log_scale0 +- max_log_scale (z: their mean +
-// log_scale_z); rotation is about the view axis; opacity logit = h + bias.
+// Geometry of the head: a gw x gh grid of splat anchors `cell` pixels apart
+// over a W x H image seen by the pinhole camera at the origin looking down
+// -z; the network's output cells hold kCellSub x kCellSub anchors each
+// (kHeadValues values per splat, kHeadChannels per cell). A splat's image
+// position is up to max_off pixels from its anchor, at a depth within
+// depth +- max_dz; its apparent log-scales (their size at `depth`) have
+// their mean within log_scale0 +- max_log_scale and differ from it by up to
+// max_aniso (z: the mean + log_scale_z, a flat splat); rotation is about the
+// view axis; opacity logit = h + bias. Splat (ax, ay) is slab element
+// ay * gw + ax.
 struct HeadGeom {
     int W, H, gw, gh;
-    float cell, max_off, depth, tan_half_fovy, aspect;
-    float log_scale0, max_log_scale, log_scale_z, logit_bias;
+    float cell, max_off, depth, max_dz, tan_half_fovy, aspect;
+    float log_scale0, max_log_scale, max_aniso, log_scale_z, logit_bias;
 };
+
+// Per head value, its weight in the supervised regression loss.
+struct HeadWeights {
+    float w[kHeadValues];
+};
+constexpr HeadWeights kDefaultHeadWeights{{1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f}};
 
 ds_status conv_fwd(const ConvShape &c, const __bf16 *in, const __bf16 *w, const float *bias,
                    void *out, bool relu, bool out_f32, hipStream_t s);
@@ -42,10 +56,19 @@ ds_status weights_to_bf16(const ConvShape &c, const float *w, __bf16 *wb, __bf16
                           hipStream_t s);
 ds_status image_to_bf16(const float *rgb, int n, __bf16 *out, hipStream_t s);
 ds_status to_bf16(const float *x, int n, __bf16 *out, hipStream_t s);
-// head values [gw*gh][kHeadChannels] -> SH-0 slab (14 x gw*gh floats).
+// head values [cells][kHeadChannels] -> SH-0 slab (14 x gw*gh floats).
 ds_status head_fwd(const float *h, const HeadGeom &g, float *slab, hipStream_t s);
 // slab gradients -> head-value gradients (fp32 copy optional, bf16 always).
 ds_status head_bwd(const float *h, const HeadGeom &g, const float *grad_slab, float *dh32,
                    __bf16 *dh, hipStream_t s);
+// The head values whose splats best match a fitted slab (image position,
+// depth, projected footprint, colour, opacity; each value through the
+// inverse of its head function, clamped inside the function's range): the
+// regression targets of supervised training, in the layout of h.
+ds_status head_targets(const float *slab, const HeadGeom &g, float *target, hipStream_t s);
+// dh = 2 w (h - target) / n over the head values (bf16), and
+// loss += sum w (h - target)^2 / n (n = splats, w per head value).
+ds_status head_regression(const float *h, const float *target, const HeadGeom &g,
+                          const HeadWeights &w, __bf16 *dh, float *loss, hipStream_t s);
 
 }  // namespace ds

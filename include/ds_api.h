@@ -541,17 +541,23 @@ ds_status ds_fitter_count(ds_fitter *f, int32_t *count);
 /* every fit from random splats, src/utils/splat_utils.h:10-51)           */
 /* ------------------------------------------------------------------ */
 
-/* A small CNN on bf16 MFMA (conv3x3 32 -> conv3x3/2 32 -> conv3x3/2 64 ->
- * conv3x3 64 -> conv1x1 16, ReLU between) maps a W x H target image to one
- * SH-0 splat per 4x4-pixel cell ((W/4)*(H/4) splats) for the pinhole camera
- * at the origin looking down -z with vertical field of view `fovy`
- * (radians): each splat sits on the plane z = -4 within 4 pixels of its
- * cell, with learned anisotropic scale, in-plane rotation, colour and
- * opacity. It is trained end to end through the differentiable renderer
- * (loss: the fitter's MSE of its own render against the image). */
+/* A CNN of six 3x3 convolutions on bf16 MFMA with fp32 accumulation
+ * (3 -> 32 /2, 32 -> 32 /2, 32 -> 64 /2, 64 -> 64 /2, 64 -> 64, 64 -> 160;
+ * ReLU between) maps a W x H target image to a grid of 16x16-pixel cells
+ * with 16 SH-0 splats each ((W/4)*(H/4) splats, one per 4x4-pixel anchor)
+ * for the pinhole camera at the origin looking down -z with vertical field
+ * of view `fovy` (radians): each splat is a flat ellipse facing the camera,
+ * its image position within 16 pixels of its anchor and its depth within
+ * 4 +- 1 (which orders the splats), with learned apparent anisotropic
+ * scale, in-plane rotation, colour and opacity (10 head values per splat:
+ * offset x, y, log-scale major, minor, angle, rgb, opacity, depth). It is
+ * trained on (image,
+ * fitted splats) pairs by regression of the head values
+ * (ds_initnet_train_step_supervised), or end to end through the
+ * differentiable renderer (ds_initnet_train_step). */
 typedef struct ds_initnet ds_initnet;
 
-/* W and H multiples of 4 in [16, 4096]; weights drawn from `seed`. */
+/* W and H multiples of 16 in [16, 4096]; weights drawn from `seed`. */
 ds_status ds_initnet_create(int width, int height, float fovy, uint64_t seed, ds_initnet **out);
 ds_status ds_initnet_destroy(ds_initnet *n);
 /* Number of weights and biases (the ds_initnet_get_params layout) and of
@@ -564,8 +570,30 @@ ds_status ds_initnet_set_params(ds_initnet *n, const float *host);
 /* Image (device, [H*W*3] in [0,1]) -> SH-0 parameter slab (device, the
  * ds_fitter_upload_params layout for num_splats splats). Asynchronous. */
 ds_status ds_initnet_infer(ds_initnet *n, const float *dev_rgb, float *dev_slab, ds_stream_t s);
-/* Head values [num_splats][16] of the last forward, host (synchronises). */
+/* Head values of the last forward, host (synchronises): [cells][160] with
+ * cells in row-major order and the 16 splats of a cell in row-major order
+ * inside it, 10 values each. */
 ds_status ds_initnet_head(ds_initnet *n, float *host_h);
+/* The regression targets of a fitted SH-0 slab (device, the
+ * ds_fitter_upload_params layout for num_splats splats): the head values
+ * (device, the ds_initnet_head layout) whose splats are closest to it, each
+ * value through the inverse of its head function (arguments of tanh and
+ * sigmoid clamped to 98% of their range): image position and depth; the
+ * fitted covariance projected along the view rays as the apparent
+ * log-scales (major axis first) and the major axis' angle; colour; opacity.
+ * Asynchronous. */
+ds_status ds_initnet_head_targets(ds_initnet *n, const float *dev_slab, float *dev_targets,
+                                  ds_stream_t s);
+/* Weights of the 10 head values in the supervised loss (NULL: the
+ * defaults). Non-negative and finite. */
+ds_status ds_initnet_set_loss_weights(ds_initnet *n, const float *weights);
+/* One supervised training step: forward, mean squared error of the head
+ * values against dev_targets (weighted per value; its mean over the splats
+ * added to *dev_loss unless NULL), backward,
+ * one Adam step (lr). Asynchronous on s. */
+ds_status ds_initnet_train_step_supervised(ds_initnet *n, const float *dev_rgb,
+                                           const float *dev_targets, float lr, float *dev_loss,
+                                           ds_stream_t s);
 /* Gradients of all weights and biases (ds_initnet_get_params layout, host)
  * for the image and a given gradient of the output slab (device).
  * Synchronises; leaves the weights unchanged. */

@@ -2,7 +2,8 @@
 tests/initnet_ref.py, which rounds to bf16 where the kernels store bf16 (so
 only the fp32 accumulation order differs, plus the rare activation that then
 rounds to the neighbouring bf16 value): head values, the splat slab, every
-weight and bias gradient, and a short training run through the renderer."""
+weight and bias gradient, the regression targets of a fitted slab, and short
+training runs, supervised on fitted splats and through the renderer."""
 from __future__ import annotations
 
 import numpy as np
@@ -23,7 +24,7 @@ def random_params(seed):
     return np.concatenate(parts).astype(np.float32)
 
 
-def setup(W, H, seed):
+def make_net(W, H, seed):
     net = ds.InitNet(W, H, seed=seed)
     p = random_params(seed)
     net.set_params(p)
@@ -33,25 +34,26 @@ def setup(W, H, seed):
 
 @pytest.mark.parametrize("W,H", [(64, 64), (96, 48)])
 def test_forward_matches_reference(W, H):
-    net, p, img, d_img = setup(W, H, 3)
+    net, p, img, d_img = make_net(W, H, 3)
     n = net.num_splats
     assert n == (W // 4) * (H // 4)
     d_slab = ds.DeviceArray(4 * 14 * n)
     net.infer(d_img, d_slab)
-    h = net.head().reshape(n, R.HEAD)
+    h = net.head().reshape(-1, R.SUB * R.SUB * R.HEAD)
     h_ref, _ = R.forward(p, img, W, H)
     err = float(np.abs(h - h_ref).max())
     scale = float(np.abs(h_ref).max())
     print(f"[parity] initnet_head_{W}x{H}: max_abs={err:.3e} scale={scale:.3e}")
     assert scale > 0.1 and err <= 1e-2 * scale
     slab = d_slab.download(np.float32, 14 * n)
-    ref = R.head_fwd(h.astype(np.float64), R.geometry(W, H))
+    g = R.geometry(W, H)
+    ref = R.head_fwd(R.per_splat(h.astype(np.float64), g), g)
     np.testing.assert_allclose(slab, ref, rtol=1e-4, atol=1e-5)
 
 
 @pytest.mark.parametrize("W,H", [(64, 64), (96, 48)])
 def test_gradients_match_reference(W, H):
-    net, p, img, d_img = setup(W, H, 5)
+    net, p, img, d_img = make_net(W, H, 5)
     n = net.num_splats
     gs = np.random.default_rng(9).normal(0, 1e-2, 14 * n).astype(np.float32)
     g = net.grads(d_img, ds.DeviceArray.from_numpy(gs))
@@ -66,6 +68,64 @@ def test_gradients_match_reference(W, H):
             assert rel <= 2e-2, (i, part, rel)
             o += size
     assert o == g.size
+
+
+def fitted_slab(W, H, seed, steps=60):
+    """A short plain fit from the initializer's neutral splats (all head
+    values 0) toward a synthetic target: a slab of fitted splats."""
+    n = (W // 4) * (H // 4)
+    net = ds.InitNet(W, H, seed=0)
+    net.set_params(np.zeros(net.num_params, np.float32))
+    img = ds.target_image(seed, W, H, 8)
+    d_img = ds.DeviceArray.from_numpy(img)
+    view, proj, cam = ds.fitter_camera(W, H)
+    fit = ds.ImageFitter(W, H, n, view=view, proj=proj, cam_pos=cam)
+    pslab, _ = fit.param_slabs()
+    net.infer(d_img, pslab, fit.stream())
+    fit.synchronize()
+    fit.reset_optimizer()
+    fit.set_target(img)
+    fit.step(steps)
+    slab = fit.params()
+    fit.close()
+    net.close()
+    return slab, img
+
+
+def test_head_targets_and_supervised_step():
+    W, H = 64, 48
+    slab, img = fitted_slab(W, H, 21)
+    net = ds.InitNet(W, H, seed=2)
+    n = net.num_splats
+    d_slab = ds.DeviceArray.from_numpy(slab)
+    d_t = ds.DeviceArray(4 * R.HEAD * n)
+    net.head_targets(d_slab, d_t)
+    ds.check(ds.lib().ds_synchronize(), "sync")
+    t = d_t.download(np.float32, R.HEAD * n).reshape(-1, R.HEAD)
+    g = R.geometry(W, H)
+    t_ref = R.per_cell(R.head_targets(slab, g), g).reshape(-1, R.HEAD)
+    err = np.abs(t - t_ref)
+    print(f"[parity] initnet_head_targets: max_abs={err.max():.3e}")
+    assert err.max() <= 1e-3 * max(1.0, float(np.abs(t_ref).max()))
+    # the first step's loss is the reference's regression loss at the
+    # starting weights; supervised steps on one pair then lower it
+    p0 = net.params()
+    h0, _ = R.forward(p0, img, W, H)
+    loss_ref, _ = R.regression(R.per_splat(h0, g), R.head_targets(slab, g))
+    d_img = ds.DeviceArray.from_numpy(img)
+    d_loss = ds.DeviceArray(4)
+    losses = []
+    for k in range(150):
+        d_loss.zero()
+        net.train_step_supervised(d_img, d_t, 2e-3, d_loss)
+        if k in (0, 149):
+            ds.check(ds.lib().ds_synchronize(), "sync")
+            losses.append(float(d_loss.download(np.float32, 1)[0]))
+    print(f"[parity] initnet_supervised: loss_first={losses[0]:.4f} (ref {loss_ref:.4f}) "
+          f"last={losses[1]:.4f}")
+    assert abs(losses[0] - loss_ref) <= 2e-2 * loss_ref
+    assert np.isfinite(losses[1]) and losses[1] < 0.5 * losses[0]
+    net.close()
 
 
 def test_training_through_the_renderer_reduces_loss():
@@ -93,5 +153,51 @@ def test_training_through_the_renderer_reduces_loss():
     after = mean_loss()
     print(f"[parity] initnet_train: loss_before={before:.5f} after={after:.5f}")
     assert np.isfinite(after) and after < 0.7 * before
+    fit.close()
+    net.close()
+
+
+def test_shipped_initializer_reaches_30db_faster():
+    """The trained weights in diffsplat_amd/data against random splats on 16
+    held-out seeds (images the training never saw): every fit reaches 30 dB
+    and the learned start takes at most half the time and half the steps."""
+    from diffsplat_amd import initnet_train as T
+    net = T.load(T.WEIGHTS)
+    plain = T.evaluate(None, 16)
+    learned = T.evaluate(net, 16)
+    net.close()
+    print(f"[parity] initnet_time_to_30db: plain {plain} learned {learned}")
+    assert plain["reached"] == learned["reached"] == 16
+    assert learned["initializer_kept"] >= 12
+    assert learned["mean_steps"] <= 0.5 * plain["mean_steps"]
+    assert learned["mean_seconds"] <= 0.5 * plain["mean_seconds"]
+
+
+def test_initializer_falls_back_to_random_splats():
+    """A net whose splats are opaque and black, on a white target: its render
+    loses to the random splats, so initialize() keeps the random ones."""
+    from diffsplat_amd import initnet_train as T
+    W = H = 64
+    net = ds.InitNet(W, H, seed=0)
+    p = np.zeros(net.num_params, np.float32)
+    cout = R.LAYERS[-1][1]
+    b0 = p.size - cout
+    for j in range(R.SUB * R.SUB):
+        p[b0 + j * R.HEAD + 5: b0 + j * R.HEAD + 8] = -6.0
+        p[b0 + j * R.HEAD + 8] = 6.0
+    net.set_params(p)
+    img = np.ones(W * H * 3, np.float32)
+    n = net.num_splats
+    view, proj, cam = ds.fitter_camera(W, H)
+    fit = ds.ImageFitter(W, H, n, view=view, proj=proj, cam_pos=cam)
+    fit.set_target(img)
+    rnd = ds.frustum_splats(5, n, aspect=1.0)
+    kept = T.initialize(fit, net, ds.DeviceArray.from_numpy(img), rnd)
+    assert not kept
+    np.testing.assert_array_equal(fit.params(), rnd)
+    # the same net on a black target wins against the random splats
+    black = np.zeros(W * H * 3, np.float32)
+    fit.set_target(black)
+    assert T.initialize(fit, net, ds.DeviceArray.from_numpy(black), rnd)
     fit.close()
     net.close()
