@@ -63,7 +63,14 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
     ap.add_argument("--graph", type=int, default=1, help="replay the step as a HIP graph")
+    ap.add_argument("--grad-mode", default="exact", choices=["exact", "reference"],
+                    help="compositing backward: exact derivative (DS_GRAD_EXACT) or the "
+                         "reference kernel's formula (DS_GRAD_REFERENCE)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-single-seconds", type=float, default=10.0,
+                    help="seconds of the one-thread CPU figure")
+    ap.add_argument("--psnr-every", type=int, default=250,
+                    help="timed steps between the PSNR curve's points (one host sync each)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-viewer", action="store_true")
@@ -130,7 +137,7 @@ def roofline_composite(ds, oracle, fit, target, W, H, N, reps=20):
     e_fwd = int(fw["E"])
     flops = e_fwd * FWD_FLOP_PER_EVAL + e_bwd * BWD_FLOP_PER_EVAL
     achieved = flops / avg_s / 1e12
-    return {"kernel": "k_composite_fused", "bound": "mfma", "unit": "TFLOP/s",
+    return {"kernel": "k_composite_fused", "bound": "valu", "unit": "TFLOP/s",
             "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
             "traffic": (FUSED_TRAFFIC_READ + FUSED_TRAFFIC_WRITE
@@ -140,8 +147,9 @@ def roofline_composite(ds, oracle, fit, target, W, H, N, reps=20):
             "avg_launch_ms": round(avg_s * 1e3, 4), "splat_evals_fwd": e_fwd,
             "splat_evals_bwd": int(e_bwd), "flop_per_eval_fwd": FWD_FLOP_PER_EVAL,
             "flop_per_eval_bwd": BWD_FLOP_PER_EVAL, "n_isects": int(st["n_isects"]),
-            "note": "fp32 vector (VALU) work priced against the 157.3 TFLOP/s fp32 peak "
-                    "(which counts packed FMA, not used here); the kernel is VALU-issue-bound"}
+            "note": "bound 'valu': the kernel's work is fp32 vector ALU (no matrix-core work), "
+                    "priced against the 157.3 TFLOP/s fp32 peak (which counts packed FMA, not "
+                    "used here); the kernel is VALU-issue-bound"}
 
 
 def learned_vs_plain(ds, n_eval=512, target_db=30.0):
@@ -274,10 +282,32 @@ def batched_fit(ds, count=64, steps=500, seq_steps=10, compare=True, graph=True)
     return out
 
 
-def viewer_fps(ds, n=1_000_000, W=1920, H=1080, n_cams=120):
+def event_ms(ds, stream, fn, reps):
+    """Average ms of fn() over reps calls, HIP events on `stream`."""
+    L = ds.lib()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    ds.check(L.ds_event_create(C.byref(e0)), "event")
+    ds.check(L.ds_event_create(C.byref(e1)), "event")
+    fn()
+    ds.check(L.ds_event_record(e0, stream), "record")
+    for _ in range(reps):
+        fn()
+    ds.check(L.ds_event_record(e1, stream), "record")
+    ds.check(L.ds_event_synchronize(e1), "sync")
+    ms = C.c_float()
+    ds.check(L.ds_event_elapsed_ms(e0, e1, C.byref(ms)), "elapsed")
+    L.ds_event_destroy(e0)
+    L.ds_event_destroy(e1)
+    return ms.value / reps
+
+
+def viewer_fps(ds, oracle=None, n=1_000_000, W=1920, H=1080, n_cams=120, reps=10):
     """Forward-only renderer on the clustered SH-3 scene over an orbit of
     n_cams cameras at radius 3 (each frame: activation+projection, tile
-    assign, sort, compositing)."""
+    assign, sort, compositing). Then, at the last camera: the compositing
+    alone (ds_raster_composite_fwd on the frame's sorted pairs) timed with
+    HIP events, its share of the frame, and (with the oracle) its algorithmic
+    work, 24 FLOP per splat evaluation, against the fp32 peak."""
     params = ds.cluster_scene(2000, n, 3, 64)
     r = ds.SplatRenderer(W, H, n, 3)
     r.upload_params(params)
@@ -291,29 +321,66 @@ def viewer_fps(ds, n=1_000_000, W=1920, H=1080, n_cams=120):
     r.synchronize()
     el = time.perf_counter() - t0
     st = r.raster_state()
+    L = ds.lib()
+    stream = r.stream()
+    frame_ms = event_ms(ds, stream, lambda: r.render(*cams[-1]), reps)
+    s2 = ds._lib.Splat2D()
+    ds.check(L.ds_renderer_views(r.h, C.byref(s2)), "ds_renderer_views")
+    ras = C.c_void_p()
+    ds.check(L.ds_renderer_get_raster(r.h, C.byref(ras)), "ds_renderer_get_raster")
+    d_rgb = ds.DeviceArray(4 * 3 * W * H)
+    comp_ms = event_ms(ds, stream, lambda: ds.check(
+        L.ds_raster_composite_fwd(ras, C.byref(s2), d_rgb.ptr, stream),
+        "ds_raster_composite_fwd"), reps)
+    out = {"workload": f"viewer {W}x{H}, {n} splats SH degree 3, {n_cams}-camera orbit",
+           "frames_per_s": round(n_cams / el, 2), "ms_per_frame": round(1e3 * el / n_cams, 3),
+           "n_isects_last_frame": int(st["n_isects"]), "compositing_tuning": tuning,
+           "last_camera": {"frame_ms": round(frame_ms, 4), "compositing_ms": round(comp_ms, 4),
+                           "compositing_share": round(comp_ms / frame_ms, 4),
+                           "project_assign_sort_share": round(1 - comp_ms / frame_ms, 4)}}
+    if oracle is not None:
+        s2_host = ds.download_ptr(s2.pos_x, np.float32, 10 * n)
+        fw = oracle.comp_fwd(s2_host, n, st["flatten_ids"], st["tile_ranges"], W, H,
+                             dtype=np.float32)
+        e = int(fw["E"])
+        tf = e * FWD_FLOP_PER_EVAL / (comp_ms * 1e-3) / 1e12
+        out["last_camera"]["compositing_roofline"] = {
+            "kernel": "k_composite_fwd", "bound": "valu", "unit": "TFLOP/s",
+            "achieved": round(tf, 3), "peak": PEAK_FP32_TFLOPS,
+            "frac": round(tf / PEAK_FP32_TFLOPS, 4), "splat_evals": e,
+            "flop_per_eval": FWD_FLOP_PER_EVAL}
     r.close()
-    return {"workload": f"viewer {W}x{H}, {n} splats SH degree 3, {n_cams}-camera orbit",
-            "frames_per_s": round(n_cams / el, 2), "ms_per_frame": round(1e3 * el / n_cams, 3),
-            "n_isects_last_frame": int(st["n_isects"]), "compositing_tuning": tuning}
+    return out
 
 
-def cpu_baseline(oracle, params, target, W, H, N, pv, cam, lr6, seconds):
-    threads = oracle.set_threads(0)
-    fit = oracle.Fit(params, target, N, W, H, 0, pv, cam, lr6)
-    for _ in range(2):
-        fit.step()
-    t0 = time.perf_counter()
-    k = 0
-    while True:
-        fit.step()
-        k += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or k >= 200:
-            break
-    fit.close()
+def cpu_baseline(oracle, params, target, W, H, N, pv, cam, lr6, seconds, single_seconds):
+    """The oracle's fp32 step on every host thread for `seconds`, then on one
+    thread for `single_seconds` (a fresh fit from the same start)."""
+    def run(threads, secs):
+        got = oracle.set_threads(threads)
+        fit = oracle.Fit(params, target, N, W, H, 0, pv, cam, lr6)
+        for _ in range(2):
+            fit.step()
+        t0 = time.perf_counter()
+        k = 0
+        while True:
+            fit.step()
+            k += 1
+            el = time.perf_counter() - t0
+            if el >= secs or k >= 200:
+                break
+        fit.close()
+        return got, k, el
+
+    threads, k, el = run(0, seconds)
+    _, k1, el1 = run(1, single_seconds)
+    oracle.set_threads(0)
     return {"value": round(k / el, 4), "unit": "steps/s", "cores": threads, "kind": "port",
             "sample": f"{k} fp32 steps of the same {W}x{H}/{N}-splat fit after 2 warm-up steps "
-                      f"(oracle/ds_oracle.cpp, OpenMP, {threads} threads), {el:.1f} s"}
+                      f"(oracle/ds_oracle.cpp, OpenMP, {threads} threads), {el:.1f} s",
+            "single_thread": {"value": round(k1 / el1, 4), "unit": "steps/s", "cores": 1,
+                              "sample": f"{k1} steps on one thread after 2 warm-up steps, "
+                                        f"{el1:.1f} s"}}
 
 
 def main():
@@ -344,7 +411,8 @@ def main():
     lr = {"lr_pos": 1e-2, "lr_sh_dc": 1e-2, "lr_scale": 5e-3, "lr_rot": 6e-4,
           "lr_opacity": 6e-3}
 
-    fit = ds.ImageFitter(W, H, N, view=view, proj=proj, cam_pos=cam, lr=lr)
+    grad_mode = {"exact": 0, "reference": 1}[args.grad_mode]
+    fit = ds.ImageFitter(W, H, N, view=view, proj=proj, cam_pos=cam, lr=lr, grad_mode=grad_mode)
     fit.set_target(target)
     fit.upload_params(params)
     if args.ppt == "auto":
@@ -359,8 +427,16 @@ def main():
     fit.step(args.warmup)
     fit.synchronize()
     print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+    # the timed steps in chunks of psnr_every: after each chunk the loss of its
+    # last step's forward is read (one host sync) for the PSNR curve
+    curve = []
+    done = 0
     t0 = time.perf_counter()
-    fit.step(args.steps)
+    while done < args.steps:
+        k = min(args.psnr_every, args.steps - done) if args.psnr_every > 0 else args.steps
+        fit.step(k)
+        done += k
+        curve.append([args.warmup + done, round(ds.psnr_from_loss(fit.loss()), 3)])
     fit.synchronize()
     el = time.perf_counter() - t0
     print(f"[bench] {args.steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
@@ -372,8 +448,10 @@ def main():
     cpu = None
     viewer = None
     ttp = None
+    if not args.no_roofline or not args.no_cpu or not args.no_viewer:
+        import oracle
     if not args.no_viewer:
-        viewer = viewer_fps(ds)
+        viewer = viewer_fps(ds, oracle if not args.no_roofline else None)
         print(f"[bench] viewer {viewer}", file=sys.stderr, flush=True)
     if not args.no_time_to_psnr:
         ttp = learned_vs_plain(ds, n_eval=args.eval_images)
@@ -383,8 +461,6 @@ def main():
         batched = batched_fit(ds, steps=args.batch_steps, compare=not args.no_batch_compare,
                               graph=bool(args.graph))
         print(f"[bench] batched {batched}", file=sys.stderr, flush=True)
-    if not args.no_roofline or not args.no_cpu:
-        import oracle
     if not args.no_roofline:
         roof = roofline_composite(ds, oracle, fit, target, W, H, N)
         print(f"[bench] roofline {roof}", file=sys.stderr, flush=True)
@@ -392,7 +468,8 @@ def main():
         pv = ds.mat_mul(proj, view)
         lr6 = np.array([lr["lr_pos"], lr["lr_scale"], lr["lr_rot"], lr["lr_sh_dc"], 6e-3,
                         lr["lr_opacity"]], np.float32)
-        cpu = cpu_baseline(oracle, params, target, W, H, N, pv, cam, lr6, args.cpu_seconds)
+        cpu = cpu_baseline(oracle, params, target, W, H, N, pv, cam, lr6, args.cpu_seconds,
+                           args.cpu_single_seconds)
         print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
 
     out = {
@@ -404,7 +481,8 @@ def main():
                                f"pinhole frustum, {n_ell}-ellipse synthetic target, "
                                "fwd+bwd+Adam per step",
                    "width": W, "height": H, "splats": N, "graph": bool(args.graph),
-                   "compositing_tuning": tuning,
+                   "compositing_tuning": tuning, "grad_mode": args.grad_mode,
+                   "psnr_curve": curve,
                    "loss_after": round(loss, 6),
                    "psnr_db_after": round(ds.psnr_from_loss(loss), 3),
                    "steps_trained": args.steps + args.warmup},

@@ -163,6 +163,7 @@ _SIGS = {
     "ds_raster_bwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, C.POINTER(Splat2D), _P]),
     "ds_raster_fwd_bwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, _P, _P, _P, C.POINTER(Splat2D),
                                     _P]),
+    "ds_raster_composite_fwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, _P]),
     "ds_raster_composite_fwd_bwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, _P, _P, _P,
                                               C.POINTER(Splat2D), _P]),
     "ds_splat2d_zero": (C.c_int, [C.POINTER(Splat2D), _P]),
@@ -273,6 +274,7 @@ _SIGS = {
     "ds_renderer_render": (C.c_int, [_P, _F, _F, _F]),
     "ds_renderer_autotune": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), _F]),
     "ds_renderer_download_image": (C.c_int, [_P, _F]),
+    "ds_renderer_views": (C.c_int, [_P, C.POINTER(Splat2D)]),
     "ds_renderer_get_raster": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
     "ds_renderer_get_stream": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
     "ds_renderer_synchronize": (C.c_int, [_P]),

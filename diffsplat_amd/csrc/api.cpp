@@ -1014,6 +1014,15 @@ ds_status ds_renderer_download_image(ds_renderer *r, float *host_rgb) {
     return ds::raster_take_error(r->impl.raster(), r->impl.getStream());
 }
 
+ds_status ds_renderer_views(ds_renderer *r, ds_splat2d *splat2d) {
+    clear_error();
+    DS_REQUIRE(r && splat2d, "NULL argument");
+    DS_GUARD_BEGIN
+    *splat2d = r->impl.splat2dView();
+    return DS_OK;
+    DS_GUARD_END
+}
+
 ds_status ds_renderer_get_raster(ds_renderer *r, ds_raster **out) {
     clear_error();
     DS_REQUIRE(r && out, "NULL argument");

@@ -238,6 +238,8 @@ public:
     int getMaxSHDegree() const { return sh_degree; }
     hipStream_t getStream() const { return stream; }
     ds_raster *raster() const { return ras_layer.raster(); }
+    // The projected splats of the last frame (the compositing's input).
+    ds_splat2d splat2dView() { return gp_layer.getOutput().view(); }
     int width() const { return w; }
     int height() const { return h; }
 

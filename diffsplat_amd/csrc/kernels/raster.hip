@@ -1268,6 +1268,18 @@ extern "C" ds_status ds_raster_composite_fwd_bwd(ds_raster *r, const ds_splat2d 
                                   target_rgb, loss_dev, grad_rgb, *grad_in, st);
 }
 
+extern "C" ds_status ds_raster_composite_fwd(ds_raster *r, const ds_splat2d *in, float *out_rgb,
+                                             ds_stream_t s) {
+    clear_error();
+    DS_REQUIRE(r && in && out_rgb, "NULL argument");
+    DS_REQUIRE(r->has_fwd, "ds_raster_composite_fwd called before a forward");
+    DS_REQUIRE(in->count == r->last_count, "splat count differs from the last forward");
+    CompositeArgs ca = composite_args(r);
+    if (!r->order_valid) ca.order = nullptr;
+    return launch_composite_fwd(ca, r->fwd_ppt, out_rgb, r->T.ptr, r->last.ptr, nullptr, nullptr,
+                                nullptr, as_stream(s));
+}
+
 extern "C" ds_status ds_raster_bwd(ds_raster *r, const ds_splat2d *in, const float *grad_rgb,
                                    const ds_splat2d *grad_in, ds_stream_t s) {
     clear_error();

@@ -292,6 +292,13 @@ ds_status ds_raster_composite_fwd_bwd(ds_raster *r, const ds_splat2d *in, float 
                                       const float *target_rgb, float *loss_dev, float *grad_rgb,
                                       const ds_splat2d *grad_in, ds_stream_t s);
 
+/* The compositing forward of ds_raster_fwd alone (no MSE), on the sorted pairs
+ * and tile ranges of the last forward: the splats must be those of that
+ * forward. Writes the image, transmittance and last ids. For timing the
+ * compositing apart from the sort. No counterpart in the reference. */
+ds_status ds_raster_composite_fwd(ds_raster *r, const ds_splat2d *in, float *out_rgb,
+                                  ds_stream_t s);
+
 /* TileRasterizeLayer::backward (src/layers/tile_rasterize_layer.cu:710-728).
  * Gradients are ACCUMULATED into grad_in (atomics), as in the reference: the
  * caller zeroes them first (ds_splat2d_zero). grad_in->pos_z is not written. */
@@ -712,6 +719,9 @@ ds_status ds_renderer_autotune(ds_renderer *r, int reps, int *fwd_pix_per_thread
                                float *ms_per_frame);
 ds_status ds_renderer_download_image(ds_renderer *r, float *host_rgb);
 ds_status ds_renderer_get_raster(ds_renderer *r, ds_raster **out);
+/* Device view of the projected splats of the last frame (the compositing's
+ * input), valid for the renderer's life. */
+ds_status ds_renderer_views(ds_renderer *r, ds_splat2d *splat2d);
 ds_status ds_renderer_get_stream(ds_renderer *r, ds_stream_t *s);
 /* This call and ds_renderer_download_image report a sticky look-back give-up
  * as ds_fitter_loss does. */
