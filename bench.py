@@ -82,6 +82,11 @@ def parse():
                     help="steps of every instance in the batched-fitting leg")
     ap.add_argument("--no-batch-compare", action="store_true",
                     help="batched leg: only ds_multifit (no per-instance stream/sequential runs)")
+    ap.add_argument("--save-params", default=None, metavar="FILE",
+                    help="write the parameter slab after the timed steps (.npy)")
+    ap.add_argument("--start-params", default=None, metavar="FILE",
+                    help="start the fit from a saved parameter slab (.npy) instead of random "
+                         "splats: a later fit state in a short run, for profiling")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (counter calibration)")
     ap.add_argument("--ppt", type=str, default=os.environ.get("DS_PPT", "auto"),
@@ -407,6 +412,9 @@ def main():
     W, H, N, n_ell, seed = CONFIGS[args.config]
     target = ds.target_image(seed, W, H, n_ell)
     params = ds.frustum_splats(seed, N, aspect=W / H)
+    if args.start_params:
+        params = np.load(args.start_params, allow_pickle=False).astype(np.float32)
+        assert params.size == 14 * N, "the saved slab is not this workload's"
     view, proj, cam = ds.fitter_camera(W, H)
     lr = {"lr_pos": 1e-2, "lr_sh_dc": 1e-2, "lr_scale": 5e-3, "lr_rot": 6e-4,
           "lr_opacity": 6e-3}
@@ -443,6 +451,8 @@ def main():
     fit.forward()
     loss = fit.loss()
     value = args.steps / el
+    if args.save_params:
+        np.save(args.save_params, fit.params(), allow_pickle=False)
 
     roof = None
     cpu = None
@@ -485,7 +495,8 @@ def main():
                    "psnr_curve": curve,
                    "loss_after": round(loss, 6),
                    "psnr_db_after": round(ds.psnr_from_loss(loss), 3),
-                   "steps_trained": args.steps + args.warmup},
+                   "steps_trained": args.steps + args.warmup,
+                   "start": "saved slab" if args.start_params else "random splats"},
         "roofline": roof, "cpu_baseline": cpu, "viewer": viewer, "time_to_30db": ttp,
         "batched": batched,
     }

@@ -164,6 +164,7 @@ _SIGS = {
     "ds_raster_fwd_bwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, _P, _P, _P, C.POINTER(Splat2D),
                                     _P]),
     "ds_raster_composite_fwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, _P]),
+    "ds_raster_release_backward": (C.c_int, [_P]),
     "ds_raster_composite_fwd_bwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, _P, _P, _P,
                                               C.POINTER(Splat2D), _P]),
     "ds_splat2d_zero": (C.c_int, [C.POINTER(Splat2D), _P]),

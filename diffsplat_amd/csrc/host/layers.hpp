@@ -170,6 +170,8 @@ public:
         h = height;
         output.allocate((size_t)width * height * 3);
     }
+    // For forward-only use: frees the backward's per-pair gradient records.
+    void releaseBackward() { check(ds_raster_release_backward(ras), "TileRasterizeLayer"); }
     void resize(int width, int height) {
         if (width == w && height == h) return;
         check(ds_raster_resize(ras, width, height), "TileRasterizeLayer::resize");

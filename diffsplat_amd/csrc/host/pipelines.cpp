@@ -997,6 +997,7 @@ void SplatRenderer::init(int width, int height, int count, int deg, int64_t isec
     gp_layer.setSHDegree(deg);
     gp_layer.allocate(count);
     ras_layer.allocate(width, height, count, isect_cap);
+    ras_layer.releaseBackward();  // frames only
     gp_layer.setInput(&gaussian_params);
     ras_layer.setInput(&gp_layer.getOutput());
     pipeline.clear();

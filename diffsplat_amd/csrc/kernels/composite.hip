@@ -345,12 +345,16 @@ __device__ __forceinline__ int composite_front_to_back(
 template <int GRAD_MODE, int P, bool BITS>
 __device__ __forceinline__ void composite_back_to_front(
     const CompositeArgs &a, int2 range, const TilePixels<P> &q, float4 *sA, float4 *sB,
-    float4 *sC, float (*sG)[kTilePix / P > kWave ? kTilePix / P : 1], int *s_last,
-    const uint64_t (*s_bits)[kSkipWords], int loaded, float (&T)[P], const int32_t (&lst)[P],
-    const float (&gr)[P], const float (&gg)[P], const float (&gb)[P], const ds_splat2d &grad) {
+    float4 *sC, float (*sG)[kTilePix / P > kWave ? kTilePix / P : 1], uint8_t *s_hit,
+    int *s_last, const uint64_t (*s_bits)[kSkipWords], int loaded, float (&T)[P],
+    const int32_t (&lst)[P], const float (&gr)[P], const float (&gg)[P], const float (&gb)[P]) {
     constexpr int NT = kTilePix / P;
     constexpr int NW = NT / kWave;
     const int tid = threadIdx.x, w = tid / kWave;
+    // the tile's row and column in its image (uniform), and the epoch tag
+    const uint32_t trow = (uint32_t)__builtin_amdgcn_readfirstlane(q.row[0] >> 4);
+    const uint32_t tcol = (uint32_t)__builtin_amdgcn_readfirstlane(q.j >> 4);
+    const uint32_t ep = *a.epoch;
     float acc[P];
     int my_last = -1;
 #pragma unroll
@@ -366,24 +370,20 @@ __device__ __forceinline__ void composite_back_to_front(
         if (tid == 0) *s_last = -1;
 #pragma unroll
         for (int f = 0; f < 9; ++f) sG[f][tid] = 0.f;
+        s_hit[tid] = 0;
         __syncthreads();
         if (__lane_id() == 0) atomicMax(s_last, wave_last);
         __syncthreads();
         block_last = __builtin_amdgcn_readfirstlane(*s_last);
     }
-    float *const dst[9] = {grad.pos_x,  grad.pos_y,   grad.cov_xx,  grad.cov_xy, grad.cov_yy,
-                           grad.color_r, grad.color_g, grad.color_b, grad.opacity};
-    // The field this lane adds after the transposed reduction, its array,
-    // and the factor that turns the wave sum into the gradient: the sums of
-    // the position and covariance fields are taken with m = -2 dL/d dist2
-    // (see below) and the record's scaled u, w, so position x, y get -1/k
+    // The field this lane holds after the transposed reduction, and the
+    // factor that turns the wave sum into the gradient: the sums of the
+    // position and covariance fields are taken with m = -2 dL/d dist2 (see
+    // below) and the record's scaled u, w, so position x, y get -1/k
     // (d dist2/d(x, y) = 2 (u, w) / k), xx and yy get 1/(2 k^2) and the
     // symmetric xy 1/k^2.
     const int lane = __lane_id();
     const int my_field = field_of_lane(lane);
-    float *my_dst = nullptr;
-#pragma unroll
-    for (int f = 0; f < 9; ++f) my_dst = my_field == f ? dst[f] : my_dst;
     constexpr float kInvK = 1.f / kFalloffScale;
     const float my_scale = (my_field == 0 || my_field == 1) ? -kInvK
                            : (my_field == 2 || my_field == 4) ? 0.5f * kInvK * kInvK
@@ -475,35 +475,111 @@ __device__ __forceinline__ void composite_back_to_front(
                     T[k] = Tb;
                 }
                 const float sum = my_scale * wave_sum9_transposed(v, lane);
-                if (my_field >= 0) {  // nine lanes of row 0, one field each
-                    if constexpr (NW == 1) {
-                        if (sum != 0.f) atomicAdd(my_dst + __float_as_uint(B.w), sum);
-                    } else {
-                        atomicAdd(&sG[my_field][t], sum);
-                    }
+                if constexpr (NW == 1) {
+                    // the wave's sums are the tile's: nine lanes of row 0
+                    // store the fields, lane 6 the epoch tag
+                    const uint32_t g = (uint32_t)__builtin_amdgcn_readfirstlane(__float_as_uint(B.w));
+                    const uint2 pb = a.pair_base[g];
+                    float *r = reinterpret_cast<float *>(a.pair_rec +
+                                                         3 * (size_t)(pb.x + trow * pb.y + tcol));
+                    if (my_field >= 0) r[my_field] = sum;
+                    if (lane == 6) r[9] = __uint_as_float(ep);
+                } else {
+                    if (my_field >= 0) atomicAdd(&sG[my_field][t], sum);
+                    if (lane == 0) s_hit[t] = 1;
                 }
             }
         }
         if constexpr (NW > 1) {
             __syncthreads();
-            if (tid < bs) {
+            if (tid < bs && s_hit[tid]) {  // the slot's record: one thread's
                 const uint32_t g = __float_as_uint(sB[tid].w);
+                const uint2 pb = a.pair_base[g];
+                float4 *r = a.pair_rec + 3 * (size_t)(pb.x + trow * pb.y + tcol);
+                r[0] = make_float4(sG[0][tid], sG[1][tid], sG[2][tid], sG[3][tid]);
+                r[1] = make_float4(sG[4][tid], sG[5][tid], sG[6][tid], sG[7][tid]);
+                r[2] = make_float4(sG[8][tid], __uint_as_float(ep), 0.f, 0.f);
 #pragma unroll
-                for (int f = 0; f < 9; ++f) {
-                    const float val = sG[f][tid];
-                    if (val != 0.f) {
-                        atomicAdd(dst[f] + g, val);
-                        sG[f][tid] = 0.f;
-                    }
-                }
+                for (int f = 0; f < 9; ++f) sG[f][tid] = 0.f;
+                s_hit[tid] = 0;
             }
         }
     }
 }
 
-// Writes a tile's image, transmittance and last index; with MSE also
-// dL/dC = 2 (C - target) / (W H) (returned in gr, gg, gb) and adds the
-// tile's share of the loss (mseLossKernel / mseGradKernel).
+// The slots' records of this epoch summed per splat and added to its
+// gradient. A splat's slots are contiguous (slot_splat holds each slot's
+// splat), so each wave takes 64 consecutive slots, one per lane, and adds
+// them with a segmented scan over the lanes (xor-free: shifts up by 1, 2, 4,
+// ... within the run of equal splats); the last lane of each run adds the
+// run's sum to the splat's gradient: a plain add when the splat's slots all
+// lie inside this wave (no other wave touches it), a float atomic for a
+// splat that continues into a neighbouring wave's slots (at most two per
+// wave).
+__global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ rec,
+                                                     const uint32_t *__restrict__ slot_splat,
+                                                     const uint32_t *nisect,
+                                                     const uint32_t *epoch, ds_splat2d grad) {
+    const uint32_t m = *nisect, ep = *epoch;
+    const int lane = __lane_id();
+    const uint32_t wave = (blockIdx.x * 256 + threadIdx.x) / kWave;
+    const uint32_t nwaves = gridDim.x * (256 / kWave);
+    float *const dst[9] = {grad.pos_x,  grad.pos_y,   grad.cov_xx,  grad.cov_xy, grad.cov_yy,
+                           grad.color_r, grad.color_g, grad.color_b, grad.opacity};
+    for (uint32_t base = wave * kWave; base < m; base += nwaves * kWave) {  // wave-uniform
+        const uint32_t p = base + lane;
+        const bool valid = p < m;
+        const uint32_t g = valid ? slot_splat[p] : 0xFFFFFFFFu;
+        // lane 0's splat, read while every lane is active (lane 0 always is)
+        const uint32_t g0 = (uint32_t)__builtin_amdgcn_readfirstlane(g);
+        float v[9];
+#pragma unroll
+        for (int f = 0; f < 9; ++f) v[f] = 0.f;
+        if (valid) {
+            const float4 r0 = rec[3 * (size_t)p], r1 = rec[3 * (size_t)p + 1],
+                         r2 = rec[3 * (size_t)p + 2];
+            if (__float_as_uint(r2.y) == ep) {
+                v[0] = r0.x;
+                v[1] = r0.y;
+                v[2] = r0.z;
+                v[3] = r0.w;
+                v[4] = r1.x;
+                v[5] = r1.y;
+                v[6] = r1.z;
+                v[7] = r1.w;
+                v[8] = r2.x;
+            }
+        }
+        // inclusive segmented scan: runs of equal g are contiguous, so lane
+        // - d holds the same splat exactly when every lane between does
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t gu = __shfl_up(g, d, kWave);
+            const bool same = lane >= d && gu == g;
+#pragma unroll
+            for (int f = 0; f < 9; ++f) {
+                const float u = __shfl_up(v[f], d, kWave);
+                v[f] += same ? u : 0.f;
+            }
+        }
+        const uint32_t gn = __shfl_down(g, 1, kWave);
+        const bool run_end = valid && (lane == kWave - 1 || gn != g);
+        if (!run_end) continue;
+        // does the run start in this wave, and end in it?
+        const bool starts_before = (g == g0) && base > 0 && slot_splat[base - 1] == g;
+        const bool ends_after = lane == kWave - 1 && p + 1 < m && slot_splat[p + 1] == g;
+        if (starts_before || ends_after) {
+#pragma unroll
+            for (int f = 0; f < 9; ++f)
+                if (v[f] != 0.f) atomicAdd(dst[f] + g, v[f]);
+        } else {
+#pragma unroll
+            for (int f = 0; f < 9; ++f)
+                if (v[f] != 0.f) dst[f][g] += v[f];
+        }
+    }
+}
+
 template <bool MSE, int P>
 __device__ __forceinline__ void composite_epilogue(
     const TileImage &im, const TilePixels<P> &q, const float (&T)[P], const float (&cr)[P],
@@ -575,12 +651,13 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
 template <int GRAD_MODE, int P, bool MULTI>
 __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
     CompositeArgs a, const float *__restrict__ Tfin, const int32_t *__restrict__ last,
-    const float *__restrict__ gout, ds_splat2d grad) {
+    const float *__restrict__ gout) {
     constexpr int NT = kTilePix / P;
     constexpr int NW = NT / kWave;
     constexpr int GW = NW > 1 ? NT : 1;
     __shared__ float4 sA[NT], sB[NT], sC[NT];
     __shared__ float sG[9][GW];
+    __shared__ uint8_t s_hit[GW];
     __shared__ int s_last;
     const int tile = tile_of_block(a);
     const int2 range = a.ranges[tile];
@@ -600,8 +677,8 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
         gg[k] = q.inside[k] ? gout[3 * pix + 1] : 0.f;
         gb[k] = q.inside[k] ? gout[3 * pix + 2] : 0.f;
     }
-    composite_back_to_front<GRAD_MODE, P, false>(a, range, q, sA, sB, sC, sG, &s_last, nullptr,
-                                                 -1, T, lst, gr, gg, gb, grad);
+    composite_back_to_front<GRAD_MODE, P, false>(a, range, q, sA, sB, sC, sG, s_hit, &s_last,
+                                                 nullptr, -1, T, lst, gr, gg, gb);
 }
 
 // Forward with the fused MSE, then the backward of the same tile in the same
@@ -611,12 +688,13 @@ template <int GRAD_MODE, int P, bool MULTI>
 __global__ __launch_bounds__(kTilePix / P) void k_composite_fused(
     CompositeArgs a, float *__restrict__ out_rgb, float *__restrict__ out_T,
     int32_t *__restrict__ out_last, const float *__restrict__ target, float *loss,
-    float *__restrict__ grad_rgb, float inv_npix_one, ds_splat2d grad) {
+    float *__restrict__ grad_rgb, float inv_npix_one) {
     constexpr int NT = kTilePix / P;
     constexpr int NW = NT / kWave;
     constexpr int GW = NW > 1 ? NT : 1;
     __shared__ float4 sA[NT], sB[NT], sC[NT];
     __shared__ float sG[9][GW];
+    __shared__ uint8_t s_hit[GW];
     __shared__ float s_red[NW];
     __shared__ int s_last;
     __shared__ uint64_t s_bits[NW][kSkipWords];
@@ -634,8 +712,8 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fused(
     int32_t lst[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) lst[k] = q.inside[k] ? cur[k] : -1;
-    composite_back_to_front<GRAD_MODE, P, true>(a, range, q, sA, sB, sC, sG, &s_last, s_bits,
-                                                loaded, T, lst, gr, gg, gb, grad);
+    composite_back_to_front<GRAD_MODE, P, true>(a, range, q, sA, sB, sC, sG, s_hit, &s_last,
+                                                s_bits, loaded, T, lst, gr, gg, gb);
 }
 
 }  // namespace
@@ -667,42 +745,41 @@ static void composite_fwd(const CompositeArgs &a, int P, float *out_rgb, float *
 
 template <int MODE, bool MULTI>
 static void composite_bwd(const CompositeArgs &a, int P, const float *T, const int32_t *last,
-                          const float *grad_rgb, const ds_splat2d &grad, hipStream_t s) {
+                          const float *grad_rgb, hipStream_t s) {
     DS_PIX_DISPATCH(P, hipLaunchKernelGGL((k_composite_bwd<MODE, PP, MULTI>), dim3(a.ntiles),
-                                          dim3(kTilePix / PP), 0, s, a, T, last, grad_rgb,
-                                          grad));
+                                          dim3(kTilePix / PP), 0, s, a, T, last, grad_rgb));
 }
 
 template <int MODE, bool MULTI>
 static void composite_fused(const CompositeArgs &a, int P, float *out_rgb, float *out_T,
                             int32_t *out_last, const float *target, float *loss, float *grad_rgb,
-                            const ds_splat2d &grad, hipStream_t s) {
+                            hipStream_t s) {
     const float inv_npix = 1.f / ((float)a.W * (float)a.H);
     DS_PIX_DISPATCH(P, hipLaunchKernelGGL((k_composite_fused<MODE, PP, MULTI>), dim3(a.ntiles),
                                           dim3(kTilePix / PP), 0, s, a, out_rgb, out_T, out_last,
-                                          target, loss, grad_rgb, inv_npix, grad));
+                                          target, loss, grad_rgb, inv_npix));
 }
 
 ds_status launch_composite_fused(const CompositeArgs &a, int pix_per_thread, int grad_mode,
                                  float *out_rgb, float *out_T, int32_t *out_last,
                                  const float *target, float *loss, float *grad_rgb,
-                                 const ds_splat2d &grad, hipStream_t s) {
+                                 hipStream_t s) {
     if (a.ntiles <= 0) return DS_OK;
     const int P = pix_per_thread;
     if (grad_mode == DS_GRAD_REFERENCE) {
         if (a.mi)
             composite_fused<DS_GRAD_REFERENCE, true>(a, P, out_rgb, out_T, out_last, target, loss,
-                                                     grad_rgb, grad, s);
+                                                     grad_rgb, s);
         else
             composite_fused<DS_GRAD_REFERENCE, false>(a, P, out_rgb, out_T, out_last, target,
-                                                      loss, grad_rgb, grad, s);
+                                                      loss, grad_rgb, s);
     } else {
         if (a.mi)
             composite_fused<DS_GRAD_EXACT, true>(a, P, out_rgb, out_T, out_last, target, loss,
-                                                 grad_rgb, grad, s);
+                                                 grad_rgb, s);
         else
             composite_fused<DS_GRAD_EXACT, false>(a, P, out_rgb, out_T, out_last, target, loss,
-                                                  grad_rgb, grad, s);
+                                                  grad_rgb, s);
     }
     DS_CHECK_LAUNCH("composite_fused");
     return DS_OK;
@@ -720,21 +797,32 @@ ds_status launch_composite_fwd(const CompositeArgs &a, int pix_per_thread, float
     return DS_OK;
 }
 
+ds_status launch_pair_reduce(const float4 *pair_rec, const uint32_t *slot_splat,
+                             const uint32_t *nisect, const uint32_t *epoch, int64_t slot_cap,
+                             const ds_splat2d &grad, hipStream_t s) {
+    if (slot_cap <= 0) return DS_OK;
+    const int blocks = (int)std::min<int64_t>((slot_cap + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_pair_reduce, dim3(blocks), dim3(256), 0, s, pair_rec, slot_splat, nisect,
+                       epoch, grad);
+    DS_CHECK_LAUNCH("pair_reduce");
+    return DS_OK;
+}
+
 ds_status launch_composite_bwd(const CompositeArgs &a, int pix_per_thread, int grad_mode,
                                const float *T, const int32_t *last, const float *grad_rgb,
-                               const ds_splat2d &grad, hipStream_t s) {
+                               hipStream_t s) {
     if (a.ntiles <= 0) return DS_OK;
     const int P = pix_per_thread;
     if (grad_mode == DS_GRAD_REFERENCE) {
         if (a.mi)
-            composite_bwd<DS_GRAD_REFERENCE, true>(a, P, T, last, grad_rgb, grad, s);
+            composite_bwd<DS_GRAD_REFERENCE, true>(a, P, T, last, grad_rgb, s);
         else
-            composite_bwd<DS_GRAD_REFERENCE, false>(a, P, T, last, grad_rgb, grad, s);
+            composite_bwd<DS_GRAD_REFERENCE, false>(a, P, T, last, grad_rgb, s);
     } else {
         if (a.mi)
-            composite_bwd<DS_GRAD_EXACT, true>(a, P, T, last, grad_rgb, grad, s);
+            composite_bwd<DS_GRAD_EXACT, true>(a, P, T, last, grad_rgb, s);
         else
-            composite_bwd<DS_GRAD_EXACT, false>(a, P, T, last, grad_rgb, grad, s);
+            composite_bwd<DS_GRAD_EXACT, false>(a, P, T, last, grad_rgb, s);
     }
     DS_CHECK_LAUNCH("composite_bwd");
     return DS_OK;

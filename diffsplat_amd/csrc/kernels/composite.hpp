@@ -33,6 +33,16 @@ struct MultiInst {
 // mi / tinst are NULL for one instance (W, H, ntx describe it); otherwise
 // tile t belongs to instance tinst[t] with the layout mi[tinst[t]].
 // Workgroup b composites tile order[b] (NULL: tile b).
+// The backward writes one gradient record per (tile, splat) pair that a pixel
+// of the tile reaches, instead of accumulating with atomics: 3 float4 at
+// pair_rec[3 p] = {g0..g3}, {g4..g7}, {g8, epoch bits, 0, 0}, the nine sums
+// of the splat's Splat2D gradient fields (pos_x, pos_y, cov_xx, cov_xy,
+// cov_yy, r, g, b, opacity), tagged with the forward's epoch (*epoch). p is
+// the pair's slot before the tile sort: the splat's pairs are the contiguous
+// slots of its tile rectangle in row-major order, so p = pair_base[g].x +
+// tile_row * pair_base[g].y + tile_col (the rectangle's width in .y, its
+// first slot minus the offset of its first tile in .x, modulo 2^32).
+// launch_pair_reduce then adds each splat's records into the gradient.
 struct CompositeArgs {
     const float4 *rec;
     const uint32_t *flat;
@@ -41,6 +51,9 @@ struct CompositeArgs {
     const MultiInst *mi;
     const uint16_t *tinst;
     const uint32_t *order;
+    float4 *pair_rec;
+    const uint2 *pair_base;
+    const uint32_t *epoch;
 };
 
 // pix_per_thread: 1, 2 or 4 pixels per thread (256, 128 or 64 threads per tile).
@@ -48,13 +61,20 @@ ds_status launch_composite_fwd(const CompositeArgs &a, int pix_per_thread, float
                                float *out_T, int32_t *out_last, const float *target,
                                float *loss, float *grad, hipStream_t s);
 // Forward with the fused MSE (target required) and the backward of each tile
-// in one launch; the backward accumulates into grad like launch_composite_bwd.
+// in one launch; the backward writes the gradient records like
+// launch_composite_bwd.
 ds_status launch_composite_fused(const CompositeArgs &a, int pix_per_thread, int grad_mode,
                                  float *out_rgb, float *out_T, int32_t *out_last,
                                  const float *target, float *loss, float *grad_rgb,
-                                 const ds_splat2d &grad, hipStream_t s);
+                                 hipStream_t s);
 ds_status launch_composite_bwd(const CompositeArgs &a, int pix_per_thread, int grad_mode,
                                const float *T, const int32_t *last, const float *grad_rgb,
-                               const ds_splat2d &grad, hipStream_t s);
+                               hipStream_t s);
+// Adds the gradient records of the current epoch into grad, per splat: slot p
+// (p < *nisect, the pairs kept) belongs to splat slot_splat[p], and a
+// splat's slots are contiguous. slot_cap bounds *nisect (grid size).
+ds_status launch_pair_reduce(const float4 *pair_rec, const uint32_t *slot_splat,
+                             const uint32_t *nisect, const uint32_t *epoch, int64_t slot_cap,
+                             const ds_splat2d &grad, hipStream_t s);
 
 }  // namespace ds

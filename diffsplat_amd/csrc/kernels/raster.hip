@@ -25,7 +25,11 @@
 //    bucketed by powers of two) so that the grid ends evenly. The MSE loss
 //    and its gradient can be fused into the forward epilogue.
 //  * Backward: wave64 reductions, then LDS float atomics across the four
-//    waves, then one global atomic per (splat, field, tile).
+//    waves, then one 48-byte gradient record per (tile, splat) pair that the
+//    tile's pixels reach, stored at the pair's slot before the tile sort
+//    (the splat's slots are contiguous) with the forward's epoch as its tag;
+//    no global atomics. A per-splat pass then adds each splat's records of
+//    this epoch into dL/dSplat2D.
 #include <float.h>
 
 #include <algorithm>
@@ -309,7 +313,9 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t *__restrict__ order
                                               uint32_t *__restrict__ tkey,
                                               uint32_t *__restrict__ tval,
                                               const MultiInst *__restrict__ mi,
-                                              const uint16_t *__restrict__ sinst) {
+                                              const uint16_t *__restrict__ sinst,
+                                              uint2 *__restrict__ pair_base,
+                                              uint32_t *__restrict__ slot_splat) {
     const uint32_t n = sc[SC_COUNT];
     const uint32_t m = sc[SC_NISECT];  // pairs kept: the total clamped at the cap
     if (gtid() == 0) {
@@ -338,6 +344,9 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t *__restrict__ order
         const uint32_t tx0 = r.x & 0xFFFF, tx1 = r.x >> 16, ty0 = r.y & 0xFFFF;
         const uint32_t w = tx1 - tx0 + 1;
         const uint32_t ty = ty0 + j / w, tx = tx0 + j % w;
+        // the splat's first slot: where its backward records go (see
+        // CompositeArgs)
+        if (j == 0 && pair_base) pair_base[g] = make_uint2((uint32_t)p - (ty0 * w + tx0), w);
         if constexpr (MULTI) {
             const MultiInst I = mi[sinst[g]];
             tkey[p] = (uint32_t)I.tile_base + ty * (uint32_t)I.ntx + tx;
@@ -345,6 +354,7 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t *__restrict__ order
             tkey[p] = ty * (uint32_t)ntx + tx;
         }
         tval[p] = g;
+        if (slot_splat) slot_splat[p] = g;  // kept: the tile passes reuse tval
     }
 }
 
@@ -407,7 +417,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_emit(
     const uint32_t *__restrict__ chunk_first, const uint2 *__restrict__ rect, uint32_t *sc,
     const uint32_t *__restrict__ ghist, int ntx, int ntiles, int tile_bits,
     uint32_t *__restrict__ pk, uint32_t *__restrict__ val, uint32_t *__restrict__ cnt,
-    uint32_t *seg) {
+    uint32_t *seg, uint2 *__restrict__ pair_base) {
     __shared__ uint16_t s_w[kTileWaves][kTileBinsMax];
     __shared__ uint16_t s_head[kTileChunk];
     __shared__ uint16_t s_first[kTileChunk];
@@ -463,6 +473,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_emit(
                 const uint32_t tx0 = r.x & 0xFFFF, tx1 = r.x >> 16, ty0 = r.y & 0xFFFF;
                 s_gid[k] = g[i];
                 s_box[k] = tx0 | (ty0 << 10) | ((tx1 - tx0) << 20);
+                // a splat whose first slot is in this chunk: where its
+                // backward records go (see CompositeArgs)
+                if (pair_base && o[i] >= cs && o[i] < cs + cc) {
+                    const uint32_t wd = tx1 - tx0 + 1;
+                    pair_base[g[i]] = make_uint2(o[i] - (ty0 * wd + tx0), wd);
+                }
                 if (k == 0) {
                     s_first0 = o[i];
                 } else if (o[i] > cs && o[i] < cs + cc) {
@@ -768,6 +784,12 @@ struct ds_raster {
     // per pair
     ds::DevBuf<uint32_t> tkey_a, tkey_b, tval_a, tval_b;
     ds::DevBuf<uint64_t> keys64;
+    // backward: per pair slot its gradient record (3 float4), per splat its
+    // record base and tile-rectangle width (see CompositeArgs); released for
+    // forward-only use
+    ds::DevBuf<float4> pair_rec;
+    ds::DevBuf<uint2> pair_base;
+    ds::DevBuf<uint32_t> pair_splat;  // radix tile sort: each slot's splat
     // one-pass tile sort (up to kTileBinsMax tiles): chunk x tile counts,
     // segment x tile sums, tile totals, the splat owning each chunk's first slot
     ds::DevBuf<uint32_t> tcnt, tseg, ttot, tfirst;
@@ -784,6 +806,11 @@ struct ds_raster {
     // results of the last forward
     bool order_valid = false;  // `order` was computed by the last forward
     const uint32_t *flat = nullptr;
+    // each pair slot's splat before the tile sort (slots of a splat are
+    // contiguous): the emitted values, or pair_splat where the tile sort
+    // overwrites them
+    const uint32_t *slot_splat = nullptr;
+    bool base_valid = false;  // the last forward wrote pair_base
     int last_count = -1;
     bool has_fwd = false;
 };
@@ -841,7 +868,27 @@ static CompositeArgs composite_args(const ds_raster *r) {
                          multi ? (const MultiInst *)r->mtab.ptr : nullptr,
                          multi ? (const uint16_t *)r->tinst.ptr : nullptr,
                          r->tile_order == DS_TILE_ORDER_HEAVY_FIRST ? (const uint32_t *)r->order.ptr
-                                                                    : nullptr};
+                                                                    : nullptr,
+                         r->pair_rec.ptr, (const uint2 *)r->pair_base.ptr,
+                         (const uint32_t *)r->scalars.ptr + SC_EPOCH};
+}
+
+// The backward needs the gradient records and the pair bases of the last
+// forward.
+static ds_status require_backward(const ds_raster *r) {
+    DS_REQUIRE(r->pair_rec.ptr && r->pair_base.ptr,
+               "the rasterizer's backward buffers were released (ds_raster_release_backward)");
+    DS_REQUIRE(r->base_valid || r->last_count == 0,
+               "the last forward ran without the backward buffers");
+    return DS_OK;
+}
+
+// Each splat's records of this epoch into grad (after a backward compositing).
+static ds_status reduce_pairs(ds_raster *r, const ds_splat2d &grad, hipStream_t s) {
+    if (r->last_count <= 0 || !r->slot_splat) return DS_OK;
+    uint32_t *sc = r->scalars.ptr;
+    return launch_pair_reduce(r->pair_rec.ptr, r->slot_splat, sc + SC_NISECT, sc + SC_EPOCH,
+                              (int64_t)r->isect_cap, grad, s);
 }
 
 // Tile assignment, sorts, ranges and dispatch order, then compositing: the
@@ -974,7 +1021,8 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
                                (const uint32_t *)r->pair_off.ptr,
                                (const uint32_t *)r->tfirst.ptr, (const uint2 *)r->rect.ptr, sc,
                                (const uint32_t *)gh, r->ntx, r->ntiles, r->tile_bits,
-                               r->tkey_a.ptr, r->tval_a.ptr, r->tcnt.ptr, r->tseg.ptr);
+                               r->tkey_a.ptr, r->tval_a.ptr, r->tcnt.ptr, r->tseg.ptr,
+                               r->pair_base.ptr);
             DS_CHECK_LAUNCH("tile_emit");
             const int tblocks = (r->ntiles + 255) / 256;
             hipLaunchKernelGGL(k_tile_segscan, dim3(tblocks), dim3(256), 0, s, r->tseg.ptr,
@@ -992,6 +1040,7 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
                                r->tval_b.ptr, r->keep_keys ? r->keys64.ptr : (uint64_t *)nullptr);
             DS_CHECK_LAUNCH("tile_scatter");
             r->flat = r->tval_b.ptr;
+            r->slot_splat = r->tval_a.ptr;  // the emitted values stay
         } else {
             const dim3 egrid(grid_cap((uint64_t)r->isect_cap, 256, 4096));
             if (multi)
@@ -999,14 +1048,15 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
                                    (const uint32_t *)r->pair_off.ptr,
                                    (const uint32_t *)r->tfirst.ptr, (const uint2 *)r->rect.ptr,
                                    sc, (const uint32_t *)gh, r->ntx, r->tkey_a.ptr,
-                                   r->tval_a.ptr, mi, si);
+                                   r->tval_a.ptr, mi, si, r->pair_base.ptr, r->pair_splat.ptr);
             else
                 hipLaunchKernelGGL(k_emit<false>, egrid, dim3(256), 0, s, order,
                                    (const uint32_t *)r->pair_off.ptr,
                                    (const uint32_t *)r->tfirst.ptr, (const uint2 *)r->rect.ptr,
                                    sc, (const uint32_t *)gh, r->ntx, r->tkey_a.ptr,
-                                   r->tval_a.ptr, mi, si);
+                                   r->tval_a.ptr, mi, si, r->pair_base.ptr, r->pair_splat.ptr);
             DS_CHECK_LAUNCH("emit");
+            r->slot_splat = r->pair_splat.ptr;
             // Stable sort of the pairs on the tile bits. With hundreds of
             // chunks the reduce-then-scan passes beat one-kernel passes,
             // whose look-back traffic grows with the chunk count.
@@ -1034,18 +1084,21 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
         DS_CHECK_LAUNCH("tile_order");
     }
     r->order_valid = r->tile_order == DS_TILE_ORDER_HEAVY_FIRST;
+    r->last_count = n;
+    r->has_fwd = true;
+    r->base_valid = r->pair_base.ptr != nullptr;
     const CompositeArgs ca = composite_args(r);
+    if (fused_grad) DS_TRY(require_backward(r));
     if (fused_grad && target && r->fwd_ppt == r->bwd_ppt)
         DS_TRY(launch_composite_fused(ca, r->fwd_ppt, r->grad_mode, out_rgb, r->T.ptr,
-                                      r->last.ptr, target, loss, grad, *fused_grad, s));
+                                      r->last.ptr, target, loss, grad, s));
     else
         DS_TRY(launch_composite_fwd(ca, r->fwd_ppt, out_rgb, r->T.ptr, r->last.ptr, target,
                                     loss, grad, s));
-    r->last_count = n;
-    r->has_fwd = true;
     if (fused_grad && !(target && r->fwd_ppt == r->bwd_ppt))
         DS_TRY(launch_composite_bwd(ca, r->bwd_ppt, r->grad_mode, r->T.ptr, r->last.ptr,
-                                    grad, *fused_grad, s));
+                                    grad, s));
+    if (fused_grad) DS_TRY(reduce_pairs(r, *fused_grad, s));
     return DS_OK;
 }
 
@@ -1073,11 +1126,12 @@ static ds_status raster_backward(ds_raster *r, const ds_splat2d &in, const float
     DS_REQUIRE(r->has_fwd, "ds_raster_bwd called before ds_raster_fwd");
     DS_REQUIRE(in.count == r->last_count, "splat count differs from the last forward");
     DS_REQUIRE(grad_rgb != nullptr, "grad_rgb is NULL");
+    DS_TRY(require_backward(r));
     CompositeArgs ca = composite_args(r);
     if (!r->order_valid) ca.order = nullptr;  // order switched on after the forward
     DS_TRY(launch_composite_bwd(ca, r->bwd_ppt, r->grad_mode, r->T.ptr, r->last.ptr,
-                                grad_rgb, gin, s));
-    return DS_OK;
+                                grad_rgb, s));
+    return reduce_pairs(r, gin, s);
 }
 
 }  // namespace ds
@@ -1104,6 +1158,11 @@ static ds_status raster_finish_create(ds_raster *r, ds_status st, ds_raster **ou
     if (st == DS_OK) st = r->tval_a.alloc(cap);
     if (st == DS_OK) st = r->tval_b.alloc(cap);
     if (st == DS_OK && r->keep_keys) st = r->keys64.alloc(cap);
+    // backward records start with tag 0: epoch 0 is never used
+    if (st == DS_OK) st = r->pair_rec.alloc(3 * (size_t)cap);
+    if (st == DS_OK) st = fill_u32((uint32_t *)r->pair_rec.ptr, 0u, 12 * (size_t)cap, nullptr);
+    if (st == DS_OK) st = r->pair_base.alloc(nc);
+    if (st == DS_OK) st = r->pair_splat.alloc(cap);
     // look-back status words start at zero: epoch 0 is never used
     // four depth passes and the last pass's pair-offset sums
     const size_t lbd = 5 * lookback_words(nc, kRadix);
@@ -1260,12 +1319,25 @@ extern "C" ds_status ds_raster_composite_fwd_bwd(ds_raster *r, const ds_splat2d 
     DS_REQUIRE(in->count == r->last_count, "splat count differs from the last forward");
     DS_REQUIRE(r->fwd_ppt == r->bwd_ppt,
                "the fused compositing needs equal forward and backward launch shapes");
+    DS_TRY(require_backward(r));
     CompositeArgs ca = composite_args(r);
     if (!r->order_valid) ca.order = nullptr;
     hipStream_t st = as_stream(s);
     DS_TRY(fill_u32(loss_dev, 0u, r->ninst > 0 ? (size_t)r->ninst : 1, st));
-    return launch_composite_fused(ca, r->fwd_ppt, r->grad_mode, out_rgb, r->T.ptr, r->last.ptr,
-                                  target_rgb, loss_dev, grad_rgb, *grad_in, st);
+    DS_TRY(launch_composite_fused(ca, r->fwd_ppt, r->grad_mode, out_rgb, r->T.ptr, r->last.ptr,
+                                  target_rgb, loss_dev, grad_rgb, st));
+    return reduce_pairs(r, *grad_in, st);
+}
+
+extern "C" ds_status ds_raster_release_backward(ds_raster *r) {
+    clear_error();
+    DS_REQUIRE(r, "NULL argument");
+    DS_TRY_HIP(hipDeviceSynchronize());
+    r->pair_rec.release();
+    r->pair_base.release();
+    r->pair_splat.release();
+    r->base_valid = false;
+    return DS_OK;
 }
 
 extern "C" ds_status ds_raster_composite_fwd(ds_raster *r, const ds_splat2d *in, float *out_rgb,

@@ -292,6 +292,11 @@ ds_status ds_raster_composite_fwd_bwd(ds_raster *r, const ds_splat2d *in, float 
                                       const float *target_rgb, float *loss_dev, float *grad_rgb,
                                       const ds_splat2d *grad_in, ds_stream_t s);
 
+/* Frees the backward's buffers (a 48-byte gradient record per pair slot of
+ * isect_cap, 8 bytes per splat) of a rasterizer used for forward passes only;
+ * every backward entry point then fails. No counterpart in the reference. */
+ds_status ds_raster_release_backward(ds_raster *r);
+
 /* The compositing forward of ds_raster_fwd alone (no MSE), on the sorted pairs
  * and tile ranges of the last forward: the splats must be those of that
  * forward. Writes the image, transmittance and last ids. For timing the
@@ -300,8 +305,11 @@ ds_status ds_raster_composite_fwd(ds_raster *r, const ds_splat2d *in, float *out
                                   ds_stream_t s);
 
 /* TileRasterizeLayer::backward (src/layers/tile_rasterize_layer.cu:710-728).
- * Gradients are ACCUMULATED into grad_in (atomics), as in the reference: the
- * caller zeroes them first (ds_splat2d_zero). grad_in->pos_z is not written. */
+ * Gradients are ACCUMULATED into grad_in, as in the reference: the caller
+ * zeroes them first (ds_splat2d_zero). grad_in->pos_z is not written. The
+ * reference adds per (tile, splat) with float atomics; here each (tile, splat)
+ * pair that the tile's pixels reach gets one gradient record, and a per-splat
+ * pass adds a splat's records in slot order (no atomics on grad_in). */
 ds_status ds_raster_bwd(ds_raster *r, const ds_splat2d *in, const float *grad_rgb,
                         const ds_splat2d *grad_in, ds_stream_t s);
 
