@@ -39,15 +39,21 @@ PEAK_HBM_GBS = 8000.0
 # backward (falloff, alpha, the colour / alpha / conic gradients).
 FWD_FLOP_PER_EVAL = 24.0
 BWD_FLOP_PER_EVAL = 60.0
-# HBM bytes per launch of the fused compositing kernel (one pixel per
-# thread), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over 30 steps of
-# this workload, scaled by a 256 MiB copy (fetch x2.000, write x1.000): see
-# profiles/fit_720p_60k_counters.md. Counters cannot be read live here; most
-# write bytes are the backward's device-scope float atomics.
-FUSED_TRAFFIC_READ = 57.66e6
-FUSED_TRAFFIC_WRITE = 178.29e6
-FUSED_TRAFFIC_STATE = ("fit step 32: the 20 roofline launches of a 30-step counter run "
-                       "(longer counter runs crash inside the profiler on the host)")
+# HBM bytes of one compositing forward + backward (the roofline's kernels,
+# k_pair_reduce included) at the final C1 state, per launch shape
+# "fwd,bwd" (pixels per thread), from rocprofv3 FETCH_SIZE and WRITE_SIZE
+# passes, each over a short eager run started from the slab a 2020-step fit
+# saved (--save-params / --start-params), averaged over the 20 roofline
+# repetitions at its end and scaled by a 256 MiB copy (fetch x2.000, write
+# x1.000): see profiles/fit_720p_60k_counters.md. Counters cannot be read
+# live here.
+COMPOSITE_TRAFFIC = {  # shape: (read bytes, write bytes)
+    "1,1": (118.67e6, 66.31e6),
+    "2,2": (129.86e6, 66.37e6),
+    "1,2": (143.22e6, 55.23e6),
+}
+COMPOSITE_TRAFFIC_STATE = ("final C1 state (650k pairs): a 30-step eager counter run started "
+                           "from the slab of a 2020-step fit")
 
 CONFIGS = {
     # name: (W, H, N, n_ellipses, seed)
@@ -95,12 +101,16 @@ def parse():
 
 
 def roofline_composite(ds, oracle, fit, target, W, H, N, reps=20):
-    """The dominant kernel of the step, k_composite_fused (each tile's
-    forward with the fused MSE, then its backward), timed with HIP events on
-    the fitter's stream over `reps` launches on the state reached after the
-    timed steps (ds_raster_composite_fwd_bwd re-runs it on the last forward's
-    sorted pairs). Algorithmic work counted by the oracle on the same state:
-    24 FLOP per forward splat evaluation, 60 per backward evaluation."""
+    """The dominant work of the step, the compositing forward + backward,
+    timed with HIP events on the fitter's stream over `reps` repetitions on
+    the state reached after the timed steps, as the fitter runs it: with
+    equal forward and backward launch shapes one k_composite_fused per tile
+    (ds_raster_composite_fwd_bwd re-runs it on the last forward's sorted
+    pairs), otherwise k_composite_fwd then k_composite_bwd
+    (ds_raster_composite_fwd, ds_raster_bwd); both end with k_pair_reduce,
+    which adds each splat's per-tile gradient records. Algorithmic work
+    counted by the oracle on the same state: 24 FLOP per forward splat
+    evaluation, 60 per backward evaluation."""
     L = ds.lib()
     g = ds._lib.Gauss3D()
     s2 = ds._lib.Splat2D()
@@ -113,24 +123,28 @@ def roofline_composite(ds, oracle, fit, target, W, H, N, reps=20):
     stream = fit.stream()
     fit.forward()
     fit.synchronize()
+    fp, bp = C.c_int(), C.c_int()
+    ds.check(L.ds_raster_get_tuning(ras, C.byref(fp), C.byref(bp)), "ds_raster_get_tuning")
+    fused = fp.value == bp.value
+    shape = f"{fp.value},{bp.value}"
+    traffic = COMPOSITE_TRAFFIC.get(shape)
     d_target = ds.DeviceArray.from_numpy(np.ascontiguousarray(target, np.float32))
     d_rgb = ds.DeviceArray(4 * 3 * W * H)
     d_loss = ds.DeviceArray(4)
-    args = (ras, C.byref(s2), d_rgb.ptr, d_target.ptr, d_loss.ptr, grad_rgb, C.byref(gs2), stream)
-    e0, e1 = C.c_void_p(), C.c_void_p()
-    ds.check(L.ds_event_create(C.byref(e0)), "event")
-    ds.check(L.ds_event_create(C.byref(e1)), "event")
-    ds.check(L.ds_raster_composite_fwd_bwd(*args), "warm")
-    ds.check(L.ds_event_record(e0, stream), "record")
-    for _ in range(reps):
-        ds.check(L.ds_raster_composite_fwd_bwd(*args), "ds_raster_composite_fwd_bwd")
-    ds.check(L.ds_event_record(e1, stream), "record")
-    ds.check(L.ds_event_synchronize(e1), "sync")
-    ms = C.c_float()
-    ds.check(L.ds_event_elapsed_ms(e0, e1, C.byref(ms)), "elapsed")
-    avg_s = ms.value / reps * 1e-3
-    L.ds_event_destroy(e0)
-    L.ds_event_destroy(e1)
+    if fused:
+        args = (ras, C.byref(s2), d_rgb.ptr, d_target.ptr, d_loss.ptr, grad_rgb, C.byref(gs2),
+                stream)
+        run = lambda: ds.check(L.ds_raster_composite_fwd_bwd(*args),  # noqa: E731
+                               "ds_raster_composite_fwd_bwd")
+        kernels = "k_composite_fused + k_pair_reduce"
+    else:
+        def run():
+            ds.check(L.ds_raster_composite_fwd(ras, C.byref(s2), d_rgb.ptr, stream),
+                     "ds_raster_composite_fwd")
+            ds.check(L.ds_raster_bwd(ras, C.byref(s2), grad_rgb, C.byref(gs2), stream),
+                     "ds_raster_bwd")
+        kernels = "k_composite_fwd + k_composite_bwd + k_pair_reduce"
+    avg_s = event_ms(ds, stream, run, reps) * 1e-3
     # algorithmic work on the same state, counted by the oracle
     st = fit.raster_state()
     s2_host = ds.download_ptr(s2.pos_x, np.float32, 10 * N)
@@ -142,13 +156,14 @@ def roofline_composite(ds, oracle, fit, target, W, H, N, reps=20):
     e_fwd = int(fw["E"])
     flops = e_fwd * FWD_FLOP_PER_EVAL + e_bwd * BWD_FLOP_PER_EVAL
     achieved = flops / avg_s / 1e12
-    return {"kernel": "k_composite_fused", "bound": "valu", "unit": "TFLOP/s",
+    return {"kernel": kernels, "launch_shape": shape, "bound": "valu", "unit": "TFLOP/s",
             "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": (FUSED_TRAFFIC_READ + FUSED_TRAFFIC_WRITE
-                        if FUSED_TRAFFIC_READ is not None else None),
-            "traffic_read_bytes": FUSED_TRAFFIC_READ, "traffic_write_bytes": FUSED_TRAFFIC_WRITE,
-            "traffic_state": FUSED_TRAFFIC_STATE,
+            "traffic": sum(traffic) if traffic else None,
+            "traffic_read_bytes": traffic[0] if traffic else None,
+            "traffic_write_bytes": traffic[1] if traffic else None,
+            "traffic_state": COMPOSITE_TRAFFIC_STATE if traffic else
+            f"no counter figures for launch shape {shape}",
             "avg_launch_ms": round(avg_s * 1e3, 4), "splat_evals_fwd": e_fwd,
             "splat_evals_bwd": int(e_bwd), "flop_per_eval_fwd": FWD_FLOP_PER_EVAL,
             "flop_per_eval_bwd": BWD_FLOP_PER_EVAL, "n_isects": int(st["n_isects"]),

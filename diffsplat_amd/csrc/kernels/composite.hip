@@ -510,12 +510,44 @@ __device__ __forceinline__ void composite_back_to_front(
 // The slots' records of this epoch summed per splat and added to its
 // gradient. A splat's slots are contiguous (slot_splat holds each slot's
 // splat), so each wave takes 64 consecutive slots, one per lane, and adds
-// them with a segmented scan over the lanes (xor-free: shifts up by 1, 2, 4,
-// ... within the run of equal splats); the last lane of each run adds the
-// run's sum to the splat's gradient: a plain add when the splat's slots all
-// lie inside this wave (no other wave touches it), a float atomic for a
-// splat that continues into a neighbouring wave's slots (at most two per
-// wave).
+// them with a segmented inclusive scan over the lanes: shifts up by 1, 2, 4
+// and 8 inside each 16-lane row (DPP), each adding only within the run of
+// equal splats, leave every lane the sum of its run from the row's start;
+// then the odd rows add the previous row's last lane and rows 2 and 3 add
+// lane 31, each when the run continues across that row boundary. The last
+// lane of each run adds the run's sum to the splat's gradient: a plain add
+// when the splat's slots all lie inside this wave (no other wave touches
+// it), a float atomic for a splat that continues into a neighbouring wave's
+// slots (at most two per wave). With STORE the gradient is known to be zero
+// (cleared by the same step), so a whole run is stored without reading it.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
+}
+
+// One in-row step: lane l adds lane l - D of its row when both hold splat g.
+template <int D>
+__device__ __forceinline__ void seg_scan_row_step(float (&v)[9], uint32_t g, int lane) {
+    constexpr int kRowShr = 0x110 + D;  // row_shr:D (lanes below D of a row read 0)
+    const uint32_t gu = dpp_u32<kRowShr>(g);
+    const bool same = (lane & 15) >= D && gu == g;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) {
+        const float u = __uint_as_float(dpp_u32<kRowShr>(__float_as_uint(v[f])));
+        v[f] += same ? u : 0.f;
+    }
+}
+
+// Lane l adds lane src's sums when take is set (src read by every lane).
+__device__ __forceinline__ void seg_scan_carry(float (&v)[9], int src, bool take) {
+#pragma unroll
+    for (int f = 0; f < 9; ++f) {
+        const float u = __shfl(v[f], src, kWave);
+        v[f] += take ? u : 0.f;
+    }
+}
+
+template <bool STORE>
 __global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ rec,
                                                      const uint32_t *__restrict__ slot_splat,
                                                      const uint32_t *nisect,
@@ -530,8 +562,9 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ 
         const uint32_t p = base + lane;
         const bool valid = p < m;
         const uint32_t g = valid ? slot_splat[p] : 0xFFFFFFFFu;
-        // lane 0's splat, read while every lane is active (lane 0 always is)
-        const uint32_t g0 = (uint32_t)__builtin_amdgcn_readfirstlane(g);
+        // the splats of the slots just before and just after the wave's
+        const uint32_t g_prev = base > 0 ? slot_splat[base - 1] : 0xFFFFFFFFu;
+        const uint32_t g_next = base + kWave < m ? slot_splat[base + kWave] : 0xFFFFFFFEu;
         float v[9];
 #pragma unroll
         for (int f = 0; f < 9; ++f) v[f] = 0.f;
@@ -550,24 +583,30 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ 
                 v[8] = r2.x;
             }
         }
-        // inclusive segmented scan: runs of equal g are contiguous, so lane
-        // - d holds the same splat exactly when every lane between does
-#pragma unroll
-        for (int d = 1; d < kWave; d <<= 1) {
-            const uint32_t gu = __shfl_up(g, d, kWave);
-            const bool same = lane >= d && gu == g;
-#pragma unroll
-            for (int f = 0; f < 9; ++f) {
-                const float u = __shfl_up(v[f], d, kWave);
-                v[f] += same ? u : 0.f;
-            }
+        // lane 0's splat, read while every lane is active (lane 0 always is)
+        const uint32_t g0 = (uint32_t)__builtin_amdgcn_readfirstlane(g);
+        // runs of equal g are contiguous: lanes a < b hold the same splat
+        // exactly when every lane between does
+        seg_scan_row_step<1>(v, g, lane);
+        seg_scan_row_step<2>(v, g, lane);
+        seg_scan_row_step<4>(v, g, lane);
+        seg_scan_row_step<8>(v, g, lane);
+        {  // rows 1 and 3: the run's part in rows 0 and 2 (lanes 15, 47)
+            const int row0 = lane & ~15;
+            const int src = row0 > 0 ? row0 - 1 : 0;
+            const uint32_t g_src = __shfl(g, src, kWave), g_row0 = __shfl(g, row0, kWave);
+            seg_scan_carry(v, src, ((lane >> 4) & 1) && g_src == g && g_row0 == g);
+        }
+        {  // rows 2 and 3: the run's part in rows 0 and 1 (lane 31)
+            const uint32_t g31 = __shfl(g, 31, kWave), g32 = __shfl(g, 32, kWave);
+            seg_scan_carry(v, 31, lane >= 32 && g31 == g && g32 == g);
         }
         const uint32_t gn = __shfl_down(g, 1, kWave);
         const bool run_end = valid && (lane == kWave - 1 || gn != g);
         if (!run_end) continue;
         // does the run start in this wave, and end in it?
-        const bool starts_before = (g == g0) && base > 0 && slot_splat[base - 1] == g;
-        const bool ends_after = lane == kWave - 1 && p + 1 < m && slot_splat[p + 1] == g;
+        const bool starts_before = g == g0 && g_prev == g;
+        const bool ends_after = lane == kWave - 1 && g_next == g;
         if (starts_before || ends_after) {
 #pragma unroll
             for (int f = 0; f < 9; ++f)
@@ -575,7 +614,12 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ 
         } else {
 #pragma unroll
             for (int f = 0; f < 9; ++f)
-                if (v[f] != 0.f) dst[f][g] += v[f];
+                if (v[f] != 0.f) {
+                    if constexpr (STORE)
+                        dst[f][g] = v[f];
+                    else
+                        dst[f][g] += v[f];
+                }
         }
     }
 }
@@ -799,11 +843,15 @@ ds_status launch_composite_fwd(const CompositeArgs &a, int pix_per_thread, float
 
 ds_status launch_pair_reduce(const float4 *pair_rec, const uint32_t *slot_splat,
                              const uint32_t *nisect, const uint32_t *epoch, int64_t slot_cap,
-                             const ds_splat2d &grad, hipStream_t s) {
+                             const ds_splat2d &grad, bool grad_is_zero, hipStream_t s) {
     if (slot_cap <= 0) return DS_OK;
     const int blocks = (int)std::min<int64_t>((slot_cap + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_pair_reduce, dim3(blocks), dim3(256), 0, s, pair_rec, slot_splat, nisect,
-                       epoch, grad);
+    if (grad_is_zero)
+        hipLaunchKernelGGL(k_pair_reduce<true>, dim3(blocks), dim3(256), 0, s, pair_rec,
+                           slot_splat, nisect, epoch, grad);
+    else
+        hipLaunchKernelGGL(k_pair_reduce<false>, dim3(blocks), dim3(256), 0, s, pair_rec,
+                           slot_splat, nisect, epoch, grad);
     DS_CHECK_LAUNCH("pair_reduce");
     return DS_OK;
 }

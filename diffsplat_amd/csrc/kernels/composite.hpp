@@ -73,8 +73,10 @@ ds_status launch_composite_bwd(const CompositeArgs &a, int pix_per_thread, int g
 // Adds the gradient records of the current epoch into grad, per splat: slot p
 // (p < *nisect, the pairs kept) belongs to splat slot_splat[p], and a
 // splat's slots are contiguous. slot_cap bounds *nisect (grid size).
+// grad_is_zero: grad was cleared earlier on the stream and nothing else adds
+// to it, so each splat's sum is stored.
 ds_status launch_pair_reduce(const float4 *pair_rec, const uint32_t *slot_splat,
                              const uint32_t *nisect, const uint32_t *epoch, int64_t slot_cap,
-                             const ds_splat2d &grad, hipStream_t s);
+                             const ds_splat2d &grad, bool grad_is_zero, hipStream_t s);
 
 }  // namespace ds

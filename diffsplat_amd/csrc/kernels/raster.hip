@@ -883,12 +883,14 @@ static ds_status require_backward(const ds_raster *r) {
     return DS_OK;
 }
 
-// Each splat's records of this epoch into grad (after a backward compositing).
-static ds_status reduce_pairs(ds_raster *r, const ds_splat2d &grad, hipStream_t s) {
+// Each splat's records of this epoch into grad (after a backward compositing);
+// grad_is_zero when the same forward's first kernel cleared it.
+static ds_status reduce_pairs(ds_raster *r, const ds_splat2d &grad, hipStream_t s,
+                              bool grad_is_zero = false) {
     if (r->last_count <= 0 || !r->slot_splat) return DS_OK;
     uint32_t *sc = r->scalars.ptr;
     return launch_pair_reduce(r->pair_rec.ptr, r->slot_splat, sc + SC_NISECT, sc + SC_EPOCH,
-                              (int64_t)r->isect_cap, grad, s);
+                              (int64_t)r->isect_cap, grad, grad_is_zero, s);
 }
 
 // Tile assignment, sorts, ranges and dispatch order, then compositing: the
@@ -1098,7 +1100,10 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
     if (fused_grad && !(target && r->fwd_ppt == r->bwd_ppt))
         DS_TRY(launch_composite_bwd(ca, r->bwd_ppt, r->grad_mode, r->T.ptr, r->last.ptr,
                                     grad, s));
-    if (fused_grad) DS_TRY(reduce_pairs(r, *fused_grad, s));
+    // a fitter step's prelude (grad_zero) cleared this gradient
+    if (fused_grad)
+        DS_TRY(reduce_pairs(r, *fused_grad, s,
+                            grad_zero != nullptr && grad_zero->pos_x == fused_grad->pos_x));
     return DS_OK;
 }
 
