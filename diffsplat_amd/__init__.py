@@ -284,6 +284,25 @@ def tile_order_name(raster_handle) -> str:
     return TILE_ORDERS[o.value]
 
 
+DEPTH_SORTS = ("onesweep_8bit", "radix_8bit", "radix_11bit")
+
+
+def depth_sort_name(raster_handle) -> str:
+    """How a ds_raster sorts splats by depth (DS_DEPTH_SORT_*); the tile pass
+    after it has 8-bit digits."""
+    o = C.c_int()
+    check(lib().ds_raster_get_depth_sort(raster_handle, C.byref(o)), "ds_raster_get_depth_sort")
+    return DEPTH_SORTS[o.value]
+
+
+def depth_sort_times(raster_handle) -> dict:
+    """ms per sort of each depth-sort mode at the last autotune (None: not
+    timed, or not available for the raster)."""
+    ms = np.zeros(3, np.float32)
+    check(lib().ds_raster_depth_sort_times(raster_handle, fptr(ms)), "ds_raster_depth_sort_times")
+    return {n: (round(float(v), 4) if v >= 0 else None) for n, v in zip(DEPTH_SORTS, ms)}
+
+
 class Rasterizer:
     """ds_raster: tile assign + sort + compositing, forward and backward."""
 
@@ -535,7 +554,8 @@ class ImageFitter:
         check(lib().ds_fitter_get_raster(self.h, C.byref(r)), "ds_fitter_get_raster")
         return {"fwd_pix_per_thread": f.value, "bwd_pix_per_thread": b.value,
                 "fwd_ms": round(tf.value, 4), "bwd_ms": round(tb.value, 4),
-                "tile_order": tile_order_name(r)}
+                "tile_order": tile_order_name(r), "depth_sort": depth_sort_name(r),
+                "depth_sort_ms": depth_sort_times(r), "tile_radix_bits": 8}
 
     def set_tuning(self, fwd_ppt: int, bwd_ppt: int) -> None:
         r = C.c_void_p()
@@ -774,7 +794,8 @@ class MultiFitter:
         check(lib().ds_multifit_get_raster(self.h, C.byref(r)), "ds_multifit_get_raster")
         return {"fwd_pix_per_thread": f.value, "bwd_pix_per_thread": b.value,
                 "fwd_ms": round(tf.value, 4), "bwd_ms": round(tb.value, 4),
-                "tile_order": tile_order_name(r)}
+                "tile_order": tile_order_name(r), "depth_sort": depth_sort_name(r),
+                "depth_sort_ms": depth_sort_times(r), "tile_radix_bits": 8}
 
     def set_tuning(self, fwd_ppt: int, bwd_ppt: int) -> None:
         r = C.c_void_p()
@@ -911,7 +932,8 @@ class SplatRenderer:
         r = C.c_void_p()
         check(lib().ds_renderer_get_raster(self.h, C.byref(r)), "ds_renderer_get_raster")
         return {"fwd_pix_per_thread": best.value, "ms_per_frame": round(ms.value, 4),
-                "tile_order": tile_order_name(r)}
+                "tile_order": tile_order_name(r), "depth_sort": depth_sort_name(r),
+                "depth_sort_ms": depth_sort_times(r), "tile_radix_bits": 8}
 
     def synchronize(self) -> None:
         check(lib().ds_renderer_synchronize(self.h), "ds_renderer_synchronize")

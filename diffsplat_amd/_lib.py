@@ -165,6 +165,12 @@ _SIGS = {
                                     _P]),
     "ds_raster_composite_fwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, _P]),
     "ds_raster_release_backward": (C.c_int, [_P]),
+    "ds_raster_set_depth_sort": (C.c_int, [_P, C.c_int]),
+    "ds_raster_get_depth_sort": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "ds_raster_sort": (C.c_int, [_P, C.POINTER(Splat2D), _P]),
+    "ds_raster_autotune_depth_sort": (C.c_int, [_P, C.POINTER(Splat2D), C.c_int, _P,
+                                                C.POINTER(C.c_int), _P]),
+    "ds_raster_depth_sort_times": (C.c_int, [_P, _P]),
     "ds_raster_composite_fwd_bwd": (C.c_int, [_P, C.POINTER(Splat2D), _P, _P, _P, _P,
                                               C.POINTER(Splat2D), _P]),
     "ds_splat2d_zero": (C.c_int, [C.POINTER(Splat2D), _P]),

@@ -42,6 +42,14 @@ static void tune_tile_order(ds_raster *r, F &&timed) {
     check(ds_raster_set_tile_order(r, t[other] < 0.95f * t[cur] ? other : cur), "autotune");
 }
 
+// Times the depth-sort modes on `in` and keeps the fastest (see
+// ds_raster_autotune_depth_sort).
+static int tune_depth_sort(ds_raster *r, const ds_splat2d &in, hipStream_t s, int reps) {
+    int mode = 0;
+    check(ds_raster_autotune_depth_sort(r, &in, reps, s, &mode, nullptr), "autotune");
+    return mode;
+}
+
 // With the forward and backward launch shapes chosen apart (bf, bb), times
 // the whole compositing (timed_fwd_bwd, ms) for (bf, bb) and for every equal
 // pair (1, 1), (2, 2), (4, 4), with which the forward and backward run in one
@@ -591,6 +599,9 @@ void ImageFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd
     };
     if (fused) tune_fused_shapes(r, bf, bb, fwd_bwd);
     tune_tile_order(r, fwd_bwd);
+    pipeline.forward();
+    tune_depth_sort(r, fused ? gp_layer.getOutput().view() : psp_layer.getOutput().view(), stream,
+                    reps);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     dropGraph();  // the captured step (if any) used the old launch shapes
@@ -967,6 +978,7 @@ void MultiFitter::autotune(int reps, int *best_fwd, int *best_bwd, float *ms_fwd
     };
     tune_fused_shapes(ras, bf, bb, fwd_bwd);
     tune_tile_order(ras, fwd_bwd);
+    tune_depth_sort(ras, sv, stream, reps);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     dropGraph();
@@ -1049,6 +1061,9 @@ int SplatRenderer::autotune(int reps, float *ms_best) {
         hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
         return ms / reps;
     });
+    pipeline.forward();
+    tune_depth_sort(r, gp_layer.getOutput().view(), stream, reps);
+    pipeline.forward();  // the frame again, with the kept sort
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (ms_best) *ms_best = t[ib];

@@ -768,6 +768,8 @@ struct ds_raster {
     int fwd_ppt = 1;  // compositing forward: pixels per thread (1, 2 or 4)
     int bwd_ppt = 1;  // compositing backward: pixels per thread
     int tile_order = DS_TILE_ORDER_HEAVY_FIRST;  // compositing dispatch order
+    int depth_sort = DS_DEPTH_SORT_ONESWEEP;     // how the splats are sorted by depth
+    float depth_sort_ms[3] = {-1.f, -1.f, -1.f};  // the last autotune's times per mode
     int tile_bits = 1;
     int tile_passes = 1;  // 8-bit radix passes over the tile ids
     // Several fit instances in one rasterizer (ninst > 0): splats, tiles and
@@ -934,12 +936,12 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
                                 const float *target, float *loss, float *grad, hipStream_t s,
                                 const ds_splat2d *fused_grad = nullptr,
                                 const ds_splat2d *grad_zero = nullptr, int32_t *tick = nullptr,
-                                bool started = false) {
+                                bool started = false, bool sort_only = false) {
     const int n = in.count;
     DS_REQUIRE(n >= 0 && n <= r->count_cap, "splat count exceeds the rasterizer's count_cap");
     DS_REQUIRE(r->ninst == 0 || n == r->count_cap,
                "a multi-instance rasterizer takes exactly the splats of all its instances");
-    DS_REQUIRE(out_rgb != nullptr, "out_rgb is NULL");
+    DS_REQUIRE(sort_only || out_rgb != nullptr, "out_rgb is NULL");
     DS_REQUIRE((target == nullptr) == (loss == nullptr) && (target == nullptr) == (grad == nullptr),
                "target, loss and grad must be given together");
     uint32_t *sc = r->scalars.ptr;
@@ -960,7 +962,12 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
         DS_CHECK_LAUNCH("begin");
     }
     if (n > 0) {
-        const bool one_kernel = (uint32_t)n <= kOneKernelMaxKeys;
+        const bool one_kernel =
+            r->depth_sort == DS_DEPTH_SORT_ONESWEEP && (uint32_t)n <= kOneKernelMaxKeys;
+        const int dbits = (!one_kernel && r->depth_sort == DS_DEPTH_SORT_RADIX11)
+                              ? kWideRadixBits
+                              : kRadixBits;
+        const int dpasses = (32 + dbits - 1) / dbits;
         if (multi)
             hipLaunchKernelGGL(k_prep<true>, dim3((n + 255) / 256), dim3(256), 0, s, in, n, r->W,
                                r->H, r->ntx, r->nty, r->rect.ptr, r->ntiles_of.ptr,
@@ -972,12 +979,13 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
                                r->dkey_a.ptr, r->didx_a.ptr, r->rec.ptr, gh, one_kernel ? 1 : 0,
                                mi, si);
         DS_CHECK_LAUNCH("prep");
-        // Depth sort of splats: four 8-bit LSD passes over the 32 depth bits,
-        // and the pair offsets in depth order (total and capped count). Up
-        // to kOneKernelMaxKeys splats each pass is one kernel with look-back
-        // (launches dominate there) and the last pass also writes the
-        // offsets; above, reduce-then-scan passes and a separate scan win, as
-        // the look-back traffic grows with the chunk count.
+        // Depth sort of splats: LSD passes over the 32 depth bits (four of 8
+        // bits, or three of 11), and the pair offsets in depth order (total
+        // and capped count). Up to kOneKernelMaxKeys splats each 8-bit pass
+        // can be one kernel with look-back (launches dominate there), the
+        // last pass also writing the offsets; otherwise reduce-then-scan
+        // passes and a separate scan (the look-back traffic grows with the
+        // chunk count). The autotuners time the choices.
         uint32_t *ka = r->dkey_a.ptr, *kb = r->dkey_b.ptr, *va = r->didx_a.ptr,
                  *vb = r->didx_b.ptr;
         const size_t dstride = lookback_words((uint64_t)r->count_cap, kRadix);
@@ -985,7 +993,7 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
         static_assert(kTileChunk == 1 << 12, "chunk heads use shift 12");
         static_assert(kEmitChunkShift == 12, "one chunk size for both emissions");
         const ScanHeads heads{r->tfirst.ptr, 12u, (uint32_t)max_chunks};
-        for (int pass = 0; pass < 4; ++pass) {
+        for (int pass = 0; pass < dpasses; ++pass) {
             if (one_kernel) {
                 // the last pass also writes the pair offsets in depth order
                 const PairOffsets po =
@@ -1006,8 +1014,8 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
                                      slot(r->lb_depth.ptr + pass * dstride, LB_DEPTH + pass), s,
                                      po));
             } else {
-                DS_TRY(radix_pass<uint32_t>(ka, va, kb, vb, sc + SC_COUNT, (uint32_t)n, 8 * pass,
-                                            r->hist.ptr, r->part.ptr, s));
+                DS_TRY(radix_pass<uint32_t>(ka, va, kb, vb, sc + SC_COUNT, (uint32_t)n,
+                                            dbits * pass, r->hist.ptr, r->part.ptr, s, dbits));
             }
             std::swap(ka, kb);
             std::swap(va, vb);
@@ -1087,8 +1095,9 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
     }
     r->order_valid = r->tile_order == DS_TILE_ORDER_HEAVY_FIRST;
     r->last_count = n;
-    r->has_fwd = true;
     r->base_valid = r->pair_base.ptr != nullptr;
+    if (sort_only) return DS_OK;
+    r->has_fwd = true;
     const CompositeArgs ca = composite_args(r);
     if (fused_grad) DS_TRY(require_backward(r));
     if (fused_grad && target && r->fwd_ppt == r->bwd_ppt)
@@ -1175,7 +1184,9 @@ static ds_status raster_finish_create(ds_raster *r, ds_status st, ds_raster **ou
     if (st == DS_OK) st = fill_u32((uint32_t *)r->lb_depth.ptr, 0u, 2 * lbd, nullptr);
     if (st == DS_OK) st = r->ghist.alloc((size_t)kGhistWords);
     if (st == DS_OK)
-        st = r->hist.alloc(sort_hist_words(std::max<uint64_t>((uint64_t)cap, (uint64_t)nc)));
+        st = r->hist.alloc(std::max<uint64_t>(
+            sort_hist_words(std::max<uint64_t>((uint64_t)cap, (uint64_t)nc)),
+            sort_hist_words((uint64_t)nc, kWideRadixBits)));
     if (st == DS_OK) st = r->part.alloc(kSpineMax);
     if (st == DS_OK) st = r->scalars.alloc(SC_NUM);
     if (st == DS_OK) st = fill_u32(r->scalars.ptr, 0u, SC_NUM, nullptr);
@@ -1202,6 +1213,8 @@ extern "C" ds_status ds_raster_create(const ds_raster_config *cfg, ds_raster **o
     auto *r = new ds_raster();
     r->count_cap = cfg->count_cap;
     r->isect_cap = (uint32_t)cap;
+    r->depth_sort = (uint32_t)r->count_cap <= kOneKernelMaxKeys ? DS_DEPTH_SORT_ONESWEEP
+                                                                 : DS_DEPTH_SORT_RADIX8;
     r->keep_keys = cfg->keep_keys;
     r->grad_mode = cfg->grad_mode;
     return raster_finish_create(r, raster_alloc_pixels(r, cfg->width, cfg->height), out);
@@ -1251,6 +1264,8 @@ extern "C" ds_status ds_raster_create_multi(const ds_raster_instance *inst, int 
     auto *r = new ds_raster();
     r->count_cap = (int)nsplat;
     r->isect_cap = (uint32_t)cap;
+    r->depth_sort = (uint32_t)r->count_cap <= kOneKernelMaxKeys ? DS_DEPTH_SORT_ONESWEEP
+                                                                 : DS_DEPTH_SORT_RADIX8;
     r->keep_keys = keep_keys;
     r->grad_mode = grad_mode;
     r->ninst = n_inst;
@@ -1434,6 +1449,88 @@ extern "C" ds_status ds_raster_set_tile_order(ds_raster *r, int order) {
     DS_REQUIRE(order == DS_TILE_ORDER_ROWS || order == DS_TILE_ORDER_HEAVY_FIRST,
                "order must be DS_TILE_ORDER_ROWS or DS_TILE_ORDER_HEAVY_FIRST");
     r->tile_order = order;
+    return DS_OK;
+}
+
+extern "C" ds_status ds_raster_set_depth_sort(ds_raster *r, int mode) {
+    clear_error();
+    DS_REQUIRE(r, "NULL raster");
+    DS_REQUIRE(mode == DS_DEPTH_SORT_ONESWEEP || mode == DS_DEPTH_SORT_RADIX8 ||
+                   mode == DS_DEPTH_SORT_RADIX11,
+               "mode must be DS_DEPTH_SORT_ONESWEEP, _RADIX8 or _RADIX11");
+    DS_REQUIRE(mode != DS_DEPTH_SORT_ONESWEEP || (uint32_t)r->count_cap <= kOneKernelMaxKeys,
+               "DS_DEPTH_SORT_ONESWEEP sorts at most 131072 splats");
+    DS_REQUIRE(mode != DS_DEPTH_SORT_RADIX11 ||
+                   (uint64_t)sort_hist_words((uint64_t)r->count_cap, kWideRadixBits) <=
+                       (uint64_t)kSpineMax * kScanChunk,
+               "DS_DEPTH_SORT_RADIX11 sorts at most 2^24 splats");
+    r->depth_sort = mode;
+    return DS_OK;
+}
+
+extern "C" ds_status ds_raster_get_depth_sort(ds_raster *r, int *mode) {
+    clear_error();
+    DS_REQUIRE(r && mode, "NULL argument");
+    *mode = r->depth_sort;
+    return DS_OK;
+}
+
+extern "C" ds_status ds_raster_sort(ds_raster *r, const ds_splat2d *in, ds_stream_t s) {
+    clear_error();
+    DS_REQUIRE(r && in, "NULL argument");
+    return raster_forward(r, *in, nullptr, nullptr, nullptr, nullptr, as_stream(s), nullptr,
+                          nullptr, nullptr, false, true);
+}
+
+extern "C" ds_status ds_raster_autotune_depth_sort(ds_raster *r, const ds_splat2d *in, int reps,
+                                                   ds_stream_t st, int *mode, float *ms) {
+    clear_error();
+    DS_REQUIRE(r && in, "NULL argument");
+    if (reps < 1) reps = 1;
+    hipStream_t s = as_stream(st);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DS_TRY_HIP(hipEventCreate(&e0));
+    DS_TRY_HIP(hipEventCreate(&e1));
+    const int cur = r->depth_sort;
+    float t[3] = {-1.f, -1.f, -1.f};
+    ds_status st_out = DS_OK;
+    for (int m = 0; m < 3 && st_out == DS_OK; ++m) {
+        if (ds_raster_set_depth_sort(r, m) != DS_OK) continue;  // not for this raster
+        st_out = raster_forward(r, *in, nullptr, nullptr, nullptr, nullptr, s, nullptr, nullptr,
+                                nullptr, false, true);  // warm-up
+        if (st_out != DS_OK) break;
+        if (hipEventRecord(e0, s) != hipSuccess) st_out = set_error(DS_ERR_HIP, "hipEventRecord");
+        for (int k = 0; k < reps && st_out == DS_OK; ++k)
+            st_out = raster_forward(r, *in, nullptr, nullptr, nullptr, nullptr, s, nullptr,
+                                    nullptr, nullptr, false, true);
+        if (st_out == DS_OK && (hipEventRecord(e1, s) != hipSuccess ||
+                                hipEventSynchronize(e1) != hipSuccess ||
+                                hipEventElapsedTime(&t[m], e0, e1) != hipSuccess))
+            st_out = set_error(DS_ERR_HIP, "timing the depth sort");
+        t[m] /= (float)reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st_out != DS_OK) {
+        r->depth_sort = cur;
+        return st_out;
+    }
+    clear_error();
+    int best = cur;  // kept unless another mode is at least 3% faster
+    for (int m = 0; m < 3; ++m)
+        if (t[m] >= 0.f && t[m] < 0.97f * t[best]) best = m;
+    r->depth_sort = best;
+    for (int m = 0; m < 3; ++m) r->depth_sort_ms[m] = t[m];
+    if (mode) *mode = best;
+    if (ms)
+        for (int m = 0; m < 3; ++m) ms[m] = t[m];
+    return DS_OK;
+}
+
+extern "C" ds_status ds_raster_depth_sort_times(ds_raster *r, float *ms) {
+    clear_error();
+    DS_REQUIRE(r && ms, "NULL argument");
+    for (int m = 0; m < 3; ++m) ms[m] = r->depth_sort_ms[m];
     return DS_OK;
 }
 

@@ -89,42 +89,45 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t *in,
     }
 }
 
-template <typename K>
+template <typename K, int RB = kRadixBits>
 __device__ __forceinline__ uint32_t digit_of(K k, int shift) {
-    return (uint32_t)(k >> shift) & (kRadix - 1);
+    return (uint32_t)(k >> shift) & ((1u << RB) - 1u);
 }
 
-template <typename K>
+template <typename K, int RB>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const K *__restrict__ kin,
                                                              const uint32_t *n_dev, int shift,
                                                              uint32_t *hist) {
-    __shared__ uint32_t s_h[kRadix];
+    constexpr int R = 1 << RB;
+    __shared__ uint32_t s_h[R];
     const uint32_t n = *n_dev;
     const uint32_t nch = (n + kSortChunk - 1) / kSortChunk;
     for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
-        s_h[threadIdx.x] = 0;
+        for (int d = threadIdx.x; d < R; d += kSortThreads) s_h[d] = 0;
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kSortItems; ++j) {
             const uint32_t e = c * kSortChunk + j * kSortThreads + threadIdx.x;
-            if (e < n) atomicAdd(&s_h[digit_of(kin[e], shift)], 1u);
+            if (e < n) atomicAdd(&s_h[digit_of<K, RB>(kin[e], shift)], 1u);
         }
         __syncthreads();
-        hist[threadIdx.x * nch + c] = s_h[threadIdx.x];
+        for (int d = threadIdx.x; d < R; d += kSortThreads) hist[d * nch + c] = s_h[d];
         __syncthreads();
     }
 }
 
-// LDS of one workgroup ranking and scattering a chunk of kSortChunk keys.
-template <typename K>
+// LDS of one workgroup ranking and scattering a chunk of kSortChunk keys by
+// RB-bit digits (2^RB digits: one per thread at 8 bits, eight at 11).
+template <typename K, int RB = kRadixBits>
 struct ChunkLds {
     static constexpr int NW = kSortThreads / kWave;
+    static constexpr int R = 1 << RB;
     K k[kSortChunk];
     uint32_t v[kSortChunk];
-    uint32_t w[NW][kRadix];  // per-wave digit counts, then per-wave digit offsets
-    uint32_t run[kRadix];    // digit counts of the chunk
-    uint32_t start[kRadix];  // exclusive prefix of run over digits
-    uint32_t gb[kRadix];     // global position of the chunk's first key of each digit
+    uint32_t w[NW][R];  // per-wave digit counts, then per-wave digit offsets
+    uint32_t run[R];    // digit counts of the chunk
+    uint32_t start[R];  // exclusive prefix of run over digits
+    uint32_t gb[R];     // global position of the chunk's first key of each digit
     uint32_t tmp[NW];
     uint32_t ticket;
 };
@@ -133,18 +136,20 @@ struct ChunkLds {
 // number of earlier keys of the chunk with the same digit. Leaves the digit
 // counts in L.run and their exclusive prefix in L.start. Wave w owns the
 // contiguous keys [w * kSortChunk / NW, (w + 1) * kSortChunk / NW) of the
-// chunk and walks them 64 at a time in order, matching the 8 digit bits
+// chunk and walks them 64 at a time in order, matching the RB digit bits
 // across the wave with ballots and keeping a running count per digit in its
 // own LDS row (lanes of one digit share one first lane, so the row's words
 // written in one step are distinct); the waves' counts are then scanned per
-// digit. Three barriers per chunk.
-template <typename K>
+// digit. Three barriers per chunk (four with more digits than threads).
+template <typename K, int RB = kRadixBits>
 __device__ __forceinline__ void chunk_rank(const K *__restrict__ kin,
                                            const uint32_t *__restrict__ vin, uint32_t base,
                                            uint32_t cnt, int shift, K (&key)[kSortItems],
                                            uint32_t (&val)[kSortItems],
-                                           uint32_t (&rank)[kSortItems], ChunkLds<K> &L) {
-    constexpr int NW = ChunkLds<K>::NW;
+                                           uint32_t (&rank)[kSortItems], ChunkLds<K, RB> &L) {
+    constexpr int NW = ChunkLds<K, RB>::NW;
+    constexpr int R = ChunkLds<K, RB>::R;
+    constexpr int DPT = R / kSortThreads;  // digits per thread
     constexpr int kWaveKeys = kSortChunk / NW;
     const int tid = threadIdx.x;
     const int w = tid / kWave, lane = __lane_id();
@@ -161,15 +166,17 @@ __device__ __forceinline__ void chunk_rank(const K *__restrict__ kin,
         }
     }
 #pragma unroll
-    for (int q = 0; q < NW; ++q) L.w[q][tid] = 0;
+    for (int i = 0; i < DPT; ++i)
+#pragma unroll
+        for (int q = 0; q < NW; ++q) L.w[q][i * kSortThreads + tid] = 0;
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
         const bool valid = (uint32_t)(w * kWaveKeys + j * kWave + lane) < cnt;
-        const uint32_t d = digit_of(key[j], shift);
+        const uint32_t d = digit_of<K, RB>(key[j], shift);
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int bit = 0; bit < kRadixBits; ++bit) {
+        for (int bit = 0; bit < RB; ++bit) {
             const bool on = (d >> bit) & 1u;
             const uint64_t b = __ballot(on);
             peers &= on ? b : ~b;
@@ -181,37 +188,54 @@ __device__ __forceinline__ void chunk_rank(const K *__restrict__ kin,
         rank[j] = before + r;  // among this wave's keys of digit d
     }
     __syncthreads();
-    {  // per digit: the waves' first ranks and the chunk's count
+#pragma unroll
+    for (int i = 0; i < DPT; ++i) {  // per digit: the waves' first ranks, the count
+        const int d = i * kSortThreads + tid;
         uint32_t run = 0;
 #pragma unroll
         for (int q = 0; q < NW; ++q) {
-            const uint32_t cq = L.w[q][tid];
-            L.w[q][tid] = run;
+            const uint32_t cq = L.w[q][d];
+            L.w[q][d] = run;
             run += cq;
         }
-        L.run[tid] = run;
+        L.run[d] = run;
     }
-    __syncthreads();
+    if constexpr (DPT > 1) __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j) rank[j] += L.w[w][digit_of(key[j], shift)];
-    L.start[tid] = block_excl_scan<kSortThreads>(L.run[tid], L.tmp, nullptr);
+    for (int j = 0; j < kSortItems; ++j) rank[j] += L.w[w][digit_of<K, RB>(key[j], shift)];
+    if constexpr (DPT == 1) {
+        L.start[tid] = block_excl_scan<kSortThreads>(L.run[tid], L.tmp, nullptr);
+    } else {  // thread t scans its DPT consecutive digits
+        uint32_t x[DPT], sum = 0;
+#pragma unroll
+        for (int i = 0; i < DPT; ++i) {
+            x[i] = L.run[tid * DPT + i];
+            sum += x[i];
+        }
+        uint32_t run = block_excl_scan<kSortThreads>(sum, L.tmp, nullptr);
+#pragma unroll
+        for (int i = 0; i < DPT; ++i) {
+            L.start[tid * DPT + i] = run;
+            run += x[i];
+        }
+    }
 }
 
 // Reorders the ranked chunk by digit in LDS, then writes each run of equal
 // digits contiguously at L.gb[digit] (which the caller has filled).
-template <typename K>
+template <typename K, int RB = kRadixBits>
 __device__ __forceinline__ void chunk_scatter(const K (&key)[kSortItems],
                                               const uint32_t (&val)[kSortItems],
                                               const uint32_t (&rank)[kSortItems], uint32_t cnt,
-                                              int shift, ChunkLds<K> &L, K *__restrict__ kout,
+                                              int shift, ChunkLds<K, RB> &L, K *__restrict__ kout,
                                               uint32_t *__restrict__ vout) {
     const int tid = threadIdx.x;
-    constexpr int kWaveKeys = kSortChunk / ChunkLds<K>::NW;
+    constexpr int kWaveKeys = kSortChunk / ChunkLds<K, RB>::NW;
     const int w = tid / kWave, lane = __lane_id();
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
         if ((uint32_t)(w * kWaveKeys + j * kWave + lane) < cnt) {
-            const uint32_t p = L.start[digit_of(key[j], shift)] + rank[j];
+            const uint32_t p = L.start[digit_of<K, RB>(key[j], shift)] + rank[j];
             L.k[p] = key[j];
             L.v[p] = val[j];
         }
@@ -219,7 +243,7 @@ __device__ __forceinline__ void chunk_scatter(const K (&key)[kSortItems],
     __syncthreads();
     for (uint32_t p = tid; p < cnt; p += kSortThreads) {
         const K k = L.k[p];
-        const uint32_t d = digit_of(k, shift);
+        const uint32_t d = digit_of<K, RB>(k, shift);
         const uint32_t pos = L.gb[d] + (p - L.start[d]);
         kout[pos] = k;
         vout[pos] = L.v[p];
@@ -227,12 +251,13 @@ __device__ __forceinline__ void chunk_scatter(const K (&key)[kSortItems],
     __syncthreads();
 }
 
-template <typename K>
+template <typename K, int RB>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const K *__restrict__ kin, const uint32_t *__restrict__ vin, K *__restrict__ kout,
     uint32_t *__restrict__ vout, const uint32_t *n_dev, int shift,
     const uint32_t *__restrict__ hist) {
-    __shared__ ChunkLds<K> L;
+    constexpr int R = 1 << RB;
+    __shared__ ChunkLds<K, RB> L;
     const uint32_t n = *n_dev;
     const uint32_t nch = (n + kSortChunk - 1) / kSortChunk;
     const int tid = threadIdx.x;
@@ -241,10 +266,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
         const uint32_t cnt = min((uint32_t)kSortChunk, n - base);
         K key[kSortItems];
         uint32_t val[kSortItems], rank[kSortItems];
-        chunk_rank(kin, vin, base, cnt, shift, key, val, rank, L);
-        L.gb[tid] = hist[tid * nch + c];
+        chunk_rank<K, RB>(kin, vin, base, cnt, shift, key, val, rank, L);
+        for (int d = tid; d < R; d += kSortThreads) L.gb[d] = hist[d * nch + c];
         __syncthreads();
-        chunk_scatter(key, val, rank, cnt, shift, L, kout, vout);
+        chunk_scatter<K, RB>(key, val, rank, cnt, shift, L, kout, vout);
     }
 }
 
@@ -528,31 +553,46 @@ ds_status scan_exclusive_u32(const uint32_t *in, const uint32_t *gather_idx, uin
     return DS_OK;
 }
 
-template <typename K>
-ds_status radix_pass(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
-                     const uint32_t *n_dev, uint32_t max_n, int shift, uint32_t *hist,
-                     uint32_t *part, hipStream_t s) {
+template <typename K, int RB>
+static ds_status radix_pass_rb(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
+                               const uint32_t *n_dev, uint32_t max_n, int shift, uint32_t *hist,
+                               uint32_t *part, hipStream_t s) {
+    constexpr int R = 1 << RB;
     const uint64_t max_chunks = ((uint64_t)max_n + kSortChunk - 1) / kSortChunk;
     if (max_chunks == 0) return DS_OK;
+    if (max_chunks * R > (uint64_t)kSpineMax * kScanChunk)
+        return set_error(DS_ERR_CAPACITY, "radix pass: too many digit counts to scan");
     const int grid = clamp_grid(max_chunks, 2048);
-    hipLaunchKernelGGL(k_radix_hist<K>, dim3(grid), dim3(kSortThreads), 0, s, kin, n_dev, shift,
-                       hist);
+    hipLaunchKernelGGL((k_radix_hist<K, RB>), dim3(grid), dim3(kSortThreads), 0, s, kin, n_dev,
+                       shift, hist);
     DS_CHECK_LAUNCH("radix_hist");
     DS_TRY(scan_exclusive_u32(hist, nullptr, hist, ScanLen{n_dev, (uint32_t)kSortChunk,
-                                                           (uint32_t)kRadix},
-                              (uint32_t)(max_chunks * kRadix), part, nullptr, nullptr, 0, s));
-    hipLaunchKernelGGL(k_radix_scatter<K>, dim3(grid), dim3(kSortThreads), 0, s, kin, vin, kout,
-                       vout, n_dev, shift, (const uint32_t *)hist);
+                                                           (uint32_t)R},
+                              (uint32_t)(max_chunks * R), part, nullptr, nullptr, 0, s));
+    hipLaunchKernelGGL((k_radix_scatter<K, RB>), dim3(grid), dim3(kSortThreads), 0, s, kin, vin,
+                       kout, vout, n_dev, shift, (const uint32_t *)hist);
     DS_CHECK_LAUNCH("radix_scatter");
     return DS_OK;
 }
 
+template <typename K>
+ds_status radix_pass(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
+                     const uint32_t *n_dev, uint32_t max_n, int shift, uint32_t *hist,
+                     uint32_t *part, hipStream_t s, int bits) {
+    if (bits == kWideRadixBits)
+        return radix_pass_rb<K, kWideRadixBits>(kin, vin, kout, vout, n_dev, max_n, shift, hist,
+                                                part, s);
+    if (bits != kRadixBits) return set_error(DS_ERR_INVALID_ARG, "radix digits of 8 or 11 bits");
+    return radix_pass_rb<K, kRadixBits>(kin, vin, kout, vout, n_dev, max_n, shift, hist, part,
+                                        s);
+}
+
 template ds_status radix_pass<uint32_t>(const uint32_t *, const uint32_t *, uint32_t *,
                                         uint32_t *, const uint32_t *, uint32_t, int, uint32_t *,
-                                        uint32_t *, hipStream_t);
+                                        uint32_t *, hipStream_t, int);
 template ds_status radix_pass<uint64_t>(const uint64_t *, const uint32_t *, uint64_t *,
                                         uint32_t *, const uint32_t *, uint32_t, int, uint32_t *,
-                                        uint32_t *, hipStream_t);
+                                        uint32_t *, hipStream_t, int);
 
 ds_status sort_pairs_u64_scratch(const uint64_t *keys_in, const uint32_t *vals_in,
                                  uint64_t *keys_out, uint32_t *vals_out, uint32_t n,

@@ -21,6 +21,8 @@ constexpr int kSortItems = 8;
 constexpr int kSortChunk = kSortThreads * kSortItems;   // 2048
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;                  // 256 == kSortThreads
+// The wide digit of radix_pass: three passes cover 32-bit keys instead of four.
+constexpr int kWideRadixBits = 11;
 
 // Length of a scan: ceil(n / div) * mul with n read from device memory.
 struct ScanLen {
@@ -90,16 +92,18 @@ ds_status scan_exclusive_u32(const uint32_t *in, const uint32_t *gather_idx, uin
                              uint32_t *clamped_out, uint32_t clamp_cap, hipStream_t s,
                              ScanHeads heads = ScanHeads{nullptr, 0, 0});
 
-// One stable LSD pass: sort (key, val) by bits [shift, shift+8) of the key.
-// n_dev: element count; hist: scratch of kRadix * ceil(max_n / kSortChunk)
-// words; part: scan scratch of kSpineMax words.
+// One stable LSD pass: sort (key, val) by bits [shift, shift + bits) of the
+// key, bits 8 or 11 (kWideRadixBits). n_dev: element count; hist: scratch of
+// 2^bits * ceil(max_n / kSortChunk) words (sort_hist_words); part: scan
+// scratch of kSpineMax words. At 11 bits the digit counts (2048 per chunk of
+// 2048 keys) must fit one scan: max_n up to 2^24.
 template <typename K>
 ds_status radix_pass(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
                      const uint32_t *n_dev, uint32_t max_n, int shift, uint32_t *hist,
-                     uint32_t *part, hipStream_t s);
+                     uint32_t *part, hipStream_t s, int bits = kRadixBits);
 
-inline uint32_t sort_hist_words(uint64_t max_n) {
-    return (uint32_t)(kRadix * ((max_n + kSortChunk - 1) / kSortChunk));
+inline uint32_t sort_hist_words(uint64_t max_n, int bits = kRadixBits) {
+    return (uint32_t)((1u << bits) * ((max_n + kSortChunk - 1) / kSortChunk));
 }
 
 // Scratch of a stand-alone (u64 key, u32 value) sort of up to n pairs: key

@@ -344,6 +344,39 @@ ds_status ds_raster_get_tuning(ds_raster *r, int *fwd_pix_per_thread, int *bwd_p
 ds_status ds_raster_set_tile_order(ds_raster *r, int order);
 ds_status ds_raster_get_tile_order(ds_raster *r, int *order);
 
+/* How the forward sorts the splats by depth before emitting their pairs
+ * (results do not depend on it; the reference sorts the 64-bit pair keys in
+ * one CUB call, tile_rasterize_layer.cu:679-688):
+ * DS_DEPTH_SORT_ONESWEEP, four 8-bit passes of one kernel each with chained
+ * look-back (count_cap up to 131072; the default there);
+ * DS_DEPTH_SORT_RADIX8, four 8-bit passes of histogram, scan and scatter
+ * kernels (the default above 131072);
+ * DS_DEPTH_SORT_RADIX11, three 11-bit passes of the same kernels (2048
+ * digits per chunk of 2048 keys; count_cap up to 2^24).
+ * The tile pass after it keeps 8-bit digits. */
+#define DS_DEPTH_SORT_ONESWEEP 0
+#define DS_DEPTH_SORT_RADIX8 1
+#define DS_DEPTH_SORT_RADIX11 2
+ds_status ds_raster_set_depth_sort(ds_raster *r, int mode);
+ds_status ds_raster_get_depth_sort(ds_raster *r, int *mode);
+/* Times ds_raster_sort on `in` (reps runs after one warm-up, HIP events on s,
+ * synchronises) with every mode the raster takes and keeps the fastest (the
+ * current one unless another is at least 3% faster). mode (optional): the
+ * kept mode; ms (optional, 3 floats): ms per sort for each mode, -1 for a
+ * mode the raster does not take. The fitters' and the renderer's autotune
+ * calls this. */
+ds_status ds_raster_autotune_depth_sort(ds_raster *r, const ds_splat2d *in, int reps,
+                                        ds_stream_t s, int *mode, float *ms);
+/* The times of the last ds_raster_autotune_depth_sort (3 floats, -1: none). */
+ds_status ds_raster_depth_sort_times(ds_raster *r, float *ms);
+
+/* The forward up to compositing: tile assignment, depth and tile sorts,
+ * tile ranges and dispatch order, for the splats `in` (no image written).
+ * For timing the sort apart from the compositing; a later ds_raster_bwd or
+ * ds_raster_composite_* call sees these sorted pairs. No counterpart in the
+ * reference. */
+ds_status ds_raster_sort(ds_raster *r, const ds_splat2d *in, ds_stream_t s);
+
 /* ------------------------------------------------------------------ */
 /* Stand-alone sort (the radix sort inside ds_raster_fwd)              */
 /* ------------------------------------------------------------------ */

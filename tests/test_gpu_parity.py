@@ -17,6 +17,8 @@ from __future__ import annotations
 import json
 import os
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -256,6 +258,48 @@ def test_tile_dispatch_order_leaves_results_unchanged(scene):
     for out in outs[1:]:
         for k in ("rgb", "T", "last_ids", "grad_rgb", "flatten_ids", "tile_ranges"):
             assert np.array_equal(out[k], ref[k]), k
+        ok, mabs, _ = close(out["gs2"], ref["gs2"], atol=1e-7, rtol=1e-5)
+        assert ok, mabs
+
+
+DEPTH_SORT_CASES = {
+    # name: (seed, n, W, H, ls_mean): one scene per depth-sort size regime
+    "small": (3, 3000, 160, 96, -2.8),
+    "many": (4, 200000, 640, 360, -4.0),  # above the one-kernel limit (131072)
+}
+
+
+@pytest.mark.parametrize("case", sorted(DEPTH_SORT_CASES))
+def test_depth_sort_modes_give_identical_results(case):
+    """Every depth-sort mode the rasterizer takes (one-kernel 8-bit passes up
+    to 131072 splats, multi-kernel 8-bit and 11-bit passes) gives the same
+    sorted pairs, ranges, image, transmittance and last ids bit for bit, and
+    the same gradients up to the order of float sums; the sort-only entry
+    leaves the same pairs as a forward."""
+    seed, n, W, H, ls = DEPTH_SORT_CASES[case]
+    params, view, proj, cam = frustum_scene(seed, n, W, H, ls)
+    target = ds.target_image(seed, W, H, 32)
+    modes = [m for m in (0, 1, 2) if not (m == 0 and n > 131072)]
+    outs = []
+    for mode in modes:
+        st = Stages(params, n, 0, W, H, view, proj, cam, pix_per_thread=1)
+        ds.check(ds.lib().ds_raster_set_depth_sort(st.ras.h, mode), "depth sort")
+        st.activ_project_fwd()
+        ds.check(ds.lib().ds_raster_sort(st.ras.h, C.byref(st.s2), None), "ds_raster_sort")
+        ds.check(ds.lib().ds_synchronize(), "sync")
+        sorted_only = st.ras.state()
+        out = st.raster_fwd(target)
+        assert np.array_equal(sorted_only["flatten_ids"], out["flatten_ids"])
+        out["gs2"] = st.raster_bwd(out["grad_rgb"])
+        outs.append(out)
+    print(f"[parity] depth_sort_{case}: modes {modes}, n_isects {outs[0]['n_isects']}")
+    if n > 131072:
+        assert ds.lib().ds_raster_set_depth_sort(st.ras.h, 0) != 0  # refused above the limit
+    ref = outs[0]
+    for out in outs[1:]:
+        for k in ("rgb", "T", "last_ids", "grad_rgb", "flatten_ids", "tile_ranges", "keys"):
+            if k in ref:
+                assert np.array_equal(out[k], ref[k]), k
         ok, mabs, _ = close(out["gs2"], ref["gs2"], atol=1e-7, rtol=1e-5)
         assert ok, mabs
 
