@@ -845,7 +845,9 @@ ds_status launch_pair_reduce(const float4 *pair_rec, const uint32_t *slot_splat,
                              const uint32_t *nisect, const uint32_t *epoch, int64_t slot_cap,
                              const ds_splat2d &grad, bool grad_is_zero, hipStream_t s) {
     if (slot_cap <= 0) return DS_OK;
-    const int blocks = (int)std::min<int64_t>((slot_cap + 255) / 256, 4096);
+    // a wave per 64 slots up to 4096 waves, each then walking on (grid-stride):
+    // fewer idle workgroups when the pairs stay far below the cap
+    const int blocks = (int)std::min<int64_t>((slot_cap + 255) / 256, 1024);
     if (grad_is_zero)
         hipLaunchKernelGGL(k_pair_reduce<true>, dim3(blocks), dim3(256), 0, s, pair_rec,
                            slot_splat, nisect, epoch, grad);
