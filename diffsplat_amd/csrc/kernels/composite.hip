@@ -55,8 +55,8 @@ struct AlphaEval {
 // tested as o g < 1/255 since 0.99 > 1/255); g and og are then meaningless
 // (g may be inf) and callers select them away rather than multiplying by
 // zero.
-__device__ __forceinline__ bool eval_alpha(const float4 &A, const float4 &B, float px, float py,
-                                           AlphaEval &e) {
+__device__ __forceinline__ void eval_falloff(const float4 &A, const float4 &B, float px,
+                                             float py, AlphaEval &e) {
     const float dx = A.x - px;
     const float dy = A.y - py;
     e.u = fmaf(A.z, dx, A.w * dy);
@@ -64,7 +64,19 @@ __device__ __forceinline__ bool eval_alpha(const float4 &A, const float4 &B, flo
     e.p = fmaf(dx, e.u, dy * e.w);  // k dist2 = -sigma / ln 2
     e.g = __builtin_amdgcn_exp2f(e.p);
     e.og = B.y * e.g;
+}
+
+__device__ __forceinline__ bool eval_alpha(const float4 &A, const float4 &B, float px, float py,
+                                           AlphaEval &e) {
+    eval_falloff(A, B, px, py, e);
     return (e.p <= 0.f) & (e.og >= kAlphaThres);  // sigma >= 0
+}
+
+// The wave mask of eval_alpha's test (bit l: lane l's pixel passes), taken as
+// two ballots so the compiler keeps both comparisons in scalar masks.
+__device__ __forceinline__ uint64_t alpha_mask(const AlphaEval &e) {
+    return __builtin_amdgcn_ballot_w64(e.p <= 0.f) &
+           __builtin_amdgcn_ballot_w64(e.og >= kAlphaThres);
 }
 
 __device__ __forceinline__ float pixel_ndc_x(int j, int W) {
@@ -301,8 +313,8 @@ __device__ __forceinline__ int composite_front_to_back(
                 bool anyv = false;
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
-                    okm[k] = __builtin_amdgcn_ballot_w64(eval_alpha(A, B, q.px, q.py[k], e[k])) &
-                             ~dm[k];
+                    eval_falloff(A, B, q.px, q.py[k], e[k]);
+                    okm[k] = alpha_mask(e[k]) & ~dm[k];
                     anyv |= okm[k] != 0;
                 }
                 if (!anyv) continue;
