@@ -559,6 +559,29 @@ __device__ __forceinline__ void seg_scan_carry(float (&v)[9], int src, bool take
     }
 }
 
+// One wave's 64 consecutive slots: each lane's slot splat and record.
+struct SlotLoad {
+    uint32_t g, g_prev, g_next;
+    float4 r0, r1, r2;
+};
+
+__device__ __forceinline__ SlotLoad load_slots(const float4 *__restrict__ rec,
+                                               const uint32_t *__restrict__ slot_splat,
+                                               uint32_t base, uint32_t m, int lane) {
+    SlotLoad L;
+    const uint32_t p = base + lane;
+    const bool valid = p < m;
+    L.g = valid ? slot_splat[p] : 0xFFFFFFFFu;
+    // the splats of the slots just before and just after the wave's
+    L.g_prev = base > 0 ? slot_splat[base - 1] : 0xFFFFFFFFu;
+    L.g_next = base + kWave < m ? slot_splat[base + kWave] : 0xFFFFFFFEu;
+    const size_t q = valid ? 3 * (size_t)p : 0;
+    L.r0 = rec[q];
+    L.r1 = rec[q + 1];
+    L.r2 = rec[q + 2];
+    return L;
+}
+
 template <bool STORE>
 __global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ rec,
                                                      const uint32_t *__restrict__ slot_splat,
@@ -568,33 +591,32 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ 
     const int lane = __lane_id();
     const uint32_t wave = (blockIdx.x * 256 + threadIdx.x) / kWave;
     const uint32_t nwaves = gridDim.x * (256 / kWave);
+    const uint32_t stride = nwaves * kWave;
     float *const dst[9] = {grad.pos_x,  grad.pos_y,   grad.cov_xx,  grad.cov_xy, grad.cov_yy,
                            grad.color_r, grad.color_g, grad.color_b, grad.opacity};
-    for (uint32_t base = wave * kWave; base < m; base += nwaves * kWave) {  // wave-uniform
+    uint32_t base = wave * kWave;
+    if (base >= m) return;  // wave-uniform
+    // software-pipelined: the next slots' loads are in flight during this
+    // slots' scan
+    SlotLoad cur = load_slots(rec, slot_splat, base, m, lane);
+    for (;;) {  // wave-uniform
+        const uint32_t next = base + stride;
+        SlotLoad nxt;
+        if (next < m) nxt = load_slots(rec, slot_splat, next, m, lane);
         const uint32_t p = base + lane;
         const bool valid = p < m;
-        const uint32_t g = valid ? slot_splat[p] : 0xFFFFFFFFu;
-        // the splats of the slots just before and just after the wave's
-        const uint32_t g_prev = base > 0 ? slot_splat[base - 1] : 0xFFFFFFFFu;
-        const uint32_t g_next = base + kWave < m ? slot_splat[base + kWave] : 0xFFFFFFFEu;
+        const uint32_t g = cur.g;
         float v[9];
-#pragma unroll
-        for (int f = 0; f < 9; ++f) v[f] = 0.f;
-        if (valid) {
-            const float4 r0 = rec[3 * (size_t)p], r1 = rec[3 * (size_t)p + 1],
-                         r2 = rec[3 * (size_t)p + 2];
-            if (__float_as_uint(r2.y) == ep) {
-                v[0] = r0.x;
-                v[1] = r0.y;
-                v[2] = r0.z;
-                v[3] = r0.w;
-                v[4] = r1.x;
-                v[5] = r1.y;
-                v[6] = r1.z;
-                v[7] = r1.w;
-                v[8] = r2.x;
-            }
-        }
+        const bool on = valid && __float_as_uint(cur.r2.y) == ep;
+        v[0] = on ? cur.r0.x : 0.f;
+        v[1] = on ? cur.r0.y : 0.f;
+        v[2] = on ? cur.r0.z : 0.f;
+        v[3] = on ? cur.r0.w : 0.f;
+        v[4] = on ? cur.r1.x : 0.f;
+        v[5] = on ? cur.r1.y : 0.f;
+        v[6] = on ? cur.r1.z : 0.f;
+        v[7] = on ? cur.r1.w : 0.f;
+        v[8] = on ? cur.r2.x : 0.f;
         // lane 0's splat, read while every lane is active (lane 0 always is)
         const uint32_t g0 = (uint32_t)__builtin_amdgcn_readfirstlane(g);
         // runs of equal g are contiguous: lanes a < b hold the same splat
@@ -615,24 +637,28 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ 
         }
         const uint32_t gn = __shfl_down(g, 1, kWave);
         const bool run_end = valid && (lane == kWave - 1 || gn != g);
-        if (!run_end) continue;
-        // does the run start in this wave, and end in it?
-        const bool starts_before = g == g0 && g_prev == g;
-        const bool ends_after = lane == kWave - 1 && g_next == g;
-        if (starts_before || ends_after) {
+        if (run_end) {
+            // does the run start in an earlier wave's slots, or end in a later one's?
+            const bool starts_before = g == g0 && cur.g_prev == g;
+            const bool ends_after = lane == kWave - 1 && cur.g_next == g;
+            if (starts_before || ends_after) {
 #pragma unroll
-            for (int f = 0; f < 9; ++f)
-                if (v[f] != 0.f) atomicAdd(dst[f] + g, v[f]);
-        } else {
+                for (int f = 0; f < 9; ++f)
+                    if (v[f] != 0.f) atomicAdd(dst[f] + g, v[f]);
+            } else {
 #pragma unroll
-            for (int f = 0; f < 9; ++f)
-                if (v[f] != 0.f) {
-                    if constexpr (STORE)
-                        dst[f][g] = v[f];
-                    else
-                        dst[f][g] += v[f];
-                }
+                for (int f = 0; f < 9; ++f)
+                    if (v[f] != 0.f) {
+                        if constexpr (STORE)
+                            dst[f][g] = v[f];
+                        else
+                            dst[f][g] += v[f];
+                    }
+            }
         }
+        if (next >= m) break;
+        base = next;
+        cur = nxt;
     }
 }
 
@@ -857,9 +883,9 @@ ds_status launch_pair_reduce(const float4 *pair_rec, const uint32_t *slot_splat,
                              const uint32_t *nisect, const uint32_t *epoch, int64_t slot_cap,
                              const ds_splat2d &grad, bool grad_is_zero, hipStream_t s) {
     if (slot_cap <= 0) return DS_OK;
-    // a wave per 64 slots up to 4096 waves, each then walking on (grid-stride):
-    // fewer idle workgroups when the pairs stay far below the cap
-    const int blocks = (int)std::min<int64_t>((slot_cap + 255) / 256, 1024);
+    // a wave per 64 slots up to 16384 waves, each then walking on (grid-stride,
+    // the next slots' loads overlapping this slots' scan)
+    const int blocks = (int)std::min<int64_t>((slot_cap + 255) / 256, 4096);
     if (grad_is_zero)
         hipLaunchKernelGGL(k_pair_reduce<true>, dim3(blocks), dim3(256), 0, s, pair_rec,
                            slot_splat, nisect, epoch, grad);
