@@ -88,8 +88,7 @@ def parse():
                     help="steps of every instance in the batched-fitting leg")
     ap.add_argument("--batch-compare", action="store_true",
                     help="batched leg: also the per-instance stream and sequential runs "
-                         "(ds_batch); off by default: its one run with per-pair gradient "
-                         "records ended in a GPU memory fault whose cause is not found yet")
+                         "(ds_batch, 64 ds_fitters); off by default to keep the run short")
     ap.add_argument("--save-params", default=None, metavar="FILE",
                     help="write the parameter slab after the timed steps (.npy)")
     ap.add_argument("--start-params", default=None, metavar="FILE",

@@ -132,28 +132,16 @@ struct ChunkLds {
     uint32_t ticket;
 };
 
-// Loads chunk c and ranks its keys stably within their digit: rank[j] is the
-// number of earlier keys of the chunk with the same digit. Leaves the digit
-// counts in L.run and their exclusive prefix in L.start. Wave w owns the
+// Loads the cnt keys and values of the chunk starting at base: wave w owns the
 // contiguous keys [w * kSortChunk / NW, (w + 1) * kSortChunk / NW) of the
-// chunk and walks them 64 at a time in order, matching the RB digit bits
-// across the wave with ballots and keeping a running count per digit in its
-// own LDS row (lanes of one digit share one first lane, so the row's words
-// written in one step are distinct); the waves' counts are then scanned per
-// digit. Three barriers per chunk (four with more digits than threads).
-template <typename K, int RB = kRadixBits>
-__device__ __forceinline__ void chunk_rank(const K *__restrict__ kin,
+// chunk, item j of a lane at w * kSortChunk / NW + 64 j + lane (zero past cnt).
+template <typename K>
+__device__ __forceinline__ void chunk_load(const K *__restrict__ kin,
                                            const uint32_t *__restrict__ vin, uint32_t base,
-                                           uint32_t cnt, int shift, K (&key)[kSortItems],
-                                           uint32_t (&val)[kSortItems],
-                                           uint32_t (&rank)[kSortItems], ChunkLds<K, RB> &L) {
-    constexpr int NW = ChunkLds<K, RB>::NW;
-    constexpr int R = ChunkLds<K, RB>::R;
-    constexpr int DPT = R / kSortThreads;  // digits per thread
-    constexpr int kWaveKeys = kSortChunk / NW;
-    const int tid = threadIdx.x;
-    const int w = tid / kWave, lane = __lane_id();
-    const uint64_t lt = lanemask_lt();
+                                           uint32_t cnt, K (&key)[kSortItems],
+                                           uint32_t (&val)[kSortItems]) {
+    constexpr int kWaveKeys = kSortChunk / (kSortThreads / kWave);
+    const int w = threadIdx.x / kWave, lane = __lane_id();
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
         const uint32_t e = (uint32_t)(w * kWaveKeys + j * kWave + lane);
@@ -165,6 +153,28 @@ __device__ __forceinline__ void chunk_rank(const K *__restrict__ kin,
             val[j] = 0;
         }
     }
+}
+
+// Ranks the loaded keys of a chunk (chunk_load) stably within their digit:
+// rank[j] is the number of earlier keys of the chunk with the same digit.
+// Leaves the digit counts in L.run and their exclusive prefix in L.start.
+// Each wave walks its keys 64 at a time in order, matching the RB digit bits
+// across the wave with ballots and keeping a running count per digit in its
+// own LDS row (lanes of one digit share one first lane, so the row's words
+// written in one step are distinct); the waves' counts are then scanned per
+// digit. Four barriers per chunk.
+template <typename K, int RB = kRadixBits>
+__device__ __forceinline__ void chunk_rank_loaded(uint32_t cnt, int shift,
+                                                  const K (&key)[kSortItems],
+                                                  uint32_t (&rank)[kSortItems],
+                                                  ChunkLds<K, RB> &L) {
+    constexpr int NW = ChunkLds<K, RB>::NW;
+    constexpr int R = ChunkLds<K, RB>::R;
+    constexpr int DPT = R / kSortThreads;  // digits per thread
+    constexpr int kWaveKeys = kSortChunk / NW;
+    const int tid = threadIdx.x;
+    const int w = tid / kWave, lane = __lane_id();
+    const uint64_t lt = lanemask_lt();
 #pragma unroll
     for (int i = 0; i < DPT; ++i)
 #pragma unroll
@@ -200,7 +210,8 @@ __device__ __forceinline__ void chunk_rank(const K *__restrict__ kin,
         }
         L.run[d] = run;
     }
-    if constexpr (DPT > 1) __syncthreads();
+    // the prefixes of every digit are written before any thread reads them
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) rank[j] += L.w[w][digit_of<K, RB>(key[j], shift)];
     if constexpr (DPT == 1) {
@@ -219,6 +230,17 @@ __device__ __forceinline__ void chunk_rank(const K *__restrict__ kin,
             run += x[i];
         }
     }
+}
+
+// chunk_load then chunk_rank_loaded.
+template <typename K, int RB = kRadixBits>
+__device__ __forceinline__ void chunk_rank(const K *__restrict__ kin,
+                                           const uint32_t *__restrict__ vin, uint32_t base,
+                                           uint32_t cnt, int shift, K (&key)[kSortItems],
+                                           uint32_t (&val)[kSortItems],
+                                           uint32_t (&rank)[kSortItems], ChunkLds<K, RB> &L) {
+    chunk_load(kin, vin, base, cnt, key, val);
+    chunk_rank_loaded<K, RB>(cnt, shift, key, rank, L);
 }
 
 // Reorders the ranked chunk by digit in LDS, then writes each run of equal
@@ -406,6 +428,13 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     unsigned long long wdig = 0;  // weight of all keys of digit tid
     if constexpr (OFF)
         for (int r = 0; r < po.whist_reps; ++r) wdig += po.whist[(size_t)r * kRadix + tid];
+    // The first ticket is usually blockIdx.x: that chunk's keys load while the
+    // ticket is in flight (and load again if the ticket differs).
+    const uint32_t guess = blockIdx.x;
+    uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+    if (guess < nch)
+        chunk_load(kin, vin, guess * kSortChunk, min((uint32_t)kSortChunk, n - guess * kSortChunk),
+                   key, val);
     if (tid == 0) L.ticket = atomicAdd(lb.ticket, 1u);
     __syncthreads();
     // first position of each digit in the output
@@ -422,6 +451,9 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     }
     for (bool first = true;; first = false) {
         if (!first) {
+            // a grid of at least one workgroup per chunk: the first tickets
+            // covered every chunk
+            if (gridDim.x >= nch) break;
             if (tid == 0) L.ticket = atomicAdd(lb.ticket, 1u);
             __syncthreads();
         }
@@ -429,8 +461,8 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
         if (c >= nch) break;  // uniform
         const uint32_t base = c * kSortChunk;
         const uint32_t cnt = min((uint32_t)kSortChunk, n - base);
-        uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
-        chunk_rank(kin, vin, base, cnt, shift, key, val, rank, L);
+        if (!first || c != guess) chunk_load(kin, vin, base, cnt, key, val);
+        chunk_rank_loaded(cnt, shift, key, rank, L);
         uint64_t *col = lb.status + (size_t)c * kRadix + tid;
         const uint32_t mine = L.run[tid];
         uint32_t excl = 0;
