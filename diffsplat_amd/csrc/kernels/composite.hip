@@ -104,13 +104,13 @@ __device__ __forceinline__ float pair_step(float a, float b, bool hi) {
     return hi ? sb : sa;
 }
 
-// Sum over the 64 lanes of a wave of nine per-lane values v[0..8], all lanes
-// active. Four DPP steps inside each 16-lane row (partners lane^1, lane^2,
-// lane-4 and lane-8 in the row) leave one field per lane, then two
-// permlane swaps add the four rows. Result: lanes 0, 8, 4, 2, 10, 1, 9, 5, 3
-// of every row hold the sums of fields 0..8 (field_of_lane); the other lanes
+// Sums over each 16-lane row of a wave of nine per-lane values v[0..8], all
+// lanes active. Four DPP steps inside each row (partners lane^1, lane^2,
+// lane-4 and lane-8 in the row) each halve the fields a lane holds
+// (9 -> 5 -> 3 -> 2 -> 1). Result: lanes 0, 8, 4, 2, 10, 1, 9, 5, 3 of every
+// row hold their row's sums of fields 0..8 (field_of_row_lane); the other lanes
 // hold garbage.
-__device__ __forceinline__ float wave_sum9_transposed(const float (&v)[9], int lane) {
+__device__ __forceinline__ float row_sum9_transposed(const float (&v)[9], int lane) {
     const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8;
     constexpr int kXor1 = 0xB1, kXor2 = 0x4E, kRor4 = 0x124, kRor8 = 0x128;
     const float r0 = pair_step<kXor1>(v[0], v[5], b0);
@@ -123,7 +123,13 @@ __device__ __forceinline__ float wave_sum9_transposed(const float (&v)[9], int l
     const float s2 = r2 + dpp<kXor2>(r2);      // valid where b1 == 0
     const float t0 = pair_step<kRor4>(s0, s2, b2);
     const float t1 = s1 + dpp<kRor4>(s1);      // valid where b2 == 0
-    float z = pair_step<kRor8>(t0, t1, b3);
+    return pair_step<kRor8>(t0, t1, b3);
+}
+
+// The same over all 64 lanes: the row sums, then two permlane swaps add the
+// four rows, so every row holds the wave's sums.
+__device__ __forceinline__ float wave_sum9_transposed(const float (&v)[9], int lane) {
+    float z = row_sum9_transposed(v, lane);
     // rows 0+1 and 2+3, then the two halves
     const auto q = __builtin_amdgcn_permlane16_swap(__float_as_uint(z), __float_as_uint(z), false,
                                                     false);
@@ -133,11 +139,10 @@ __device__ __forceinline__ float wave_sum9_transposed(const float (&v)[9], int l
     return __uint_as_float(h[0]) + __uint_as_float(h[1]);
 }
 
-// Field whose wave sum a lane holds after wave_sum9_transposed (-1: none).
-// Every row holds the same sums; row 0 issues the atomics.
-__device__ __forceinline__ int field_of_lane(int lane) {
-    if (lane >= 16) return -1;
-    switch (lane) {
+// Field whose row sum a lane holds after row_sum9_transposed (-1: none); row
+// 0's lanes hold the same fields of the wave sum after wave_sum9_transposed.
+__device__ __forceinline__ int field_of_row_lane(int lane) {
+    switch (lane & 15) {
         case 0: return 0;
         case 8: return 1;
         case 4: return 2;
@@ -149,6 +154,19 @@ __device__ __forceinline__ int field_of_lane(int lane) {
         case 3: return 8;
         default: return -1;
     }
+}
+
+// The factor that turns a field's sum into the gradient: the sums of the
+// position and covariance fields are taken with m = -2 dL/d dist2 and the
+// record's scaled u, w (k = kFalloffScale), so position x, y get -1/k
+// (d dist2/d(x, y) = 2 (u, w) / k), xx and yy get 1/(2 k^2) and the
+// symmetric xy 1/k^2; colour and opacity sums are the gradients.
+__device__ __forceinline__ float field_scale(int f) {
+    constexpr float kInvK = 1.f / kFalloffScale;
+    return (f == 0 || f == 1)   ? -kInvK
+           : (f == 2 || f == 4) ? 0.5f * kInvK * kInvK
+           : f == 3             ? kInvK * kInvK
+                                : 1.f;
 }
 
 // The image a tile belongs to: its size, tiles per row, the tile's index in
@@ -172,6 +190,9 @@ __device__ __forceinline__ TileImage tile_image(const CompositeArgs &a, int tile
 }
 
 // Pixels of one thread: column j (NDC px), P rows (NDC py), inside the image.
+// Wave w owns rows [4 P w, 4 P (w + 1)) of the tile, as P bands of 16 x 4
+// pixels: pixel k of lane l lies in band k, row 4 P w + 4 k + l / 16. The
+// compositing loops test, skip and record reach per band.
 template <int P>
 struct TilePixels {
     int j;
@@ -187,9 +208,10 @@ __device__ __forceinline__ TilePixels<P> tile_pixels(const TileImage &im, int ti
     const int trow = im.local / im.ntx, tcol = im.local % im.ntx;
     q.j = tcol * kTile + (tid & 15);
     q.px = pixel_ndc_x(q.j, im.W);
+    const int w = tid / kWave, lrow = (tid & (kWave - 1)) >> 4;
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        q.row[k] = trow * kTile + (tid >> 4) * P + k;
+        q.row[k] = trow * kTile + w * 4 * P + 4 * k + lrow;
         q.py[k] = pixel_ndc_y(q.row[k], im.H);
         q.inside[k] = q.row[k] < im.H && q.j < im.W;
     }
@@ -227,15 +249,17 @@ __device__ __forceinline__ bool reaches_band(const float4 &A, const float4 &B, f
     return !(m > B.z);
 }
 
-// Per wave, one bit per splat of the tile's list for its first 64 *
-// kSkipWords splats: did the forward find a pixel of this wave that the splat reaches
-// (alpha >= 1/255, pixel not yet finished)? The backward's test for the
-// same (wave, splat) is the same test (a pixel's last contributor is where
-// it finished), so it skips the cleared bits without evaluating them.
+// Per band (16 x 4 pixels of a wave), one bit per splat of the tile's list
+// for its first 64 * kSkipWords splats: did the forward find a pixel of the
+// band that the splat reaches (alpha >= 1/255, pixel not yet finished)? The
+// backward's test for the same (band, splat) is the same test (a pixel's
+// last contributor is where it finished), so it skips the cleared bits
+// without evaluating them, and evaluates a splat only on the bands it
+// reaches.
 constexpr int kSkipWords = 64;
 
 // Front to back over the tile's list (compositeFwdKernel). With BITS the
-// per-wave reach bits are recorded in s_bits. Every exit is wave-uniform (a
+// per-band reach bits are recorded in s_bits. Every exit is wave-uniform (a
 // ballot), so the loop runs on scalar branches; lanes whose pixels are done
 // ride along adding exact zeros. A wave whose pixels are all done skips the
 // batch, and leaves it once the last of them finishes: only a splat that
@@ -244,8 +268,8 @@ constexpr int kSkipWords = 64;
 template <int P, bool BITS>
 __device__ __forceinline__ int composite_front_to_back(
     const CompositeArgs &a, int2 range, const TilePixels<P> &q, float4 *sA, float4 *sB,
-    float4 *sC, uint64_t (*s_bits)[kSkipWords], float (&T)[P], float (&cr)[P], float (&cg)[P],
-    float (&cb)[P], int32_t (&cur)[P]) {
+    float4 *sC, uint64_t (*s_bits)[P][kSkipWords], float (&T)[P], float (&cr)[P],
+    float (&cg)[P], float (&cb)[P], int32_t (&cur)[P]) {
     constexpr int NT = kTilePix / P;
     const int tid = threadIdx.x, w = tid / kWave;
     // Finished pixels as wave masks (scalar registers): bit l of dm[k] is
@@ -266,13 +290,16 @@ __device__ __forceinline__ int composite_front_to_back(
         for (int k = 0; k < P; ++k) live |= dm[k] != ~0ull;
         return live;
     };
-    // The wave's band of pixel centres (NDC): lane 0 holds its first column
+    // The bands' pixel centres (NDC): lane 0 holds each band's first column
     // and top row, lane 63 its last column and bottom row.
     const float wx0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.px), 0));
     const float wx1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.px), kWave - 1));
-    const float wy1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.py[0]), 0));
-    const float wy0 =
-        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.py[P - 1]), kWave - 1));
+    float wy0[P], wy1[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        wy1[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.py[k]), 0));
+        wy0[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.py[k]), kWave - 1));
+    }
     int loaded = -1;
     for (int b = range.x; b < range.y; b += NT) {
         if (__syncthreads_count(wave_live()) == 0) break;
@@ -290,39 +317,52 @@ __device__ __forceinline__ int composite_front_to_back(
         // 64 splats at a time: one word of reach bits (NT is a multiple of 64)
         for (int t0 = 0; t0 < bs && !finished; t0 += 64) {
             const int t1 = min(bs, t0 + 64);
-            // Candidates: the splats whose reach ellipse meets the band (no
-            // pixel outside it can pass the alpha test); the others are
-            // skipped without evaluation. Lane l tests splat t0 + l.
-            uint64_t cand;
+            // Candidates per band: the splats whose reach ellipse meets the
+            // band (no pixel outside it can pass the alpha test); a splat is
+            // evaluated only on those bands. Lane l tests splat t0 + l.
+            uint64_t cand[P], any = 0;
             {
                 const int t = t0 + __lane_id();
-                bool hit = false;
-                if (t < t1) hit = reaches_band(sA[t], sB[t], wx0, wx1, wy0, wy1);
-                cand = __ballot(hit);
+                const bool in = t < t1;
+                const float4 A = sA[in ? t : t0], B = sB[in ? t : t0];
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    // a band whose pixels are all done has no candidates
+                    cand[k] = __ballot(in && reaches_band(A, B, wx0, wx1, wy0[k], wy1[k]));
+                    cand[k] = dm[k] == ~0ull ? 0 : cand[k];
+                    any |= cand[k];
+                }
             }
-            uint64_t reach = 0;
-            while (cand) {
-                const int s = (int)__builtin_ctzll(cand);
-                cand &= cand - 1;
+            uint64_t reach[P];
+#pragma unroll
+            for (int k = 0; k < P; ++k) reach[k] = 0;
+            while (any) {
+                const int s = (int)__builtin_ctzll(any);
+                any &= any - 1;
                 const int t = t0 + s;
                 const float4 A = sA[t], B = sB[t];
-                // Pass 1: which of this lane's pixels the splat reaches; a
-                // wave whose pixels it reaches none of skips the rest.
+                // Pass 1: which pixels of the candidate bands the splat
+                // reaches; a wave whose pixels it reaches none of skips the
+                // rest.
                 AlphaEval e[P];
                 uint64_t okm[P];
                 bool anyv = false;
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
-                    eval_falloff(A, B, q.px, q.py[k], e[k]);
-                    okm[k] = alpha_mask(e[k]) & ~dm[k];
+                    okm[k] = 0;
+                    if ((cand[k] >> s) & 1) {
+                        eval_falloff(A, B, q.px, q.py[k], e[k]);
+                        okm[k] = alpha_mask(e[k]) & ~dm[k];
+                    }
                     anyv |= okm[k] != 0;
                 }
                 if (!anyv) continue;
-                reach |= 1ull << s;
                 const float4 C = sC[t];
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
-                    // A skipped splat gets alpha 0, which leaves T and the
+                    if (okm[k] == 0) continue;  // band k unchanged
+                    reach[k] |= 1ull << s;
+                    // A skipped pixel gets alpha 0, which leaves T and the
                     // colour exactly unchanged (T * (1 - 0) == T).
                     const bool ok = __builtin_amdgcn_inverse_ballot_w64(okm[k]);
                     const float alpha = ok ? e[k].alpha() : 0.f;
@@ -333,15 +373,25 @@ __device__ __forceinline__ int composite_front_to_back(
                     cur[k] = ok ? b + t : cur[k];
                     T[k] = T[k] * (1.f - alpha);
                     dm[k] |= __builtin_amdgcn_ballot_w64(T[k] <= kTThres);
+                    if (dm[k] == ~0ull) cand[k] = 0;  // band k done
                 }
                 if (!wave_live()) {
                     finished = true;
                     break;
                 }
+                if constexpr (P > 1) {  // drop the candidates of bands done meanwhile
+                    uint64_t left = 0;
+#pragma unroll
+                    for (int k = 0; k < P; ++k) left |= cand[k];
+                    any &= left;
+                }
             }
             if constexpr (BITS) {
                 const int word = (b - range.x + t0) >> 6;
-                if (word < kSkipWords && __lane_id() == 0) s_bits[w][word] = reach;
+                if (word < kSkipWords && __lane_id() == 0) {
+#pragma unroll
+                    for (int k = 0; k < P; ++k) s_bits[w][k][word] = reach[k];
+                }
             }
         }
     }
@@ -352,13 +402,14 @@ __device__ __forceinline__ int composite_front_to_back(
 // (compositeBwdKernel), from the final transmittance T, last index lst and
 // dL/dC (gr, gg, gb) of each pixel, in the forward's batches of NT splats.
 // The batch starting at list index `loaded` is already in sA/sB/sC (-1:
-// none). With BITS, (wave, splat) pairs whose reach bit the forward
-// cleared are skipped without evaluation.
+// none). A splat is evaluated band by band (16 x 4 pixels), and its nine
+// sums are reduced once over the wave's bands; with BITS, (band, splat)
+// pairs whose reach bit the forward cleared are skipped without evaluation.
 template <int GRAD_MODE, int P, bool BITS>
 __device__ __forceinline__ void composite_back_to_front(
     const CompositeArgs &a, int2 range, const TilePixels<P> &q, float4 *sA, float4 *sB,
-    float4 *sC, float (*sG)[kTilePix / P > kWave ? kTilePix / P : 1], uint8_t *s_hit,
-    int *s_last, const uint64_t (*s_bits)[kSkipWords], int loaded, float (&T)[P],
+    float4 *sC, float (*sG)[kTilePix / P > kWave ? kTilePix / P : 1], int *s_last,
+    const uint64_t (*s_bits)[P][kSkipWords], int loaded, float (&T)[P],
     const int32_t (&lst)[P], const float (&gr)[P], const float (&gg)[P], const float (&gb)[P]) {
     constexpr int NT = kTilePix / P;
     constexpr int NW = NT / kWave;
@@ -382,25 +433,17 @@ __device__ __forceinline__ void composite_back_to_front(
         if (tid == 0) *s_last = -1;
 #pragma unroll
         for (int f = 0; f < 9; ++f) sG[f][tid] = 0.f;
-        s_hit[tid] = 0;
         __syncthreads();
         if (__lane_id() == 0) atomicMax(s_last, wave_last);
         __syncthreads();
         block_last = __builtin_amdgcn_readfirstlane(*s_last);
     }
-    // The field this lane holds after the transposed reduction, and the
-    // factor that turns the wave sum into the gradient: the sums of the
-    // position and covariance fields are taken with m = -2 dL/d dist2 (see
-    // below) and the record's scaled u, w, so position x, y get -1/k
-    // (d dist2/d(x, y) = 2 (u, w) / k), xx and yy get 1/(2 k^2) and the
-    // symmetric xy 1/k^2.
+    // The field this lane holds after the transposed reduction (of its row,
+    // or of the wave in row 0), and the factor that turns the sum into the
+    // gradient (field_scale).
     const int lane = __lane_id();
-    const int my_field = field_of_lane(lane);
-    constexpr float kInvK = 1.f / kFalloffScale;
-    const float my_scale = (my_field == 0 || my_field == 1) ? -kInvK
-                           : (my_field == 2 || my_field == 4) ? 0.5f * kInvK * kInvK
-                           : my_field == 3                    ? kInvK * kInvK
-                                                              : 1.f;
+    const int my_field = field_of_row_lane(lane);
+    const float my_scale = field_scale(my_field);
     const int end = min(range.y, block_last + 1);
     if (end <= range.x) return;  // workgroup-uniform
     for (int bb = range.x + (end - 1 - range.x) / NT * NT; bb >= range.x; bb -= NT) {
@@ -422,46 +465,51 @@ __device__ __forceinline__ void composite_back_to_front(
         const int tmax = min(bs - 1, wave_last - bb);
         for (int c0 = tmax < 0 ? -64 : tmax & ~63; c0 >= 0; c0 -= 64) {
             const int lim = tmax - c0;  // highest slot of the chunk in range
-            uint64_t mask = lim >= 63 ? ~0ull : (2ull << lim) - 1ull;
-            if constexpr (BITS) {
-                const int word = (bb - range.x + c0) >> 6;
-                if (word < kSkipWords) {
-                    const uint64_t v = s_bits[w][word];
-                    mask &= ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
-                            __builtin_amdgcn_readfirstlane((uint32_t)v);
+            const uint64_t mask = lim >= 63 ? ~0ull : (2ull << lim) - 1ull;
+            // per band: the chunk's slots to visit (the forward's reach bits
+            // where recorded), and their union
+            uint64_t mk[P], any = 0;
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                mk[k] = mask;
+                if constexpr (BITS) {
+                    const int word = (bb - range.x + c0) >> 6;
+                    if (word < kSkipWords) {
+                        const uint64_t v = s_bits[w][k][word];
+                        mk[k] &= ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32))
+                                  << 32) |
+                                 __builtin_amdgcn_readfirstlane((uint32_t)v);
+                    }
                 }
+                any |= mk[k];
             }
-            while (mask) {
-                const int sl = 63 - (int)__builtin_clzll(mask);
-                mask &= ~(1ull << sl);
+            while (any) {
+                const int sl = 63 - (int)__builtin_clzll(any);
+                any &= ~(1ull << sl);
                 const int t = c0 + sl;
                 const int idx = bb + t;
-                const float4 A = sA[t], B = sB[t];
-                // Pass 1: which of this lane's pixels the splat reaches. A wave
-                // whose pixels it reaches none of skips the rest.
-                AlphaEval e[P];
-                bool ok[P];
-                bool anyv = false;
-#pragma unroll
-                for (int k = 0; k < P; ++k) {
-                    ok[k] = eval_alpha(A, B, q.px, q.py[k], e[k]) & (idx <= lst[k]);
-                    anyv |= ok[k];
-                }
-                if (__ballot(anyv) == 0) continue;
-                // Pass 2, without branches: a skipped pixel gets alpha 0 and gate
-                // false, which adds exact zeros and keeps T and acc unchanged.
-                const float4 C = sC[t];
+                const float4 A = sA[t], B = sB[t], C = sC[t];
                 float v[9];
 #pragma unroll
                 for (int f = 0; f < 9; ++f) v[f] = 0.f;
+                bool anyk = false;
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
-                    const float alpha = ok[k] ? e[k].alpha() : 0.f;
+                    if (!((mk[k] >> sl) & 1)) continue;  // band k not reached
+                    // which of the band's pixels the splat reaches; a band it
+                    // reaches none of is skipped
+                    AlphaEval e;
+                    const bool ok = eval_alpha(A, B, q.px, q.py[k], e) & (idx <= lst[k]);
+                    if (__ballot(ok) == 0) continue;
+                    anyk = true;
+                    // without branches: a skipped pixel gets alpha 0 and gate
+                    // false, which adds exact zeros and keeps T and acc unchanged
+                    const float alpha = ok ? e.alpha() : 0.f;
                     // 1 / max(1 - alpha, 1e-6) of the reference: alpha <= 0.99, so the
                     // floor never applies
                     const float ra = __builtin_amdgcn_rcpf(1.f - alpha);
                     // transmittance in front of the splat
-                    const float Tb = ok[k] ? T[k] * ra : T[k];
+                    const float Tb = ok ? T[k] * ra : T[k];
                     const float fac = alpha * Tb;
                     v[5] = fmaf(fac, gr[k], v[5]);
                     v[6] = fmaf(fac, gg[k], v[6]);
@@ -472,48 +520,65 @@ __device__ __forceinline__ void composite_back_to_front(
                         v_alpha = Tb * (cgk - acc[k]);
                     else
                         v_alpha = Tb * cgk - ra * acc[k];
-                    const bool gate = ok[k] & (e[k].og <= kMaxAlpha);  // alpha below its clamp
-                    v[8] = fmaf(gate ? e[k].g : 0.f, v_alpha, v[8]);
+                    const bool gate = ok & (e.og <= kMaxAlpha);  // alpha below its clamp
+                    v[8] = fmaf(gate ? e.g : 0.f, v_alpha, v[8]);
                     // m = -2 dL/d dist2 (dL/d dist2 = -o g v_alpha / 2); the
                     // per-field factors are applied once, after the reduction
-                    const float m = gate ? e[k].og * v_alpha : 0.f;
-                    const float pu = m * e[k].u, pw = m * e[k].w;
+                    const float m = gate ? e.og * v_alpha : 0.f;
+                    const float pu = m * e.u, pw = m * e.w;
                     v[0] += pu;
                     v[1] += pw;
-                    v[2] = fmaf(pu, e[k].u, v[2]);
-                    v[3] = fmaf(pu, e[k].w, v[3]);
-                    v[4] = fmaf(pw, e[k].w, v[4]);
+                    v[2] = fmaf(pu, e.u, v[2]);
+                    v[3] = fmaf(pu, e.w, v[3]);
+                    v[4] = fmaf(pw, e.w, v[4]);
                     acc[k] = fmaf(fac, cgk, acc[k]);
                     T[k] = Tb;
                 }
-                const float sum = my_scale * wave_sum9_transposed(v, lane);
+                if (!anyk) continue;
                 if constexpr (NW == 1) {
                     // the wave's sums are the tile's: nine lanes of row 0
                     // store the fields, lane 6 the epoch tag
+                    const float sum = my_scale * wave_sum9_transposed(v, lane);
                     const uint32_t g = (uint32_t)__builtin_amdgcn_readfirstlane(__float_as_uint(B.w));
                     const uint2 pb = a.pair_base[g];
                     float *r = reinterpret_cast<float *>(a.pair_rec +
                                                          3 * (size_t)(pb.x + trow * pb.y + tcol));
-                    if (my_field >= 0) r[my_field] = sum;
+                    if (lane < 16 && my_field >= 0) r[my_field] = sum;
                     if (lane == 6) r[9] = __uint_as_float(ep);
                 } else {
-                    if (my_field >= 0) atomicAdd(&sG[my_field][t], sum);
-                    if (lane == 0) s_hit[t] = 1;
+                    // the tile's waves add their sums in LDS (nine lanes of
+                    // row 0, one field each); the field factors are applied
+                    // when the slot's record is written. (Adding the four
+                    // rows' sums there instead of shuffling them together
+                    // was 40-50% slower: same-address LDS atomics serialise.)
+                    const float sum = wave_sum9_transposed(v, lane);
+                    if (lane < 16 && my_field >= 0) atomicAdd(&sG[my_field][t], sum);
                 }
             }
         }
         if constexpr (NW > 1) {
             __syncthreads();
-            if (tid < bs && s_hit[tid]) {  // the slot's record: one thread's
-                const uint32_t g = __float_as_uint(sB[tid].w);
-                const uint2 pb = a.pair_base[g];
-                float4 *r = a.pair_rec + 3 * (size_t)(pb.x + trow * pb.y + tcol);
-                r[0] = make_float4(sG[0][tid], sG[1][tid], sG[2][tid], sG[3][tid]);
-                r[1] = make_float4(sG[4][tid], sG[5][tid], sG[6][tid], sG[7][tid]);
-                r[2] = make_float4(sG[8][tid], __uint_as_float(ep), 0.f, 0.f);
+            if (tid < bs) {  // the slot's record: one thread's
+                float gs[9];
+                bool hit = false;
 #pragma unroll
-                for (int f = 0; f < 9; ++f) sG[f][tid] = 0.f;
-                s_hit[tid] = 0;
+                for (int f = 0; f < 9; ++f) {
+                    gs[f] = sG[f][tid];
+                    hit |= gs[f] != 0.f;
+                }
+                // A slot no pixel reached keeps its sums at zero and its
+                // record unwritten (an older epoch, which counts as zero).
+                if (hit) {
+                    const uint32_t g = __float_as_uint(sB[tid].w);
+                    const uint2 pb = a.pair_base[g];
+                    float4 *r = a.pair_rec + 3 * (size_t)(pb.x + trow * pb.y + tcol);
+                    r[0] = make_float4(gs[0] * field_scale(0), gs[1] * field_scale(1),
+                                       gs[2] * field_scale(2), gs[3] * field_scale(3));
+                    r[1] = make_float4(gs[4] * field_scale(4), gs[5], gs[6], gs[7]);
+                    r[2] = make_float4(gs[8], __uint_as_float(ep), 0.f, 0.f);
+#pragma unroll
+                    for (int f = 0; f < 9; ++f) sG[f][tid] = 0.f;
+                }
             }
         }
     }
@@ -739,7 +804,6 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
     constexpr int GW = NW > 1 ? NT : 1;
     __shared__ float4 sA[NT], sB[NT], sC[NT];
     __shared__ float sG[9][GW];
-    __shared__ uint8_t s_hit[GW];
     __shared__ int s_last;
     const int tile = tile_of_block(a);
     const int2 range = a.ranges[tile];
@@ -759,13 +823,13 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
         gg[k] = q.inside[k] ? gout[3 * pix + 1] : 0.f;
         gb[k] = q.inside[k] ? gout[3 * pix + 2] : 0.f;
     }
-    composite_back_to_front<GRAD_MODE, P, false>(a, range, q, sA, sB, sC, sG, s_hit, &s_last,
+    composite_back_to_front<GRAD_MODE, P, false>(a, range, q, sA, sB, sC, sG, &s_last,
                                                  nullptr, -1, T, lst, gr, gg, gb);
 }
 
 // Forward with the fused MSE, then the backward of the same tile in the same
 // workgroup: the per-pixel state (T, last index, dL/dC) stays in registers
-// and the backward skips the (wave, splat) pairs the forward found unreached.
+// and the backward skips the (band, splat) pairs the forward found unreached.
 template <int GRAD_MODE, int P, bool MULTI>
 __global__ __launch_bounds__(kTilePix / P) void k_composite_fused(
     CompositeArgs a, float *__restrict__ out_rgb, float *__restrict__ out_T,
@@ -776,10 +840,9 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fused(
     constexpr int GW = NW > 1 ? NT : 1;
     __shared__ float4 sA[NT], sB[NT], sC[NT];
     __shared__ float sG[9][GW];
-    __shared__ uint8_t s_hit[GW];
     __shared__ float s_red[NW];
     __shared__ int s_last;
-    __shared__ uint64_t s_bits[NW][kSkipWords];
+    __shared__ uint64_t s_bits[NW][P][kSkipWords];
     const int tile = tile_of_block(a);
     const TileImage im = tile_image<MULTI>(a, tile, inv_npix_one);
     const TilePixels<P> q = tile_pixels<P>(im, threadIdx.x);
@@ -794,7 +857,7 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fused(
     int32_t lst[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) lst[k] = q.inside[k] ? cur[k] : -1;
-    composite_back_to_front<GRAD_MODE, P, true>(a, range, q, sA, sB, sC, sG, s_hit, &s_last,
+    composite_back_to_front<GRAD_MODE, P, true>(a, range, q, sA, sB, sC, sG, &s_last,
                                                 s_bits, loaded, T, lst, gr, gg, gb);
 }
 
