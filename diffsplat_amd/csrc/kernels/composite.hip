@@ -489,9 +489,11 @@ __device__ __forceinline__ void composite_back_to_front(
                 const int t = c0 + sl;
                 const int idx = bb + t;
                 const float4 A = sA[t], B = sB[t], C = sC[t];
+                // sums start at -0, the identity of float addition (+0 is not:
+                // +0 + -0 = +0), so the first band's terms need no add
                 float v[9];
 #pragma unroll
-                for (int f = 0; f < 9; ++f) v[f] = 0.f;
+                for (int f = 0; f < 9; ++f) v[f] = -0.f;
                 bool anyk = false;
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
