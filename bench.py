@@ -39,21 +39,32 @@ PEAK_HBM_GBS = 8000.0
 # backward (falloff, alpha, the colour / alpha / conic gradients).
 FWD_FLOP_PER_EVAL = 24.0
 BWD_FLOP_PER_EVAL = 60.0
-# HBM bytes of one compositing forward + backward (the roofline's kernels,
-# k_pair_reduce included) at the final C1 state, per launch shape
-# "fwd,bwd" (pixels per thread), from rocprofv3 FETCH_SIZE and WRITE_SIZE
-# passes, each over a short eager run started from the slab a 2020-step fit
-# saved (--save-params / --start-params), averaged over the 20 roofline
-# repetitions at its end and scaled by a 256 MiB copy (fetch x2.000, write
-# x1.000): see profiles/fit_720p_60k_counters.md. Counters cannot be read
-# live here.
-COMPOSITE_TRAFFIC = {  # shape: (read bytes, write bytes)
-    "1,1": (118.67e6, 66.31e6),
-    "2,2": (129.86e6, 66.37e6),
-    "1,2": (143.22e6, 55.23e6),
+# HBM bytes of one compositing forward + backward (the roofline's kernels:
+# k_composite_fused and k_pair_reduce, the API's accumulating form) per
+# launch shape "fwd,bwd" (pixels per thread) at two C1 fit states, from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE passes averaged over the 20 roofline
+# repetitions at the end of a short eager run, scaled by a 256 MiB copy
+# (fetch x2.000, write x1.000): see profiles/fit_720p_60k_counters.md.
+# Counters cannot be read live here; the line reports the state whose pair
+# count is nearest the measured one.
+COMPOSITE_TRAFFIC = {  # state: (pairs, {shape: (read bytes, write bytes)}, how it was reached)
+    "early": (1327660, {"1,1": (165.00e6, 73.56e6)},
+              "25 steps from random splats (the default bench flags' state), 1.33M pairs"),
+    "final": (647839, {"1,1": (117.77e6, 66.27e6)},
+              "a 30-step run from the slab of a 2020-step fit, 648k pairs"),
 }
-COMPOSITE_TRAFFIC_STATE = ("final C1 state (650k pairs): a 30-step eager counter run started "
-                           "from the slab of a 2020-step fit")
+
+
+def composite_traffic(shape, n_isects):
+    """(read, write, state description) of the profiled state nearest in pair
+    count, or None when that launch shape was not profiled."""
+    name = min(COMPOSITE_TRAFFIC,
+               key=lambda k: abs(math.log(max(n_isects, 1) / COMPOSITE_TRAFFIC[k][0])))
+    pairs, shapes, how = COMPOSITE_TRAFFIC[name]
+    if shape not in shapes:
+        return None
+    return shapes[shape] + (f"{name} C1 state: {how}",)
+
 
 CONFIGS = {
     # name: (W, H, N, n_ellipses, seed)
@@ -128,7 +139,6 @@ def roofline_composite(ds, oracle, fit, target, W, H, N, reps=20):
     ds.check(L.ds_raster_get_tuning(ras, C.byref(fp), C.byref(bp)), "ds_raster_get_tuning")
     fused = fp.value == bp.value
     shape = f"{fp.value},{bp.value}"
-    traffic = COMPOSITE_TRAFFIC.get(shape)
     d_target = ds.DeviceArray.from_numpy(np.ascontiguousarray(target, np.float32))
     d_rgb = ds.DeviceArray(4 * 3 * W * H)
     d_loss = ds.DeviceArray(4)
@@ -157,13 +167,14 @@ def roofline_composite(ds, oracle, fit, target, W, H, N, reps=20):
     e_fwd = int(fw["E"])
     flops = e_fwd * FWD_FLOP_PER_EVAL + e_bwd * BWD_FLOP_PER_EVAL
     achieved = flops / avg_s / 1e12
+    traffic = composite_traffic(shape, int(st["n_isects"]))
     return {"kernel": kernels, "launch_shape": shape, "bound": "valu", "unit": "TFLOP/s",
             "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": sum(traffic) if traffic else None,
+            "traffic": traffic[0] + traffic[1] if traffic else None,
             "traffic_read_bytes": traffic[0] if traffic else None,
             "traffic_write_bytes": traffic[1] if traffic else None,
-            "traffic_state": COMPOSITE_TRAFFIC_STATE if traffic else
+            "traffic_state": traffic[2] if traffic else
             f"no counter figures for launch shape {shape}",
             "avg_launch_ms": round(avg_s * 1e3, 4), "splat_evals_fwd": e_fwd,
             "splat_evals_bwd": int(e_bwd), "flop_per_eval_fwd": FWD_FLOP_PER_EVAL,
