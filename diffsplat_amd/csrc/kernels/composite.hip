@@ -340,7 +340,10 @@ __device__ __forceinline__ int composite_front_to_back(
                 const int s = (int)__builtin_ctzll(any);
                 any &= any - 1;
                 const int t = t0 + s;
-                const float4 A = sA[t], B = sB[t];
+                const float4 A = sA[t], B = sB[t], C = sC[t];
+                // the colour is read with the record's other parts, through
+                // the same LDS address (most candidates reach a pixel)
+                asm volatile("" ::"v"(C.x), "v"(C.y), "v"(C.z));
                 // Pass 1: which pixels of the candidate bands the splat
                 // reaches; a wave whose pixels it reaches none of skips the
                 // rest.
@@ -357,7 +360,6 @@ __device__ __forceinline__ int composite_front_to_back(
                     anyv |= okm[k] != 0;
                 }
                 if (!anyv) continue;
-                const float4 C = sC[t];
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
                     if (okm[k] == 0) continue;  // band k unchanged
@@ -489,6 +491,7 @@ __device__ __forceinline__ void composite_back_to_front(
                 const int t = c0 + sl;
                 const int idx = bb + t;
                 const float4 A = sA[t], B = sB[t], C = sC[t];
+                asm volatile("" ::"v"(C.x), "v"(C.y), "v"(C.z));  // read with A and B
                 // sums start at -0, the identity of float addition (+0 is not:
                 // +0 + -0 = +0), so the first band's terms need no add
                 float v[9];
