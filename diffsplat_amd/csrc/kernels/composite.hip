@@ -258,6 +258,27 @@ __device__ __forceinline__ bool reaches_band(const float4 &A, const float4 &B, f
 // reaches.
 constexpr int kSkipWords = 64;
 
+// Stages entry idx of the tile's list in LDS slot t: the splat's record and,
+// with SLOT, in the colour part's spare word the pair slot of (splat, this
+// tile) where the backward writes the pair's gradient record (see
+// CompositeArgs), so that no global load sits between a reached pair's
+// sums and their store.
+template <bool SLOT>
+__device__ __forceinline__ void stage_record(const CompositeArgs &a, int idx, int t,
+                                             uint32_t trow, uint32_t tcol, float4 *sA,
+                                             float4 *sB, float4 *sC) {
+    const uint32_t g = a.flat[idx];
+    const float4 *r = a.rec + 3 * (size_t)g;
+    float4 c = r[2];
+    if constexpr (SLOT) {
+        const uint2 pb = a.pair_base[g];
+        c.w = __uint_as_float(pb.x + trow * pb.y + tcol);
+    }
+    sA[t] = r[0];
+    sB[t] = r[1];
+    sC[t] = c;
+}
+
 // Front to back over the tile's list (compositeFwdKernel). With BITS the
 // per-band reach bits are recorded in s_bits. Every exit is wave-uniform (a
 // ballot), so the loop runs on scalar branches; lanes whose pixels are done
@@ -300,16 +321,15 @@ __device__ __forceinline__ int composite_front_to_back(
         wy1[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.py[k]), 0));
         wy0[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.py[k]), kWave - 1));
     }
+    // the tile's row and column in its image (uniform): with BITS the
+    // backward may reuse the last batch, so its pair slots are staged too
+    const uint32_t trow = (uint32_t)__builtin_amdgcn_readfirstlane(q.row[0] >> 4);
+    const uint32_t tcol = (uint32_t)__builtin_amdgcn_readfirstlane(q.j >> 4);
     int loaded = -1;
     for (int b = range.x; b < range.y; b += NT) {
         if (__syncthreads_count(wave_live()) == 0) break;
         const int idx = b + tid;
-        if (idx < range.y) {
-            const float4 *r = a.rec + 3 * (size_t)a.flat[idx];
-            sA[tid] = r[0];
-            sB[tid] = r[1];
-            sC[tid] = r[2];
-        }
+        if (idx < range.y) stage_record<BITS>(a, idx, tid, trow, tcol, sA, sB, sC);
         __syncthreads();
         loaded = b;
         const int bs = wave_live() ? min(NT, range.y - b) : 0;
@@ -452,12 +472,7 @@ __device__ __forceinline__ void composite_back_to_front(
         const int bs = min(NT, end - bb);  // slots [0, bs) hold list indices bb + slot
         if (bb != loaded) {
             __syncthreads();
-            if (tid < bs) {
-                const float4 *r = a.rec + 3 * (size_t)a.flat[bb + tid];
-                sA[tid] = r[0];
-                sB[tid] = r[1];
-                sC[tid] = r[2];
-            }
+            if (tid < bs) stage_record<true>(a, bb + tid, tid, trow, tcol, sA, sB, sC);
             __syncthreads();
         }
         loaded = -1;  // only the forward's last batch is reused
@@ -544,10 +559,9 @@ __device__ __forceinline__ void composite_back_to_front(
                     // the wave's sums are the tile's: nine lanes of row 0
                     // store the fields, lane 6 the epoch tag
                     const float sum = my_scale * wave_sum9_transposed(v, lane);
-                    const uint32_t g = (uint32_t)__builtin_amdgcn_readfirstlane(__float_as_uint(B.w));
-                    const uint2 pb = a.pair_base[g];
-                    float *r = reinterpret_cast<float *>(a.pair_rec +
-                                                         3 * (size_t)(pb.x + trow * pb.y + tcol));
+                    const uint32_t slot =
+                        (uint32_t)__builtin_amdgcn_readfirstlane(__float_as_uint(C.w));
+                    float *r = reinterpret_cast<float *>(a.pair_rec + 3 * (size_t)slot);
                     if (lane < 16 && my_field >= 0) r[my_field] = sum;
                     if (lane == 6) r[9] = __uint_as_float(ep);
                 } else {
@@ -574,9 +588,7 @@ __device__ __forceinline__ void composite_back_to_front(
                 // A slot no pixel reached keeps its sums at zero and its
                 // record unwritten (an older epoch, which counts as zero).
                 if (hit) {
-                    const uint32_t g = __float_as_uint(sB[tid].w);
-                    const uint2 pb = a.pair_base[g];
-                    float4 *r = a.pair_rec + 3 * (size_t)(pb.x + trow * pb.y + tcol);
+                    float4 *r = a.pair_rec + 3 * (size_t)__float_as_uint(sC[tid].w);
                     r[0] = make_float4(gs[0] * field_scale(0), gs[1] * field_scale(1),
                                        gs[2] * field_scale(2), gs[3] * field_scale(3));
                     r[1] = make_float4(gs[4] * field_scale(4), gs[5], gs[6], gs[7]);
