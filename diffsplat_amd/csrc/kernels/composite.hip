@@ -258,16 +258,16 @@ __device__ __forceinline__ bool reaches_band(const float4 &A, const float4 &B, f
 // reaches.
 constexpr int kSkipWords = 64;
 
-// Stages entry idx of the tile's list in LDS slot t: the splat's record and,
+// Stages an entry of the tile's list in LDS slot t: the splat's record and,
 // with SLOT, in the colour part's spare word the pair slot of (splat, this
 // tile) where the backward writes the pair's gradient record (see
 // CompositeArgs), so that no global load sits between a reached pair's
 // sums and their store.
+// g is the entry's splat (flat[idx]), read a batch ahead.
 template <bool SLOT>
-__device__ __forceinline__ void stage_record(const CompositeArgs &a, int idx, int t,
+__device__ __forceinline__ void stage_record(const CompositeArgs &a, uint32_t g, int t,
                                              uint32_t trow, uint32_t tcol, float4 *sA,
                                              float4 *sB, float4 *sC) {
-    const uint32_t g = a.flat[idx];
     const float4 *r = a.rec + 3 * (size_t)g;
     float4 c = r[2];
     if constexpr (SLOT) {
@@ -326,10 +326,14 @@ __device__ __forceinline__ int composite_front_to_back(
     const uint32_t trow = (uint32_t)__builtin_amdgcn_readfirstlane(q.row[0] >> 4);
     const uint32_t tcol = (uint32_t)__builtin_amdgcn_readfirstlane(q.j >> 4);
     int loaded = -1;
+    // the list entries (splat ids) of the next batch are read a batch ahead
+    uint32_t gnext = range.x + tid < range.y ? a.flat[range.x + tid] : 0u;
     for (int b = range.x; b < range.y; b += NT) {
         if (__syncthreads_count(wave_live()) == 0) break;
         const int idx = b + tid;
-        if (idx < range.y) stage_record<BITS>(a, idx, tid, trow, tcol, sA, sB, sC);
+        const uint32_t g = gnext;
+        if (idx + NT < range.y) gnext = a.flat[idx + NT];
+        if (idx < range.y) stage_record<BITS>(a, g, tid, trow, tcol, sA, sB, sC);
         __syncthreads();
         loaded = b;
         const int bs = wave_live() ? min(NT, range.y - b) : 0;
@@ -472,7 +476,7 @@ __device__ __forceinline__ void composite_back_to_front(
         const int bs = min(NT, end - bb);  // slots [0, bs) hold list indices bb + slot
         if (bb != loaded) {
             __syncthreads();
-            if (tid < bs) stage_record<true>(a, bb + tid, tid, trow, tcol, sA, sB, sC);
+            if (tid < bs) stage_record<true>(a, a.flat[bb + tid], tid, trow, tcol, sA, sB, sC);
             __syncthreads();
         }
         loaded = -1;  // only the forward's last batch is reused
@@ -744,12 +748,40 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ 
     }
 }
 
+// The MSE target of this thread's pixels, loaded when the kernel starts so
+// that its latency passes while the tile composites (zero outside the image
+// or without a target).
+template <bool MSE, int P>
+struct PixelTarget {
+    float r[P], g[P], b[P];
+};
+
+template <bool MSE, int P>
+__device__ __forceinline__ PixelTarget<MSE, P> load_target(const TileImage &im,
+                                                           const TilePixels<P> &q,
+                                                           const float *__restrict__ target) {
+    PixelTarget<MSE, P> t;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        t.r[k] = t.g[k] = t.b[k] = 0.f;
+        if constexpr (MSE) {
+            if (q.inside[k]) {
+                const int pix = im.pix_base + q.row[k] * im.W + q.j;
+                t.r[k] = target[3 * pix + 0];
+                t.g[k] = target[3 * pix + 1];
+                t.b[k] = target[3 * pix + 2];
+            }
+        }
+    }
+    return t;
+}
+
 template <bool MSE, int P>
 __device__ __forceinline__ void composite_epilogue(
     const TileImage &im, const TilePixels<P> &q, const float (&T)[P], const float (&cr)[P],
     const float (&cg)[P], const float (&cb)[P], const int32_t (&cur)[P],
     float *__restrict__ out_rgb, float *__restrict__ out_T, int32_t *__restrict__ out_last,
-    const float *__restrict__ target, float *loss, float *__restrict__ grad, float *s_red,
+    const PixelTarget<MSE, P> &tg, float *loss, float *__restrict__ grad, float *s_red,
     float (&gr)[P], float (&gg)[P], float (&gb)[P]) {
     constexpr int NW = kTilePix / P / kWave;
     const int tid = threadIdx.x;
@@ -765,9 +797,9 @@ __device__ __forceinline__ void composite_epilogue(
         out_T[pix] = T[k];
         out_last[pix] = cur[k];
         if constexpr (MSE) {
-            const float dr = cr[k] - target[3 * pix + 0];
-            const float dg = cg[k] - target[3 * pix + 1];
-            const float db = cb[k] - target[3 * pix + 2];
+            const float dr = cr[k] - tg.r[k];
+            const float dg = cg[k] - tg.g[k];
+            const float db = cb[k] - tg.b[k];
             const float s2 = 2.f * im.inv_npix;
             gr[k] = s2 * dr;
             gg[k] = s2 * dg;
@@ -804,11 +836,12 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
     const TileImage im = tile_image<MULTI>(a, tile, inv_npix_one);
     const TilePixels<P> q = tile_pixels<P>(im, threadIdx.x);
     const int2 range = a.ranges[tile];
+    const PixelTarget<MSE, P> tg = load_target<MSE, P>(im, q, target);
     float T[P], cr[P], cg[P], cb[P], gr[P], gg[P], gb[P];
     int32_t cur[P];
     (void)composite_front_to_back<P, false>(a, range, q, sA, sB, sC, nullptr, T, cr, cg, cb,
                                             cur);
-    composite_epilogue<MSE, P>(im, q, T, cr, cg, cb, cur, out_rgb, out_T, out_last, target, loss,
+    composite_epilogue<MSE, P>(im, q, T, cr, cg, cb, cur, out_rgb, out_T, out_last, tg, loss,
                                grad, s_red, gr, gg, gb);
 }
 
@@ -864,11 +897,12 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fused(
     const TileImage im = tile_image<MULTI>(a, tile, inv_npix_one);
     const TilePixels<P> q = tile_pixels<P>(im, threadIdx.x);
     const int2 range = a.ranges[tile];
+    const PixelTarget<true, P> tg = load_target<true, P>(im, q, target);
     float T[P], cr[P], cg[P], cb[P], gr[P], gg[P], gb[P];
     int32_t cur[P];
     const int loaded =
         composite_front_to_back<P, true>(a, range, q, sA, sB, sC, s_bits, T, cr, cg, cb, cur);
-    composite_epilogue<true, P>(im, q, T, cr, cg, cb, cur, out_rgb, out_T, out_last, target,
+    composite_epilogue<true, P>(im, q, T, cr, cg, cb, cur, out_rgb, out_T, out_last, tg,
                                 loss, grad_rgb, s_red, gr, gg, gb);
     if (range.y <= range.x) return;
     int32_t lst[P];
