@@ -32,6 +32,11 @@ __device__ __forceinline__ int tile_of_block(const CompositeArgs &a) {
     return a.order ? (int)a.order[blockIdx.x] : (int)blockIdx.x;
 }
 
+// Its range in the sorted list (read beside the order, not through it).
+__device__ __forceinline__ int2 range_of_block(const CompositeArgs &a) {
+    return a.order ? a.order_range[blockIdx.x] : a.ranges[blockIdx.x];
+}
+
 // u = k (Sigma^-1 d)_x, w = k (Sigma^-1 d)_y for d = splat centre - pixel
 // and the record's scale k, so that p = dx u + dy w = k dist2,
 // d dist2/d(x,y) = 2 (u, w) / k, and the covariance gradients of the
@@ -835,7 +840,7 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fwd(
     const int tile = tile_of_block(a);
     const TileImage im = tile_image<MULTI>(a, tile, inv_npix_one);
     const TilePixels<P> q = tile_pixels<P>(im, threadIdx.x);
-    const int2 range = a.ranges[tile];
+    const int2 range = range_of_block(a);
     const PixelTarget<MSE, P> tg = load_target<MSE, P>(im, q, target);
     float T[P], cr[P], cg[P], cb[P], gr[P], gg[P], gb[P];
     int32_t cur[P];
@@ -856,7 +861,7 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_bwd(
     __shared__ float sG[9][GW];
     __shared__ int s_last;
     const int tile = tile_of_block(a);
-    const int2 range = a.ranges[tile];
+    const int2 range = range_of_block(a);
     if (range.y <= range.x) return;
     const TileImage im = tile_image<MULTI>(a, tile, 0.f);
     const TilePixels<P> q = tile_pixels<P>(im, threadIdx.x);
@@ -896,7 +901,7 @@ __global__ __launch_bounds__(kTilePix / P) void k_composite_fused(
     const int tile = tile_of_block(a);
     const TileImage im = tile_image<MULTI>(a, tile, inv_npix_one);
     const TilePixels<P> q = tile_pixels<P>(im, threadIdx.x);
-    const int2 range = a.ranges[tile];
+    const int2 range = range_of_block(a);
     const PixelTarget<true, P> tg = load_target<true, P>(im, q, target);
     float T[P], cr[P], cg[P], cb[P], gr[P], gg[P], gb[P];
     int32_t cur[P];

@@ -54,6 +54,7 @@ struct CompositeArgs {
     float4 *pair_rec;
     const uint2 *pair_base;
     const uint32_t *epoch;
+    const int2 *order_range;  // with order: the range of tile order[b], at b
 };
 
 // pix_per_thread: 1, 2 or 4 pixels per thread (256, 128 or 64 threads per tile).

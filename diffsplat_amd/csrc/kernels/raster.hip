@@ -691,7 +691,8 @@ __device__ __forceinline__ uint64_t same_bucket(uint32_t b, bool valid) {
 
 __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const int2 *__restrict__ ranges,
                                                               int ntiles,
-                                                              uint32_t *__restrict__ order) {
+                                                              uint32_t *__restrict__ order,
+                                                              int2 *__restrict__ order_range) {
     __shared__ uint32_t h[32];
     __shared__ uint32_t wh[kOrderWaves][32];
     __shared__ uint8_t s_b[kOrderLdsTiles];
@@ -737,14 +738,18 @@ __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const int2 *__rest
     for (int t0 = 0; t0 < ntiles; t0 += kOrderThreads) {
         const int t = t0 + tid;
         const bool valid = t < ntiles;
-        const uint32_t b = !valid ? 0u : cache ? (uint32_t)s_b[t] : tile_bucket(ranges[t]);
+        const int2 rt = valid ? ranges[t] : make_int2(0, 0);
+        const uint32_t b = !valid ? 0u : cache ? (uint32_t)s_b[t] : tile_bucket(rt);
         const uint64_t peers = same_bucket(b, valid);
         const uint32_t rank = (uint32_t)__popcll(peers & lt);
         uint32_t base = 0;
         if (valid && rank == 0) base = atomicAdd(&h[b], (uint32_t)__popcll(peers));
         const int leader = valid ? __ffsll((unsigned long long)peers) - 1 : 0;
         base = (uint32_t)__shfl((int)base, leader, kWave);
-        if (valid) order[base + rank] = (uint32_t)t;
+        if (valid) {
+            order[base + rank] = (uint32_t)t;
+            order_range[base + rank] = rt;  // read with the order: one load level less
+        }
     }
 }
 
@@ -798,6 +803,7 @@ struct ds_raster {
     // per tile / pixel
     ds::DevBuf<int2> ranges;
     ds::DevBuf<uint32_t> order;  // compositing dispatch order (heavy-first)
+    ds::DevBuf<int2> order_range;  // the range of tile order[b], at b
     ds::DevBuf<float> T;
     ds::DevBuf<int32_t> last;
     // scratch: device scalars, depth-digit histograms, look-back status words
@@ -857,6 +863,7 @@ static ds_status raster_alloc_tiles(ds_raster *r) {
     }
     DS_TRY(r->ranges.alloc((size_t)r->ntiles));
     DS_TRY(r->order.alloc((size_t)r->ntiles));
+    DS_TRY(r->order_range.alloc((size_t)r->ntiles));
     DS_TRY(r->T.alloc((size_t)r->npix));
     DS_TRY(r->last.alloc((size_t)r->npix));
     r->has_fwd = false;
@@ -872,7 +879,10 @@ static CompositeArgs composite_args(const ds_raster *r) {
                          r->tile_order == DS_TILE_ORDER_HEAVY_FIRST ? (const uint32_t *)r->order.ptr
                                                                     : nullptr,
                          r->pair_rec.ptr, (const uint2 *)r->pair_base.ptr,
-                         (const uint32_t *)r->scalars.ptr + SC_EPOCH};
+                         (const uint32_t *)r->scalars.ptr + SC_EPOCH,
+                         r->tile_order == DS_TILE_ORDER_HEAVY_FIRST
+                             ? (const int2 *)r->order_range.ptr
+                             : nullptr};
 }
 
 // The backward needs the gradient records and the pair bases of the last
@@ -1090,7 +1100,8 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
     }
     if (r->tile_order == DS_TILE_ORDER_HEAVY_FIRST) {
         hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, s,
-                           (const int2 *)r->ranges.ptr, r->ntiles, r->order.ptr);
+                           (const int2 *)r->ranges.ptr, r->ntiles, r->order.ptr,
+                           r->order_range.ptr);
         DS_CHECK_LAUNCH("tile_order");
     }
     r->order_valid = r->tile_order == DS_TILE_ORDER_HEAVY_FIRST;
@@ -1142,7 +1153,7 @@ static ds_status raster_backward(ds_raster *r, const ds_splat2d &in, const float
     DS_REQUIRE(grad_rgb != nullptr, "grad_rgb is NULL");
     DS_TRY(require_backward(r));
     CompositeArgs ca = composite_args(r);
-    if (!r->order_valid) ca.order = nullptr;  // order switched on after the forward
+    if (!r->order_valid) ca.order = nullptr, ca.order_range = nullptr;  // order switched on after the forward
     DS_TRY(launch_composite_bwd(ca, r->bwd_ppt, r->grad_mode, r->T.ptr, r->last.ptr,
                                 grad_rgb, s));
     return reduce_pairs(r, gin, s);
@@ -1341,7 +1352,7 @@ extern "C" ds_status ds_raster_composite_fwd_bwd(ds_raster *r, const ds_splat2d 
                "the fused compositing needs equal forward and backward launch shapes");
     DS_TRY(require_backward(r));
     CompositeArgs ca = composite_args(r);
-    if (!r->order_valid) ca.order = nullptr;
+    if (!r->order_valid) ca.order = nullptr, ca.order_range = nullptr;
     hipStream_t st = as_stream(s);
     DS_TRY(fill_u32(loss_dev, 0u, r->ninst > 0 ? (size_t)r->ninst : 1, st));
     DS_TRY(launch_composite_fused(ca, r->fwd_ppt, r->grad_mode, out_rgb, r->T.ptr, r->last.ptr,
@@ -1367,7 +1378,7 @@ extern "C" ds_status ds_raster_composite_fwd(ds_raster *r, const ds_splat2d *in,
     DS_REQUIRE(r->has_fwd, "ds_raster_composite_fwd called before a forward");
     DS_REQUIRE(in->count == r->last_count, "splat count differs from the last forward");
     CompositeArgs ca = composite_args(r);
-    if (!r->order_valid) ca.order = nullptr;
+    if (!r->order_valid) ca.order = nullptr, ca.order_range = nullptr;
     return launch_composite_fwd(ca, r->fwd_ppt, out_rgb, r->T.ptr, r->last.ptr, nullptr, nullptr,
                                 nullptr, as_stream(s));
 }
