@@ -488,23 +488,33 @@ void ImageFitter::setUseGraph(bool on) {
     dropGraph();
 }
 
+// Captures nsteps steps into a graph (recorded, not run) unless captured.
+// The optimizer's host step count is left as it was: the captured steps run
+// when the graph is launched.
+void ImageFitter::capture(int nsteps, hipGraph_t &g, hipGraphExec_t &ge) {
+    if (ge) return;
+    const int host_steps = optimizer.getStepCount();
+    hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal),
+              "hipStreamBeginCapture");
+    try {
+        for (int k = 0; k < nsteps; ++k) stepOnce();
+    } catch (...) {
+        hipGraph_t bad = nullptr;
+        (void)hipStreamEndCapture(stream, &bad);
+        if (bad) (void)hipGraphDestroy(bad);
+        optimizer.setStepCount(host_steps);
+        throw;
+    }
+    optimizer.setStepCount(host_steps);
+    hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
 // Replays nsteps captured steps (captured on first use).
 void ImageFitter::launchCaptured(int nsteps, hipGraph_t &g, hipGraphExec_t &ge) {
-    if (!ge) {
-        hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal),
-                  "hipStreamBeginCapture");
-        try {
-            for (int k = 0; k < nsteps; ++k) stepOnce();
-        } catch (...) {
-            hipGraph_t bad = nullptr;
-            (void)hipStreamEndCapture(stream, &bad);
-            if (bad) (void)hipGraphDestroy(bad);
-            throw;
-        }
-        hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
-        hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
-    }
+    capture(nsteps, g, ge);
     hip_check(hipGraphLaunch(ge, stream), "hipGraphLaunch");
+    for (int k = 0; k < nsteps; ++k) optimizer.countStep();
 }
 
 void ImageFitter::step() { steps(1); }
@@ -517,7 +527,11 @@ void ImageFitter::steps(int n) {
         return;
     }
     // Each graph launch costs the GPU a few microseconds between graphs, so
-    // long runs replay a graph of kGraphSteps steps.
+    // long runs replay a graph of kGraphSteps steps. Both graphs are captured
+    // on the first graph step, so that a later long run (a timed one, say)
+    // does not pay the capture of the long graph inside it.
+    capture(1, graph, graph_exec);
+    capture(kGraphSteps, graph_k, graph_exec_k);
     for (; n >= kGraphSteps; n -= kGraphSteps) launchCaptured(kGraphSteps, graph_k, graph_exec_k);
     for (; n > 0; --n) launchCaptured(1, graph, graph_exec);
 }

@@ -104,6 +104,7 @@ private:
     static constexpr int kGraphSteps = 16;
     hipGraphExec_t graph_exec = nullptr, graph_exec_k = nullptr;
     hipGraph_t graph = nullptr, graph_k = nullptr;
+    void capture(int nsteps, hipGraph_t &g, hipGraphExec_t &ge);
     void launchCaptured(int nsteps, hipGraph_t &g, hipGraphExec_t &ge);
 };
 
