@@ -508,6 +508,8 @@ void ImageFitter::capture(int nsteps, hipGraph_t &g, hipGraphExec_t &ge) {
     optimizer.setStepCount(host_steps);
     hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
     hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    // its device-side setup now, not at its first launch
+    hip_check(hipGraphUpload(ge, stream), "hipGraphUpload");
 }
 
 // Replays nsteps captured steps (captured on first use).
