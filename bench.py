@@ -460,6 +460,12 @@ def main():
     print(f"[bench] tuning {tuning}", file=sys.stderr, flush=True)
     fit.set_graph(bool(args.graph))
     fit.step(args.warmup)
+    # graph mode: the graphs the timed chunks replay are captured now (setup,
+    # not steps: nothing runs), so the timed region holds launches only
+    chunk = min(args.psnr_every, args.steps) if args.psnr_every > 0 else args.steps
+    fit.prepare(chunk)
+    if args.steps % chunk:
+        fit.prepare(args.steps % chunk)
     fit.synchronize()
     print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
     # the timed steps in chunks of psnr_every: after each chunk the loss of its

@@ -544,6 +544,10 @@ class ImageFitter:
     def set_graph(self, on: bool) -> None:
         check(lib().ds_fitter_set_graph(self.h, int(on)), "ds_fitter_set_graph")
 
+    def prepare(self, n: int) -> None:
+        """Graph mode: capture what step(n) replays, without running it."""
+        check(lib().ds_fitter_prepare(self.h, n), "ds_fitter_prepare")
+
     def autotune(self, reps: int = 3) -> dict:
         """Picks the fastest compositing launch shapes on the current state."""
         f, b = C.c_int(), C.c_int()

@@ -213,6 +213,7 @@ _SIGS = {
     "ds_fitter_get_raster": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
     "ds_fitter_get_stream": (C.c_int, [_P, C.POINTER(C.c_void_p)]),
     "ds_fitter_set_graph": (C.c_int, [_P, C.c_int]),
+    "ds_fitter_prepare": (C.c_int, [_P, C.c_int]),
     "ds_fitter_step_count": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "ds_fitter_views": (C.c_int, [_P, C.POINTER(Gauss3D), C.POINTER(Splat2D), C.POINTER(Splat2D),
                                   C.POINTER(C.c_void_p)]),

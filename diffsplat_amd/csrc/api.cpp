@@ -524,6 +524,15 @@ ds_status ds_fitter_step(ds_fitter *f, int n_steps) {
     DS_GUARD_END
 }
 
+ds_status ds_fitter_prepare(ds_fitter *f, int n_steps) {
+    clear_error();
+    DS_REQUIRE(f && n_steps >= 0, "bad argument");
+    DS_GUARD_BEGIN
+    f->impl.prepare(n_steps);
+    return DS_OK;
+    DS_GUARD_END
+}
+
 ds_status ds_fitter_forward(ds_fitter *f) {
     clear_error();
     DS_REQUIRE(f, "NULL fitter");

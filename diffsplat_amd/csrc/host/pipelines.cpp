@@ -529,13 +529,17 @@ void ImageFitter::steps(int n) {
         return;
     }
     // Each graph launch costs the GPU a few microseconds between graphs, so
-    // long runs replay a graph of kGraphSteps steps. Both graphs are captured
-    // on the first graph step, so that a later long run (a timed one, say)
-    // does not pay the capture of the long graph inside it.
-    capture(1, graph, graph_exec);
-    capture(kGraphSteps, graph_k, graph_exec_k);
+    // long runs replay a graph of kGraphSteps steps.
     for (; n >= kGraphSteps; n -= kGraphSteps) launchCaptured(kGraphSteps, graph_k, graph_exec_k);
     for (; n > 0; --n) launchCaptured(1, graph, graph_exec);
+}
+
+// Captures (without running) the graphs that steps(n) replays, so that a
+// later steps(n) pays no capture or instantiation.
+void ImageFitter::prepare(int n) {
+    if (!use_graph || n <= 0) return;
+    if (n >= kGraphSteps) capture(kGraphSteps, graph_k, graph_exec_k);
+    if (n % kGraphSteps) capture(1, graph, graph_exec);
 }
 
 void ImageFitter::forwardOnly() {

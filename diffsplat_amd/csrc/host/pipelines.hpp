@@ -45,6 +45,7 @@ public:
 
     void step();          // zeroGrad, forward, backward, Adam
     void steps(int n);    // n steps (graph mode: replays of a multi-step graph)
+    void prepare(int n);  // graph mode: capture what steps(n) replays, without running it
     void forwardOnly();   // forward (render + loss)
     void forwardBackward();  // zeroGrad, forward, backward (no Adam update)
     void setTargetDevice(const float *dev_rgb);  // device-to-device, on the fitter's stream

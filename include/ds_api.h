@@ -505,6 +505,13 @@ ds_status ds_fitter_get_stream(ds_fitter *f, ds_stream_t *s);
  * call drops a captured step, so call it again after changing the
  * rasterizer's launch shapes or dispatch order directly (ds_raster_set_*). */
 ds_status ds_fitter_set_graph(ds_fitter *f, int enable);
+/* Graph mode: captures (records, does not run) the graphs that
+ * ds_fitter_step(f, n_steps) replays, so that the next such call launches
+ * them without capturing or instantiating; a no-op without graphs. For
+ * callers that time or latency-bound runs (a setup call, like an
+ * allocation). No counterpart in the reference, which launches layer by
+ * layer. */
+ds_status ds_fitter_prepare(ds_fitter *f, int n_steps);
 /* Autotuner: times every compositing launch shape on the current parameters
  * (forward and backward only; parameters and Adam state are unchanged; `reps`
  * timed runs each) and switches to the fastest forward and backward shapes
