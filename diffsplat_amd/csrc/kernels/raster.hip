@@ -294,16 +294,35 @@ __global__ __launch_bounds__(256) void k_prep(ds_splat2d in, int n, int W, int H
 
 // Writes the pairs of the splats, taken in depth order, each splat's tiles in
 // row-major order starting at its scanned offset; pairs at or past the cap
-// are dropped. One thread per pair slot, so a splat covering thousands of
-// tiles costs no more than thousands of small ones: slot p belongs to the
-// last depth rank d with off[d] <= p (off is the exclusive scan of the tile
-// counts in depth order, so that splat's range [off[d], off[d+1]) holds p and
-// is not empty), found by binary search between the owners of the first
-// slots of p's chunk of kEmitChunk slots and of the next chunk (recorded by
-// the offset scan). The first thread also totals the visible-count
+// are dropped. Slot p belongs to the last depth rank d with off[d] <= p (off
+// is the exclusive scan of the tile counts in depth order, so that splat's
+// range [off[d], off[d+1]) holds p and is not empty). Each wave takes a span
+// of kEmitSpan consecutive slots, 64 at a time: it finds the owner of the
+// span's first slot by one uniform binary search between the owners of the
+// first slots of its chunk of 2^kEmitChunkShift slots and of the next chunk
+// (recorded by the offset scan); every window then covers slots owned by the
+// 64 depth ranks from the owner of its first slot on (each visible splat has
+// a pair), so lane i reads off[lo + i], marks the slot where rank lo + i
+// starts (an LDS max, per wave) and a max-scan over the lanes gives each
+// slot's owner. The next window starts from the owner of the last slot. A
+// splat covering thousands of tiles costs no more than thousands of small
+// ones, and the dependent loads per slot are a few per window instead of a
+// binary search per slot. The first thread also totals the visible-count
 // replicas into the device scalar. With MULTI the tile id is global: the
 // splat's instance's first tile plus the tile in its image.
 constexpr int kEmitChunkShift = 12;
+constexpr int kEmitSpan = 512;  // consecutive slots per wave and grid sweep
+
+__device__ __forceinline__ uint32_t wave_incl_max_u32(uint32_t x) {
+    const int l = __lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, kWave);
+        if (l >= d) x = max(x, y);
+    }
+    return x;
+}
+
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_emit(const uint32_t *__restrict__ order,
                                               const uint32_t *__restrict__ off,
@@ -316,6 +335,7 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t *__restrict__ order
                                               const uint16_t *__restrict__ sinst,
                                               uint2 *__restrict__ pair_base,
                                               uint32_t *__restrict__ slot_splat) {
+    __shared__ uint32_t s_head[256 / kWave][kWave];
     const uint32_t n = sc[SC_COUNT];
     const uint32_t m = sc[SC_NISECT];  // pairs kept: the total clamped at the cap
     if (gtid() == 0) {
@@ -324,37 +344,60 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t *__restrict__ order
         for (int r = 0; r < kVisReps; ++r) v += ghist[kVisOffset + r * kVisStride];
         sc[SC_VISIBLE] = v;
     }
+    const int lane = __lane_id();
+    uint32_t *const head = s_head[threadIdx.x / kWave];
     const uint32_t nch = (m + (1u << kEmitChunkShift) - 1) >> kEmitChunkShift;
-    for (int64_t p = gtid(); p < m; p += gstride()) {
-        // off[lo] <= p; the answer lies in [lo, hi): the owner of the chunk's
-        // first slot up to the owner of the next chunk's first slot
-        const uint32_t c = (uint32_t)p >> kEmitChunkShift;
+    const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x / kWave);
+    for (uint64_t s0 = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) *
+                       kEmitSpan;
+         s0 < m; s0 += nwaves * kEmitSpan) {  // wave-uniform
+        const uint32_t s1 = (uint32_t)min((uint64_t)m, s0 + kEmitSpan);
+        // the owner of the span's first slot (uniform binary search)
+        const uint32_t c = (uint32_t)s0 >> kEmitChunkShift;
         uint32_t lo = chunk_first[c];
         uint32_t hi = (c + 1 < nch) ? chunk_first[c + 1] + 1 : n;
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (off[mid] <= (uint32_t)p)
+            if (off[mid] <= (uint32_t)s0)
                 lo = mid;
             else
                 hi = mid;
         }
-        const uint32_t g = order[lo];
-        const uint32_t j = (uint32_t)p - off[lo];  // index inside the splat's rectangle
-        const uint2 r = rect[g];
-        const uint32_t tx0 = r.x & 0xFFFF, tx1 = r.x >> 16, ty0 = r.y & 0xFFFF;
-        const uint32_t w = tx1 - tx0 + 1;
-        const uint32_t ty = ty0 + j / w, tx = tx0 + j % w;
-        // the splat's first slot: where its backward records go (see
-        // CompositeArgs)
-        if (j == 0 && pair_base) pair_base[g] = make_uint2((uint32_t)p - (ty0 * w + tx0), w);
-        if constexpr (MULTI) {
-            const MultiInst I = mi[sinst[g]];
-            tkey[p] = (uint32_t)I.tile_base + ty * (uint32_t)I.ntx + tx;
-        } else {
-            tkey[p] = ty * (uint32_t)ntx + tx;
+        for (uint32_t P = (uint32_t)s0; P < s1; P += kWave) {
+            // where ranks lo + 1 .. lo + 63 start, relative to the window
+            const uint32_t k = lo + (uint32_t)lane;
+            const int d = k < n ? (int)(off[k] - P) : kWave;
+            head[lane] = 0u;
+            __builtin_amdgcn_wave_barrier();
+            if (lane > 0 && d >= 0 && d < kWave) atomicMax(&head[d], (uint32_t)lane);
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t rel = wave_incl_max_u32(head[lane]);  // owner of slot P + lane
+            const int d_own = __shfl(d, (int)rel, kWave);
+            const uint32_t p = P + (uint32_t)lane;
+            if (p < s1) {
+                const uint32_t ko = lo + rel;
+                const uint32_t g = order[ko];
+                const uint32_t j = (uint32_t)(lane - d_own);  // index inside the splat's rectangle
+                const uint2 r = rect[g];
+                const uint32_t tx0 = r.x & 0xFFFF, tx1 = r.x >> 16, ty0 = r.y & 0xFFFF;
+                const uint32_t w = tx1 - tx0 + 1;
+                const uint32_t ty = ty0 + j / w, tx = tx0 + j % w;
+                // the splat's first slot: where its backward records go (see
+                // CompositeArgs)
+                if (j == 0 && pair_base) pair_base[g] = make_uint2(p - (ty0 * w + tx0), w);
+                if constexpr (MULTI) {
+                    const MultiInst I = mi[sinst[g]];
+                    tkey[p] = (uint32_t)I.tile_base + ty * (uint32_t)I.ntx + tx;
+                } else {
+                    tkey[p] = ty * (uint32_t)ntx + tx;
+                }
+                tval[p] = g;
+                if (slot_splat) slot_splat[p] = g;  // kept: the tile passes reuse tval
+            }
+            // the next window starts with the owner of this window's last slot
+            lo += (uint32_t)__builtin_amdgcn_readlane((int)rel, kWave - 1);
+            __builtin_amdgcn_wave_barrier();
         }
-        tval[p] = g;
-        if (slot_splat) slot_splat[p] = g;  // kept: the tile passes reuse tval
     }
 }
 
