@@ -452,9 +452,14 @@ __device__ __forceinline__ uint32_t tile_seg_len(uint32_t nch) {
 }
 
 // LDS: per-wave tile counters (64 KB at kTileBinsMax), the owner of each slot
-// and each owner's first slot (16-bit: both stay below kTileChunk), and per
-// splat of the chunk its id and tile box. Three memory round trips per chunk:
-// the chunk's first and last splats, their offsets and ids, their boxes.
+// (16-bit: it stays below kTileChunk) and, for a chunk of at most kTileKMax
+// splats, each splat's id, tile box and first slot: under 80 KB, so that two
+// chunks' workgroups share a CU (one per CU left half the fit's chunks to a
+// second round). A chunk of more splats (about six pairs each or fewer)
+// reads a slot's owner's first slot, id and box from global memory instead.
+// Memory round trips per chunk: the chunk's first and last splats; their
+// first slots and ids; their boxes (two more per slot, past kTileKMax).
+constexpr int kTileKMax = 640;
 __global__ __launch_bounds__(kTileThreads) void k_tile_emit(
     const uint32_t *__restrict__ order, const uint32_t *__restrict__ off,
     const uint32_t *__restrict__ chunk_first, const uint2 *__restrict__ rect, uint32_t *sc,
@@ -463,9 +468,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_emit(
     uint32_t *seg, uint2 *__restrict__ pair_base) {
     __shared__ uint16_t s_w[kTileWaves][kTileBinsMax];
     __shared__ uint16_t s_head[kTileChunk];
-    __shared__ uint16_t s_first[kTileChunk];
-    __shared__ uint32_t s_gid[kTileChunk];
-    __shared__ uint32_t s_box[kTileChunk];  // tx0 | ty0 << 10 | (tx1 - tx0) << 20
+    __shared__ uint32_t s_gid[kTileKMax];
+    __shared__ uint32_t s_box[kTileKMax];  // tx0 | ty0 << 10 | (tx1 - tx0) << 20
+    __shared__ uint16_t s_first[kTileKMax];
     __shared__ uint32_t s_red[kTileWaves];
     __shared__ uint32_t s_first0;
     const int tid = threadIdx.x, w = tid / kWave, lane = __lane_id();
@@ -493,14 +498,15 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_emit(
         const uint32_t d0 = min(chunk_first[c], n - 1);
         const uint32_t dn = (c + 1 < nch) ? min(chunk_first[c + 1], n - 1) : n - 1;
         const uint32_t K = (dn >= d0) ? min((uint32_t)kTileChunk, dn - d0 + 1) : 1u;
-        uint32_t o[kTileItems], g[kTileItems];
+        const bool in_lds = K <= (uint32_t)kTileKMax;  // uniform
+        uint32_t o[kTileItems], g[kTileItems];  // first slots and ids of the chunk's splats
 #pragma unroll
         for (int i = 0; i < kTileItems; ++i) {
             const uint32_t k = (uint32_t)(tid + i * kTileThreads);
             o[i] = g[i] = 0;
             if (k < K) {
                 o[i] = off[d0 + k];
-                g[i] = order[d0 + k];
+                if (in_lds) g[i] = order[d0 + k];
             }
         }
         __syncthreads();  // s_head zeroed
@@ -511,7 +517,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_emit(
 #pragma unroll
         for (int i = 0; i < kTileItems; ++i) {
             const uint32_t k = (uint32_t)(tid + i * kTileThreads);
-            if (k < K) {
+            if (in_lds && k < K) {
                 const uint2 r = rect[g[i]];
                 const uint32_t tx0 = r.x & 0xFFFF, tx1 = r.x >> 16, ty0 = r.y & 0xFFFF;
                 s_gid[k] = g[i];
@@ -522,13 +528,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_emit(
                     const uint32_t wd = tx1 - tx0 + 1;
                     pair_base[g[i]] = make_uint2(o[i] - (ty0 * wd + tx0), wd);
                 }
-                if (k == 0) {
+                if (k == 0)
                     s_first0 = o[i];
-                } else if (o[i] > cs && o[i] < cs + cc) {
-                    s_head[o[i] - cs] = (uint16_t)k;
+                else if (o[i] > cs && o[i] < cs + cc)
                     s_first[k] = (uint16_t)(o[i] - cs);
-                }
             }
+            if (k > 0 && k < K && o[i] > cs && o[i] < cs + cc) s_head[o[i] - cs] = (uint16_t)k;
         }
         __syncthreads();
         // Inclusive max-scan of the heads: the owner of every slot.
@@ -561,17 +566,46 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_emit(
         // Wave w takes slots [w * kTileWaveSlots, (w + 1) * kTileWaveSlots)
         // in groups of 64, in order.
         uint32_t tile[kTileGroups], gid[kTileGroups], rnk[kTileGroups];
+        if (in_lds) {
 #pragma unroll
-        for (int j = 0; j < kTileGroups; ++j) {
-            const uint32_t q = (uint32_t)(w * kTileWaveSlots + j * kWave + lane);
-            tile[j] = gid[j] = 0;
-            if (q < cc) {
-                const uint32_t k = s_head[q];
-                const uint32_t jj = (k == 0) ? cs + q - s_first0 : q - s_first[k];
-                const uint32_t b = s_box[k];
-                const uint32_t tx0 = b & 1023u, ty0 = (b >> 10) & 1023u, wd = (b >> 20) + 1;
-                gid[j] = s_gid[k];
-                tile[j] = (ty0 + jj / wd) * (uint32_t)ntx + tx0 + jj % wd;
+            for (int j = 0; j < kTileGroups; ++j) {
+                const uint32_t q = (uint32_t)(w * kTileWaveSlots + j * kWave + lane);
+                tile[j] = gid[j] = 0;
+                if (q < cc) {
+                    const uint32_t k = s_head[q];
+                    // the slot's index among its owner's pairs
+                    const uint32_t jj = (k == 0) ? cs + q - s_first0 : q - s_first[k];
+                    const uint32_t b = s_box[k];
+                    const uint32_t tx0 = b & 1023u, ty0 = (b >> 10) & 1023u, wd = (b >> 20) + 1;
+                    gid[j] = s_gid[k];
+                    tile[j] = (ty0 + jj / wd) * (uint32_t)ntx + tx0 + jj % wd;
+                }
+            }
+        } else {
+            uint32_t jj[kTileGroups];
+#pragma unroll
+            for (int j = 0; j < kTileGroups; ++j) {
+                const uint32_t q = (uint32_t)(w * kTileWaveSlots + j * kWave + lane);
+                gid[j] = jj[j] = 0;
+                if (q < cc) {
+                    const uint32_t d = d0 + s_head[q];
+                    gid[j] = order[d];
+                    jj[j] = cs + q - off[d];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kTileGroups; ++j) {
+                const uint32_t q = (uint32_t)(w * kTileWaveSlots + j * kWave + lane);
+                tile[j] = 0;
+                if (q < cc) {
+                    const uint2 r = rect[gid[j]];
+                    const uint32_t tx0 = r.x & 0xFFFF, tx1 = r.x >> 16, ty0 = r.y & 0xFFFF;
+                    const uint32_t wd = tx1 - tx0 + 1;
+                    tile[j] = (ty0 + jj[j] / wd) * (uint32_t)ntx + tx0 + jj[j] % wd;
+                    // the owner's first slot: where its backward records go
+                    if (pair_base && jj[j] == 0)
+                        pair_base[gid[j]] = make_uint2(cs + q - (ty0 * wd + tx0), wd);
+                }
             }
         }
         // Rank among the wave's earlier slots of the same tile: lanes of one
