@@ -612,17 +612,25 @@ __device__ __forceinline__ void composite_back_to_front(
 
 // The slots' records of this epoch summed per splat and added to its
 // gradient. A splat's slots are contiguous (slot_splat holds each slot's
-// splat), so each wave takes 64 consecutive slots, one per lane, and adds
-// them with a segmented inclusive scan over the lanes: shifts up by 1, 2, 4
-// and 8 inside each 16-lane row (DPP), each adding only within the run of
-// equal splats, leave every lane the sum of its run from the row's start;
-// then the odd rows add the previous row's last lane and rows 2 and 3 add
-// lane 31, each when the run continues across that row boundary. The last
-// lane of each run adds the run's sum to the splat's gradient: a plain add
-// when the splat's slots all lie inside this wave (no other wave touches
-// it), a float atomic for a splat that continues into a neighbouring wave's
-// slots (at most two per wave). With STORE the gradient is known to be zero
-// (cleared by the same step), so a whole run is stored without reading it.
+// splat): a run. Each wave takes spans of kReduceSpan consecutive slots,
+// walks a span 64 slots at a time, one per lane, and adds them with a
+// segmented inclusive scan over the lanes: shifts up by 1, 2, 4 and 8 inside
+// each 16-lane row (DPP), each adding only within the run, leave every lane
+// the sum of its run from the row's start; then the odd rows add the
+// previous row's last lane and rows 2 and 3 add lane 31, each when the run
+// continues across that row boundary. A run going on past lane 63 inside the
+// span carries its sums to the next 64 slots (the run's first lanes add
+// them). The last lane of each run's part in the span writes the sums: a
+// plain store (or add) for a run that lies inside the span (no other wave
+// touches it), a float atomic for a run that crosses the span's start or end
+// (at most two per span; device-scope float atomics on scattered words are
+// slow, which is why a span is longer than 64 slots). With STORE the gradient
+// is known to be zero (cleared by the same step), so a whole run is stored
+// without reading it.
+constexpr int kSpanWindows = 4;  // 64-slot windows per span
+constexpr uint32_t kReduceSpan = kSpanWindows * kWave;
+constexpr uint32_t kNoSplat = 0xFFFFFFFFu;
+
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
@@ -650,9 +658,10 @@ __device__ __forceinline__ void seg_scan_carry(float (&v)[9], int src, bool take
     }
 }
 
-// One wave's 64 consecutive slots: each lane's slot splat and record.
+// 64 consecutive slots of a wave: each lane's slot splat and record, and the
+// splat of the slot after the last.
 struct SlotLoad {
-    uint32_t g, g_prev, g_next;
+    uint32_t g, g_next;
     float4 r0, r1, r2;
 };
 
@@ -662,10 +671,8 @@ __device__ __forceinline__ SlotLoad load_slots(const float4 *__restrict__ rec,
     SlotLoad L;
     const uint32_t p = base + lane;
     const bool valid = p < m;
-    L.g = valid ? slot_splat[p] : 0xFFFFFFFFu;
-    // the splats of the slots just before and just after the wave's
-    L.g_prev = base > 0 ? slot_splat[base - 1] : 0xFFFFFFFFu;
-    L.g_next = base + kWave < m ? slot_splat[base + kWave] : 0xFFFFFFFEu;
+    L.g = valid ? slot_splat[p] : kNoSplat;
+    L.g_next = base + kWave < m ? slot_splat[base + kWave] : kNoSplat;
     const size_t q = valid ? 3 * (size_t)p : 0;
     L.r0 = rec[q];
     L.r1 = rec[q + 1];
@@ -682,74 +689,89 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ 
     const int lane = __lane_id();
     const uint32_t wave = (blockIdx.x * 256 + threadIdx.x) / kWave;
     const uint32_t nwaves = gridDim.x * (256 / kWave);
-    const uint32_t stride = nwaves * kWave;
     float *const dst[9] = {grad.pos_x,  grad.pos_y,   grad.cov_xx,  grad.cov_xy, grad.cov_yy,
                            grad.color_r, grad.color_g, grad.color_b, grad.opacity};
-    uint32_t base = wave * kWave;
-    if (base >= m) return;  // wave-uniform
-    // software-pipelined: the next slots' loads are in flight during this
-    // slots' scan
-    SlotLoad cur = load_slots(rec, slot_splat, base, m, lane);
-    for (;;) {  // wave-uniform
-        const uint32_t next = base + stride;
-        SlotLoad nxt;
-        if (next < m) nxt = load_slots(rec, slot_splat, next, m, lane);
-        const uint32_t p = base + lane;
-        const bool valid = p < m;
-        const uint32_t g = cur.g;
-        float v[9];
-        const bool on = valid && __float_as_uint(cur.r2.y) == ep;
-        v[0] = on ? cur.r0.x : 0.f;
-        v[1] = on ? cur.r0.y : 0.f;
-        v[2] = on ? cur.r0.z : 0.f;
-        v[3] = on ? cur.r0.w : 0.f;
-        v[4] = on ? cur.r1.x : 0.f;
-        v[5] = on ? cur.r1.y : 0.f;
-        v[6] = on ? cur.r1.z : 0.f;
-        v[7] = on ? cur.r1.w : 0.f;
-        v[8] = on ? cur.r2.x : 0.f;
-        // lane 0's splat, read while every lane is active (lane 0 always is)
-        const uint32_t g0 = (uint32_t)__builtin_amdgcn_readfirstlane(g);
-        // runs of equal g are contiguous: lanes a < b hold the same splat
-        // exactly when every lane between does
-        seg_scan_row_step<1>(v, g, lane);
-        seg_scan_row_step<2>(v, g, lane);
-        seg_scan_row_step<4>(v, g, lane);
-        seg_scan_row_step<8>(v, g, lane);
-        {  // rows 1 and 3: the run's part in rows 0 and 2 (lanes 15, 47)
-            const int row0 = lane & ~15;
-            const int src = row0 > 0 ? row0 - 1 : 0;
-            const uint32_t g_src = __shfl(g, src, kWave), g_row0 = __shfl(g, row0, kWave);
-            seg_scan_carry(v, src, ((lane >> 4) & 1) && g_src == g && g_row0 == g);
-        }
-        {  // rows 2 and 3: the run's part in rows 0 and 1 (lane 31)
-            const uint32_t g31 = __shfl(g, 31, kWave), g32 = __shfl(g, 32, kWave);
-            seg_scan_carry(v, 31, lane >= 32 && g31 == g && g32 == g);
-        }
-        const uint32_t gn = __shfl_down(g, 1, kWave);
-        const bool run_end = valid && (lane == kWave - 1 || gn != g);
-        if (run_end) {
-            // does the run start in an earlier wave's slots, or end in a later one's?
-            const bool starts_before = g == g0 && cur.g_prev == g;
-            const bool ends_after = lane == kWave - 1 && cur.g_next == g;
-            if (starts_before || ends_after) {
-#pragma unroll
-                for (int f = 0; f < 9; ++f)
-                    if (v[f] != 0.f) atomicAdd(dst[f] + g, v[f]);
-            } else {
-#pragma unroll
-                for (int f = 0; f < 9; ++f)
-                    if (v[f] != 0.f) {
-                        if constexpr (STORE)
-                            dst[f][g] = v[f];
-                        else
-                            dst[f][g] += v[f];
-                    }
+    for (uint32_t s0 = wave * kReduceSpan; s0 < m; s0 += nwaves * kReduceSpan) {  // uniform
+        const uint32_t s1 = min(s0 + kReduceSpan, m);
+        // the runs crossing the span's start and its end (if any)
+        const uint32_t g_lead = s0 > 0 ? slot_splat[s0 - 1] : kNoSplat;
+        const uint32_t g_after = s1 < m ? slot_splat[s1] : kNoSplat;
+        float carry[9];
+        uint32_t carry_g = kNoSplat;  // the run whose sums carry into the next slots
+        // software-pipelined: the span's next slots load during this slots'
+        // scan (loading the whole span at once was slower)
+        SlotLoad cur = load_slots(rec, slot_splat, s0, m, lane);
+        for (uint32_t base = s0;;) {  // wave-uniform
+            const uint32_t next = base + kWave;
+            SlotLoad nxt;
+            if (next < s1) nxt = load_slots(rec, slot_splat, next, m, lane);
+            const uint32_t p = base + lane;
+            const bool valid = p < m;
+            const uint32_t g = cur.g;
+            float v[9];
+            const bool on = valid && __float_as_uint(cur.r2.y) == ep;
+            v[0] = on ? cur.r0.x : 0.f;
+            v[1] = on ? cur.r0.y : 0.f;
+            v[2] = on ? cur.r0.z : 0.f;
+            v[3] = on ? cur.r0.w : 0.f;
+            v[4] = on ? cur.r1.x : 0.f;
+            v[5] = on ? cur.r1.y : 0.f;
+            v[6] = on ? cur.r1.z : 0.f;
+            v[7] = on ? cur.r1.w : 0.f;
+            v[8] = on ? cur.r2.x : 0.f;
+            // runs of equal g are contiguous: lanes a < b hold the same splat
+            // exactly when every lane between does
+            seg_scan_row_step<1>(v, g, lane);
+            seg_scan_row_step<2>(v, g, lane);
+            seg_scan_row_step<4>(v, g, lane);
+            seg_scan_row_step<8>(v, g, lane);
+            {  // rows 1 and 3: the run's part in rows 0 and 2 (lanes 15, 47)
+                const int row0 = lane & ~15;
+                const int src = row0 > 0 ? row0 - 1 : 0;
+                const uint32_t g_src = __shfl(g, src, kWave), g_row0 = __shfl(g, row0, kWave);
+                seg_scan_carry(v, src, ((lane >> 4) & 1) && g_src == g && g_row0 == g);
             }
+            {  // rows 2 and 3: the run's part in rows 0 and 1 (lane 31)
+                const uint32_t g31 = __shfl(g, 31, kWave), g32 = __shfl(g, 32, kWave);
+                seg_scan_carry(v, 31, lane >= 32 && g31 == g && g32 == g);
+            }
+            if (carry_g != kNoSplat) {  // the run continuing from the previous slots
+#pragma unroll
+                for (int f = 0; f < 9; ++f) v[f] += g == carry_g ? carry[f] : 0.f;
+            }
+            // the last lane of a run's part in the span: the next slot holds
+            // another splat or lies past the span
+            const uint32_t gn = __shfl_down(g, 1, kWave);
+            const bool last = p + 1 == s1 || (lane == kWave - 1 ? cur.g_next != g : gn != g);
+            if (valid && last) {
+                if (g == g_lead || (p + 1 == s1 && g == g_after)) {
+#pragma unroll
+                    for (int f = 0; f < 9; ++f)
+                        if (v[f] != 0.f) atomicAdd(dst[f] + g, v[f]);
+                } else {
+#pragma unroll
+                    for (int f = 0; f < 9; ++f)
+                        if (v[f] != 0.f) {
+                            if constexpr (STORE)
+                                dst[f][g] = v[f];
+                            else
+                                dst[f][g] += v[f];
+                        }
+                }
+            }
+            if (next >= s1) break;
+            // a run going on past lane 63 (inside the span) carries its sums
+            const uint32_t g63 = (uint32_t)__builtin_amdgcn_readlane((int)g, kWave - 1);
+            carry_g = cur.g_next == g63 ? g63 : kNoSplat;
+            if (carry_g != kNoSplat) {
+#pragma unroll
+                for (int f = 0; f < 9; ++f)
+                    carry[f] = __int_as_float(
+                        __builtin_amdgcn_readlane(__float_as_int(v[f]), kWave - 1));
+            }
+            base = next;
+            cur = nxt;
         }
-        if (next >= m) break;
-        base = next;
-        cur = nxt;
     }
 }
 
@@ -1002,9 +1024,10 @@ ds_status launch_pair_reduce(const float4 *pair_rec, const uint32_t *slot_splat,
                              const uint32_t *nisect, const uint32_t *epoch, int64_t slot_cap,
                              const ds_splat2d &grad, bool grad_is_zero, hipStream_t s) {
     if (slot_cap <= 0) return DS_OK;
-    // a wave per 64 slots up to 16384 waves, each then walking on (grid-stride,
-    // the next slots' loads overlapping this slots' scan)
-    const int blocks = (int)std::min<int64_t>((slot_cap + 255) / 256, 4096);
+    // a wave per span of kReduceSpan slots up to 16384 waves, each then
+    // taking the span a grid further on
+    const int64_t spans = (slot_cap + kReduceSpan - 1) / kReduceSpan;
+    const int blocks = (int)std::min<int64_t>((spans + 3) / 4, 4096);
     if (grad_is_zero)
         hipLaunchKernelGGL(k_pair_reduce<true>, dim3(blocks), dim3(256), 0, s, pair_rec,
                            slot_splat, nisect, epoch, grad);

@@ -59,6 +59,9 @@ SCENES = {
     "c0": (0, 4096, 256, 256, -3.0),
     "ragged": (5, 1500, 200, 120, -2.6),
     "needles": (9, 2000, 200, 120, -2.8),
+    # splats covering up to 960 of 1024 tiles: a splat's pair slots span
+    # several of the gradient reduction's 256-slot spans
+    "wide": (12, 400, 512, 512, -1.2),
 }
 
 
@@ -211,7 +214,7 @@ def test_composite_forward(scene, ppt):
 
 @pytest.mark.parametrize("ppt", [1, 2, 4])
 @pytest.mark.parametrize("grad_mode", [0, 1])
-@pytest.mark.parametrize("scene", ["c0", "ragged", "needles"])
+@pytest.mark.parametrize("scene", ["c0", "ragged", "needles", "wide"])
 def test_composite_backward(scene, grad_mode, ppt):
     seed, n, W, H, ls = SCENES[scene]
     params, view, proj, cam = scene_params(scene)
@@ -472,11 +475,15 @@ def test_mse_and_adam():
         check_close(f"adam_v_{len(p0)}", dev[3].download(np.float32, len(p0)), rv)
 
 
-@pytest.mark.parametrize("fused,grad_mode", [(True, 0), (False, 0), (True, 1), (False, 1)])
-def test_fitter_step_matches_oracle_chain(fused, grad_mode):
-    """One full ImageFitter step: gradients and post-Adam parameters."""
-    seed, n, W, H, ls = SCENES["c0"]
-    params, view, proj, cam = scene_params("c0")
+@pytest.mark.parametrize("scene,fused,grad_mode", [("c0", True, 0), ("c0", False, 0),
+                                                   ("c0", True, 1), ("c0", False, 1),
+                                                   ("wide", True, 0), ("wide", False, 1)])
+def test_fitter_step_matches_oracle_chain(scene, fused, grad_mode):
+    """One full ImageFitter step: gradients and post-Adam parameters. The step
+    stores each splat's summed gradient records into the cleared gradient
+    ("wide": sums over runs of pair slots longer than a reduction span)."""
+    seed, n, W, H, ls = SCENES[scene]
+    params, view, proj, cam = scene_params(scene)
     target = ds.target_image(seed, W, H, 32)
     lr = {"lr_pos": 1e-2, "lr_sh_dc": 1e-2, "lr_scale": 5e-3}
     fit = ds.ImageFitter(W, H, n, view=view, proj=proj, cam_pos=cam, lr=lr, fused=fused,
@@ -490,7 +497,7 @@ def test_fitter_step_matches_oracle_chain(fused, grad_mode):
     fw = fp64_forward(params.astype(np.float64), n, 0, cam, pv, W, H, target)
     bw = fp64_backward(fw, params.astype(np.float64), n, 0, cam, pv, W, H, grad_mode=grad_mode)
     mask = np.ones(14 * n, bool)
-    check_close(f"fitter_grads_fused{int(fused)}_mode{grad_mode}", g_gpu, bw["gparams"])
+    check_close(f"fitter_grads_{scene}_fused{int(fused)}_mode{grad_mode}", g_gpu, bw["gparams"])
     # post-Adam: first step moves each parameter by about lr * sign(g); compare where
     # |g| is well above the fp32 noise floor of the gradient
     lr_f = np.concatenate([np.full(3 * n, 1e-2), np.full(3 * n, 5e-3), np.full(4 * n, 6e-4),
@@ -500,7 +507,7 @@ def test_fitter_step_matches_oracle_chain(fused, grad_mode):
     v = 0.001 * g64 * g64
     p_ref = params.astype(np.float64) - lr_f * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
     strong = mask & (np.abs(g64) > 1e-3 * np.abs(g64).max())
-    check_close(f"fitter_adam_fused{int(fused)}_mode{grad_mode}", p_gpu, p_ref, strong)
+    check_close(f"fitter_adam_{scene}_fused{int(fused)}_mode{grad_mode}", p_gpu, p_ref, strong)
 
 
 def test_gradcheck_scene_against_finite_differences():
