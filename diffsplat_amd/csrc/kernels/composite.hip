@@ -719,6 +719,31 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ 
             v[6] = on ? cur.r1.z : 0.f;
             v[7] = on ? cur.r1.w : 0.f;
             v[8] = on ? cur.r2.x : 0.f;
+            // the last lane of a run's part in the span (the next slot holds
+            // another splat or lies past the span) writes it: atomically if
+            // the run crosses the span's start or end
+            const uint32_t gn = __shfl_down(g, 1, kWave);
+            const bool last = p + 1 == s1 || (lane == kWave - 1 ? cur.g_next != g : gn != g);
+            const bool atomic = g == g_lead || (p + 1 == s1 && g == g_after);
+            const bool plain = valid && last && !atomic;
+            // adding to the gradient: its old values load during the scan,
+            // for the runs with a record of this epoch (or a carry) only
+            float old[9];
+#pragma unroll
+            for (int f = 0; f < 9; ++f) old[f] = 0.f;
+            if constexpr (!STORE) {
+                const uint64_t onm = __ballot(on);
+                const uint32_t gp = __shfl_up(g, 1, kWave);
+                const uint64_t firsts = __ballot(lane == 0 || gp != g);
+                // lanes [s, lane] hold the run ending at this lane
+                const uint64_t upto = lane == kWave - 1 ? ~0ull : (2ull << lane) - 1ull;
+                const int s = 63 - (int)__builtin_clzll(firsts & upto);
+                const bool any = ((onm & upto) >> s) != 0 || g == carry_g;
+                if (plain && any) {
+#pragma unroll
+                    for (int f = 0; f < 9; ++f) old[f] = dst[f][g];
+                }
+            }
             // runs of equal g are contiguous: lanes a < b hold the same splat
             // exactly when every lane between does
             seg_scan_row_step<1>(v, g, lane);
@@ -739,25 +764,14 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ 
 #pragma unroll
                 for (int f = 0; f < 9; ++f) v[f] += g == carry_g ? carry[f] : 0.f;
             }
-            // the last lane of a run's part in the span: the next slot holds
-            // another splat or lies past the span
-            const uint32_t gn = __shfl_down(g, 1, kWave);
-            const bool last = p + 1 == s1 || (lane == kWave - 1 ? cur.g_next != g : gn != g);
-            if (valid && last) {
-                if (g == g_lead || (p + 1 == s1 && g == g_after)) {
+            if (plain) {
 #pragma unroll
-                    for (int f = 0; f < 9; ++f)
-                        if (v[f] != 0.f) atomicAdd(dst[f] + g, v[f]);
-                } else {
+                for (int f = 0; f < 9; ++f)
+                    if (v[f] != 0.f) dst[f][g] = STORE ? v[f] : old[f] + v[f];
+            } else if (valid && last) {
 #pragma unroll
-                    for (int f = 0; f < 9; ++f)
-                        if (v[f] != 0.f) {
-                            if constexpr (STORE)
-                                dst[f][g] = v[f];
-                            else
-                                dst[f][g] += v[f];
-                        }
-                }
+                for (int f = 0; f < 9; ++f)
+                    if (v[f] != 0.f) atomicAdd(dst[f] + g, v[f]);
             }
             if (next >= s1) break;
             // a run going on past lane 63 (inside the span) carries its sums
