@@ -612,22 +612,22 @@ __device__ __forceinline__ void composite_back_to_front(
 
 // The slots' records of this epoch summed per splat and added to its
 // gradient. A splat's slots are contiguous (slot_splat holds each slot's
-// splat): a run. Each wave takes spans of kReduceSpan consecutive slots,
-// walks a span 64 slots at a time, one per lane, and adds them with a
+// splat): a run. Each workgroup takes spans of kReduceSpan consecutive slots,
+// one 64-slot window per wave, one slot per lane, and adds them with a
 // segmented inclusive scan over the lanes: shifts up by 1, 2, 4 and 8 inside
 // each 16-lane row (DPP), each adding only within the run, leave every lane
 // the sum of its run from the row's start; then the odd rows add the
 // previous row's last lane and rows 2 and 3 add lane 31, each when the run
-// continues across that row boundary. A run going on past lane 63 inside the
-// span carries its sums to the next 64 slots (the run's first lanes add
-// them). The last lane of each run's part in the span writes the sums: a
-// plain store (or add) for a run that lies inside the span (no other wave
-// touches it), a float atomic for a run that crosses the span's start or end
-// (at most two per span; device-scope float atomics on scattered words are
-// slow, which is why a span is longer than 64 slots). With STORE the gradient
-// is known to be zero (cleared by the same step), so a whole run is stored
-// without reading it.
-constexpr int kSpanWindows = 4;  // 64-slot windows per span
+// continues across that row boundary. A run going on from an earlier window
+// of the span gets those windows' sums through LDS (each wave posts its last
+// lane's sum and splat). The last lane of each run's part in the span writes
+// the sums: a plain store (or add) for a run that lies inside the span (no
+// other workgroup touches it), a float atomic for a run that crosses the
+// span's start or end (at most two per span; device-scope float atomics on
+// scattered words are slow, which is why a span is longer than one window).
+// With STORE the gradient is known to be zero (cleared by the same step), so a
+// whole run is stored without reading it.
+constexpr int kSpanWindows = 4;  // 64-slot windows per span, one per wave
 constexpr uint32_t kReduceSpan = kSpanWindows * kWave;
 constexpr uint32_t kNoSplat = 0xFFFFFFFFu;
 
@@ -681,34 +681,39 @@ __device__ __forceinline__ SlotLoad load_slots(const float4 *__restrict__ rec,
 }
 
 template <bool STORE>
-__global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ rec,
-                                                     const uint32_t *__restrict__ slot_splat,
-                                                     const uint32_t *nisect,
-                                                     const uint32_t *epoch, ds_splat2d grad) {
+__global__ __launch_bounds__(kReduceSpan) void k_pair_reduce(
+    const float4 *__restrict__ rec, const uint32_t *__restrict__ slot_splat,
+    const uint32_t *nisect, const uint32_t *epoch, ds_splat2d grad) {
+    // per wave of the span: its last lane's splat, whether that lane's run
+    // covers the whole window, and its sums
+    __shared__ uint32_t s_g[kSpanWindows];
+    __shared__ uint32_t s_whole[kSpanWindows];
+    __shared__ float s_sum[kSpanWindows][9];
     const uint32_t m = *nisect, ep = *epoch;
-    const int lane = __lane_id();
-    const uint32_t wave = (blockIdx.x * 256 + threadIdx.x) / kWave;
-    const uint32_t nwaves = gridDim.x * (256 / kWave);
+    const int lane = __lane_id(), w = threadIdx.x / kWave;
     float *const dst[9] = {grad.pos_x,  grad.pos_y,   grad.cov_xx,  grad.cov_xy, grad.cov_yy,
                            grad.color_r, grad.color_g, grad.color_b, grad.opacity};
-    for (uint32_t s0 = wave * kReduceSpan; s0 < m; s0 += nwaves * kReduceSpan) {  // uniform
+    for (uint32_t s0 = blockIdx.x * kReduceSpan; s0 < m;
+         s0 += gridDim.x * kReduceSpan) {  // workgroup-uniform
         const uint32_t s1 = min(s0 + kReduceSpan, m);
+        const uint32_t base = s0 + w * kWave;
         // the runs crossing the span's start and its end (if any)
         const uint32_t g_lead = s0 > 0 ? slot_splat[s0 - 1] : kNoSplat;
         const uint32_t g_after = s1 < m ? slot_splat[s1] : kNoSplat;
-        float carry[9];
-        uint32_t carry_g = kNoSplat;  // the run whose sums carry into the next slots
-        // software-pipelined: the span's next slots load during this slots'
-        // scan (loading the whole span at once was slower)
-        SlotLoad cur = load_slots(rec, slot_splat, s0, m, lane);
-        for (uint32_t base = s0;;) {  // wave-uniform
-            const uint32_t next = base + kWave;
-            SlotLoad nxt;
-            if (next < s1) nxt = load_slots(rec, slot_splat, next, m, lane);
+        const bool live = base < s1;  // wave-uniform
+        uint32_t g = kNoSplat;
+        float v[9];
+#pragma unroll
+        for (int f = 0; f < 9; ++f) v[f] = 0.f;
+        bool valid = false, last = false, atomic = false, plain = false;
+        float old[9];
+#pragma unroll
+        for (int f = 0; f < 9; ++f) old[f] = 0.f;
+        if (live) {
+            const SlotLoad cur = load_slots(rec, slot_splat, base, m, lane);
             const uint32_t p = base + lane;
-            const bool valid = p < m;
-            const uint32_t g = cur.g;
-            float v[9];
+            valid = p < m;
+            g = cur.g;
             const bool on = valid && __float_as_uint(cur.r2.y) == ep;
             v[0] = on ? cur.r0.x : 0.f;
             v[1] = on ? cur.r0.y : 0.f;
@@ -723,22 +728,19 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ 
             // another splat or lies past the span) writes it: atomically if
             // the run crosses the span's start or end
             const uint32_t gn = __shfl_down(g, 1, kWave);
-            const bool last = p + 1 == s1 || (lane == kWave - 1 ? cur.g_next != g : gn != g);
-            const bool atomic = g == g_lead || (p + 1 == s1 && g == g_after);
-            const bool plain = valid && last && !atomic;
-            // adding to the gradient: its old values load during the scan,
-            // for the runs with a record of this epoch (or a carry) only
-            float old[9];
-#pragma unroll
-            for (int f = 0; f < 9; ++f) old[f] = 0.f;
+            last = p + 1 == s1 || (lane == kWave - 1 ? cur.g_next != g : gn != g);
+            atomic = g == g_lead || (p + 1 == s1 && g == g_after);
+            plain = valid && last && !atomic;
+            const uint32_t gp = __shfl_up(g, 1, kWave);
+            const uint64_t firsts = __ballot(lane == 0 || gp != g);
             if constexpr (!STORE) {
+                // adding to the gradient: its old values load during the scan,
+                // for the runs with a record of this epoch or an earlier
+                // window's part
                 const uint64_t onm = __ballot(on);
-                const uint32_t gp = __shfl_up(g, 1, kWave);
-                const uint64_t firsts = __ballot(lane == 0 || gp != g);
-                // lanes [s, lane] hold the run ending at this lane
                 const uint64_t upto = lane == kWave - 1 ? ~0ull : (2ull << lane) - 1ull;
-                const int s = 63 - (int)__builtin_clzll(firsts & upto);
-                const bool any = ((onm & upto) >> s) != 0 || g == carry_g;
+                const int st = 63 - (int)__builtin_clzll(firsts & upto);
+                const bool any = ((onm & upto) >> st) != 0 || st == 0;
                 if (plain && any) {
 #pragma unroll
                     for (int f = 0; f < 9; ++f) old[f] = dst[f][g];
@@ -760,10 +762,27 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ 
                 const uint32_t g31 = __shfl(g, 31, kWave), g32 = __shfl(g, 32, kWave);
                 seg_scan_carry(v, 31, lane >= 32 && g31 == g && g32 == g);
             }
-            if (carry_g != kNoSplat) {  // the run continuing from the previous slots
+            if (lane == kWave - 1) {
+                s_g[w] = g;
+                s_whole[w] = firsts == 1ull;  // one run over the whole window
 #pragma unroll
-                for (int f = 0; f < 9; ++f) v[f] += g == carry_g ? carry[f] : 0.f;
+                for (int f = 0; f < 9; ++f) s_sum[w][f] = v[f];
             }
+        }
+        __syncthreads();
+        if (live) {
+            // the part of this window's first run in the span's earlier windows
+            const uint32_t g0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)g);
+            float carry[9];
+#pragma unroll
+            for (int f = 0; f < 9; ++f) carry[f] = 0.f;
+            for (int k = w - 1; k >= 0 && s_g[k] == g0; --k) {  // uniform
+#pragma unroll
+                for (int f = 0; f < 9; ++f) carry[f] += s_sum[k][f];
+                if (!s_whole[k]) break;
+            }
+#pragma unroll
+            for (int f = 0; f < 9; ++f) v[f] += g == g0 ? carry[f] : 0.f;
             if (plain) {
 #pragma unroll
                 for (int f = 0; f < 9; ++f)
@@ -773,19 +792,8 @@ __global__ __launch_bounds__(256) void k_pair_reduce(const float4 *__restrict__ 
                 for (int f = 0; f < 9; ++f)
                     if (v[f] != 0.f) atomicAdd(dst[f] + g, v[f]);
             }
-            if (next >= s1) break;
-            // a run going on past lane 63 (inside the span) carries its sums
-            const uint32_t g63 = (uint32_t)__builtin_amdgcn_readlane((int)g, kWave - 1);
-            carry_g = cur.g_next == g63 ? g63 : kNoSplat;
-            if (carry_g != kNoSplat) {
-#pragma unroll
-                for (int f = 0; f < 9; ++f)
-                    carry[f] = __int_as_float(
-                        __builtin_amdgcn_readlane(__float_as_int(v[f]), kWave - 1));
-            }
-            base = next;
-            cur = nxt;
         }
+        __syncthreads();  // the LDS is reused by the next span
     }
 }
 
@@ -1038,16 +1046,16 @@ ds_status launch_pair_reduce(const float4 *pair_rec, const uint32_t *slot_splat,
                              const uint32_t *nisect, const uint32_t *epoch, int64_t slot_cap,
                              const ds_splat2d &grad, bool grad_is_zero, hipStream_t s) {
     if (slot_cap <= 0) return DS_OK;
-    // a wave per span of kReduceSpan slots up to 16384 waves, each then
-    // taking the span a grid further on
+    // a workgroup per span of kReduceSpan slots up to 16384 workgroups, each
+    // then taking the span a grid further on
     const int64_t spans = (slot_cap + kReduceSpan - 1) / kReduceSpan;
-    const int blocks = (int)std::min<int64_t>((spans + 3) / 4, 4096);
+    const int blocks = (int)std::min<int64_t>(spans, 16384);
     if (grad_is_zero)
-        hipLaunchKernelGGL(k_pair_reduce<true>, dim3(blocks), dim3(256), 0, s, pair_rec,
+        hipLaunchKernelGGL(k_pair_reduce<true>, dim3(blocks), dim3(kReduceSpan), 0, s, pair_rec,
                            slot_splat, nisect, epoch, grad);
     else
-        hipLaunchKernelGGL(k_pair_reduce<false>, dim3(blocks), dim3(256), 0, s, pair_rec,
-                           slot_splat, nisect, epoch, grad);
+        hipLaunchKernelGGL(k_pair_reduce<false>, dim3(blocks), dim3(kReduceSpan), 0, s,
+                           pair_rec, slot_splat, nisect, epoch, grad);
     DS_CHECK_LAUNCH("pair_reduce");
     return DS_OK;
 }
