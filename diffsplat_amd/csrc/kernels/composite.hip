@@ -680,10 +680,80 @@ __device__ __forceinline__ SlotLoad load_slots(const float4 *__restrict__ rec,
     return L;
 }
 
+// The next forward's dispatch order (NextOrder) by one workgroup: pass 1
+// buckets every tile (kept in LDS) and counts the buckets per wave (the lanes
+// of one bucket found with five ballots, their first lane adding their
+// number to the wave's row); a descending scan gives each bucket's first
+// position; pass 2 reserves each wave's positions with one LDS atomic per
+// bucket.
+constexpr int kOrderLoads = 8;  // ranges in flight per thread
+__device__ __forceinline__ void next_order_block(const NextOrder o) {
+    constexpr int NT = kReduceSpan, NW = NT / kWave;
+    __shared__ uint32_t h[32];
+    __shared__ uint32_t wh[NW][32];
+    __shared__ uint8_t s_b[kNextOrderMaxTiles];
+    const int tid = threadIdx.x, w = tid / kWave;
+    const uint64_t lt = lanemask_lt();
+    for (int i = tid; i < NW * 32; i += NT) (&wh[0][0])[i] = 0u;
+    __syncthreads();
+    for (int t0 = 0; t0 < o.ntiles; t0 += NT * kOrderLoads) {
+        int2 r[kOrderLoads];
+#pragma unroll
+        for (int u = 0; u < kOrderLoads; ++u) {
+            const int t = t0 + u * NT + tid;
+            r[u] = t < o.ntiles ? o.ranges[t] : make_int2(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < kOrderLoads; ++u) {
+            const int t = t0 + u * NT + tid;
+            const bool valid = t < o.ntiles;
+            const uint32_t b = tile_bucket(r[u]);
+            if (valid) s_b[t] = (uint8_t)b;
+            const uint64_t peers = same_bucket(b, valid);
+            if (valid && (peers & lt) == 0) wh[w][b] += (uint32_t)__popcll(peers);
+        }
+    }
+    __syncthreads();
+    if (tid < 32) {  // bucket totals
+        uint32_t v = 0;
+        for (int q = 0; q < NW; ++q) v += wh[q][tid];
+        h[tid] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {  // descending exclusive scan: the first position of each bucket
+        uint32_t run = 0;
+        for (int b = 31; b >= 0; --b) {
+            const uint32_t v = h[b];
+            h[b] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    for (int t0 = 0; t0 < o.ntiles; t0 += NT) {
+        const int t = t0 + tid;
+        const bool valid = t < o.ntiles;
+        const uint32_t b = valid ? (uint32_t)s_b[t] : 0u;
+        const uint64_t peers = same_bucket(b, valid);
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        uint32_t base = 0;
+        if (valid && rank == 0) base = atomicAdd(&h[b], (uint32_t)__popcll(peers));
+        const int leader = valid ? __ffsll((unsigned long long)peers) - 1 : 0;
+        base = (uint32_t)__shfl((int)base, leader, kWave);
+        if (valid) o.inv[t] = base + rank;
+    }
+}
+
 template <bool STORE>
 __global__ __launch_bounds__(kReduceSpan) void k_pair_reduce(
     const float4 *__restrict__ rec, const uint32_t *__restrict__ slot_splat,
-    const uint32_t *nisect, const uint32_t *epoch, ds_splat2d grad) {
+    const uint32_t *nisect, const uint32_t *epoch, ds_splat2d grad, NextOrder next) {
+    // with a next order to compute, workgroup 0 does that and the spans
+    // start at workgroup 1
+    const uint32_t first = next.ranges ? 1u : 0u;
+    if (blockIdx.x < first) {
+        next_order_block(next);
+        return;
+    }
     // per wave of the span: its last lane's splat, whether that lane's run
     // covers the whole window, and its sums
     __shared__ uint32_t s_g[kSpanWindows];
@@ -693,8 +763,8 @@ __global__ __launch_bounds__(kReduceSpan) void k_pair_reduce(
     const int lane = __lane_id(), w = threadIdx.x / kWave;
     float *const dst[9] = {grad.pos_x,  grad.pos_y,   grad.cov_xx,  grad.cov_xy, grad.cov_yy,
                            grad.color_r, grad.color_g, grad.color_b, grad.opacity};
-    for (uint32_t s0 = blockIdx.x * kReduceSpan; s0 < m;
-         s0 += gridDim.x * kReduceSpan) {  // workgroup-uniform
+    for (uint32_t s0 = (blockIdx.x - first) * kReduceSpan; s0 < m;
+         s0 += (gridDim.x - first) * kReduceSpan) {  // workgroup-uniform
         const uint32_t s1 = min(s0 + kReduceSpan, m);
         const uint32_t base = s0 + w * kWave;
         // the runs crossing the span's start and its end (if any)
@@ -1044,18 +1114,21 @@ ds_status launch_composite_fwd(const CompositeArgs &a, int pix_per_thread, float
 
 ds_status launch_pair_reduce(const float4 *pair_rec, const uint32_t *slot_splat,
                              const uint32_t *nisect, const uint32_t *epoch, int64_t slot_cap,
-                             const ds_splat2d &grad, bool grad_is_zero, hipStream_t s) {
-    if (slot_cap <= 0) return DS_OK;
+                             const ds_splat2d &grad, bool grad_is_zero, hipStream_t s,
+                             NextOrder next) {
+    if (next.ranges && next.ntiles > kNextOrderMaxTiles)
+        return set_error(DS_ERR_INVALID_ARG, "next dispatch order over more than 4096 tiles");
+    if (slot_cap <= 0 && !next.ranges) return DS_OK;
     // a workgroup per span of kReduceSpan slots up to 16384 workgroups, each
     // then taking the span a grid further on
-    const int64_t spans = (slot_cap + kReduceSpan - 1) / kReduceSpan;
-    const int blocks = (int)std::min<int64_t>(spans, 16384);
+    const int64_t spans = (std::max<int64_t>(slot_cap, 1) + kReduceSpan - 1) / kReduceSpan;
+    const int blocks = (int)std::min<int64_t>(spans, 16384) + (next.ranges ? 1 : 0);
     if (grad_is_zero)
         hipLaunchKernelGGL(k_pair_reduce<true>, dim3(blocks), dim3(kReduceSpan), 0, s, pair_rec,
-                           slot_splat, nisect, epoch, grad);
+                           slot_splat, nisect, epoch, grad, next);
     else
         hipLaunchKernelGGL(k_pair_reduce<false>, dim3(blocks), dim3(kReduceSpan), 0, s,
-                           pair_rec, slot_splat, nisect, epoch, grad);
+                           pair_rec, slot_splat, nisect, epoch, grad, next);
     DS_CHECK_LAUNCH("pair_reduce");
     return DS_OK;
 }

@@ -76,8 +76,41 @@ ds_status launch_composite_bwd(const CompositeArgs &a, int pix_per_thread, int g
 // splat's slots are contiguous. slot_cap bounds *nisect (grid size).
 // grad_is_zero: grad was cleared earlier on the stream and nothing else adds
 // to it, so each splat's sum is stored.
+// Heavy-first dispatch order of the compositing workgroups (tiles bucketed by
+// floor(log2 pairs) + 1, empty tiles 0, buckets in descending order; the
+// order within a bucket is arbitrary), computed by one workgroup of the pair
+// reduction from the tile ranges of the forward it follows, for the next
+// forward: inv[t] is the dispatch position of tile t (a permutation; the
+// next forward writes the order and each position's range from it).
+// ranges == nullptr: nothing to compute. At most kNextOrderMaxTiles tiles.
+constexpr int kNextOrderMaxTiles = 4096;
+struct NextOrder {
+    const int2 *ranges;
+    int ntiles;
+    uint32_t *inv;
+};
+
+__device__ __forceinline__ uint32_t tile_bucket(int2 r) {
+    const uint32_t c = (uint32_t)max(r.y - r.x, 0);
+    return c ? (uint32_t)min(31, 32 - __clz(c)) : 0u;
+}
+
+// Lanes of this wave (among `valid` ones) whose bucket equals this lane's.
+__device__ __forceinline__ uint64_t same_bucket(uint32_t b, bool valid) {
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 5; ++bit) {
+        const bool on = (b >> bit) & 1u;
+        const uint64_t bb = __ballot(on);
+        peers &= on ? bb : ~bb;
+    }
+    return peers;
+}
+
+// next: also compute the next forward's dispatch order (see NextOrder).
 ds_status launch_pair_reduce(const float4 *pair_rec, const uint32_t *slot_splat,
                              const uint32_t *nisect, const uint32_t *epoch, int64_t slot_cap,
-                             const ds_splat2d &grad, bool grad_is_zero, hipStream_t s);
+                             const ds_splat2d &grad, bool grad_is_zero, hipStream_t s,
+                             NextOrder next = NextOrder{nullptr, 0, nullptr});
 
 }  // namespace ds

@@ -674,13 +674,17 @@ __global__ __launch_bounds__(256) void k_tile_segscan(uint32_t *__restrict__ seg
 
 // Workgroup (x, s): tiles [256 x, 256 x + 256) in segment s. The tile's first
 // output slot is the exclusive scan of the totals; segment 0 writes the
-// ranges of the non-empty tiles (empty ones stay (0, 0)); the counts of the
+// ranges of the non-empty tiles (empty ones stay (0, 0)) and, with inv, every
+// tile's range at its position in the dispatch order; the counts of the
 // segment's chunks become first slots.
 __global__ __launch_bounds__(256) void k_tile_offsets(uint32_t *__restrict__ cnt,
                                                       const uint32_t *__restrict__ seg,
                                                       const uint32_t *__restrict__ tot,
                                                       const uint32_t *sc, int ntiles,
-                                                      int2 *__restrict__ ranges) {
+                                                      int2 *__restrict__ ranges,
+                                                      const uint32_t *__restrict__ inv,
+                                                      uint32_t *__restrict__ order,
+                                                      int2 *__restrict__ order_range) {
     __shared__ uint32_t s_tmp[256 / kWave];
     const uint32_t m = sc[SC_NISECT];
     const uint32_t nch = (m + kTileChunk - 1) / kTileChunk;
@@ -695,7 +699,15 @@ __global__ __launch_bounds__(256) void k_tile_offsets(uint32_t *__restrict__ cnt
     const uint32_t mine = (t < ntiles) ? tot[t] : 0u;
     const uint32_t start = base + block_excl_scan<256>(mine, s_tmp, nullptr);
     if (t >= ntiles) return;
-    if (s == 0 && mine) ranges[t] = make_int2((int)start, (int)(start + mine));
+    const int2 rt = mine ? make_int2((int)start, (int)(start + mine)) : make_int2(0, 0);
+    if (s == 0 && mine) ranges[t] = rt;
+    // a dispatch order computed before these ranges (NextOrder): each tile
+    // and its range at its position
+    if (s == 0 && inv) {
+        const uint32_t pos = inv[t];
+        order[pos] = (uint32_t)t;
+        order_range[pos] = rt;
+    }
     uint32_t run = start + seg[(size_t)s * ntiles + t];
     const uint32_t c1 = min(s * G + G, nch);
     for (uint32_t c = s * G; c < c1; c += 8) {
@@ -749,26 +761,10 @@ constexpr int kOrderThreads = 1024;
 constexpr int kOrderWaves = kOrderThreads / kWave;
 constexpr int kOrderUnroll = 4;
 constexpr int kOrderLdsTiles = 49152;
-__device__ __forceinline__ uint32_t tile_bucket(int2 r) {
-    const uint32_t c = (uint32_t)max(r.y - r.x, 0);
-    return c ? (uint32_t)min(31, 32 - __clz(c)) : 0u;
-}
-
-// Lanes of this wave (among `valid` ones) whose bucket equals this lane's.
-__device__ __forceinline__ uint64_t same_bucket(uint32_t b, bool valid) {
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int bit = 0; bit < 5; ++bit) {
-        const bool on = (b >> bit) & 1u;
-        const uint64_t bb = __ballot(on);
-        peers &= on ? bb : ~bb;
-    }
-    return peers;
-}
-
 __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const int2 *__restrict__ ranges,
                                                               int ntiles,
                                                               uint32_t *__restrict__ order,
+                                                              uint32_t *__restrict__ inv,
                                                               int2 *__restrict__ order_range) {
     __shared__ uint32_t h[32];
     __shared__ uint32_t wh[kOrderWaves][32];
@@ -825,6 +821,7 @@ __global__ __launch_bounds__(kOrderThreads) void k_tile_order(const int2 *__rest
         base = (uint32_t)__shfl((int)base, leader, kWave);
         if (valid) {
             order[base + rank] = (uint32_t)t;
+            inv[t] = base + rank;
             order_range[base + rank] = rt;  // read with the order: one load level less
         }
     }
@@ -879,7 +876,7 @@ struct ds_raster {
     ds::DevBuf<uint32_t> tcnt, tseg, ttot, tfirst;
     // per tile / pixel
     ds::DevBuf<int2> ranges;
-    ds::DevBuf<uint32_t> order;  // compositing dispatch order (heavy-first)
+    ds::DevBuf<uint32_t> order, order_inv;  // compositing dispatch order (heavy-first), inverse
     ds::DevBuf<int2> order_range;  // the range of tile order[b], at b
     ds::DevBuf<float> T;
     ds::DevBuf<int32_t> last;
@@ -890,6 +887,12 @@ struct ds_raster {
     ds::DevBuf<uint64_t> lb_depth;
     // results of the last forward
     bool order_valid = false;  // `order` was computed by the last forward
+    // The fitter step's dispatch order: the pair reduction computes each
+    // tile's position (order_inv) from the ranges of the step before
+    // (NextOrder), and the next forward's k_tile_offsets writes the order and
+    // the positions' ranges from it, instead of k_tile_order (one launch less
+    // per step; the pair counts change little between steps).
+    bool lag_order = false;
     const uint32_t *flat = nullptr;
     // each pair slot's splat before the tile sort (slots of a splat are
     // contiguous): the emitted values, or pair_splat where the tile sort
@@ -905,6 +908,12 @@ namespace ds {
 // The one-pass tile sort handles up to kTileBinsMax tiles of one instance.
 static bool one_pass_tile_sort(const ds_raster *r) {
     return r->ninst == 0 && r->ntiles <= kTileBinsMax;
+}
+
+// The dispatch order comes from the step before (see ds_raster::lag_order).
+static bool lagged_order(const ds_raster *r) {
+    static_assert(kTileBinsMax <= kNextOrderMaxTiles, "one-pass tile counts fit NextOrder");
+    return r->lag_order && one_pass_tile_sort(r) && r->tile_order == DS_TILE_ORDER_HEAVY_FIRST;
 }
 
 static ds_status raster_alloc_tiles(ds_raster *r);
@@ -940,7 +949,17 @@ static ds_status raster_alloc_tiles(ds_raster *r) {
     }
     DS_TRY(r->ranges.alloc((size_t)r->ntiles));
     DS_TRY(r->order.alloc((size_t)r->ntiles));
+    DS_TRY(r->order_inv.alloc((size_t)r->ntiles));
     DS_TRY(r->order_range.alloc((size_t)r->ntiles));
+    {  // the dispatch order is a permutation from the start: the identity
+        std::vector<uint32_t> id((size_t)r->ntiles);
+        for (size_t t = 0; t < id.size(); ++t) id[t] = (uint32_t)t;
+        if (!id.empty()) {
+            DS_TRY_HIP(hipMemcpy(r->order.ptr, id.data(), id.size() * 4, hipMemcpyHostToDevice));
+            DS_TRY_HIP(
+                hipMemcpy(r->order_inv.ptr, id.data(), id.size() * 4, hipMemcpyHostToDevice));
+        }
+    }
     DS_TRY(r->T.alloc((size_t)r->npix));
     DS_TRY(r->last.alloc((size_t)r->npix));
     r->has_fwd = false;
@@ -978,8 +997,11 @@ static ds_status reduce_pairs(ds_raster *r, const ds_splat2d &grad, hipStream_t 
                               bool grad_is_zero = false) {
     if (r->last_count <= 0 || !r->slot_splat) return DS_OK;
     uint32_t *sc = r->scalars.ptr;
+    const NextOrder next = lagged_order(r) ? NextOrder{(const int2 *)r->ranges.ptr, r->ntiles,
+                                                       r->order_inv.ptr}
+                                           : NextOrder{nullptr, 0, nullptr};
     return launch_pair_reduce(r->pair_rec.ptr, r->slot_splat, sc + SC_NISECT, sc + SC_EPOCH,
-                              (int64_t)r->isect_cap, grad, grad_is_zero, s);
+                              (int64_t)r->isect_cap, grad, grad_is_zero, s, next);
 }
 
 // Tile assignment, sorts, ranges and dispatch order, then compositing: the
@@ -1125,10 +1147,13 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
             hipLaunchKernelGGL(k_tile_segscan, dim3(tblocks), dim3(256), 0, s, r->tseg.ptr,
                                r->ttot.ptr, r->ntiles);
             DS_CHECK_LAUNCH("tile_segscan");
+            const bool lag = lagged_order(r);
             hipLaunchKernelGGL(k_tile_offsets, dim3(tblocks, kTileSegs), dim3(256), 0, s,
                                r->tcnt.ptr, (const uint32_t *)r->tseg.ptr,
                                (const uint32_t *)r->ttot.ptr, (const uint32_t *)sc, r->ntiles,
-                               r->ranges.ptr);
+                               r->ranges.ptr, lag ? (const uint32_t *)r->order_inv.ptr : nullptr,
+                               lag ? r->order.ptr : (uint32_t *)nullptr,
+                               lag ? r->order_range.ptr : (int2 *)nullptr);
             DS_CHECK_LAUNCH("tile_offsets");
             hipLaunchKernelGGL(k_tile_scatter, dim3(grid_cap(r->isect_cap, 256, 4096)),
                                dim3(256), 0, s, (const uint32_t *)r->tkey_a.ptr,
@@ -1175,10 +1200,10 @@ static ds_status raster_forward(ds_raster *r, const ds_splat2d &in, float *out_r
     } else {
         r->flat = r->tval_a.ptr;
     }
-    if (r->tile_order == DS_TILE_ORDER_HEAVY_FIRST) {
+    if (r->tile_order == DS_TILE_ORDER_HEAVY_FIRST && !lagged_order(r)) {
         hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, s,
                            (const int2 *)r->ranges.ptr, r->ntiles, r->order.ptr,
-                           r->order_range.ptr);
+                           r->order_inv.ptr, r->order_range.ptr);
         DS_CHECK_LAUNCH("tile_order");
     }
     r->order_valid = r->tile_order == DS_TILE_ORDER_HEAVY_FIRST;
@@ -1211,6 +1236,7 @@ ds_status raster_step_fwd_bwd(ds_raster *r, const ds_splat2d &in, float *out_rgb
     DS_REQUIRE(r, "NULL raster");
     DS_REQUIRE(target && loss && grad_rgb, "the fitter step needs the MSE target");
     DS_REQUIRE(grad_in.count == in.count, "grad_in count differs from the input's");
+    r->lag_order = true;  // each step's reduction orders the next step's tiles
     return raster_forward(r, in, out_rgb, target, loss, grad_rgb, s, &grad_in, &grad_in,
                           adam_tick, started);
 }
