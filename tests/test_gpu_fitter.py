@@ -141,3 +141,57 @@ def test_fused_adam_steps_the_reported_gradients(deg, pos_decay):
         assert ok, (t, mabs, mrel)
         p = got  # the next step starts from the GPU's parameters
     fit.close()
+
+
+def test_dispatch_order_from_the_step_before():
+    """The fitter step takes its heavy-first dispatch order from the step
+    before (the pair reduction computes each tile's position, the next
+    forward writes the order and each position's range). Whatever the order,
+    each tile is composited once with its own range: the image of a forward
+    after some steps, and that of a compositing re-run right after a step (the
+    reduction has already written the next order), equal a fresh
+    rasterizer's image of the same splats."""
+    import ctypes as C
+    from gpu_stages import Stages
+    fit, params, target, pv, cam = make(graph=True)
+    fit.set_tuning(1, 1)
+    view, proj, _ = ds.fitter_camera(W, H)
+    L = ds.lib()
+
+    def fresh_image(p):
+        st = Stages(p, N, 0, W, H, view, proj, cam, pix_per_thread=1)
+        st.activ_project_fwd()
+        return st.raster_fwd()["rgb"]
+
+    for k in range(3):
+        fit.step(1)
+        fit.synchronize()
+        fit.forward()
+        st = fit.raster_state()
+        order = st["tile_order"]
+        assert np.array_equal(np.sort(order), np.arange(len(order)))
+        ok, mabs, _ = close(fit.image(), fresh_image(fit.params()), atol=1e-6, rtol=1e-6)
+        print(f"[parity] lagged_order_forward_step{k}: max_abs={mabs:.3e}")
+        assert ok, (k, mabs)
+    # a compositing re-run right after a step, on the step's own sorted pairs
+    before = fit.params()
+    fit.step(1)
+    fit.synchronize()
+    g, s2, gs2 = ds._lib.Gauss3D(), ds._lib.Splat2D(), ds._lib.Splat2D()
+    grad_rgb = C.c_void_p()
+    ds.check(L.ds_fitter_views(fit.h, C.byref(g), C.byref(s2), C.byref(gs2), C.byref(grad_rgb)),
+             "ds_fitter_views")
+    ras = C.c_void_p()
+    ds.check(L.ds_fitter_get_raster(fit.h, C.byref(ras)), "ds_fitter_get_raster")
+    d_target = ds.DeviceArray.from_numpy(np.ascontiguousarray(target, np.float32))
+    d_rgb = ds.DeviceArray(4 * 3 * W * H)
+    d_loss = ds.DeviceArray(4)
+    ds.check(L.ds_raster_composite_fwd_bwd(ras, C.byref(s2), d_rgb.ptr, d_target.ptr, d_loss.ptr,
+                                           grad_rgb, C.byref(gs2), fit.stream()),
+             "ds_raster_composite_fwd_bwd")
+    fit.synchronize()
+    rerun = d_rgb.download(np.float32, 3 * W * H)
+    ok, mabs, _ = close(rerun, fresh_image(before), atol=1e-6, rtol=1e-6)
+    print(f"[parity] lagged_order_rerun: max_abs={mabs:.3e}")
+    assert ok, mabs
+    fit.close()
