@@ -537,8 +537,9 @@ __device__ __forceinline__ void composite_back_to_front(
                     // 1 / max(1 - alpha, 1e-6) of the reference: alpha <= 0.99, so the
                     // floor never applies
                     const float ra = __builtin_amdgcn_rcpf(1.f - alpha);
-                    // transmittance in front of the splat
-                    const float Tb = ok ? T[k] * ra : T[k];
+                    // transmittance in front of the splat (a skipped pixel: alpha 0,
+                    // ra = 1 exactly, T unchanged)
+                    const float Tb = T[k] * ra;
                     const float fac = alpha * Tb;
                     v[5] = fmaf(fac, gr[k], v[5]);
                     v[6] = fmaf(fac, gg[k], v[6]);
@@ -550,10 +551,11 @@ __device__ __forceinline__ void composite_back_to_front(
                     else
                         v_alpha = Tb * cgk - ra * acc[k];
                     const bool gate = ok & (e.og <= kMaxAlpha);  // alpha below its clamp
-                    v[8] = fmaf(gate ? e.g : 0.f, v_alpha, v[8]);
+                    const float gva = (gate ? e.g : 0.f) * v_alpha;
+                    v[8] += gva;
                     // m = -2 dL/d dist2 (dL/d dist2 = -o g v_alpha / 2); the
                     // per-field factors are applied once, after the reduction
-                    const float m = gate ? e.og * v_alpha : 0.f;
+                    const float m = B.y * gva;
                     const float pu = m * e.u, pw = m * e.w;
                     v[0] += pu;
                     v[1] += pw;
