@@ -48,9 +48,9 @@ BWD_FLOP_PER_EVAL = 60.0
 # Counters cannot be read live here; the line reports the state whose pair
 # count is nearest the measured one.
 COMPOSITE_TRAFFIC = {  # state: (pairs, {shape: (read bytes, write bytes)}, how it was reached)
-    "early": (1327683, {"1,1": (172.47e6, 71.44e6)},
+    "early": (1327685, {"1,1": (169.99e6, 72.07e6)},
               "25 steps from random splats (the default bench flags' state), 1.33M pairs"),
-    "final": (648148, {"1,1": (124.35e6, 65.27e6)},
+    "final": (648137, {"1,1": (120.65e6, 65.00e6)},
               "a 30-step run from the slab of a 2020-step fit, 648k pairs"),
 }
 
