@@ -15,9 +15,10 @@
 // broadcasts. The per-pixel work has no branches: a skipped splat is a
 // select that adds exact zeros. In the backward the nine per-splat gradient
 // sums are reduced across the wave by a transposed DPP reduction that leaves
-// one field per lane (no LDS traffic), amortised over P pixels per lane, and
-// nine lanes then issue one global float atomic instruction together
-// (through an LDS pre-reduction when the tile has more than one wave).
+// one field per lane (no LDS traffic), amortised over P pixels per lane; the
+// tile's waves add theirs in LDS, and each reached (tile, splat) pair's sums
+// go to one gradient record, which k_pair_reduce adds per splat (no global
+// float atomics in the compositing).
 // The transmittance is stored directly (not as 1 - T) so the backward's
 // division chain starts from an exact value.
 #include "composite.hpp"

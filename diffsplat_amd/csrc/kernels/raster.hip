@@ -756,7 +756,9 @@ __global__ __launch_bounds__(256) void k_tile_scatter(const uint32_t *__restrict
 // buckets per wave: the lanes of one bucket are found with five ballots and
 // their first lane adds their number to the wave's row (distinct buckets,
 // distinct words: no atomics). Pass 2 reserves each wave's slots with one
-// LDS atomic per bucket.
+// LDS atomic per bucket, and writes the order, its inverse and each
+// position's range. (A fitter's steps get the same order from the pair
+// reduction of the step before instead: see ds_raster::lag_order.)
 constexpr int kOrderThreads = 1024;
 constexpr int kOrderWaves = kOrderThreads / kWave;
 constexpr int kOrderUnroll = 4;
